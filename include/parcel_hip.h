@@ -1,0 +1,180 @@
+/*
+ * parcel_hip.h -- C ABI of the MI355X-native PARCEL Krylov library
+ * (libparcel_hip.so).  Plain C types only; every entry point returns an
+ * error code (PARCEL_HIP_OK = 0) instead of stopping the process.
+ *
+ * Array conventions follow the reference API (src/wrapper.f90:1995-2010):
+ * CRS arrays are 1-based (crsRow_ptr(n+1) offsets, crsCol global column ids),
+ * istart is the 1-based first global row of this rank; with one GPU holding
+ * one rank's whole subdomain, istart = 1 and gn = n.  Vector and matrix
+ * pointers may be host or device (HBM) pointers: the library inspects each
+ * one (the reference GPU build takes device pointers, src/wrapper.f90:569-573,
+ * the CPU build host arrays).  reshistory is always a host array of itrmax
+ * entries, as in the reference.
+ */
+#ifndef PARCEL_HIP_H
+#define PARCEL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define PARCEL_HIP_OK 0
+#define PARCEL_HIP_ERR_ARG (-1)        /* invalid argument */
+#define PARCEL_HIP_ERR_HIP (-2)        /* HIP runtime error (no device, OOM ...) */
+#define PARCEL_HIP_ERR_ZERO_DIAG (-3)  /* missing diagonal in a factorised block */
+#define PARCEL_HIP_ERR_UNSUPPORTED (-4)
+#define PARCEL_HIP_ERR_NOT_INIT (-5)
+
+/* solver ids, src/define.f90:14-22 */
+#define PARCEL_SOLVER_CG 1
+#define PARCEL_SOLVER_BICG 2
+#define PARCEL_SOLVER_GMRES 3
+#define PARCEL_SOLVER_CAGMRES 4
+#define PARCEL_SOLVER_CBCG 5
+
+/* preconditioner ids, src/define.f90:30-35 */
+#define PARCEL_PRECON_NONE 0
+#define PARCEL_PRECON_POINTJACOBI 1
+#define PARCEL_PRECON_BLOCKJACOBI 2
+#define PARCEL_PRECON_ADDITIVESCHWARZ 3
+
+/* ILU_method, src/define.f90:45-47 */
+#define PARCEL_ILU0 0
+#define PARCEL_ILUD 1
+#define PARCEL_MLUD 2
+
+/* return codes of the solvers (iret), src/define.f90:101-102 */
+#define PARCEL_RETURNCODE_NORMAL 0
+#define PARCEL_RETURNCODE_MAXITERATION 1
+
+/*
+ * Solver parameters: the fields of KSP_INPUT_PARAM (src/type.f90:907-946)
+ * that the solver wrappers fill (src/wrapper.f90:2040-2070).  A value < 0
+ * means "unset, use the default" exactly as INIT_PARCEL_INPUT
+ * (src/wrapper.f90:5-44) marks unset fields with -1.
+ */
+typedef struct parcel_hip_param {
+  int solver;
+  int ipreflag;
+  int ILU_method;
+  int addL;
+  int iflagAS;
+  int iovlflag;
+  int precon_thblock; /* block-Jacobi blocks; < 1: one per compute unit */
+  int Neuman_itr;
+  int independ_nvec;
+  int nBlock;
+  int GMRES_m;
+  int GMRES_GSflag;
+  int CAGMRES_sstep;
+  int CAGMRES_tstep;
+  int CAGMRES_basis;
+  int CAGMRES_QRflag;
+  int CBCG_kstep;
+  int CBCG_Eigenflag;
+  int power_method_itrmax;
+  int CAARNOLDI_sstep;
+  int CAARNOLDI_tstep;
+  int CAARNOLDI_basis;
+  int CAARNOLDI_QRflag;
+  int profile; /* 1: time kernel classes with HIP events during solves */
+} parcel_hip_param;
+
+/* Per-plan measurements (filled by setup and by every solve). */
+typedef struct parcel_hip_stats {
+  double setup_s;          /* host wall time of plan setup (upload + factorisation) */
+  double precon_setup_s;   /* part of setup_s spent on the preconditioner */
+  double solve_s;          /* host wall time of the last solve (inputs already in HBM) */
+  double solve_device_ms;  /* HIP-event time of the last solve on the plan's stream */
+  int niter;
+  int launches;            /* kernel launches issued by the last solve */
+  /* with param.profile = 1: summed HIP-event times of kernel classes */
+  double spmv_ms;          /* fused SpMV + CG update kernel */
+  int spmv_calls;
+  double precon_ms;        /* preconditioner apply */
+  int precon_calls;
+  double vec_ms;           /* vector update + reduction kernels */
+  int vec_calls;
+  /* preconditioner structure */
+  int bj_blocks;
+  int bj_levels_fwd;
+  int bj_levels_bwd;
+  long bj_entries;         /* stored factor entries (both sweeps) */
+} parcel_hip_stats;
+
+/* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */
+/* Selects device myrank % ndevices (src/CUDA/src/cu_common.cu:489-518). */
+int parcel_hip_init(int myrank);
+int parcel_hip_finalize(void);
+const char* parcel_hip_last_error(void);
+void parcel_hip_default_param(parcel_hip_param* prm);
+
+/* --- one-shot solver entry points, same argument lists and meaning as the
+ * reference wrappers minus the MPI communicator handle's semantics (icomm is
+ * accepted and ignored: one GPU = one rank).  itrmax is in: maximum
+ * iterations, out: iterations done.  Returns an error code; the solver's own
+ * status goes to *iret. */
+
+/* parcel_dcg (src/wrapper.f90:544-632 GPU build, :1995-2083 CPU build) */
+int parcel_hip_dcg(int icomm, double* vecx, const double* vecb, int n, int gn,
+                   int nnz, int istart, const double* crsA, const int* crsRow_ptr,
+                   const int* crsCol, int ipreflag, int ILU_method, int addL,
+                   int iflagAS, int* itrmax, double rtolmax, double* reshistory,
+                   int iovlflag, int precon_thblock, int Neuman_itr,
+                   int independ_nvec, int nBlock, int* iret);
+
+/* --- plan API: setup once (upload, factorise), solve many times.  This is the
+ * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
+ * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident
+ * in HBM across right-hand sides. */
+typedef struct parcel_hip_plan parcel_hip_plan;
+
+int parcel_hip_plan_create(parcel_hip_plan** plan, int n, int gn, int nnz, int istart,
+                           const double* crsA, const int* crsRow_ptr,
+                           const int* crsCol, const parcel_hip_param* prm);
+int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb,
+                          int* itrmax, double rtolmax, double* reshistory, int* iret);
+int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats);
+int parcel_hip_plan_destroy(parcel_hip_plan* plan);
+
+/* --- kernels exposed for unit parity tests and measurement --------------- */
+/* y = A x with the plan's matrix (src/matrix.f90:9051 MatVec_crs_FP64_CPU;
+ * GPU reference: cusparseSpMV, src/CUDA/src/cusparse.cu:632).  Device or host
+ * pointers of length n. */
+int parcel_hip_plan_spmv(parcel_hip_plan* plan, const double* x, double* y);
+/* s = M^-1 r with the plan's preconditioner (src/krylov_h.f90:1684 precon_crs) */
+int parcel_hip_plan_precon(parcel_hip_plan* plan, const double* r, double* s);
+/* <x,y> in the library's fixed reduction order (src/vec.f90:1027
+ * calc_dot_FP64_CPU; GPU reference src/CUDA/src/cu_common.cu:412 cuda_dot). */
+int parcel_hip_dot(int n, const double* x, const double* y, double* result);
+/* Time `reps` launches of the plan's dominant kernel class (1: fused CG SpMV,
+ * 2: preconditioner apply) with HIP events on the plan's stream; returns the
+ * mean duration in milliseconds. */
+int parcel_hip_plan_time_kernel(parcel_hip_plan* plan, int which, int reps, double* mean_ms);
+
+/* --- synthetic problem generators (host code): the counterpart of the
+ * reference's example generators (src/example/problem.f90) plus the build's
+ * own seeded generators.  Output CSR is 1-based with sorted columns. ------ */
+/* make_poisson3D_CRS (src/example/problem.f90:448-618): diag -6, off +1,
+ * whole domain (one rank). nnz of the result is returned in *nnz; pass NULL
+ * arrays to query the size. */
+int parcel_gen_poisson3d_parcel(int gnx, int gny, int gnz, int* nnz, int* rowptr,
+                                int* col, double* val);
+/* make_poisson3D_right_vector (src/example/problem.f90:772-875) */
+int parcel_gen_poisson3d_rhs_parcel(int gnx, int gny, int gnz, double* vecb, double* vecx_exact);
+/* make_poisson2D_CRS (src/example/problem.f90:210-340) scaled by `scale`
+ * (scale -1 gives the SPD 4/-1 operator) */
+int parcel_gen_poisson2d(int gnx, int gny, double scale, int* nnz, int* rowptr, int* col,
+                         double* val);
+/* 3D 7-point constant Poisson, diag 6 off -1 (SPD) */
+int parcel_gen_poisson3d_spd(int gnx, int gny, int gnz, int* nnz, int* rowptr, int* col,
+                             double* val);
+/* x ~ U[-1,1] from the counter-based stream `seed` (SplitMix64, 53-bit) */
+int parcel_gen_uniform(long n, unsigned long long seed, double lo, double hi, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,118 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+RED_SEQ, RED_TILED = 0, 1
+# the HIP library's reduction layout (parcel-legacy_amd/csrc/common.hpp)
+GPU_NT, GPU_NBMAX = 256, 2048
+
+
+class Red(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("nt", ctypes.c_int), ("nbmax", ctypes.c_int)]
+
+
+TILED = Red(RED_TILED, GPU_NT, GPU_NBMAX)
+SEQ = Red(RED_SEQ, GPU_NT, GPU_NBMAX)
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+_RP = ctypes.POINTER(Red)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError("oracle not built: %s" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "oracle_dot": (_D, [_RP, _I, _P, _P]),
+        "oracle_reduce_partials": (_D, [_P, _I]),
+        "oracle_divide_region": (None, [_I, _I, _I, _IP, _IP]),
+        "oracle_spmv": (None, [_I, _P, _P, _P, _P, _P]),
+        "oracle_precon_create": (_P, [_I, _I, _I, _I, _I, _P, _P, _P, _IP]),
+        "oracle_precon_apply": (None, [_P, _P, _P]),
+        "oracle_precon_free": (None, [_P]),
+        "oracle_precon_nblocks": (_I, [_P]),
+        "oracle_pcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def dot(x, y, red=TILED):
+    return load().oracle_dot(ctypes.byref(red), int(x.shape[0]), _p(x), _p(y))
+
+
+def reduce_partials(part):
+    return load().oracle_reduce_partials(_p(part), int(part.shape[0]))
+
+
+def divide_region(rank, nproc, n):
+    lo, hi = ctypes.c_int(), ctypes.c_int()
+    load().oracle_divide_region(rank, nproc, n, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def spmv(rowptr, col, val, x):
+    n = rowptr.shape[0] - 1
+    y = np.empty(n, np.float64)
+    load().oracle_spmv(n, _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+class Precon:
+    def __init__(self, rowptr, col, val, ipreflag, ilu_method=1, nblocks=1, independ_nvec=1):
+        self._keep = (rowptr, col, val)
+        self.n = rowptr.shape[0] - 1
+        err = ctypes.c_int(0)
+        self.h = load().oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, self.n, _p(rowptr),
+                                             _p(col), _p(val), ctypes.byref(err))
+        if not self.h:
+            raise RuntimeError("oracle precon setup failed: %d" % err.value)
+
+    def apply(self, r):
+        s = np.empty(self.n, np.float64)
+        load().oracle_precon_apply(self.h, _p(r), _p(s))
+        return s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_precon_free(self.h)
+            self.h = None
+
+
+def pcg(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=1000,
+        rtol=1e-8, red=TILED):
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 1))
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    rc = load().oracle_pcg(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+                           independ_nvec, itrmax, rtol, ctypes.byref(red), _p(hist), ctypes.byref(niter),
+                           ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_pcg failed: %d" % rc)
+    return x, hist[: niter.value].copy(), niter.value, iret.value

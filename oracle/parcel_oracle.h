@@ -1,0 +1,86 @@
+/*
+ * parcel_oracle.h -- CPU oracle for the PARCEL Krylov hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, as the checker and as the CPU baseline.
+ *
+ * Every routine restates the reference algorithm in the reference's own
+ * operation order; each one cites the reference file:line it follows
+ * (paths relative to the PARCEL 1.2 source tree).
+ *
+ * Conventions shared with the reference (src/wrapper.f90:1995-2010):
+ *   - CSR is 1-based: rowptr[0..n] holds 1-based offsets, col[] holds 1-based
+ *     global column ids; the oracle is single-rank, so istart = 1, gn = n.
+ *   - every `a*b + c` of the reference is evaluated as one fused multiply-add
+ *     fma(a,b,c); the reference is built with -O3 -march=native
+ *     (arch/make_config:17), which contracts these loops the same way.
+ *     The oracle itself is compiled with -ffp-contract=off so that no other
+ *     contraction happens.
+ *
+ * Reduction order.  The reference's dots are OpenMP/MPI reductions whose
+ * order depends on the thread and rank count (src/vec.f90:1008-1045).  The
+ * oracle commits to one of two documented orders, chosen per call:
+ *   ORACLE_RED_SEQ   : left-to-right fma accumulation over i = 0..n-1;
+ *   ORACLE_RED_TILED : the tiled tree order the HIP kernels use (see
+ *                      oracle_dot() for the exact definition), so GPU and
+ *                      oracle agree bit for bit.
+ */
+#ifndef PARCEL_ORACLE_H
+#define PARCEL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_RED_SEQ = 0, ORACLE_RED_TILED = 1 };
+
+/* reduction layout: mode, threads per tile (nt), max number of tiles owners
+ * (nbmax). For ORACLE_RED_TILED the number of owner blocks is
+ * nb = min(ceil(n/nt), nbmax). */
+typedef struct {
+  int mode;
+  int nt;
+  int nbmax;
+} oracle_red;
+
+/* --- reductions --------------------------------------------------------- */
+double oracle_dot(const oracle_red* red, int n, const double* x, const double* y);
+/* Reduce nb partial sums in the tiled order used by the consumers. */
+double oracle_reduce_partials(const double* part, int nb);
+
+/* --- block layout (src/common.f90:125-182) ------------------------------ */
+/* start/end (1-based, inclusive) of block `rank` of `nproc` over [1, n] */
+void oracle_divide_region(int rank, int nproc, int n, int* lo, int* hi);
+
+/* --- SpMV (src/matrix.f90:9051-9110) ------------------------------------ */
+void oracle_spmv(int n, const int* rowptr, const int* col, const double* val,
+                 const double* x, double* y);
+
+/* --- preconditioners (src/krylov.f90:1278-1350, src/krylov_h.f90:1684) --- */
+typedef struct oracle_precon oracle_precon;
+/* ipreflag: 0 none, 1 point Jacobi, 2 block Jacobi.
+ * ilu_method: 0 ILU(0), 1 D-ILU (ILUD), 2 D-ILU (MLUD).
+ * nblocks: number of contiguous row blocks (divide_region), >= 1.
+ * Returns NULL on error (zero diagonal inside a block), with *err set. */
+oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
+                                    int independ_nvec, int n, const int* rowptr,
+                                    const int* col, const double* val, int* err);
+void oracle_precon_apply(const oracle_precon* p, const double* r, double* s);
+void oracle_precon_free(oracle_precon* p);
+/* Inspect the factor data (for tests). */
+int oracle_precon_nblocks(const oracle_precon* p);
+const double* oracle_precon_dinv(const oracle_precon* p);
+
+/* --- solvers ----------------------------------------------------------- */
+/* KSP_pCG (src/pcg.f90:224-892).  x is the initial guess in, solution out.
+ * hist must hold itrmax entries.  Returns 0, or a negative error code. */
+int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
+               const double* b, double* x, int ipreflag, int ilu_method,
+               int nblocks, int independ_nvec, int itrmax, double rtol,
+               const oracle_red* red, double* hist, int* niter, int* iret);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

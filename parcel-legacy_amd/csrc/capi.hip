@@ -1,0 +1,339 @@
+// capi.hip -- extern "C" entry points of libparcel_hip.so (include/parcel_hip.h).
+//
+// The one-shot entry points keep the argument lists of the reference Fortran
+// wrappers (src/wrapper.f90); internally each one is plan_create +
+// plan_solve + plan_destroy, the split KSP_CRS performs (src/krylov.f90:447-735).
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+using namespace parcel;
+
+struct parcel_hip_plan {
+  Plan P;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+int g_device = -1;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+template <class T>
+void fetch(const T* src, size_t count, std::vector<T>& dst) {
+  dst.resize(count);
+  if (!count) return;
+  if (is_device_ptr(src))
+    PARCEL_HIP_CHECK(hipMemcpy(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost));
+  else
+    std::memcpy(dst.data(), src, count * sizeof(T));
+}
+
+int ensure_init() {
+  if (g_device >= 0) return PARCEL_HIP_OK;
+  return parcel_hip_init(0);
+}
+
+#define PARCEL_API_BEGIN try {
+#define PARCEL_API_END                                               \
+  }                                                                  \
+  catch (const hip_error& e) {                                       \
+    return fail(e.code < 0 ? e.code : PARCEL_HIP_ERR_HIP, e.what()); \
+  }                                                                  \
+  catch (const std::exception& e) {                                  \
+    return fail(PARCEL_HIP_ERR_HIP, e.what());                       \
+  }
+
+void fill_defaults(parcel_hip_param* p) {
+  // PARCEL_KSP_Default_Setting (src/type.f90:1007-1069)
+  p->solver = PARCEL_SOLVER_CG;
+  p->ipreflag = PARCEL_PRECON_NONE;
+  p->ILU_method = PARCEL_ILUD;
+  p->addL = 0;
+  p->iflagAS = 4;
+  p->iovlflag = 0;
+  p->precon_thblock = -1;
+  p->Neuman_itr = 1;
+  p->independ_nvec = 1;
+  p->nBlock = 2000;
+  p->GMRES_m = 4;
+  p->GMRES_GSflag = 1;
+  p->CAGMRES_sstep = 4;
+  p->CAGMRES_tstep = 1;
+  p->CAGMRES_basis = 0;
+  p->CAGMRES_QRflag = 4;
+  p->CBCG_kstep = 4;
+  p->CBCG_Eigenflag = 2;
+  p->power_method_itrmax = 100;
+  p->CAARNOLDI_sstep = 4;
+  p->CAARNOLDI_tstep = 10;
+  p->CAARNOLDI_basis = 1;
+  p->CAARNOLDI_QRflag = 3;
+  p->profile = 0;
+}
+
+// Merge user values over defaults (values < 0 keep the default), as
+// SET_KSP_PARAM does with the -1 markers of INIT_PARCEL_INPUT.
+parcel_hip_param merged(const parcel_hip_param* user) {
+  parcel_hip_param d;
+  fill_defaults(&d);
+  if (!user) return d;
+  parcel_hip_param r = *user;
+  const int* u = reinterpret_cast<const int*>(user);
+  int* o = reinterpret_cast<int*>(&r);
+  const int* df = reinterpret_cast<const int*>(&d);
+  for (size_t k = 0; k < sizeof(parcel_hip_param) / sizeof(int); ++k) o[k] = u[k] < 0 ? df[k] : u[k];
+  // precon_thblock < 1 keeps its meaning (one block per compute unit)
+  r.precon_thblock = user->precon_thblock;
+  if (r.independ_nvec < 1) r.independ_nvec = 1;  // type.f90:1377
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int parcel_hip_init(int myrank) {
+  PARCEL_API_BEGIN
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1) return fail(PARCEL_HIP_ERR_HIP, "no HIP device available");
+  int dev = (myrank < 0 ? 0 : myrank) % count;
+  PARCEL_HIP_CHECK(hipSetDevice(dev));
+  g_device = dev;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_finalize(void) {
+  g_device = -1;
+  return PARCEL_HIP_OK;
+}
+
+const char* parcel_hip_last_error(void) { return g_last_error.c_str(); }
+
+void parcel_hip_default_param(parcel_hip_param* prm) {
+  if (prm) fill_defaults(prm);
+}
+
+int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int istart,
+                           const double* crsA, const int* crsRow_ptr, const int* crsCol,
+                           const parcel_hip_param* prm) {
+  PARCEL_API_BEGIN
+  if (!out) return fail(PARCEL_HIP_ERR_ARG, "plan pointer is NULL");
+  *out = nullptr;
+  if (n < 1 || nnz < 0 || !crsA || !crsRow_ptr || !crsCol)
+    return fail(PARCEL_HIP_ERR_ARG, "invalid matrix arguments");
+  if (gn != n || istart != 1)
+    return fail(PARCEL_HIP_ERR_UNSUPPORTED,
+                "one GPU holds one rank's whole system: expected gn == n and istart == 1");
+  int rc = ensure_init();
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  auto* h = new parcel_hip_plan();
+  Plan& P = h->P;
+  P.device = g_device;
+  P.prm = merged(prm);
+  try {
+    PARCEL_HIP_CHECK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    HostCSR& A = P.hA;
+    A.n = n;
+    fetch(crsRow_ptr, (size_t)n + 1, A.rowptr);
+    fetch(crsCol, (size_t)nnz, A.col);
+    fetch(crsA, (size_t)nnz, A.val);
+    if (A.rowptr[0] != 1 || A.rowptr[n] != nnz + 1)
+      throw hip_error("crsRow_ptr must be 1-based with crsRow_ptr(n+1) = nnz+1", PARCEL_HIP_ERR_ARG);
+    for (int i = 0; i <= n; ++i) A.rowptr[i] -= 1;
+    for (int i = 0; i < n; ++i)
+      if (A.rowptr[i + 1] < A.rowptr[i]) throw hip_error("crsRow_ptr is not monotone", PARCEL_HIP_ERR_ARG);
+    for (long j = 0; j < nnz; ++j) {
+      int c = A.col[j] - istart;
+      if (c < 0 || c >= n)
+        throw hip_error("column index outside this rank's rows (halo columns need >1 rank)",
+                        PARCEL_HIP_ERR_UNSUPPORTED);
+      A.col[j] = c;
+    }
+    P.A.n = n;
+    P.A.nnz = nnz;
+    P.A.rowptr.alloc(n + 1);
+    P.A.col.alloc(nnz);
+    P.A.val.alloc(nnz);
+    P.A.rowptr.upload(A.rowptr.data(), n + 1, P.stream);
+    P.A.col.upload(A.col.data(), nnz, P.stream);
+    P.A.val.upload(A.val.data(), nnz, P.stream);
+    PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+    if (P.prm.solver != PARCEL_SOLVER_CG)
+      throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
+    precon_setup(P);
+    P.stats.precon_setup_s = P.M.setup_seconds;
+    P.stats.bj_blocks = P.M.nblocks;
+    P.stats.bj_levels_fwd = P.M.fwd.nlev;
+    P.stats.bj_levels_bwd = P.M.bwd.nlev;
+    P.stats.bj_entries = P.M.fwd.entries + P.M.bwd.entries;
+    P.hA = HostCSR();  // setup copy no longer needed
+    plan_workspace(P, 1);
+    PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  } catch (...) {
+    if (P.stream) (void)hipStreamDestroy(P.stream);
+    delete h;
+    throw;
+  }
+  P.stats.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = h;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_plan_destroy(parcel_hip_plan* plan) {
+  PARCEL_API_BEGIN
+  if (!plan) return PARCEL_HIP_OK;
+  hipStream_t s = plan->P.stream;
+  if (s) (void)hipStreamSynchronize(s);
+  delete plan;
+  if (s) (void)hipStreamDestroy(s);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats) {
+  if (!plan || !stats) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  *stats = plan->P.stats;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb, int* itrmax,
+                          double rtolmax, double* reshistory, int* iret) {
+  PARCEL_API_BEGIN
+  if (!plan || !vecx || !vecb || !itrmax || !iret) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  Plan& P = plan->P;
+  PARCEL_HIP_CHECK(hipSetDevice(P.device));
+  const int n = P.A.n;
+  const int maxit = *itrmax;
+  const bool xdev = is_device_ptr(vecx), bdev = is_device_ptr(vecb);
+  double* x = vecx;
+  const double* b = vecb;
+  if (!xdev) {
+    if (P.x.n < (size_t)n) P.x.alloc(n);
+    P.x.upload(vecx, n, P.stream);
+    x = P.x.p;
+  }
+  if (!bdev) {
+    if (P.b.n < (size_t)n) P.b.alloc(n);
+    P.b.upload(vecb, n, P.stream);
+    b = P.b.p;
+  }
+  int niter = 0, ret = 0;
+  int rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret);
+  if (rc) return rc;
+  if (!xdev) {
+    PARCEL_HIP_CHECK(hipMemcpyAsync(vecx, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, P.stream));
+  }
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  *itrmax = niter;  // wrapper.f90:630 itrmax = niter
+  *iret = ret;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_dcg(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                   const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                   int ILU_method, int addL, int iflagAS, int* itrmax, double rtolmax,
+                   double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                   int independ_nvec, int nBlock, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  for (size_t k = 0; k < sizeof(prm) / sizeof(int); ++k) reinterpret_cast<int*>(&prm)[k] = -1;
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  prm.profile = 0;
+  parcel_hip_plan* plan = nullptr;
+  int rc = parcel_hip_plan_create(&plan, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &prm);
+  if (rc) return rc;
+  rc = parcel_hip_plan_solve(plan, vecx, vecb, itrmax, rtolmax, reshistory, iret);
+  parcel_hip_plan_destroy(plan);
+  return rc;
+}
+
+int parcel_hip_plan_spmv(parcel_hip_plan* plan, const double* x, double* y) {
+  PARCEL_API_BEGIN
+  if (!plan || !x || !y) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  Plan& P = plan->P;
+  const int n = P.A.n;
+  DevBuf<double> tx, ty;
+  const double* dx = x;
+  double* dy = y;
+  if (!is_device_ptr(x)) { tx.alloc(n); tx.upload(x, n, P.stream); dx = tx.p; }
+  bool ydev = is_device_ptr(y);
+  if (!ydev) { ty.alloc(n); dy = ty.p; }
+  spmv(P, dx, dy);
+  if (!ydev) PARCEL_HIP_CHECK(hipMemcpyAsync(y, dy, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, P.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_plan_precon(parcel_hip_plan* plan, const double* r, double* s) {
+  PARCEL_API_BEGIN
+  if (!plan || !r || !s) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  Plan& P = plan->P;
+  const int n = P.A.n;
+  DevBuf<double> tr, ts;
+  const double* dr = r;
+  double* ds = s;
+  if (!is_device_ptr(r)) { tr.alloc(n); tr.upload(r, n, P.stream); dr = tr.p; }
+  bool sdev = is_device_ptr(s);
+  if (!sdev) { ts.alloc(n); ds = ts.p; }
+  precon_apply(P, dr, ds, nullptr);
+  if (!sdev) PARCEL_HIP_CHECK(hipMemcpyAsync(s, ds, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, P.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_dot(int n, const double* x, const double* y, double* result) {
+  PARCEL_API_BEGIN
+  if (n < 0 || !x || !y || !result) return fail(PARCEL_HIP_ERR_ARG, "invalid argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  DevBuf<double> tx, ty;
+  const double* dx = x;
+  const double* dy = y;
+  if (!is_device_ptr(x)) { tx.alloc(n); tx.upload(x, n, nullptr); dx = tx.p; }
+  if (!is_device_ptr(y)) { ty.alloc(n); ty.upload(y, n, nullptr); dy = ty.p; }
+  *result = dot_device(n, dx, dy, nullptr);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_plan_time_kernel(parcel_hip_plan* plan, int which, int reps, double* mean_ms) {
+  PARCEL_API_BEGIN
+  if (!plan || !mean_ms || reps < 1 || (which != 1 && which != 2))
+    return fail(PARCEL_HIP_ERR_ARG, "invalid argument");
+  *mean_ms = time_kernel(plan->P, which, reps);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+"""ctypes binding of libparcel_hip.so (C ABI declared in include/parcel_hip.h).
+
+Host-side mirror of the reference's solver interface: `dcg` keeps the
+argument meaning of `parcel_dcg` (src/wrapper.f90:1995-2083) -- 1-based CRS,
+`itrmax` in/out, residual history, `iret` -- and `Plan` exposes the
+setup/solve split of KSP_CRS (src/krylov.f90:447-735).
+
+Arrays may be numpy arrays (host) or objects with `data_ptr()` (torch device
+tensors, resident in HBM).  The library is loaded from lib/ next to this file;
+if it is missing, loading raises instead of falling back to anything.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libparcel_hip.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_ZERO_DIAG = -3
+ERR_UNSUPPORTED = -4
+
+SOLVER_CG, SOLVER_BICG, SOLVER_GMRES, SOLVER_CAGMRES, SOLVER_CBCG = 1, 2, 3, 4, 5
+PRECON_NONE, PRECON_PJ, PRECON_BJ, PRECON_AS = 0, 1, 2, 3
+ILU0, ILUD, MLUD = 0, 1, 2
+
+_PARAM_FIELDS = [
+    "solver", "ipreflag", "ILU_method", "addL", "iflagAS", "iovlflag", "precon_thblock",
+    "Neuman_itr", "independ_nvec", "nBlock", "GMRES_m", "GMRES_GSflag", "CAGMRES_sstep",
+    "CAGMRES_tstep", "CAGMRES_basis", "CAGMRES_QRflag", "CBCG_kstep", "CBCG_Eigenflag",
+    "power_method_itrmax", "CAARNOLDI_sstep", "CAARNOLDI_tstep", "CAARNOLDI_basis",
+    "CAARNOLDI_QRflag", "profile",
+]
+
+
+class Param(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int) for f in _PARAM_FIELDS]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("setup_s", ctypes.c_double),
+        ("precon_setup_s", ctypes.c_double),
+        ("solve_s", ctypes.c_double),
+        ("solve_device_ms", ctypes.c_double),
+        ("niter", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("spmv_ms", ctypes.c_double),
+        ("spmv_calls", ctypes.c_int),
+        ("precon_ms", ctypes.c_double),
+        ("precon_calls", ctypes.c_int),
+        ("vec_ms", ctypes.c_double),
+        ("vec_calls", ctypes.c_int),
+        ("bj_blocks", ctypes.c_int),
+        ("bj_levels_fwd", ctypes.c_int),
+        ("bj_levels_bwd", ctypes.c_int),
+        ("bj_entries", ctypes.c_long),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every symbol the header declares, with (restype, argtypes).
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_IP = ctypes.POINTER(ctypes.c_int)
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+SYMBOLS = {
+    "parcel_hip_init": (_I, [_I]),
+    "parcel_hip_finalize": (_I, []),
+    "parcel_hip_last_error": (ctypes.c_char_p, []),
+    "parcel_hip_default_param": (None, [ctypes.POINTER(Param)]),
+    "parcel_hip_dcg": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                            _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
+                                    ctypes.POINTER(Param)]),
+    "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
+    "parcel_hip_plan_stats": (_I, [_P, ctypes.POINTER(Stats)]),
+    "parcel_hip_plan_destroy": (_I, [_P]),
+    "parcel_hip_plan_spmv": (_I, [_P, _P, _P]),
+    "parcel_hip_plan_precon": (_I, [_P, _P, _P]),
+    "parcel_hip_dot": (_I, [_I, _P, _P, _DP]),
+    "parcel_hip_plan_time_kernel": (_I, [_P, _I, _I, _DP]),
+    "parcel_gen_poisson3d_parcel": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
+    "parcel_gen_poisson3d_rhs_parcel": (_I, [_I, _I, _I, _P, _P]),
+    "parcel_gen_poisson2d": (_I, [_I, _I, _D, _IP, _P, _P, _P]),
+    "parcel_gen_poisson3d_spd": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
+    "parcel_gen_uniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
+}
+
+_lib = None
+
+
+class ParcelError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("parcel_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libparcel_hip.so; raises if the library was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError("libparcel_hip.so not built: %s (run __graft_entry__.build())" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != OK:
+        msg = load().parcel_hip_last_error()
+        raise ParcelError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    """Pointer of a numpy array (host) or a torch tensor (device or host)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def default_param(**kw):
+    p = Param()
+    load().parcel_hip_default_param(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, int(v))
+    return p
+
+
+# --- generators ---------------------------------------------------------------
+
+def _gen_csr(fn, *args):
+    lib = load()
+    nnz = ctypes.c_int(0)
+    _check(fn(*args, ctypes.byref(nnz), None, None, None))
+    return nnz.value
+
+
+def poisson3d_parcel(gnx, gny, gnz):
+    """make_poisson3D_CRS (src/example/problem.f90:448-618): diag -6, off +1."""
+    lib = load()
+    nnz = _gen_csr(lib.parcel_gen_poisson3d_parcel, gnx, gny, gnz)
+    n = gnx * gny * gnz
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    c = ctypes.c_int(0)
+    _check(lib.parcel_gen_poisson3d_parcel(gnx, gny, gnz, ctypes.byref(c), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def poisson3d_spd(gnx, gny, gnz):
+    """3D 7-point constant-coefficient Poisson, diag 6, off -1."""
+    lib = load()
+    nnz = _gen_csr(lib.parcel_gen_poisson3d_spd, gnx, gny, gnz)
+    n = gnx * gny * gnz
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    c = ctypes.c_int(0)
+    _check(lib.parcel_gen_poisson3d_spd(gnx, gny, gnz, ctypes.byref(c), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def poisson3d_rhs_parcel(gnx, gny, gnz):
+    """make_poisson3D_right_vector (src/example/problem.f90:772-875): (b, x_exact)."""
+    n = gnx * gny * gnz
+    b = np.empty(n, np.float64)
+    x = np.empty(n, np.float64)
+    _check(load().parcel_gen_poisson3d_rhs_parcel(gnx, gny, gnz, _ptr(b), _ptr(x)))
+    return b, x
+
+
+def poisson2d(gnx, gny, scale=1.0):
+    """make_poisson2D_CRS (src/example/problem.f90:210-340) times `scale`."""
+    lib = load()
+    nnz = ctypes.c_int(0)
+    _check(lib.parcel_gen_poisson2d(gnx, gny, scale, ctypes.byref(nnz), None, None, None))
+    n = gnx * gny
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    _check(lib.parcel_gen_poisson2d(gnx, gny, scale, ctypes.byref(nnz), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def uniform(n, seed, lo=-1.0, hi=1.0):
+    x = np.empty(n, np.float64)
+    _check(load().parcel_gen_uniform(n, seed, lo, hi, _ptr(x)))
+    return x
+
+
+# --- solvers -------------------------------------------------------------------
+
+def init(myrank=0):
+    _check(load().parcel_hip_init(myrank))
+
+
+def dcg(x, b, rowptr, col, val, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=1000, rtolmax=1e-8,
+        precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000):
+    """parcel_dcg: solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1), np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    rc = lib.parcel_hip_dcg(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+                            ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                            iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock,
+                            ctypes.byref(iret))
+    _check(rc)
+    return hist[: it.value].copy(), it.value, iret.value
+
+
+class Plan:
+    """Setup once, solve many (parcel_hip_plan_*)."""
+
+    def __init__(self, rowptr, col, val, param=None, **kw):
+        lib = load()
+        self.lib = lib
+        self.n = int(rowptr.shape[0]) - 1
+        nnz = int(col.shape[0])
+        p = param if param is not None else default_param(**kw)
+        h = ctypes.c_void_p()
+        _check(lib.parcel_hip_plan_create(ctypes.byref(h), self.n, self.n, nnz, 1, _ptr(val), _ptr(rowptr),
+                                          _ptr(col), ctypes.byref(p)))
+        self.h = h
+
+    def solve(self, x, b, itrmax, rtolmax):
+        hist = np.zeros(max(itrmax, 1), np.float64)
+        it = ctypes.c_int(itrmax)
+        iret = ctypes.c_int(-1)
+        _check(self.lib.parcel_hip_plan_solve(self.h, _ptr(x), _ptr(b), ctypes.byref(it), rtolmax, _ptr(hist),
+                                              ctypes.byref(iret)))
+        return hist[: it.value].copy(), it.value, iret.value
+
+    def spmv(self, x, y):
+        _check(self.lib.parcel_hip_plan_spmv(self.h, _ptr(x), _ptr(y)))
+
+    def precon(self, r, s):
+        _check(self.lib.parcel_hip_plan_precon(self.h, _ptr(r), _ptr(s)))
+
+    def stats(self):
+        st = Stats()
+        _check(self.lib.parcel_hip_plan_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def time_kernel(self, which, reps):
+        ms = ctypes.c_double(0.0)
+        _check(self.lib.parcel_hip_plan_time_kernel(self.h, which, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            self.lib.parcel_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dot(x, y):
+    r = ctypes.c_double(0.0)
+    _check(load().parcel_hip_dot(int(x.shape[0]), _ptr(x), _ptr(y), ctypes.byref(r)))
+    return r.value
