@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-solution of preconditioned CG at relative residual 1e-10
+on one MI355X, with HBM GB/s against the roofline.
+
+Workload (default): 3D 7-point constant-coefficient Poisson 256^3
+(16,777,216 unknowns, 117,047,296 nonzeros, diag 6 / off -1), x_true ~ U[-1,1]
+from SplitMix64 seed 3, b = A x_true, x0 = 0, rtol 1e-10.  Methods:
+  bj  FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks (headline)
+  pj  FP64 PCG, point Jacobi (the SpMV/CG path)
+A "step" is one full solve from x0 = 0 with the matrix, b and x already in
+HBM; `value` is the mean time-to-solution of the headline method.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "parcel-legacy_amd"), os.path.join(ROOT, "oracle")]
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "time-to-solution at rel res 1e-10 (s) and HBM GB/s: learned/mixed vs plain PCG"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def k1_bytes(n, nnz):
+    """Fused CG kernel (SpMV + p/x update + <p,q>): CSR 12 B/nnz + row pointers
+    4(n+1) + x read/write 16n + p_old 8n + s 8n + p_new 8n + q 8n."""
+    return 12 * nnz + 4 * (n + 1) + 48 * n
+
+
+def ilu0_apply_bytes(n, nnz_l, nnz_u):
+    """Block-Jacobi ILU(0) apply (forward + backward sweep): 12 B per factor
+    entry, row ids 4n per sweep, r 8n, s written 8n (forward), s read+write
+    16n and u_ii 8n (backward)."""
+    return 12 * (nnz_l + nnz_u) + 8 * n + 8 * n + 8 * n + 16 * n + 8 * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--methods", default="bj,pj")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one GPU holds one rank's whole subdomain); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import parcel_hip as ph
+
+    ph.init(0)
+    g = args.grid
+    n = g ** 3
+    t0 = time.perf_counter()
+    rp, col, val = ph.poisson3d_spd(g, g, g)
+    nnz = int(col.shape[0])
+    xt = ph.uniform(n, 3)
+    log("generated %d^3: n=%d nnz=%d in %.2fs" % (g, n, nnz, time.perf_counter() - t0))
+    D = ph.DeviceArray
+    xt_d = D(xt)
+    b_d = D(n=n)
+    x_d = D(n=n)
+    r_d = D(n=n)
+
+    methods = {}
+    for m in args.methods.split(","):
+        if m == "bj":
+            kw = dict(ipreflag=2, ILU_method=0, precon_thblock=256)
+        elif m == "pj":
+            kw = dict(ipreflag=1)
+        else:
+            raise SystemExit("unknown method " + m)
+        t0 = time.perf_counter()
+        plan = ph.Plan(rp, col, val, profile=1, **kw)
+        setup = time.perf_counter() - t0
+        plan.spmv(xt_d, b_d)
+        for _ in range(args.warmup):
+            x_d.zero()
+            plan.solve(x_d, b_d, 20000, 1e-10)
+        times = []
+        dev_ms = []
+        st = None
+        for k in range(args.steps):
+            x_d.zero()
+            ph.load().parcel_hip_device_synchronize()
+            t1 = time.perf_counter()
+            hist, niter, iret = plan.solve(x_d, b_d, 20000, 1e-10)
+            times.append(time.perf_counter() - t1)
+            st = plan.stats()
+            dev_ms.append(st["solve_device_ms"])
+            log("%s step %d: %.4f s, niter %d, iret %d, final %.3e" % (m, k, times[-1], niter, iret, hist[-1]))
+        # true residual of the last solve
+        plan.spmv(x_d, r_d)
+        bb = b_d.numpy()
+        true_res = float(np.linalg.norm(bb - r_d.numpy()) / np.linalg.norm(bb))
+        k1_ms = st["spmv_ms"] / max(st["spmv_calls"], 1)
+        pre_ms = st["precon_ms"] / max(st["precon_calls"], 1) if st["precon_calls"] else 0.0
+        vec_ms = st["vec_ms"] / max(st["vec_calls"], 1) if st["vec_calls"] else 0.0
+        kernels = {"cg_spmv_fused": {"mean_ms": k1_ms, "calls_per_solve": st["spmv_calls"],
+                                     "bytes": k1_bytes(n, nnz),
+                                     "GBs": k1_bytes(n, nnz) / (k1_ms * 1e-3) / 1e9}}
+        if m == "bj":
+            pb = ilu0_apply_bytes(n, st["bj_nnz_l"], st["bj_nnz_u"])
+            kernels["bj_ilu0_apply"] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"], "bytes": pb,
+                                        "GBs": pb / (pre_ms * 1e-3) / 1e9, "levels_fwd": st["bj_levels_fwd"],
+                                        "levels_bwd": st["bj_levels_bwd"]}
+        kernels["vector_reductions"] = {"mean_ms": vec_ms, "calls_per_solve": st["vec_calls"]}
+        methods[m] = {"time_to_solution_s": float(np.mean(times)), "device_ms": float(np.mean(dev_ms)),
+                      "niter": niter, "iret": iret, "final_recurrence_res": float(hist[-1]),
+                      "true_rel_res": true_res, "setup_s": setup, "precon_setup_s": st["precon_setup_s"],
+                      "kernels": kernels}
+        plan.close()
+
+    head = "bj" if "bj" in methods else list(methods)[0]
+    H = methods[head]
+    # dominant kernel of the headline method: the one with most time per solve
+    dom_name, dom = max(((k, v) for k, v in H["kernels"].items() if "bytes" in v),
+                        key=lambda kv: kv[1]["mean_ms"] * kv[1]["calls_per_solve"])
+    achieved = dom["GBs"]
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        hist_len = args.cpu_iters
+        kwm = dict(ipreflag=2, ilu_method=0, nblocks=256) if head == "bj" else dict(ipreflag=1)
+        b_h = b_d.numpy()
+        t0 = time.perf_counter()
+        _, ho, no, _ = orc.pcg(rp, col, val, b_h, itrmax=hist_len, rtol=1e-10, **kwm)
+        tset, titer = orc.last_times()
+        per_it = titer / no
+        cpu = {"value": round(per_it * H["niter"], 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": "oracle PCG (%s) on the same 256^3 system for %d iterations (%.3f s/iteration, "
+                         "setup %.1f s excluded), scaled to the GPU's %d iterations" %
+                         (head, no, per_it, tset, H["niter"])}
+        log("cpu baseline: %.3f s/it over %d its" % (per_it, no))
+
+    line = {
+        "metric": METRIC,
+        "value": round(H["time_to_solution_s"], 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(H["time_to_solution_s"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D 7-point Poisson %d^3 (n=%d, nnz=%d), x_true~U[-1,1] seed 3, b=A x_true, x0=0, "
+                        "rtol 1e-10" % (g, n, nnz),
+            "method": {"bj": "FP64 PCG + block-Jacobi ILU(0), 256 row blocks",
+                       "pj": "FP64 PCG + point Jacobi"}[head],
+            "methods": methods,
+            "parallelism": "1 GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -101,7 +101,9 @@ typedef struct parcel_hip_stats {
   int bj_blocks;
   int bj_levels_fwd;
   int bj_levels_bwd;
-  long bj_entries;         /* stored factor entries (both sweeps) */
+  long bj_entries;         /* stored factor entries (both sweeps, with padding) */
+  long bj_nnz_l;           /* off-diagonal entries read by the forward sweep */
+  long bj_nnz_u;           /* off-diagonal entries read by the backward sweep */
 } parcel_hip_stats;
 
 /* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */
@@ -153,6 +155,17 @@ int parcel_hip_dot(int n, const double* x, const double* y, double* result);
  * 2: preconditioner apply) with HIP events on the plan's stream; returns the
  * mean duration in milliseconds. */
 int parcel_hip_plan_time_kernel(parcel_hip_plan* plan, int which, int reps, double* mean_ms);
+
+/* --- device memory helpers: the counterpart of cudaMalloc_FP64 /
+ * cudaMemcpy_h2d_FP64 / cudaFree of the reference drivers
+ * (src/CUDA/src/cu_common.cu:573,850; src/c_common.f90:206), which put the
+ * matrix and vectors in device memory before calling the GPU-build wrappers.
+ * kind: 1 host->device, 2 device->host, 3 device->device. */
+int parcel_hip_malloc(void** ptr, unsigned long bytes);
+int parcel_hip_free(void* ptr);
+int parcel_hip_memcpy(void* dst, const void* src, unsigned long bytes, int kind);
+int parcel_hip_memset(void* ptr, int value, unsigned long bytes);
+int parcel_hip_device_synchronize(void);
 
 /* --- synthetic problem generators (host code): the counterpart of the
  * reference's example generators (src/example/problem.f90) plus the build's

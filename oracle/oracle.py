@@ -48,6 +48,7 @@ def load():
         "oracle_precon_free": (None, [_P]),
         "oracle_precon_nblocks": (_I, [_P]),
         "oracle_pcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
+        "oracle_last_times": (None, [_P]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -116,3 +117,10 @@ def pcg(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, indep
     if rc != 0:
         raise RuntimeError("oracle_pcg failed: %d" % rc)
     return x, hist[: niter.value].copy(), niter.value, iret.value
+
+
+def last_times():
+    """(setup_s, iterations_s) of the last pcg() call."""
+    t = np.zeros(2)
+    load().oracle_last_times(_p(t))
+    return float(t[0]), float(t[1])

@@ -21,6 +21,7 @@
 #include "parcel_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -443,6 +444,7 @@ void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
     memcpy(s, r, sizeof(double) * (size_t)n); /* xcpy, krylov_h.f90:1708 */
   } else if (p->ipreflag == 1) {
     /* vec_scaling (src/vec.f90:1050): s = r * Dinv */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) s[i] = r[i] * p->dinv[i];
   } else if (p->ilu_method == 0) {
 #pragma omp parallel for schedule(dynamic)
@@ -474,14 +476,19 @@ void oracle_precon_free(oracle_precon* p) {
 /* ------------------------------------------------------------------------ */
 /* KSP_pCG (src/pcg.f90:224-892).                                            */
 /* ------------------------------------------------------------------------ */
+static double g_times[2];
+void oracle_last_times(double* t) { t[0] = g_times[0]; t[1] = g_times[1]; }
+
 int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
                const double* b, double* x, int ipreflag, int ilu_method,
                int nblocks, int independ_nvec, int itrmax, double rtol,
                const oracle_red* red, double* hist, int* niter, int* iret) {
   int err = 0;
+  double t0 = omp_get_wtime();
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec,
                                           n, rowptr, col, val, &err);
   if (!M) return err;
+  double t1 = omp_get_wtime();
   size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
   double* r = (double*)calloc(1, bytes);
   double* p = (double*)calloc(1, bytes);
@@ -489,6 +496,7 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   double* s = (double*)calloc(1, bytes);
   /* r = b - A x  (calc_res_CRS, krylov_h.f90:343: r = Ax; r = 1*b + (-1)*r) */
   oracle_spmv(n, rowptr, col, val, x, r);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
   double bnorm = sqrt(oracle_dot(red, n, b, b));
   double rnorm = sqrt(oracle_dot(red, n, r, r));
@@ -498,6 +506,7 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   int itr;
   for (itr = 1; itr <= itrmax; ++itr) {
     hist[itr - 1] = rnorm / bnorm;                 /* pcg.f90:512-518 */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {                  /* CG_axpy2, pcg.f90:8 */
       x[i] = fma(alpha, p[i], x[i]);
       p[i] = fma(beta, p[i], s[i]);
@@ -505,6 +514,7 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
     oracle_spmv(n, rowptr, col, val, p, q);        /* q = A p */
     double gamma = oracle_dot(red, n, p, q);       /* gamma = <p,q> */
     alpha = rho0 / gamma;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) r[i] = fma(-alpha, q[i], r[i]); /* r -= alpha q */
     oracle_precon_apply(M, r, s);                  /* s = M^-1 r */
     double rho1 = oracle_dot(red, n, s, r);        /* CG_calc_dot2 */
@@ -517,6 +527,8 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   }
   for (int i = 0; i < n; ++i) x[i] = fma(alpha, p[i], x[i]); /* pcg.f90:733 */
   *niter = itr < itrmax ? itr : itrmax;             /* pcg.f90:850 */
+  g_times[0] = t1 - t0;
+  g_times[1] = omp_get_wtime() - t1;
   *iret = (itrmax <= itr) ? 1 : 0;                  /* pcg.f90:851-855 */
   free(r); free(p); free(q); free(s);
   oracle_precon_free(M);

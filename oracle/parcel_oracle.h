@@ -79,6 +79,9 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
                const double* b, double* x, int ipreflag, int ilu_method,
                int nblocks, int independ_nvec, int itrmax, double rtol,
                const oracle_red* red, double* hist, int* niter, int* iret);
+/* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
+ * [1] iterations (everything after setup). */
+void oracle_last_times(double* t);
 
 #ifdef __cplusplus
 }

@@ -184,6 +184,8 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
     P.stats.bj_levels_fwd = P.M.fwd.nlev;
     P.stats.bj_levels_bwd = P.M.bwd.nlev;
     P.stats.bj_entries = P.M.fwd.entries + P.M.bwd.entries;
+    P.stats.bj_nnz_l = P.M.fwd.real_entries;
+    P.stats.bj_nnz_u = P.M.bwd.real_entries;
     P.hA = HostCSR();  // setup copy no longer needed
     plan_workspace(P, 1);
     PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
@@ -332,6 +334,54 @@ int parcel_hip_plan_time_kernel(parcel_hip_plan* plan, int which, int reps, doub
   if (!plan || !mean_ms || reps < 1 || (which != 1 && which != 2))
     return fail(PARCEL_HIP_ERR_ARG, "invalid argument");
   *mean_ms = time_kernel(plan->P, which, reps);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_malloc(void** ptr, unsigned long bytes) {
+  PARCEL_API_BEGIN
+  if (!ptr) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  *ptr = nullptr;
+  PARCEL_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_free(void* ptr) {
+  PARCEL_API_BEGIN
+  if (ptr) PARCEL_HIP_CHECK(hipFree(ptr));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_memcpy(void* dst, const void* src, unsigned long bytes, int kind) {
+  PARCEL_API_BEGIN
+  if ((!dst || !src) && bytes) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                  : kind == 2 ? hipMemcpyDeviceToHost
+                  : kind == 3 ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+  if (kind < 1 || kind > 3) return fail(PARCEL_HIP_ERR_ARG, "kind must be 1, 2 or 3");
+  if (bytes) PARCEL_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_memset(void* ptr, int value, unsigned long bytes) {
+  PARCEL_API_BEGIN
+  if (!ptr && bytes) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  if (bytes) {
+    PARCEL_HIP_CHECK(hipMemset(ptr, value, bytes));
+    PARCEL_HIP_CHECK(hipDeviceSynchronize());
+  }
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_device_synchronize(void) {
+  PARCEL_API_BEGIN
+  PARCEL_HIP_CHECK(hipDeviceSynchronize());
   return PARCEL_HIP_OK;
   PARCEL_API_END
 }

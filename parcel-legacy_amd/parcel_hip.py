@@ -5,8 +5,8 @@ argument meaning of `parcel_dcg` (src/wrapper.f90:1995-2083) -- 1-based CRS,
 `itrmax` in/out, residual history, `iret` -- and `Plan` exposes the
 setup/solve split of KSP_CRS (src/krylov.f90:447-735).
 
-Arrays may be numpy arrays (host) or objects with `data_ptr()` (torch device
-tensors, resident in HBM).  The library is loaded from lib/ next to this file;
+Arrays may be numpy arrays (host memory: the library copies them) or
+DeviceArray buffers (HBM, allocated through the library's own HIP runtime).  The library is loaded from lib/ next to this file;
 if it is missing, loading raises instead of falling back to anything.
 """
 import ctypes
@@ -58,6 +58,8 @@ class Stats(ctypes.Structure):
         ("bj_levels_fwd", ctypes.c_int),
         ("bj_levels_bwd", ctypes.c_int),
         ("bj_entries", ctypes.c_long),
+        ("bj_nnz_l", ctypes.c_long),
+        ("bj_nnz_u", ctypes.c_long),
     ]
 
     def as_dict(self):
@@ -86,6 +88,11 @@ SYMBOLS = {
     "parcel_hip_plan_precon": (_I, [_P, _P, _P]),
     "parcel_hip_dot": (_I, [_I, _P, _P, _DP]),
     "parcel_hip_plan_time_kernel": (_I, [_P, _I, _I, _DP]),
+    "parcel_hip_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_ulong]),
+    "parcel_hip_free": (_I, [_P]),
+    "parcel_hip_memcpy": (_I, [_P, _P, ctypes.c_ulong, _I]),
+    "parcel_hip_memset": (_I, [_P, _I, ctypes.c_ulong]),
+    "parcel_hip_device_synchronize": (_I, []),
     "parcel_gen_poisson3d_parcel": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
     "parcel_gen_poisson3d_rhs_parcel": (_I, [_I, _I, _I, _P, _P]),
     "parcel_gen_poisson2d": (_I, [_I, _I, _D, _IP, _P, _P, _P]),
@@ -124,12 +131,57 @@ def _check(rc):
         raise ParcelError(rc, msg.decode() if msg else "")
 
 
+class DeviceArray:
+    """A buffer in HBM owned through the library's own runtime
+    (parcel_hip_malloc); `shape`/`dtype` mirror the numpy array it copies."""
+
+    def __init__(self, host=None, n=None, dtype=np.float64):
+        lib = load()
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            self.dtype, self.shape = host.dtype, host.shape
+        else:
+            self.dtype, self.shape = np.dtype(dtype), (int(n),)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib.parcel_hip_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+        if host is not None:
+            _check(lib.parcel_hip_memcpy(self.ptr, ctypes.c_void_p(host.ctypes.data), self.nbytes, 1))
+        else:
+            _check(lib.parcel_hip_memset(self.ptr, 0, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(load().parcel_hip_memcpy(ctypes.c_void_p(out.ctypes.data), self.ptr, self.nbytes, 2))
+        return out
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes == self.nbytes
+        _check(load().parcel_hip_memcpy(self.ptr, ctypes.c_void_p(host.ctypes.data), self.nbytes, 1))
+
+    def zero(self):
+        _check(load().parcel_hip_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            load().parcel_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _ptr(a):
-    """Pointer of a numpy array (host) or a torch tensor (device or host)."""
+    """Pointer of a numpy array (host) or a DeviceArray (HBM)."""
     if a is None:
         return None
-    if hasattr(a, "data_ptr"):
-        return ctypes.c_void_p(a.data_ptr())
+    if isinstance(a, DeviceArray):
+        return a.ptr
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
     return ctypes.c_void_p(a.ctypes.data)

@@ -7,7 +7,6 @@ for sub in ("parcel-legacy_amd", "oracle"):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
     config.addinivalue_line("markers", "slow: long-running CPU test")

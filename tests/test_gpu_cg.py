@@ -1,0 +1,209 @@
+"""GPU parity tests: HIP kernels (through the C ABI) vs the CPU oracle and the
+reference's golden histories.  Integer/index work and per-row sums must be
+bit-exact; whole-solve residual histories use the oracle's tiled reduction
+order, which is the GPU's order, so they are bit-exact too."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as orc
+import parcel_hip as ph
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "templates.json")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    ph.init(0)
+
+
+def random_csr(n, nnz_per_row, seed, diag=10.0, long_rows=0):
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        k = int(rng.integers(0, nnz_per_row + 1))
+        if long_rows and i % long_rows == 0:
+            k = 300
+        cs = set(int(c) for c in rng.integers(0, n, size=k))
+        cs.add(i)
+        for c in sorted(cs):
+            rows.append(i)
+            cols.append(c)
+            vals.append(diag if c == i else rng.uniform(-1, 1))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.astype(np.float64)
+
+
+CASES = {
+    "poisson2d_64": lambda: ph.poisson2d(64, 64, -1.0),
+    "parcel3d_20": lambda: ph.poisson3d_parcel(20, 20, 20),
+    "random_5000": lambda: random_csr(5000, 9, 11),
+    "random_long_rows": lambda: random_csr(3000, 5, 12, long_rows=97),
+    "single_row": lambda: (np.array([1, 2], np.int32), np.array([1], np.int32), np.array([3.0])),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_spmv_bit_exact(case):
+    say("spmv", case)
+    rp, col, val = CASES[case]()
+    n = rp.shape[0] - 1
+    x = ph.uniform(n, 5)
+    with ph.Plan(rp, col, val, ipreflag=0) as P:
+        y = np.zeros(n)
+        P.spmv(x, y)
+    assert np.array_equal(y, orc.spmv(rp, col, val, x))
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 100000, 3000001])
+def test_dot_bit_exact(n):
+    say("dot", n)
+    x = ph.uniform(n, 1)
+    y = ph.uniform(n, 2)
+    assert ph.dot(x, y) == orc.dot(x, y, orc.TILED)
+
+
+PRECONS = [
+    ("pj", dict(ipreflag=1)),
+    ("ilu0_1", dict(ipreflag=2, ILU_method=0, precon_thblock=1)),
+    ("ilu0_7", dict(ipreflag=2, ILU_method=0, precon_thblock=7)),
+    ("ilud_13", dict(ipreflag=2, ILU_method=1, precon_thblock=13)),
+    ("mlud_5", dict(ipreflag=2, ILU_method=2, precon_thblock=5)),
+    ("ilu0_nvec4", dict(ipreflag=2, ILU_method=0, precon_thblock=3, independ_nvec=4)),
+    ("ilud_nvec3", dict(ipreflag=2, ILU_method=1, precon_thblock=6, independ_nvec=3)),
+]
+
+
+@pytest.mark.parametrize("case", ["parcel3d_20", "random_5000", "poisson2d_64"])
+@pytest.mark.parametrize("name,kw", PRECONS, ids=[p[0] for p in PRECONS])
+def test_precon_apply_bit_exact(case, name, kw):
+    say("precon", case, name)
+    rp, col, val = CASES[case]()
+    n = rp.shape[0] - 1
+    r = ph.uniform(n, 9)
+    with ph.Plan(rp, col, val, **kw) as P:
+        s = np.zeros(n)
+        P.precon(r, s)
+    ilu = kw.get("ILU_method", 1)
+    ref = orc.Precon(rp, col, val, kw["ipreflag"], ilu, kw.get("precon_thblock", 1),
+                     kw.get("independ_nvec", 1)).apply(r)
+    assert np.array_equal(s, ref)
+
+
+@pytest.mark.parametrize("case", ["poisson2d_64", "parcel3d_20", "random_5000"])
+@pytest.mark.parametrize("name,kw", [PRECONS[0], PRECONS[2], PRECONS[3], ("none", dict(ipreflag=0))],
+                         ids=["pj", "ilu0_7", "ilud_13", "none"])
+def test_pcg_history_bit_exact(case, name, kw):
+    say("pcg", case, name)
+    rp, col, val = CASES[case]()
+    n = rp.shape[0] - 1
+    b = ph.uniform(n, 21)
+    x = np.zeros(n)
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, itrmax=3000, rtolmax=1e-10,
+                               ipreflag=kw["ipreflag"], ILU_method=kw.get("ILU_method", 1),
+                               precon_thblock=kw.get("precon_thblock", 1))
+    xo, ho, no, io = orc.pcg(rp, col, val, b, ipreflag=kw["ipreflag"], ilu_method=kw.get("ILU_method", 1),
+                             nblocks=kw.get("precon_thblock", 1), itrmax=3000, rtol=1e-10)
+    assert (niter, iret) == (no, io)
+    assert np.array_equal(hist, ho)
+    assert np.array_equal(x, xo)
+
+
+def test_pcg_c1_point_jacobi_bit_exact_and_converges():
+    """Workload C1: 2D Poisson 256^2 (diag 4, off -1), x_true ~ U[-1,1] seed 0,
+    b = A x_true, x0 = 0, point Jacobi, rtol 1e-10, itrmax 5000."""
+    rp, col, val = ph.poisson2d(256, 256, -1.0)
+    n = 65536
+    xt = ph.uniform(n, 0)
+    b = orc.spmv(rp, col, val, xt)
+    x = np.zeros(n)
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=1, itrmax=5000, rtolmax=1e-10)
+    xo, ho, no, io = orc.pcg(rp, col, val, b, ipreflag=1, itrmax=5000, rtol=1e-10)
+    say("C1 niter", niter)
+    assert iret == 0 and niter == no and np.array_equal(hist, ho) and np.array_equal(x, xo)
+    A = sp.csr_matrix((val, col - 1, rp - 1), shape=(n, n))
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
+
+
+def test_pcg_edge_cases():
+    rp, col, val = ph.poisson2d(16, 16, -1.0)
+    n = 256
+    b = ph.uniform(n, 3)
+    # itrmax 0: nothing done, iret = 1
+    x = np.zeros(n)
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=1, itrmax=0, rtolmax=1e-10)
+    assert niter == 0 and iret == 1 and np.all(x == 0)
+    # itrmax reached
+    x = np.zeros(n)
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=1, itrmax=3, rtolmax=1e-14)
+    xo, ho, no, io = orc.pcg(rp, col, val, b, ipreflag=1, itrmax=3, rtol=1e-14)
+    assert (niter, iret) == (3, 1) == (no, io) and np.array_equal(hist, ho) and np.array_equal(x, xo)
+    # nonzero initial guess
+    x0 = ph.uniform(n, 4)
+    x = x0.copy()
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=2, ILU_method=0, precon_thblock=4, itrmax=500,
+                               rtolmax=1e-11)
+    xo, ho, no, io = orc.pcg(rp, col, val, b, x0=x0, ipreflag=2, ilu_method=0, nblocks=4, itrmax=500, rtol=1e-11)
+    assert niter == no and np.array_equal(hist, ho) and np.array_equal(x, xo)
+
+
+def test_errors_are_reported():
+    # zero diagonal inside a factorised block
+    rp = np.array([1, 2, 4], np.int32)
+    col = np.array([2, 1, 2], np.int32)
+    val = np.array([1.0, 1.0, 2.0])
+    with pytest.raises(ph.ParcelError) as e:
+        ph.Plan(rp, col, val, ipreflag=2, ILU_method=0, precon_thblock=1)
+    assert e.value.code == ph.ERR_ZERO_DIAG
+    # halo columns (a multi-rank system) are rejected on one GPU
+    rp = np.array([1, 3], np.int32)
+    col = np.array([1, 2], np.int32)
+    with pytest.raises(ph.ParcelError) as e:
+        ph.Plan(rp, col, np.array([1.0, 1.0]), ipreflag=0)
+    assert e.value.code == ph.ERR_UNSUPPORTED
+
+
+def test_device_pointers_match_host_pointers():
+    rp, col, val = ph.poisson3d_parcel(24, 24, 24)
+    n = 24 ** 3
+    b = ph.uniform(n, 8)
+    xh = np.zeros(n)
+    hh, nh, ih = ph.dcg(xh, b, rp, col, val, ipreflag=2, ILU_method=1, precon_thblock=9, itrmax=1000,
+                        rtolmax=1e-10)
+    D = ph.DeviceArray
+    xd = D(n=n)
+    hd, nd, idd = ph.dcg(xd, D(b), D(rp), D(col), D(val), ipreflag=2, ILU_method=1, precon_thblock=9,
+                         itrmax=1000, rtolmax=1e-10)
+    assert nd == nh and np.array_equal(hd, hh) and np.array_equal(xd.numpy(), xh)
+
+
+@pytest.mark.parametrize("ilu,nblk,gname", [(1, 80, "template_cpu/STDOUT_run"), (0, 4, "template_gpu/STDOUT_run")])
+def test_pcg_matches_reference_golden_160(ilu, nblk, gname):
+    """The reference's own golden histories (160^3 Poisson, rtol 1e-9): the GPU
+    path must reproduce every printed residual within 1e-12 relative."""
+    say("golden", gname)
+    rp, col, val = ph.poisson3d_parcel(160, 160, 160)
+    b, xe = ph.poisson3d_rhs_parcel(160, 160, 160)
+    x = np.zeros(b.shape[0])
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=2, ILU_method=ilu, precon_thblock=nblk,
+                               itrmax=2000, rtolmax=1e-9)
+    hists = GOLD[gname]["histories"][0]
+    gold = np.array(hists)
+    assert niter == gold.shape[0] and iret == 0
+    for c in range(1, gold.shape[1]):
+        assert np.max(np.abs(hist - gold[:, c]) / gold[:, c]) <= 1e-12
+    err = np.abs(x - xe).max()
+    assert abs(err - GOLD[gname]["extras"]["max_abs_err"][0]) <= 1e-6 * err
