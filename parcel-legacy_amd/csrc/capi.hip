@@ -167,15 +167,7 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
                         PARCEL_HIP_ERR_UNSUPPORTED);
       A.col[j] = c;
     }
-    P.A.n = n;
-    P.A.nnz = nnz;
-    P.A.rowptr.alloc(n + 1);
-    P.A.col.alloc(nnz);
-    P.A.val.alloc(nnz);
-    P.A.rowptr.upload(A.rowptr.data(), n + 1, P.stream);
-    P.A.col.upload(A.col.data(), nnz, P.stream);
-    P.A.val.upload(A.val.data(), nnz, P.stream);
-    PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+    sell_build(P);
     if (P.prm.solver != PARCEL_SOLVER_CG)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
     precon_setup(P);

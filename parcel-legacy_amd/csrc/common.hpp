@@ -8,9 +8,10 @@
 //     i = k*kNT + t for its tiles in increasing k, starting from 0;
 //   * each 64-lane wavefront reduces with an xor butterfly (offsets 32..1),
 //     then the wave totals of a workgroup are added left to right;
-//   * the nb workgroup partials are reduced by the last workgroup to finish,
-//     thread t adding partials t, t+256, ... left to right, then the same
-//     butterfly and left-to-right wave sum.
+//   * the nb workgroup partials are reduced (by the kernel that consumes the
+//     result, in each of its workgroups) with thread t adding partials
+//     t, t+256, ... left to right, then the same butterfly and
+//     left-to-right wave sum.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -65,38 +66,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 #pragma unroll
   for (int k = 1; k < kWaves; ++k) s = s + sh[k];
   return s;
-}
-
-// Device-coherent load of a value written by another workgroup of the same
-// kernel (bypasses any stale per-CU cache line).
-__device__ __forceinline__ double coherent_load(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Reduce nb partials (written by other workgroups) in the layout's order.
-__device__ __forceinline__ double reduce_partials(const double* part, int nb, double* sh) {
-  double a = 0.0;
-  for (int j = threadIdx.x; j < nb; j += kNT) a = a + coherent_load(part + j);
-  return block_sum(a, sh);
-}
-
-// Last-workgroup-done protocol.  Every workgroup calls this once after it has
-// written its partials; exactly one (the last to arrive) gets true.  That
-// workgroup resets the counter for the next launch.
-__device__ __forceinline__ bool arrive_last(unsigned* counter, int* flag_sh) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    *flag_sh = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  bool last = *flag_sh != 0;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return last;
 }
 
 }  // namespace parcel

@@ -42,13 +42,21 @@ struct DevBuf {
   }
 };
 
-// 0-based CSR resident in HBM.
-struct DevCSR {
+// Sliced ELLPACK with slice height 64 (one wavefront), resident in HBM.
+// Row i lives in slice i/64, lane i%64; its j-th stored entry (in the CSR
+// order of the row) is at sptr[slice] + j*64 + lane; rows shorter than the
+// slice width are padded with column -1.  A wavefront reads entry j of its
+// 64 rows as one contiguous 512-byte (values) / 256-byte (columns) access,
+// and every row still sums its entries in the reference's order.
+struct DevSELL {
   int n = 0;
-  long nnz = 0;
-  DevBuf<int> rowptr;   // n+1
-  DevBuf<int> col;      // nnz, 0-based
-  DevBuf<double> val;   // nnz
+  long nnz = 0;          // real entries
+  long entries = 0;      // stored entries (with padding)
+  long nslices = 0;
+  DevBuf<long> sptr;     // nslices+1
+  DevBuf<int> sk;        // nslices: width of each slice
+  DevBuf<int> col;       // entries, 0-based, -1 pads
+  DevBuf<double> val;    // entries
 };
 
 // 0-based CSR copy on the host (setup work: factorisations, level analysis).
@@ -61,33 +69,50 @@ struct HostCSR {
 // Level-scheduled triangular sweep of one block-Jacobi factor.
 // Rows of every block are grouped into levels (a row's level is one more
 // than the highest level of the rows it reads).  Level l of the whole
-// preconditioner holds rows rows[row_off[l] .. row_off[l+1]) and an ELL slab
-// of kmax[l] entries per row, stored entry-major: entry e of the t-th row is
-// at ent_off[l] + e*m + t (m = rows in the level); a column id of -1 pads.
+// preconditioner is described by meta[l] = {m, k, eo, ro}: m rows
+// rows[ro .. ro+m) and an ELL slab of k entries per row stored entry-major
+// (entry e of the t-th row at eo + e*m + t).  An entry's dependency is
+// encoded in dep[]: >= 0 a global row id (read from the output vector),
+// < 0 the LDS ring slot -(dep+1) holding a value computed `ring` levels or
+// fewer before, kPad a padding entry.
+constexpr int kPad = 0x7fffffff;
 struct Sweep {
   int nlev = 0;
   DevBuf<int> blk_lev;   // nblocks+1: levels of block b are [blk_lev[b], blk_lev[b+1])
-  DevBuf<int> row_off;   // nlev+1
-  DevBuf<int> ent_off;   // nlev+1
-  DevBuf<int> kmax;      // nlev
+  DevBuf<int4> meta;     // nlev
   DevBuf<int> rows;      // global row ids in level order
-  DevBuf<int> ecol;      // global column ids (dependencies)
+  DevBuf<int> dep;       // encoded dependencies
   DevBuf<double> eval;   // factor values in accumulation order
   DevBuf<double> diag;   // per level-ordered row: scale (see mode)
   int max_width = 0;
+  int max_k = 0;
+  int ring = 0;          // levels kept in the LDS ring (0: none)
+  int ring_stride = 0;   // doubles per ring level (>= max_width)
   long entries = 0;      // stored (padded) entries
   long real_entries = 0; // entries without padding
+  long global_deps = 0;  // dependencies read from HBM instead of the ring
 };
 
 // Device-resident scalars of one CG solve (src/pcg.f90:240-255 keeps them in
 // host variables and round-trips every one through PCIe; here they never
-// leave HBM until the solve ends).
+// leave HBM until the solve ends).  Iteration i reads slot (i-1)&1 and writes
+// slot i&1, so no workgroup ever reads a value another workgroup of the same
+// kernel is writing.
+struct CGSlot {
+  double rho0;   // rho0 of iteration `itr` (rho1 of itr-1)
+  double alpha;  // alpha of iteration `itr` (written by its second kernel)
+  double beta;   // beta used by iteration `itr`
+  double rnorm;  // ||r|| entering iteration `itr`
+  int itr;
+  int pad[3];
+};
 struct CGState {
-  double alpha, beta, rho0, bnorm, rnorm, gamma, rho1, rr;
-  int itr;   // iterations started
-  int done;  // 1 once the loop has exited (converged, diverged or itrmax)
-  int iret;  // PARCEL_RETURNCODE_*
+  double bnorm;
+  int done_at;   // index of the K1 launch that ended the loop (0: running)
+  int niter;
+  int iret;
   int pad;
+  CGSlot slot[2];
 };
 
 enum PreconKind { PRECON_NONE = 0, PRECON_PJ = 1, PRECON_BJ = 2 };
@@ -97,6 +122,7 @@ struct Precon {
   int kind = PRECON_NONE;
   int mode = BJ_ILU0;
   int nblocks = 0;
+  int nth = 256;        // workgroup size of the block-Jacobi apply kernel
   DevBuf<double> dinv;  // point Jacobi
   Sweep fwd, bwd;       // block Jacobi
   double setup_seconds = 0.0;
@@ -106,22 +132,37 @@ struct Plan {
   int device = 0;
   hipStream_t stream = nullptr;
   parcel_hip_param prm{};
-  DevCSR A;
+  DevSELL A;
   HostCSR hA;  // kept only while setting up
   Precon M;
   // workspace
   DevBuf<double> r, q, s, p0, p1, x, b, t0;
   DevBuf<CGState> cg;       // device-resident solver scalars
-  DevBuf<double> part;      // tiled partial sums (4 * kNBMax)
-  DevBuf<unsigned> counter; // last-workgroup counters (8)
+  DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
   DevBuf<double> hist;      // residual history
   int hist_cap = 0;
   parcel_hip_stats stats{};
 };
 
+// Partial-sum areas of P.part (each kNBMax doubles).
+enum PartSlot { PART_BB = 0, PART_RR0 = 1, PART_RHO0 = 2, PART_GAMMA = 3, PART_RHO1 = 4, PART_RR = 5, PART_N = 8 };
+
+// Fused residual update for the block-Jacobi apply inside CG iteration `it`:
+// r -= alpha*q with alpha = slot.rho0 / gamma, gamma reduced from the
+// partials of the preceding SpMV kernel.
+struct CGFuse {
+  const double* q = nullptr;
+  double* r = nullptr;
+  CGState* st = nullptr;
+  const double* part_gamma = nullptr;
+  int nb_prod = 0;
+  int it = 0;
+};
+
 // setup helpers (precon.hip)
 void precon_setup(Plan& P);
-void precon_apply(Plan& P, const double* r, double* s, const int* skip);
+void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse);
+void sell_build(Plan& P);
 
 // solvers and kernels (krylov.hip)
 void plan_workspace(Plan& P, int itrmax);

@@ -1,21 +1,30 @@
-// krylov.hip -- CSR SpMV, reductions and the preconditioned CG solver.
+// krylov.hip -- SpMV (SELL-64), reductions and the preconditioned CG solver.
 //
 // KSP_pCG (src/pcg.f90:224-892) keeps alpha, beta, rho and the norms in host
 // variables and moves each one through a 1-element memcpy plus a device
 // synchronisation (src/parcel_vec.f90:98-214), ~30 host round trips per
-// iteration.  Here every scalar lives in a CGState in HBM: the last workgroup
-// of each reducing kernel finishes the reduction and updates the scalars, the
-// host only enqueues kernels and polls the `done` flag once per chunk of
-// iterations.  Kernels launched after the loop has exited return at once.
+// iteration.  Here the scalars never leave HBM while the solve runs:
+//   * every reducing kernel writes one partial sum per workgroup (the fixed
+//     tiled order of common.hpp);
+//   * the NEXT kernel reduces those partials in its prologue, every
+//     workgroup redundantly and identically (kernel boundaries make the
+//     partials visible; no atomics, no fences);
+//   * scalars that must survive a kernel go to CGState::slot[i & 1], written
+//     by workgroup 0 of iteration i's kernels and read by iteration i+1.
+// The host only enqueues kernels and polls CGState::done_at once per chunk.
 //
-// One CG iteration is two kernels with point Jacobi / no preconditioner:
-//   K1 k_cg_spmv      x += alpha p_old; p_new = s + beta p_old; q = A p_new;
-//                     gamma = <p_new, q>; alpha = rho0 / gamma
-//   K2 k_cg_update_pj r -= alpha q; s = Dinv r; rho1 = <s,r>; rr = <r,r>;
-//                     convergence test; beta = rho1 / rho0
-// and four with block Jacobi (r update, forward sweep, backward sweep, dots).
-// K1 reads p_new of neighbour rows as fma(beta, p_old[c], s[c]), which is the
-// same rounding as the owner's p_new, so p never needs a separate pass.
+// One iteration i (point Jacobi / none) is two kernels:
+//   K1 k_cg_spmv(i): reduce (rho1, rr) of iteration i-1 -> exit test, beta;
+//      x += alpha p_old; p_new = s + beta p_old; q = A p_new; gamma partials.
+//      If the loop has ended it only applies the final x += alpha p.
+//   K2 k_cg_update_pj(i): alpha = rho0 / gamma; r -= alpha q; s = Dinv r;
+//      (rho1, rr) partials.
+// With block Jacobi, K2 is the fused sweep kernel (precon.hip) followed by
+// k_cg_dot2(i).  K1 reads p_new of neighbour rows as fma(beta, p_old[c], s[c]),
+// the same rounding the owning row uses, so p never needs its own pass.
+#include <omp.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -24,43 +33,102 @@
 
 namespace parcel {
 
-namespace {
-
-__device__ __forceinline__ void cg_check(CGState* st, double rho1, double rr, int itrmax, double rtol) {
-  // pcg.f90:640-690: rnorm = sqrt(<r,r>); exit tests; beta = rho1/rho0
-  const double rnorm = sqrt(rr);
-  st->rnorm = rnorm;
-  st->rho1 = rho1;
-  st->rr = rr;
-  const double ratio = rnorm / st->bnorm;
-  const int itr = st->itr;
-  if (ratio < rtol || ratio > 1.0e8 || itr >= itrmax) {
-    st->done = 1;
-    st->iret = (itr >= itrmax) ? 1 : 0;
-  } else {
-    st->beta = rho1 / st->rho0;
-    st->rho0 = rho1;
+// --- SELL-64 build (host) ----------------------------------------------------
+void sell_build(Plan& P) {
+  const HostCSR& A = P.hA;
+  const int n = A.n;
+  DevSELL& S = P.A;
+  S.n = n;
+  S.nnz = A.rowptr[n];
+  const long ns = ((long)n + 63) / 64;
+  S.nslices = ns;
+  std::vector<long> sptr(ns + 1, 0);
+  std::vector<int> sk(ns, 0);
+  for (long s = 0; s < ns; ++s) {
+    int k = 0;
+    const long r1 = std::min((long)n, s * 64 + 64);
+    for (long r = s * 64; r < r1; ++r) k = std::max(k, A.rowptr[r + 1] - A.rowptr[r]);
+    sk[s] = k;
+    sptr[s + 1] = sptr[s] + 64L * k;
   }
+  const long tot = sptr[ns];
+  S.entries = tot;
+  std::vector<int> col((size_t)tot);
+  std::vector<double> val((size_t)tot);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < ns; ++s) {
+    const long base = sptr[s];
+    const int k = sk[s];
+    for (int l = 0; l < 64; ++l) {
+      const long r = s * 64 + l;
+      const int len = r < n ? A.rowptr[r + 1] - A.rowptr[r] : 0;
+      for (int j = 0; j < k; ++j) {
+        const long at = base + (long)j * 64 + l;
+        if (j < len) {
+          col[at] = A.col[A.rowptr[r] + j];
+          val[at] = A.val[A.rowptr[r] + j];
+        } else {
+          col[at] = -1;
+          val[at] = 0.0;
+        }
+      }
+    }
+  }
+  S.sptr.upload(sptr, P.stream);
+  S.sk.upload(sk, P.stream);
+  S.col.upload(col, P.stream);
+  S.val.upload(val, P.stream);
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 
-// r = b - A x ;  bnorm = sqrt(<b,b>) ; rnorm = sqrt(<r,r>)   (calc_res_CRS,
-// src/krylov_h.f90:343 and pcg.f90:456-470)
-__global__ void __launch_bounds__(kNT) k_residual(int n, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ col,
-                                                  const double* __restrict__ val,
-                                                  const double* __restrict__ x,
+namespace {
+
+struct SellArgs {
+  const long* sptr;
+  const int* sk;
+  const int* col;
+  const double* val;
+};
+
+SellArgs sell_args(const DevSELL& S) { return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p}; }
+
+// Row i of the SELL matrix times the vector produced by `g(c)`, entries in
+// stored order.
+template <class G>
+__device__ __forceinline__ double sell_row(const SellArgs& A, long i, G g) {
+  const long s = i >> 6;
+  const long base = A.sptr[s] + (i & 63);
+  const int k = A.sk[s];
+  double t = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const int c = A.col[base + (long)j * 64];
+    const double v = A.val[base + (long)j * 64];
+    if (c >= 0) t = fma(v, g(c), t);
+  }
+  return t;
+}
+
+// Reduce the nb partials of a producer kernel (reference order of
+// common.hpp) in a workgroup of kNT threads.
+__device__ __forceinline__ double reduce_part(const double* part, int nb, double* sh) {
+  double a = 0.0;
+  for (int j = threadIdx.x; j < nb; j += kNT) a = a + part[j];
+  return block_sum(a, sh);
+}
+
+// r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,
+// src/krylov_h.f90:343; pcg.f90:456-470)
+__global__ void __launch_bounds__(kNT) k_residual(int n, SellArgs A, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r,
-                                                  double* part, unsigned* counter, CGState* st) {
+                                                  double* part) {
   __shared__ double sh[kWaves];
-  __shared__ int flag;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
-      double t = 0.0;
-      for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) t = fma(val[j], x[col[j]], t);
+      const double t = sell_row(A, i, [&](int c) { return x[c]; });
       const double bi = b[i];
       const double ri = bi - t;
       r[i] = ri;
@@ -70,103 +138,142 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, const int* __restrict__
   }
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) { part[blockIdx.x] = s0; part[kNBMax + blockIdx.x] = s1; }
-  if (arrive_last(counter, &flag)) {
-    double bb = reduce_partials(part, nb, sh);
-    double rr = reduce_partials(part + kNBMax, nb, sh);
-    if (threadIdx.x == 0) {
-      st->bnorm = sqrt(bb);
-      st->rnorm = sqrt(rr);
-    }
+  if (threadIdx.x == 0) {
+    part[PART_BB * kNBMax + blockIdx.x] = s0;
+    part[PART_RR0 * kNBMax + blockIdx.x] = s1;
   }
 }
 
-// rho0 = <s,r>; alpha = beta = 0 (pcg.f90:478-490)
-__global__ void __launch_bounds__(kNT) k_cg_init(int n, const double* __restrict__ s,
-                                                 const double* __restrict__ r, double* part,
-                                                 unsigned* counter, CGState* st) {
+// partials of <x,y> into part[slot]
+__global__ void __launch_bounds__(kNT) k_dot_part(int n, const double* __restrict__ x,
+                                                  const double* __restrict__ y, double* part) {
   __shared__ double sh[kWaves];
-  __shared__ int flag;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0;
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
-    if (i < n) a0 = fma(s[i], r[i], a0);
+    if (i < n) a0 = fma(x[i], y[i], a0);
   }
   double s0 = block_sum(a0, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s0;
-  if (arrive_last(counter, &flag)) {
-    double rho0 = reduce_partials(part, nb, sh);
-    if (threadIdx.x == 0) {
-      st->rho0 = rho0;
-      st->alpha = 0.0;
-      st->beta = 0.0;
-      st->gamma = 0.0;
-      st->itr = 0;
-      st->done = 0;
-      st->iret = 0;
-    }
+}
+
+__global__ void __launch_bounds__(kNT) k_reduce_one(const double* part, int nb, double* out) {
+  __shared__ double sh[kWaves];
+  double v = reduce_part(part, nb, sh);
+  if (threadIdx.x == 0) *out = v;
+}
+
+// bnorm, rnorm, rho0 of the initial residual (pcg.f90:456-490); slot 0.
+__global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGState* st) {
+  __shared__ double sh[kWaves];
+  const double bb = reduce_part(part + PART_BB * kNBMax, nb, sh);
+  const double rr = reduce_part(part + PART_RR0 * kNBMax, nb, sh);
+  const double rho0 = reduce_part(part + PART_RHO0 * kNBMax, nb, sh);
+  if (threadIdx.x == 0) {
+    st->bnorm = sqrt(bb);
+    st->done_at = 0;
+    st->niter = 0;
+    st->iret = 0;
+    CGSlot& s = st->slot[0];
+    s.rho0 = rho0;
+    s.alpha = 0.0;
+    s.beta = 0.0;
+    s.rnorm = sqrt(rr);
+    s.itr = 0;
   }
 }
 
-// K1: fused CG_axpy2 (pcg.f90:8) + SpMV (matrix.f90:9051) + gamma = <p,q>
-__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, const int* __restrict__ rowptr,
-                                                 const int* __restrict__ col,
-                                                 const double* __restrict__ val, double* __restrict__ x,
-                                                 const double* __restrict__ p_old,
+// K1 of iteration `it`
+__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgs A,
+                                                 double* __restrict__ x, const double* __restrict__ p_old,
                                                  const double* __restrict__ s, double* __restrict__ p_new,
-                                                 double* __restrict__ q, double* part, unsigned* counter,
-                                                 CGState* st, double* hist, int hist_cap) {
-  if (st->done) return;
+                                                 double* __restrict__ q, double* part, CGState* st,
+                                                 double* hist, int hist_cap) {
   __shared__ double sh[kWaves];
-  __shared__ int flag;
-  const double alpha = st->alpha, beta = st->beta;
+  {
+    const int d = st->done_at;
+    if (d != 0 && d < it) return;
+  }
+  const CGSlot& prev = st->slot[(it - 1) & 1];
+  const double bnorm = st->bnorm;
+  double alpha_prev, beta, rho0, rnorm;
+  bool exit_loop = false;
   const int nb = gridDim.x;
+  if (it == 1) {
+    alpha_prev = 0.0;
+    beta = 0.0;
+    rho0 = prev.rho0;
+    rnorm = prev.rnorm;
+  } else {
+    // pcg.f90:640-700: rnorm = sqrt(<r,r>); exit tests; beta = rho1/rho0
+    const double rho1 = reduce_part(part + PART_RHO1 * kNBMax, nb, sh);
+    const double rr = reduce_part(part + PART_RR * kNBMax, nb, sh);
+    rnorm = sqrt(rr);
+    const double ratio = rnorm / bnorm;
+    alpha_prev = prev.alpha;
+    exit_loop = ratio < rtol || ratio > 1.0e8 || it - 1 >= itrmax;
+    beta = rho1 / prev.rho0;
+    rho0 = rho1;
+  }
   const long ntiles = ((long)n + kNT - 1) / kNT;
+  if (exit_loop) {
+    // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
+    for (long k = blockIdx.x; k < ntiles; k += nb) {
+      const long i = k * kNT + threadIdx.x;
+      if (i < n) x[i] = fma(alpha_prev, p_old[i], x[i]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done_at = it;
+      st->niter = it - 1;                       // pcg.f90:850
+      st->iret = (it - 1 >= itrmax) ? 1 : 0;    // pcg.f90:851-855
+      st->slot[it & 1].rnorm = rnorm;
+      st->slot[it & 1].itr = it - 1;
+    }
+    return;
+  }
   double acc = 0.0;
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
       const double po = p_old[i];
-      x[i] = fma(alpha, po, x[i]);
+      x[i] = fma(alpha_prev, po, x[i]);
       const double pn = fma(beta, po, s[i]);
       p_new[i] = pn;
-      double t = 0.0;
-      const int j1 = rowptr[i + 1];
-      for (int j = rowptr[i]; j < j1; ++j) {
-        const int c = col[j];
-        t = fma(val[j], fma(beta, p_old[c], s[c]), t);
-      }
+      const double t = sell_row(A, i, [&](int c) { return fma(beta, p_old[c], s[c]); });
       q[i] = t;
       acc = fma(pn, t, acc);
     }
   }
   double s0 = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s0;
-  if (arrive_last(counter, &flag)) {
-    double gamma = reduce_partials(part, nb, sh);
-    if (threadIdx.x == 0) {
-      const int itr = st->itr + 1;
-      if (itr - 1 < hist_cap) hist[itr - 1] = st->rnorm / st->bnorm;  // pcg.f90:512-518
-      st->gamma = gamma;
-      st->alpha = st->rho0 / gamma;
-      st->itr = itr;
-    }
+  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + blockIdx.x] = s0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it - 1 < hist_cap) hist[it - 1] = rnorm / bnorm;  // pcg.f90:512-518
+    CGSlot& cur = st->slot[it & 1];
+    cur.rho0 = rho0;
+    cur.beta = beta;
+    cur.rnorm = rnorm;
+    cur.itr = it;
   }
 }
 
-// K2 (point Jacobi / none): r -= alpha q; s = Dinv r; rho1, rr; exit test
-__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, double* __restrict__ r,
+// K2 of iteration `it` (point Jacobi or none): alpha; r -= alpha q;
+// s = Dinv r; (rho1, rr) partials
+__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, double* __restrict__ r,
                                                       const double* __restrict__ q,
                                                       const double* __restrict__ dinv,
-                                                      double* __restrict__ s, double* part,
-                                                      unsigned* counter, CGState* st, int itrmax,
-                                                      double rtol) {
-  if (st->done) return;
+                                                      double* __restrict__ s, double* part, CGState* st,
+                                                      int nb_prod) {
   __shared__ double sh[kWaves];
-  __shared__ int flag;
-  const double alpha = st->alpha;
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
+  CGSlot& cur = st->slot[it & 1];
+  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
+  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
@@ -183,34 +290,21 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, double* __restrict_
   }
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) { part[blockIdx.x] = s0; part[kNBMax + blockIdx.x] = s1; }
-  if (arrive_last(counter, &flag)) {
-    double rho1 = reduce_partials(part, nb, sh);
-    double rr = reduce_partials(part + kNBMax, nb, sh);
-    if (threadIdx.x == 0) cg_check(st, rho1, rr, itrmax, rtol);
+  if (threadIdx.x == 0) {
+    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
+    part[PART_RR * kNBMax + blockIdx.x] = s1;
   }
 }
 
-// r -= alpha q (block-Jacobi path: the sweeps follow)
-__global__ void __launch_bounds__(kNT) k_cg_update_r(int n, double* __restrict__ r,
-                                                     const double* __restrict__ q, const CGState* st) {
-  if (st->done) return;
-  const double alpha = st->alpha;
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) r[i] = fma(-alpha, q[i], r[i]);
-  }
-}
-
-// rho1 = <s,r>, rr = <r,r> (CG_calc_dot2, pcg.f90:30) + exit test
-__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, const double* __restrict__ s,
+// rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30)
+__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const double* __restrict__ s,
                                                  const double* __restrict__ r, double* part,
-                                                 unsigned* counter, CGState* st, int itrmax, double rtol) {
-  if (st->done) return;
+                                                 const CGState* st) {
   __shared__ double sh[kWaves];
-  __shared__ int flag;
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
@@ -224,55 +318,19 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, const double* __restrict
   }
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) { part[blockIdx.x] = s0; part[kNBMax + blockIdx.x] = s1; }
-  if (arrive_last(counter, &flag)) {
-    double rho1 = reduce_partials(part, nb, sh);
-    double rr = reduce_partials(part + kNBMax, nb, sh);
-    if (threadIdx.x == 0) cg_check(st, rho1, rr, itrmax, rtol);
+  if (threadIdx.x == 0) {
+    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
+    part[PART_RR * kNBMax + blockIdx.x] = s1;
   }
 }
 
-// final x += alpha p (pcg.f90:733) with p the buffer written last
-__global__ void k_cg_final(int n, double* __restrict__ x, const double* __restrict__ p0,
-                           const double* __restrict__ p1, const CGState* st) {
-  const double* p = (st->itr & 1) ? p1 : p0;
-  const double alpha = st->alpha;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    x[i] = fma(alpha, p[i], x[i]);
-}
-
-__global__ void __launch_bounds__(kNT) k_spmv(int n, const int* __restrict__ rowptr,
-                                              const int* __restrict__ col, const double* __restrict__ val,
-                                              const double* __restrict__ x, double* __restrict__ y) {
+__global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgs A, const double* __restrict__ x,
+                                              double* __restrict__ y) {
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      double t = 0.0;
-      for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) t = fma(val[j], x[col[j]], t);
-      y[i] = t;
-    }
-  }
-}
-
-__global__ void __launch_bounds__(kNT) k_dot(int n, const double* __restrict__ x,
-                                             const double* __restrict__ y, double* part,
-                                             unsigned* counter, double* out) {
-  __shared__ double sh[kWaves];
-  __shared__ int flag;
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) a0 = fma(x[i], y[i], a0);
-  }
-  double s0 = block_sum(a0, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s0;
-  if (arrive_last(counter, &flag)) {
-    double v = reduce_partials(part, nb, sh);
-    if (threadIdx.x == 0) *out = v;
+    if (i < n) y[i] = sell_row(A, i, [&](int c) { return x[c]; });
   }
 }
 
@@ -284,11 +342,7 @@ void plan_workspace(Plan& P, int itrmax) {
     P.r.alloc(n); P.q.alloc(n); P.s.alloc(n); P.p0.alloc(n); P.p1.alloc(n);
   }
   if (!P.cg.p) P.cg.alloc(1);
-  if (!P.part.p) P.part.alloc(4 * kNBMax);
-  if (!P.counter.p) {
-    P.counter.alloc(8);
-    PARCEL_HIP_CHECK(hipMemsetAsync(P.counter.p, 0, 8 * sizeof(unsigned), P.stream));
-  }
+  if (!P.part.p) P.part.alloc(PART_N * kNBMax);
   if (P.hist_cap < itrmax) {
     P.hist.alloc(itrmax > 0 ? itrmax : 1);
     P.hist_cap = itrmax;
@@ -297,16 +351,15 @@ void plan_workspace(Plan& P, int itrmax) {
 
 void spmv(Plan& P, const double* x, double* y) {
   const int n = P.A.n;
-  hipLaunchKernelGGL(k_spmv, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, P.A.rowptr.p, P.A.col.p,
-                     P.A.val.p, x, y);
+  hipLaunchKernelGGL(k_spmv, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, y);
   PARCEL_HIP_CHECK(hipGetLastError());
 }
 
 double dot_device(int n, const double* x, const double* y, hipStream_t st) {
   DevBuf<double> part(kNBMax + 1);
-  DevBuf<unsigned> cnt(1);
-  PARCEL_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), st));
-  hipLaunchKernelGGL(k_dot, dim3(tiled_nb(n)), dim3(kNT), 0, st, n, x, y, part.p, cnt.p, part.p + kNBMax);
+  const int nb = tiled_nb(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kNT), 0, st, n, x, y, part.p);
+  hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kNT), 0, st, (const double*)part.p, nb, part.p + kNBMax);
   PARCEL_HIP_CHECK(hipGetLastError());
   double h = 0.0;
   PARCEL_HIP_CHECK(hipMemcpyAsync(&h, part.p + kNBMax, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -326,13 +379,15 @@ struct Timer {
   std::vector<EventPair> spmv, prec, vec;
   std::vector<hipEvent_t> pool;
   size_t used = 0;
-  hipEvent_t get() {
+  hipEvent_t rec(hipStream_t st) {
     if (used == pool.size()) {
       hipEvent_t e;
       PARCEL_HIP_CHECK(hipEventCreate(&e));
       pool.push_back(e);
     }
-    return pool[used++];
+    hipEvent_t e = pool[used++];
+    PARCEL_HIP_CHECK(hipEventRecord(e, st));
+    return e;
   }
   ~Timer() {
     for (auto e : pool) (void)hipEventDestroy(e);
@@ -350,9 +405,8 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   const int nb = tiled_nb(n);
   const dim3 grid(nb), block(kNT);
   CGState* S = P.cg.p;
-  const int* done = &S->done;
   double* part = P.part.p;
-  unsigned* cnt = P.counter.p;
+  const SellArgs A = sell_args(P.A);
   int launches = 0;
   Timer T;
   T.on = P.prm.profile == 1;
@@ -361,68 +415,82 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   PARCEL_HIP_CHECK(hipEventCreate(&ev1));
   PARCEL_HIP_CHECK(hipEventRecord(ev0, st));
 
+  if (itrmax <= 0) {
+    // the loop body never runs (pcg.f90:488 with itrmax = 0): niter 0, iret 1
+    PARCEL_HIP_CHECK(hipEventRecord(ev1, st));
+    PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    *niter = 0;
+    *iret = 1;
+    P.stats.niter = 0;
+    return PARCEL_HIP_OK;
+  }
+
   PARCEL_HIP_CHECK(hipMemsetAsync(P.p0.p, 0, sizeof(double) * (size_t)n, st));
-  hipLaunchKernelGGL(k_residual, grid, block, 0, st, n, P.A.rowptr.p, P.A.col.p, P.A.val.p, x, b,
-                     P.r.p, part, cnt, S);
+  hipLaunchKernelGGL(k_residual, grid, block, 0, st, n, A, x, b, P.r.p, part);
   precon_apply(P, P.r.p, P.s.p, nullptr);
-  hipLaunchKernelGGL(k_cg_init, grid, block, 0, st, n, P.s.p, P.r.p, part, cnt, S);
-  launches += 3;
+  hipLaunchKernelGGL(k_dot_part, grid, block, 0, st, n, P.s.p, P.r.p, part + PART_RHO0 * kNBMax);
+  hipLaunchKernelGGL(k_cg_init, dim3(1), block, 0, st, (const double*)part, nb, S);
+  launches += 4;
   PARCEL_HIP_CHECK(hipGetLastError());
 
   CGState* h = nullptr;
   PARCEL_HIP_CHECK(hipHostMalloc((void**)&h, sizeof(CGState), hipHostMallocDefault));
-  h->done = 0;
-  h->itr = 0;
-  int enq = 0;
+  h->done_at = 0;
+  int enq = 0;  // K1 launches enqueued
   int chunk = 8;
   const bool bj = P.M.kind == PRECON_BJ;
   const double* dinv = P.M.kind == PRECON_PJ ? P.M.dinv.p : nullptr;
-  while (enq < itrmax) {
-    const int k = std::min(chunk, itrmax - enq);
+  while (enq < itrmax + 1) {
+    const int k = std::min(chunk, itrmax + 1 - enq);
     for (int j = 1; j <= k; ++j) {
       const int it = enq + j;
       double* pold = (it - 1) & 1 ? P.p1.p : P.p0.p;
       double* pnew = it & 1 ? P.p1.p : P.p0.p;
-      hipEvent_t ea = nullptr, eb = nullptr, ec = nullptr, ed = nullptr;
-      if (T.on) { ea = T.get(); PARCEL_HIP_CHECK(hipEventRecord(ea, st)); }
-      hipLaunchKernelGGL(k_cg_spmv, grid, block, 0, st, n, P.A.rowptr.p, P.A.col.p, P.A.val.p, x, pold,
-                         P.s.p, pnew, P.q.p, part, cnt, S, P.hist.p, P.hist_cap);
-      if (T.on) { eb = T.get(); PARCEL_HIP_CHECK(hipEventRecord(eb, st)); T.spmv.push_back({ea, eb, it}); }
+      hipEvent_t ea = T.on ? T.rec(st) : nullptr;
+      hipLaunchKernelGGL(k_cg_spmv, grid, block, 0, st, n, it, itrmax, rtol, A, x, pold, P.s.p, pnew, P.q.p,
+                         part, S, P.hist.p, P.hist_cap);
+      launches++;
+      hipEvent_t eb = T.on ? T.rec(st) : nullptr;
+      if (T.on) T.spmv.push_back({ea, eb, it});
+      if (it > itrmax) break;  // the last K1 only closes the loop
       if (!bj) {
-        hipLaunchKernelGGL(k_cg_update_pj, grid, block, 0, st, n, P.r.p, P.q.p, dinv, P.s.p, part, cnt, S,
-                           itrmax, rtol);
-        if (T.on) { ec = T.get(); PARCEL_HIP_CHECK(hipEventRecord(ec, st)); T.vec.push_back({eb, ec, it}); }
-        launches += 2;
+        hipLaunchKernelGGL(k_cg_update_pj, grid, block, 0, st, n, it, P.r.p, P.q.p, dinv, P.s.p, part, S, nb);
+        launches++;
+        if (T.on) T.vec.push_back({eb, T.rec(st), it});
       } else {
-        hipLaunchKernelGGL(k_cg_update_r, grid, block, 0, st, n, P.r.p, P.q.p, S);
-        if (T.on) { ec = T.get(); PARCEL_HIP_CHECK(hipEventRecord(ec, st)); T.vec.push_back({eb, ec, it}); }
-        precon_apply(P, P.r.p, P.s.p, done);
-        if (T.on) { ed = T.get(); PARCEL_HIP_CHECK(hipEventRecord(ed, st)); T.prec.push_back({ec, ed, it}); }
-        hipLaunchKernelGGL(k_cg_dot2, grid, block, 0, st, n, P.s.p, P.r.p, part, cnt, S, itrmax, rtol);
-        if (T.on) {
-          hipEvent_t ee = T.get(); PARCEL_HIP_CHECK(hipEventRecord(ee, st));
-          T.vec.push_back({ed, ee, it});
-        }
-        launches += 5;
+        CGFuse f;
+        f.q = P.q.p;
+        f.r = P.r.p;
+        f.st = S;
+        f.part_gamma = part + PART_GAMMA * kNBMax;
+        f.nb_prod = nb;
+        f.it = it;
+        precon_apply(P, P.r.p, P.s.p, &f);
+        hipEvent_t ec = T.on ? T.rec(st) : nullptr;
+        if (T.on) T.prec.push_back({eb, ec, it});
+        hipLaunchKernelGGL(k_cg_dot2, grid, block, 0, st, n, it, P.s.p, P.r.p, part, S);
+        launches += 2;
+        if (T.on) T.vec.push_back({ec, T.rec(st), it});
       }
     }
     PARCEL_HIP_CHECK(hipGetLastError());
     enq += k;
     PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState), hipMemcpyDeviceToHost, st));
     PARCEL_HIP_CHECK(hipStreamSynchronize(st));
-    if (h->done) break;
+    if (h->done_at) break;
     if (chunk < 64) chunk *= 2;
-  }
-  if (itrmax > 0) {
-    hipLaunchKernelGGL(k_cg_final, grid, block, 0, st, n, x, P.p0.p, P.p1.p, S);
-    launches++;
   }
   PARCEL_HIP_CHECK(hipEventRecord(ev1, st));
   PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState), hipMemcpyDeviceToHost, st));
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
-  int nit = itrmax > 0 ? h->itr : 0;
-  int ret = itrmax > 0 ? h->iret : 1;
-  if (itrmax > 0 && !h->done) { ret = 1; }
+  if (!h->done_at) {
+    (void)hipHostFree(h);
+    throw hip_error("CG loop did not terminate", PARCEL_HIP_ERR_HIP);
+  }
+  const int nit = h->niter;
+  const int ret = h->iret;
   if (nit > 0 && hist_host)
     PARCEL_HIP_CHECK(hipMemcpy(hist_host, P.hist.p, sizeof(double) * (size_t)nit, hipMemcpyDeviceToHost));
   float ms = 0.f;
@@ -459,11 +527,13 @@ double time_kernel(Plan& P, int which, int reps) {
   plan_workspace(P, 1);
   const int nb = tiled_nb(n);
   CGState h{};
-  h.alpha = 0.0; h.beta = 0.5; h.rho0 = 1.0; h.bnorm = 1.0; h.rnorm = 1.0;
+  h.bnorm = 1.0;
+  h.slot[0].rho0 = 1.0; h.slot[1].rho0 = 1.0;
+  h.slot[0].rnorm = 1.0; h.slot[1].rnorm = 1.0;
   PARCEL_HIP_CHECK(hipMemcpyAsync(P.cg.p, &h, sizeof(CGState), hipMemcpyHostToDevice, st));
-  PARCEL_HIP_CHECK(hipMemsetAsync(P.p0.p, 0, sizeof(double) * (size_t)n, st));
-  PARCEL_HIP_CHECK(hipMemsetAsync(P.s.p, 0, sizeof(double) * (size_t)n, st));
-  PARCEL_HIP_CHECK(hipMemsetAsync(P.r.p, 0, sizeof(double) * (size_t)n, st));
+  for (double* v : {P.p0.p, P.p1.p, P.s.p, P.r.p, P.q.p})
+    PARCEL_HIP_CHECK(hipMemsetAsync(v, 0, sizeof(double) * (size_t)n, st));
+  PARCEL_HIP_CHECK(hipMemsetAsync(P.part.p, 0, sizeof(double) * PART_N * kNBMax, st));
   if (P.x.n < (size_t)n) P.x.alloc(n);
   PARCEL_HIP_CHECK(hipMemsetAsync(P.x.p, 0, sizeof(double) * (size_t)n, st));
   hipEvent_t a, b;
@@ -471,9 +541,10 @@ double time_kernel(Plan& P, int which, int reps) {
   PARCEL_HIP_CHECK(hipEventCreate(&b));
   auto launch = [&](int i) {
     if (which == 1) {
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kNT), 0, st, n, P.A.rowptr.p, P.A.col.p, P.A.val.p,
-                         P.x.p, (i & 1) ? P.p1.p : P.p0.p, P.s.p, (i & 1) ? P.p0.p : P.p1.p, P.q.p,
-                         P.part.p, P.counter.p, P.cg.p, P.hist.p, 0);
+      // iteration 1 semantics every time (no exit test), no history write
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kNT), 0, st, n, 1, 1 << 30, 0.0, sell_args(P.A), P.x.p,
+                         (i & 1) ? P.p1.p : P.p0.p, P.s.p, (i & 1) ? P.p0.p : P.p1.p, P.q.p, P.part.p, P.cg.p,
+                         P.hist.p, 0);
     } else {
       precon_apply(P, P.r.p, P.s.p, nullptr);
     }

@@ -27,6 +27,17 @@ namespace parcel {
 
 namespace {
 
+constexpr int kRingBudget = 6144;  // doubles of LDS for the ring (48 KiB)
+
+// Entry count per row the apply kernel is instantiated for (1..4, 6, 8),
+// or the row length itself above 8 (generic kernel).
+int apply_k(int k) {
+  if (k <= 4) return std::max(k, 1);
+  if (k <= 6) return 6;
+  if (k <= 8) return 8;
+  return k;
+}
+
 // divide_region (src/common.f90:125-182), 0-based half-open result.
 void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   if (n == 1) { *lo = 0; *hi = 1; return; }
@@ -36,13 +47,6 @@ void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   else { l = 1 + iblock * rank + over; h = l + iblock - 1; }
   *lo = l - 1; *hi = h;
 }
-
-// One row of a sweep: global row id, dependencies and values in
-// accumulation order, and the row's scale.
-struct RowEntries {
-  std::vector<int> dep;
-  std::vector<double> val;
-};
 
 struct BlockSweep {
   // rows of the block in level order, with per-level boundaries
@@ -78,11 +82,7 @@ void build_levels(BlockSweep& S, int nrow, bool forward) {
   for (int l = 0; l < nlev; ++l) S.lev_start[l + 1] += S.lev_start[l];
   S.order.assign(nrow, 0);
   std::vector<int> fill(S.lev_start.begin(), S.lev_start.end() - (nlev ? 1 : 0));
-  if (forward) {
-    for (int i = 0; i < nrow; ++i) S.order[fill[lev[i]]++] = i;
-  } else {
-    for (int i = 0; i < nrow; ++i) S.order[fill[lev[i]]++] = i;
-  }
+  for (int i = 0; i < nrow; ++i) S.order[fill[lev[i]]++] = i;
 }
 
 // ILU(0) of one block, reference semantics (see file header).
@@ -220,15 +220,38 @@ bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSwe
 }
 
 // Pack per-block level-ordered sweeps into the device layout of `Sweep`.
+// ring_budget: doubles of LDS available for the ring of recent levels.
+// Every level is stored with `padk` entries per row (at least the widest
+// row of the sweep).
+// `nth`: workgroup size of the apply kernel; the LDS ring stores a level's
+// values at stride nth (every thread writes its slot).
 void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& blo, Sweep& S,
-                hipStream_t st) {
+                int ring_budget, int padk, int nth, hipStream_t st) {
   const int nbk = (int)blocks.size();
   std::vector<int> blk_lev(nbk + 1, 0);
-  for (int b = 0; b < nbk; ++b)
-    blk_lev[b + 1] = blk_lev[b] + (int)blocks[b].lev_start.size() - (blocks[b].lev_start.empty() ? 0 : 1);
+  int maxw = 0, maxk = 0;
+  for (int b = 0; b < nbk; ++b) {
+    const BlockSweep& B = blocks[b];
+    int nl = B.lev_start.empty() ? 0 : (int)B.lev_start.size() - 1;
+    blk_lev[b + 1] = blk_lev[b] + nl;
+    for (int l = 0; l < nl; ++l) {
+      int m = B.lev_start[l + 1] - B.lev_start[l];
+      maxw = std::max(maxw, m);
+      for (int t = 0; t < m; ++t) {
+        int i = B.order[B.lev_start[l] + t];
+        maxk = std::max(maxk, B.ptr[i + 1] - B.ptr[i]);
+      }
+    }
+  }
   const int nlev = blk_lev[nbk];
-  std::vector<int> row_off(nlev + 1, 0), ent_off(nlev + 1, 0), kmax(nlev, 0);
-  // sizes
+  maxk = std::max(maxk, padk);
+  // ring of the last W levels in LDS (W >= 2 to be useful: level l reads l-1)
+  int W = (maxw > 0 && maxw <= nth) ? ring_budget / nth : 0;
+  if (W > 4) W = 4;
+  if (W < 2) W = 0;
+  std::vector<int4> meta(nlev);
+  std::vector<long> eo(nlev + 1, 0);
+  std::vector<int> ro(nlev + 1, 0);
   for (int b = 0; b < nbk; ++b) {
     const BlockSweep& B = blocks[b];
     int nl = blk_lev[b + 1] - blk_lev[b];
@@ -240,40 +263,74 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
         int i = B.order[B.lev_start[l] + t];
         k = std::max(k, B.ptr[i + 1] - B.ptr[i]);
       }
-      kmax[g] = k;
-      row_off[g + 1] = m;
-      ent_off[g + 1] = k * m;
+      (void)k;
+      // every level of the sweep gets the sweep's widest row count, so the
+      // apply kernel can load a fixed number of entries per row
+      meta[g] = make_int4(m, maxk, 0, 0);
+      eo[g + 1] = (long)maxk * m;
+      ro[g + 1] = m;
     }
   }
-  std::vector<long> ent_off_l(nlev + 1, 0);
-  for (int g = 0; g < nlev; ++g) {
-    row_off[g + 1] += row_off[g];
-    ent_off_l[g + 1] = ent_off_l[g] + ent_off[g + 1];
+  for (int g = 0; g < nlev; ++g) { eo[g + 1] += eo[g]; ro[g + 1] += ro[g]; }
+  const long nent = eo[nlev];
+  if (nent > 0x7fffffffL) throw hip_error("block-Jacobi factor too large for int32 offsets", PARCEL_HIP_ERR_UNSUPPORTED);
+  for (int g = 0; g < nlev; ++g) { meta[g].z = (int)eo[g]; meta[g].w = ro[g]; }
+  const int nrows = ro[nlev];
+  // ring mode needs every dependency within W-1 levels; check first
+  bool ring_ok = W > 0;
+  if (ring_ok) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+    for (int b = 0; b < nbk; ++b) {
+      const BlockSweep& B = blocks[b];
+      const int nrow = (int)B.ptr.size() - 1;
+      const int nl = blk_lev[b + 1] - blk_lev[b];
+      std::vector<int> lev(nrow);
+      for (int l = 0; l < nl; ++l)
+        for (int t = B.lev_start[l]; t < B.lev_start[l + 1]; ++t) lev[B.order[t]] = l;
+      for (int i = 0; i < nrow && !bad; ++i)
+        for (int q = B.ptr[i]; q < B.ptr[i + 1]; ++q) {
+          int d = lev[i] - lev[B.dep[q]];
+          if (d < 1 || d > W - 1) { bad = 1; break; }
+        }
+    }
+    ring_ok = !bad;
   }
-  const long nent = ent_off_l[nlev];
-  if (nent > 0x7fffffffL) throw hip_error("block-Jacobi factor too large for int32 offsets", -1);
-  for (int g = 0; g <= nlev; ++g) ent_off[g] = (int)ent_off_l[g];
-  const int nrows = row_off[nlev];
-  std::vector<int> rows(nrows), ecol((size_t)nent, -1);
+  if (!ring_ok) W = 0;
+  const int zero_slot = W * nth;  // ring slot that always holds 0.0
+  const int pad = W ? -(1 + zero_slot) : kPad;
+  std::vector<int> rows(nrows), dep((size_t)nent, pad);
   std::vector<double> eval((size_t)nent, 0.0), diag(nrows);
-  long real = 0;
-  int maxw = 0;
-#pragma omp parallel for schedule(dynamic) reduction(+ : real) reduction(max : maxw)
+  long real = 0, gdeps = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : real, gdeps)
   for (int b = 0; b < nbk; ++b) {
     const BlockSweep& B = blocks[b];
+    const int nrow = (int)B.ptr.size() - 1;
     int nl = blk_lev[b + 1] - blk_lev[b];
+    // level and position of every local row
+    std::vector<int> lev(nrow), pos(nrow);
+    for (int l = 0; l < nl; ++l)
+      for (int t = B.lev_start[l]; t < B.lev_start[l + 1]; ++t) {
+        lev[B.order[t]] = l;
+        pos[B.order[t]] = t - B.lev_start[l];
+      }
     for (int l = 0; l < nl; ++l) {
       int g = blk_lev[b] + l;
-      int m = row_off[g + 1] - row_off[g];
-      maxw = std::max(maxw, m);
+      int m = meta[g].x;
       for (int t = 0; t < m; ++t) {
         int i = B.order[B.lev_start[l] + t];
-        rows[row_off[g] + t] = blo[b] + i;
-        diag[row_off[g] + t] = B.diag[i];
+        rows[ro[g] + t] = blo[b] + i;
+        diag[ro[g] + t] = B.diag[i];
         int e = 0;
         for (int q = B.ptr[i]; q < B.ptr[i + 1]; ++q, ++e) {
-          size_t at = (size_t)ent_off[g] + (size_t)e * m + t;
-          ecol[at] = blo[b] + B.dep[q];
+          size_t at = (size_t)eo[g] + (size_t)e * m + t;
+          int d = B.dep[q];
+          if (W) {
+            dep[at] = -(1 + (lev[d] % W) * nth + pos[d]);
+          } else {
+            dep[at] = blo[b] + d;
+            gdeps++;
+          }
           eval[at] = B.val[q];
           real++;
         }
@@ -282,14 +339,16 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
   }
   S.nlev = nlev;
   S.max_width = maxw;
+  S.max_k = maxk;
+  S.ring = W;
+  S.ring_stride = nth;
   S.entries = nent;
   S.real_entries = real;
+  S.global_deps = gdeps;
   S.blk_lev.upload(blk_lev, st);
-  S.row_off.upload(row_off, st);
-  S.ent_off.upload(ent_off, st);
-  S.kmax.upload(kmax, st);
+  S.meta.upload(meta, st);
   S.rows.upload(rows, st);
-  S.ecol.upload(ecol, st);
+  S.dep.upload(dep, st);
   S.eval.upload(eval, st);
   S.diag.upload(diag, st);
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
@@ -298,47 +357,297 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
 // --- kernels -----------------------------------------------------------------
 
 __global__ void k_scale_pj(int n, const double* __restrict__ r, const double* __restrict__ dinv,
-                           double* __restrict__ s, const int* skip) {
-  if (skip && *skip) return;
+                           double* __restrict__ s) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     s[i] = dinv ? r[i] * dinv[i] : r[i];
 }
 
-// MODE 0: forward, unit diagonal (ILU(0) L):   s = r - sum s*l
-// MODE 1: forward, D-ILU:                        s = dinv * (r - sum s*l)
-// MODE 2: backward, ILU(0) U:                    s = (s - sum s*u) / u_ii
-// MODE 3: backward, D-ILU (entries descending):  s = s - dinv * (sum s*u)
-template <int MODE>
-__global__ void __launch_bounds__(256) k_sweep(const int* __restrict__ blk_lev, const int* __restrict__ row_off,
-                                               const int* __restrict__ ent_off, const int* __restrict__ kmax,
-                                               const int* __restrict__ rows, const int* __restrict__ ecol,
-                                               const double* __restrict__ eval, const double* __restrict__ diag,
-                                               const double* __restrict__ rin, double* s, const int* skip) {
-  if (skip && *skip) return;
-  const int b = blockIdx.x;
-  const int l0 = blk_lev[b], l1 = blk_lev[b + 1];
-  for (int l = l0; l < l1; ++l) {
-    const int ro = row_off[l], m = row_off[l + 1] - ro, eo = ent_off[l], k = kmax[l];
-    for (int t = threadIdx.x; t < m; t += blockDim.x) {
-      const int row = rows[ro + t];
-      double z;
-      if (MODE == 0 || MODE == 1) z = rin[row];
-      else if (MODE == 2) z = s[row];
-      else z = 0.0;
-      for (int e = 0; e < k; ++e) {
-        const int c = ecol[eo + e * m + t];
-        if (c < 0) break;
-        const double v = eval[eo + e * m + t];
-        if (MODE == 3) z = fma(s[c], v, z);
-        else z = fma(-s[c], v, z);
-      }
-      if (MODE == 0) s[row] = z;
-      else if (MODE == 1) s[row] = diag[ro + t] * z;
-      else if (MODE == 2) s[row] = z / diag[ro + t];
-      else s[row] = fma(-diag[ro + t], z, s[row]);
+struct SweepArgs {
+  const int* blk_lev;
+  const int4* meta;
+  const int* rows;
+  const int* dep;
+  const double* eval;
+  const double* diag;
+  int ring;
+  int stride;
+  int ldsbar;  // every dependency is in the LDS ring: level barriers need no HBM wait
+};
+
+// Barrier that orders LDS accesses only.  __syncthreads() also waits for
+// every outstanding global load and store of the wave (CDNA counts both in
+// vmcnt), which would drain the prefetch pipeline at every level.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+constexpr int kKR = 8;  // widest rows handled by the register-pipelined path
+
+// Level data of one thread, filled in two pipeline stages: structure
+// (row id, dependencies, values, scale) two levels ahead, vector operands
+// (r and q, or the forward value s) one level ahead.
+template <int K>
+struct LevelRegs {
+  int row;
+  int m;     // rows in the level (threads >= m only shadow the last row)
+  int dep[K];
+  double val[K];
+  double d;
+  double a;  // forward: r[row]; backward: s[row] (the forward value)
+  double b;  // forward with the fused update: q[row]
+};
+
+constexpr int kMetaLds = 2048;  // levels of metadata held in LDS at a time
+
+// The pipelined path keeps its loads branch-free: threads beyond the
+// level's width load and compute the level's last row as shadows, and
+// look-ahead levels past the block's end re-read its last level.  Only the
+// owning thread (t < m) stores to HBM: a shadow's inputs may be read after
+// the owner has already updated them (r in the fused update, s in the
+// backward sweep), so its result must never be written.
+template <bool FWD, int MODE, int K>
+__device__ __forceinline__ void stage_struct(const SweepArgs& A, const int4* sm, int base, int l, int l1,
+                                             LevelRegs<K>& R) {
+  const int4 mt = sm[min(l, l1 - 1) - base];
+  const int t = min((int)threadIdx.x, mt.x - 1);
+  R.m = mt.x;
+  R.row = A.rows[mt.w + t];
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    R.dep[e] = A.dep[mt.z + e * mt.x + t];
+    R.val[e] = A.eval[mt.z + e * mt.x + t];
+  }
+  if (!(FWD && MODE == BJ_ILU0)) R.d = A.diag[mt.w + t];
+}
+
+template <bool FWD, bool FUSED, int K>
+__device__ __forceinline__ void stage_vec(LevelRegs<K>& R, const double* rin, const double* q, const double* s) {
+  if (FWD) {
+    R.a = rin[R.row];
+    if (FUSED) R.b = q[R.row];
+  } else {
+    R.a = s[R.row];
+  }
+}
+
+template <bool FWD, int MODE, int K, bool RING, bool FUSED>
+__device__ __forceinline__ void compute_level(const SweepArgs& A, const int4* sm, int base, int l0, int L,
+                                              const LevelRegs<K>& C, double* s, double* rout, double alpha,
+                                              double* ring) {
+  const int t = threadIdx.x;
+  const bool own = t < C.m;
+  double z0 = C.a;
+  if (FWD && FUSED) {
+    z0 = fma(-alpha, C.b, C.a);
+    if (own) rout[C.row] = z0;
+  }
+  // FWD:      z = z0 - sum dep*val ; ILU0: z ; DILU: d*z
+  // BWD ILU0: z = z0 - sum dep*val ; z / d
+  // BWD DILU: z = sum dep*val ; z0 - d*z
+  // Padding entries only follow a row's real entries, so skipping them
+  // keeps the stored order.
+  double z = (!FWD && MODE == BJ_DILU) ? 0.0 : z0;
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    const int c = C.dep[e];
+    double v;
+    if (RING) {
+      // padding points at a slot holding 0.0 with value 0.0:
+      // fma(+-0, 0, z) == z exactly, so no branch is needed
+      v = ring[-c - 1];
+    } else {
+      if (c == kPad) continue;
+      v = s[c];
     }
+    if (!FWD && MODE == BJ_DILU) z = fma(v, C.val[e], z);
+    else z = fma(-v, C.val[e], z);
+  }
+  double v;
+  if (FWD) v = MODE == BJ_ILU0 ? z : C.d * z;
+  else v = MODE == BJ_ILU0 ? z / C.d : fma(-C.d, z, z0);
+  if (own) s[C.row] = v;
+  if (RING) ring[((L - l0) % A.ring) * A.stride + t] = v;
+  (void)sm; (void)base;
+}
+
+// Generic level (wide levels, long rows): direct loads, no pipelining.
+template <bool FWD, int MODE>
+__device__ void compute_level_generic(const SweepArgs& A, const int4 mt, int l0, int L, const double* rin,
+                                      double* s, const double* q, double* rout, double alpha, double* ring) {
+  const int m = mt.x, k = mt.y;
+  const int W = A.ring;
+  for (int tt = threadIdx.x; tt < m; tt += blockDim.x) {
+    const int row = A.rows[mt.w + tt];
+    double z0;
+    if (FWD) {
+      z0 = rin[row];
+      if (q) { z0 = fma(-alpha, q[row], z0); rout[row] = z0; }
+    } else {
+      z0 = s[row];
+    }
+    double z = (!FWD && MODE == BJ_DILU) ? 0.0 : z0;
+    for (int e = 0; e < k; ++e) {
+      const int c = A.dep[mt.z + e * m + tt];
+      const double a = A.eval[mt.z + e * m + tt];
+      double v;
+      if (W) {
+        v = ring[-c - 1];
+      } else {
+        if (c == kPad) break;
+        v = s[c];
+      }
+      if (!FWD && MODE == BJ_DILU) z = fma(v, a, z);
+      else z = fma(-v, a, z);
+    }
+    double v;
+    if (FWD) v = MODE == BJ_ILU0 ? z : A.diag[mt.w + tt] * z;
+    else v = MODE == BJ_ILU0 ? z / A.diag[mt.w + tt] : fma(-A.diag[mt.w + tt], z, z0);
+    s[row] = v;
+    if (W) ring[((L - l0) % W) * A.stride + tt] = v;
+  }
+}
+
+__device__ __forceinline__ void level_barrier(const SweepArgs& A) {
+  if (A.ldsbar) lds_barrier();
+  else __syncthreads();
+}
+
+// Load the metadata of levels [c0, min(c0 + kMetaLds, l1)) into LDS.
+__device__ __forceinline__ void load_meta(const SweepArgs& A, int4* sm, int c0, int l1) {
+  __syncthreads();
+  const int cnt = min(kMetaLds, l1 - c0);
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) sm[j] = A.meta[c0 + j];
+  __syncthreads();
+}
+
+// Register-pipelined sweep: every row of the sweep has K entries (padded)
+// and every level fits the workgroup.  Level l computes with data loaded
+// while levels l-2 (structure) and l-1 (vector operands) were computed.
+template <bool FWD, int MODE, int K, bool RING, bool FUSED>
+__device__ void run_sweep_fast(const SweepArgs& A, const double* rin, double* s, const double* q,
+                               double* rout, double alpha, double* ring, int4* sm) {
+  const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
+  if (l1 <= l0) return;
+  // metadata window of kMetaLds levels starting at `base`; it is moved
+  // forward before the look-ahead would leave it
+  int base = l0;
+  if (RING && threadIdx.x == 0) ring[A.ring * A.stride] = 0.0;  // zero slot for padding
+  load_meta(A, sm, base, l1);  // its barriers also publish the zero slot
+  LevelRegs<K> R0, R1, R2;
+  stage_struct<FWD, MODE, K>(A, sm, base, l0, l1, R0);
+  stage_struct<FWD, MODE, K>(A, sm, base, l0 + 1, l1, R1);
+  stage_vec<FWD, FUSED, K>(R0, rin, q, s);
+  int l = l0;
+  for (; l + 2 < l1; l += 3) {
+    if (l + 4 >= base + kMetaLds) {
+      base = l;
+      load_meta(A, sm, base, l1);
+    }
+    stage_struct<FWD, MODE, K>(A, sm, base, l + 2, l1, R2);
+    stage_vec<FWD, FUSED, K>(R1, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
+    level_barrier(A);
+    stage_struct<FWD, MODE, K>(A, sm, base, l + 3, l1, R0);
+    stage_vec<FWD, FUSED, K>(R2, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
+    level_barrier(A);
+    stage_struct<FWD, MODE, K>(A, sm, base, l + 4, l1, R1);
+    stage_vec<FWD, FUSED, K>(R0, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 2, R2, s, rout, alpha, ring);
+    level_barrier(A);
+  }
+  // tail: at most two levels; R0 holds level l, R1 the structure of l+1
+  if (l < l1) {
+    if (l + 1 < l1) stage_vec<FWD, FUSED, K>(R1, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
+    level_barrier(A);
+    if (l + 1 < l1) {
+      compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
+      level_barrier(A);
+    }
+  }
+}
+
+template <bool FWD, int MODE>
+__device__ void run_sweep_generic(const SweepArgs& A, const double* rin, double* s, const double* q,
+                                  double* rout, double alpha, double* ring) {
+  const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
+  if (A.ring && threadIdx.x == 0) ring[A.ring * A.stride] = 0.0;  // zero slot for padding
+  __syncthreads();
+  for (int l = l0; l < l1; ++l) {
+    compute_level_generic<FWD, MODE>(A, A.meta[l], l0, l, rin, s, q, rout, alpha, ring);
     __syncthreads();
   }
+}
+
+// One workgroup per block-Jacobi block: forward then backward sweep of the
+// block (blocks are independent, so no grid-wide synchronisation).  With
+// `fuse`, the forward sweep first applies r -= alpha*q of CG iteration
+// fuse.it (alpha = slot.rho0 / gamma, gamma reduced from the SpMV partials,
+// the same value every workgroup computes) and returns at once if the
+// solve has already ended.  K > 0: every sweep row has at most K entries and
+// every level fits the workgroup (register-pipelined path); K = 0: generic.
+template <int MODE, int NTH, int K>
+__global__ void __launch_bounds__(NTH) k_bj_apply(SweepArgs F, SweepArgs B, const double* rin, double* s,
+                                                  CGFuse fuse) {
+  extern __shared__ double ring[];
+  __shared__ int4 smeta[kMetaLds];
+  __shared__ double sh[NTH / 64];
+  double alpha = 0.0;
+  const double* q = nullptr;
+  double* rout = nullptr;
+  if (fuse.st) {
+    const int d = fuse.st->done_at;
+    if (d != 0 && d <= fuse.it) return;
+    CGSlot* sl = &fuse.st->slot[fuse.it & 1];
+    // gamma in the fixed order: threads 0..kNT-1 sum partials t, t+kNT, ...
+    double a = 0.0;
+    if (threadIdx.x < kNT)
+      for (int j = threadIdx.x; j < fuse.nb_prod; j += kNT) a = a + fuse.part_gamma[j];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    double gamma = sh[0];
+    for (int k = 1; k < kWaves; ++k) gamma = gamma + sh[k];
+    alpha = sl->rho0 / gamma;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sl->alpha = alpha;
+    q = fuse.q;
+    rout = fuse.r;
+    rin = fuse.r;
+  }
+  if (K > 0) {
+    constexpr int KK = K > 0 ? K : 1;
+    if (q) {
+      if (F.ring) run_sweep_fast<true, MODE, KK, true, true>(F, rin, s, q, rout, alpha, ring, smeta);
+      else run_sweep_fast<true, MODE, KK, false, true>(F, rin, s, q, rout, alpha, ring, smeta);
+    } else {
+      if (F.ring) run_sweep_fast<true, MODE, KK, true, false>(F, rin, s, q, rout, alpha, ring, smeta);
+      else run_sweep_fast<true, MODE, KK, false, false>(F, rin, s, q, rout, alpha, ring, smeta);
+    }
+    __syncthreads();  // forward values in HBM before the backward sweep reads them
+    if (B.ring) run_sweep_fast<false, MODE, KK, true, false>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
+    else run_sweep_fast<false, MODE, KK, false, false>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
+  } else {
+    run_sweep_generic<true, MODE>(F, rin, s, q, rout, alpha, ring);
+    __syncthreads();
+    run_sweep_generic<false, MODE>(B, nullptr, s, nullptr, nullptr, 0.0, ring);
+  }
+}
+
+SweepArgs sweep_args(const Sweep& S) {
+  SweepArgs a;
+  a.blk_lev = S.blk_lev.p;
+  a.meta = S.meta.p;
+  a.rows = S.rows.p;
+  a.dep = S.dep.p;
+  a.eval = S.eval.p;
+  a.diag = S.diag.p;
+  a.ring = S.ring;
+  a.stride = S.ring_stride;
+  a.ldsbar = (S.ring > 0 && S.global_deps == 0) ? 1 : 0;
+  return a;
 }
 
 }  // namespace
@@ -383,38 +692,70 @@ void precon_setup(Plan& P) {
       build_levels(B[b], bhi[b] - blo[b], false);
     }
     if (bad) throw hip_error("zero diagonal in a block-Jacobi block", PARCEL_HIP_ERR_ZERO_DIAG);
-    pack_sweep(F, blo, M.fwd, P.stream);
-    pack_sweep(B, blo, M.bwd, P.stream);
+    // both sweeps padded to the entry count the apply kernel is built for
+    int kmax = 1;
+    for (int b = 0; b < nblk; ++b)
+      for (const BlockSweep* S : {&F[b], &B[b]})
+        for (size_t i = 0; i + 1 < S->ptr.size(); ++i) kmax = std::max(kmax, S->ptr[i + 1] - S->ptr[i]);
+    int maxw = 1;
+    for (int b = 0; b < nblk; ++b)
+      for (const BlockSweep* S : {&F[b], &B[b]})
+        for (size_t l = 0; l + 1 < S->lev_start.size(); ++l)
+          maxw = std::max(maxw, S->lev_start[l + 1] - S->lev_start[l]);
+    M.nth = maxw <= 256 ? 256 : maxw <= 512 ? 512 : 1024;
+    pack_sweep(F, blo, M.fwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
+    pack_sweep(B, blo, M.bwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
   } else if (M.kind != PRECON_NONE) {
     throw hip_error("preconditioner not supported on this path", PARCEL_HIP_ERR_UNSUPPORTED);
   }
   M.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void precon_apply(Plan& P, const double* r, double* s, const int* skip) {
+void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse) {
   Precon& M = P.M;
   const int n = P.A.n;
   if (M.kind == PRECON_NONE || M.kind == PRECON_PJ) {
+    if (fuse) throw hip_error("fused update needs the block-Jacobi preconditioner", PARCEL_HIP_ERR_ARG);
     int grid = tiled_nb(n);
     hipLaunchKernelGGL(k_scale_pj, dim3(grid), dim3(kNT), 0, P.stream, n, r,
-                       M.kind == PRECON_PJ ? (const double*)M.dinv.p : nullptr, s, skip);
+                       M.kind == PRECON_PJ ? (const double*)M.dinv.p : nullptr, s);
   } else {
     const int nb = M.nblocks;
-    if (M.mode == BJ_ILU0) {
-      hipLaunchKernelGGL(k_sweep<0>, dim3(nb), dim3(256), 0, P.stream, M.fwd.blk_lev.p, M.fwd.row_off.p,
-                         M.fwd.ent_off.p, M.fwd.kmax.p, M.fwd.rows.p, M.fwd.ecol.p, M.fwd.eval.p,
-                         M.fwd.diag.p, r, s, skip);
-      hipLaunchKernelGGL(k_sweep<2>, dim3(nb), dim3(256), 0, P.stream, M.bwd.blk_lev.p, M.bwd.row_off.p,
-                         M.bwd.ent_off.p, M.bwd.kmax.p, M.bwd.rows.p, M.bwd.ecol.p, M.bwd.eval.p,
-                         M.bwd.diag.p, (const double*)nullptr, s, skip);
-    } else {
-      hipLaunchKernelGGL(k_sweep<1>, dim3(nb), dim3(256), 0, P.stream, M.fwd.blk_lev.p, M.fwd.row_off.p,
-                         M.fwd.ent_off.p, M.fwd.kmax.p, M.fwd.rows.p, M.fwd.ecol.p, M.fwd.eval.p,
-                         M.fwd.diag.p, r, s, skip);
-      hipLaunchKernelGGL(k_sweep<3>, dim3(nb), dim3(256), 0, P.stream, M.bwd.blk_lev.p, M.bwd.row_off.p,
-                         M.bwd.ent_off.p, M.bwd.kmax.p, M.bwd.rows.p, M.bwd.ecol.p, M.bwd.eval.p,
-                         M.bwd.diag.p, (const double*)nullptr, s, skip);
-    }
+    const int nth = M.nth;  // >= kNT: the fused gamma reduction uses kNT threads
+    size_t shm = sizeof(double) * (size_t)(std::max(M.fwd.ring * M.fwd.ring_stride, M.bwd.ring * M.bwd.ring_stride) + 1);
+    CGFuse f;
+    if (fuse) f = *fuse;
+    SweepArgs F = sweep_args(M.fwd), B = sweep_args(M.bwd);
+    // register-pipelined path when every level fits the workgroup and rows
+    // are at most kKR long; the generic kernel otherwise
+    int kk = std::max(M.fwd.max_k, M.bwd.max_k);
+    if (kk < 1) kk = 1;
+    const bool fast = kk <= kKR && nth <= 512;
+    if (!fast) kk = 0;
+#define PARCEL_BJ_K(NTH, KK)                                                                            \
+  do {                                                                                                  \
+    if (M.mode == BJ_ILU0)                                                                              \
+      hipLaunchKernelGGL((k_bj_apply<BJ_ILU0, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f); \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_bj_apply<BJ_DILU, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f); \
+  } while (0)
+#define PARCEL_BJ_LAUNCH(NTH)                                                     \
+  do {                                                                            \
+    switch (kk) {                                                                 \
+      case 1: PARCEL_BJ_K(NTH, 1); break;                                         \
+      case 2: PARCEL_BJ_K(NTH, 2); break;                                         \
+      case 3: PARCEL_BJ_K(NTH, 3); break;                                         \
+      case 4: PARCEL_BJ_K(NTH, 4); break;                                         \
+      case 6: PARCEL_BJ_K(NTH, 6); break;                                         \
+      case 8: PARCEL_BJ_K(NTH, 8); break;                                         \
+      default: PARCEL_BJ_K(NTH, 0); break;                                        \
+    }                                                                             \
+  } while (0)
+    if (nth <= 256) PARCEL_BJ_LAUNCH(256);
+    else if (nth <= 512) PARCEL_BJ_LAUNCH(512);
+    else PARCEL_BJ_K(1024, 0);
+#undef PARCEL_BJ_LAUNCH
+#undef PARCEL_BJ_K
   }
   PARCEL_HIP_CHECK(hipGetLastError());
 }
