@@ -5,10 +5,12 @@ on one MI355X, with HBM GB/s against the roofline.
 Workload (default): 3D 7-point constant-coefficient Poisson 256^3
 (16,777,216 unknowns, 117,047,296 nonzeros, diag 6 / off -1), x_true ~ U[-1,1]
 from SplitMix64 seed 3, b = A x_true, x0 = 0, rtol 1e-10.  Methods:
-  bj  FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks (headline)
+  bj  FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks
   pj  FP64 PCG, point Jacobi (the SpMV/CG path)
 A "step" is one full solve from x0 = 0 with the matrix, b and x already in
-HBM; `value` is the mean time-to-solution of the headline method.
+HBM; `value` is the mean time-to-solution of the fastest method.  The timed
+solves run without per-kernel events; one more solve with HIP events around
+every kernel gives the per-kernel means behind `roofline`.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -51,6 +53,8 @@ def main():
     ap.add_argument("--methods", default="bj,pj")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one plain 2 GiB device-to-device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -74,6 +78,15 @@ def main():
     x_d = D(n=n)
     r_d = D(n=n)
 
+    if args.calib:
+        src = D(n=1 << 28)
+        dst = D(n=1 << 28)
+        ph._check(ph.load().parcel_hip_memcpy(dst.ptr, src.ptr, src.nbytes, 3))
+        ph.load().parcel_hip_device_synchronize()
+        log("calibration copy: %d bytes read, %d bytes written" % (src.nbytes, src.nbytes))
+        src.free()
+        dst.free()
+
     methods = {}
     for m in args.methods.split(","):
         if m == "bj":
@@ -83,7 +96,7 @@ def main():
         else:
             raise SystemExit("unknown method " + m)
         t0 = time.perf_counter()
-        plan = ph.Plan(rp, col, val, profile=1, **kw)
+        plan = ph.Plan(rp, col, val, profile=0, **kw)
         setup = time.perf_counter() - t0
         plan.spmv(xt_d, b_d)
         for _ in range(args.warmup):
@@ -101,6 +114,12 @@ def main():
             st = plan.stats()
             dev_ms.append(st["solve_device_ms"])
             log("%s step %d: %.4f s, niter %d, iret %d, final %.3e" % (m, k, times[-1], niter, iret, hist[-1]))
+        # kernel timing pass: the same solve with HIP events around each kernel
+        plan.set_profile(True)
+        x_d.zero()
+        plan.solve(x_d, b_d, 20000, 1e-10)
+        st = plan.stats()
+        plan.set_profile(False)
         # true residual of the last solve
         plan.spmv(x_d, r_d)
         bb = b_d.numpy()
@@ -123,14 +142,23 @@ def main():
                       "kernels": kernels}
         plan.close()
 
-    head = "bj" if "bj" in methods else list(methods)[0]
+    head = min(methods, key=lambda k: methods[k]["time_to_solution_s"])
     H = methods[head]
     # dominant kernel of the headline method: the one with most time per solve
     dom_name, dom = max(((k, v) for k, v in H["kernels"].items() if "bytes" in v),
                         key=lambda kv: kv[1]["mean_ms"] * kv[1]["calls_per_solve"])
     achieved = dom["GBs"]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pcg_c4_poisson256_traffic.json")
+    if g == 256 and os.path.exists(tf):
+        # HBM bytes per launch of the same kernel on the same workload, from the
+        # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy kernel)
+        tk = json.load(open(tf))["kernels"].get(dom_name)
+        if tk:
+            traffic = tk["traffic_bytes"]
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": dom["bytes"], "mean_ms": round(dom["mean_ms"], 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -167,6 +195,7 @@ def main():
                         "rtol 1e-10" % (g, n, nnz),
             "method": {"bj": "FP64 PCG + block-Jacobi ILU(0), 256 row blocks",
                        "pj": "FP64 PCG + point Jacobi"}[head],
+            "headline": head,
             "methods": methods,
             "parallelism": "1 GPU",
         },

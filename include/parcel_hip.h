@@ -127,6 +127,15 @@ int parcel_hip_dcg(int icomm, double* vecx, const double* vecb, int n, int gn,
                    int iovlflag, int precon_thblock, int Neuman_itr,
                    int independ_nvec, int nBlock, int* iret);
 
+/* parcel_dbicgstab (src/wrapper.f90:362 GPU build, :1813 CPU build): same
+ * arguments as parcel_dcg; BiCGStab of src/pbicg.f90:280-698. */
+int parcel_hip_dbicgstab(int icomm, double* vecx, const double* vecb, int n, int gn,
+                         int nnz, int istart, const double* crsA, const int* crsRow_ptr,
+                         const int* crsCol, int ipreflag, int ILU_method, int addL,
+                         int iflagAS, int* itrmax, double rtolmax, double* reshistory,
+                         int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, int* iret);
+
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
  * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident
@@ -139,6 +148,8 @@ int parcel_hip_plan_create(parcel_hip_plan** plan, int n, int gn, int nnz, int i
 int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb,
                           int* itrmax, double rtolmax, double* reshistory, int* iret);
 int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats);
+/* Switch per-kernel HIP-event timing of later solves on (1) or off (0). */
+int parcel_hip_plan_set_profile(parcel_hip_plan* plan, int on);
 int parcel_hip_plan_destroy(parcel_hip_plan* plan);
 
 /* --- kernels exposed for unit parity tests and measurement --------------- */

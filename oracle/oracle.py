@@ -49,6 +49,7 @@ def load():
         "oracle_precon_nblocks": (_I, [_P]),
         "oracle_pcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_last_times": (None, [_P]),
+        "oracle_bicgstab": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -105,13 +106,17 @@ class Precon:
             self.h = None
 
 
+def bicgstab(rowptr, col, val, b, x0=None, **kw):
+    return pcg(rowptr, col, val, b, x0, _fn="oracle_bicgstab", **kw)
+
+
 def pcg(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=1000,
-        rtol=1e-8, red=TILED):
+        rtol=1e-8, red=TILED, _fn="oracle_pcg"):
     n = rowptr.shape[0] - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     hist = np.zeros(max(itrmax, 1))
     niter, iret = ctypes.c_int(0), ctypes.c_int(0)
-    rc = load().oracle_pcg(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+    rc = getattr(load(), _fn)(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
                            independ_nvec, itrmax, rtol, ctypes.byref(red), _p(hist), ctypes.byref(niter),
                            ctypes.byref(iret))
     if rc != 0:

@@ -534,3 +534,63 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   oracle_precon_free(M);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* KSP_pBiCG (src/pbicg.f90:280-698).                                        */
+/* ------------------------------------------------------------------------ */
+int oracle_bicgstab(int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int ipreflag, int ilu_method,
+                    int nblocks, int independ_nvec, int itrmax, double rtol,
+                    const oracle_red* red, double* hist, int* niter, int* iret) {
+  int err = 0;
+  oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec,
+                                          n, rowptr, col, val, &err);
+  if (!M) return err;
+  size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
+  double* r = (double*)calloc(1, bytes);
+  double* r0 = (double*)calloc(1, bytes);
+  double* p = (double*)calloc(1, bytes);
+  double* q = (double*)calloc(1, bytes);
+  double* pp = (double*)calloc(1, bytes);
+  double* e = (double*)calloc(1, bytes);
+  double* pe = (double*)calloc(1, bytes);
+  double* v = (double*)calloc(1, bytes);
+  if (itrmax <= 0) { *niter = 0; *iret = 1; goto done; }
+  /* r = b - A x, bnorm, rnorm (pbicg.f90:430-445) */
+  oracle_spmv(n, rowptr, col, val, x, r);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  {
+    double bnorm = sqrt(oracle_dot(red, n, b, b));
+    double rnorm = sqrt(oracle_dot(red, n, r, r));
+    memcpy(r0, r, bytes);                                 /* xcpy r -> r0 */
+    memcpy(p, r, bytes);                                  /* xcpy r -> p */
+    double c1 = oracle_dot(red, n, r0, r0), c2 = 0.0, c3 = 0.0, alpha = 0.0, beta = 0.0;
+    int itr;
+    for (itr = 1; itr <= itrmax; ++itr) {
+      hist[itr - 1] = rnorm / bnorm;                      /* pbicg.f90:470 */
+      oracle_precon_apply(M, p, pp);                      /* pp = M^-1 p */
+      oracle_spmv(n, rowptr, col, val, pp, q);            /* q = A pp */
+      c2 = oracle_dot(red, n, r0, q);
+      alpha = c1 / c2;
+      for (int i = 0; i < n; ++i) e[i] = fma(-alpha, q[i], r[i]);
+      oracle_precon_apply(M, e, pe);                      /* pe = M^-1 e */
+      oracle_spmv(n, rowptr, col, val, pe, v);            /* v = A pe */
+      double ev = oracle_dot(red, n, e, v), vv = oracle_dot(red, n, v, v);
+      c3 = ev / vv;
+      for (int i = 0; i < n; ++i) x[i] = fma(c3, pe[i], fma(alpha, pp[i], x[i]));
+      for (int i = 0; i < n; ++i) r[i] = fma(-c3, v[i], e[i]);   /* BiCG_axpy3_dot */
+      c1 = oracle_dot(red, n, r, r0);
+      rnorm = sqrt(oracle_dot(red, n, r, r));
+      if (rnorm / bnorm < rtol) break;                    /* pbicg.f90:598 */
+      beta = c1 / (c3 * c2);
+      double bc = -(beta * c3);
+      for (int i = 0; i < n; ++i) p[i] = fma(beta, p[i], fma(bc, q[i], r[i]));
+    }
+    *niter = itr < itrmax ? itr : itrmax;
+    *iret = (itrmax <= itr) ? 1 : 0;
+  }
+done:
+  free(r); free(r0); free(p); free(q); free(pp); free(e); free(pe); free(v);
+  oracle_precon_free(M);
+  return 0;
+}

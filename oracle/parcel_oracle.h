@@ -79,6 +79,15 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
                const double* b, double* x, int ipreflag, int ilu_method,
                int nblocks, int independ_nvec, int itrmax, double rtol,
                const oracle_red* red, double* hist, int* niter, int* iret);
+/* KSP_pBiCG (src/pbicg.f90:280-698), BiCGStab with right preconditioning.
+ * The 3-term updates of axpy_vec3 are evaluated as
+ *   x = fma(c3, pe, fma(alpha, pp, x)),  p = fma(beta, p, fma(-(beta*c3), q, r)),
+ *   e = fma(-alpha, q, r),  r = fma(-c3, v, e). */
+int oracle_bicgstab(int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int ipreflag, int ilu_method,
+                    int nblocks, int independ_nvec, int itrmax, double rtol,
+                    const oracle_red* red, double* hist, int* niter, int* iret);
+
 /* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
  * [1] iterations (everything after setup). */
 void oracle_last_times(double* t);

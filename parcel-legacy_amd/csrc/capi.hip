@@ -168,7 +168,7 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
       A.col[j] = c;
     }
     sell_build(P);
-    if (P.prm.solver != PARCEL_SOLVER_CG)
+    if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
     precon_setup(P);
     P.stats.precon_setup_s = P.M.setup_seconds;
@@ -209,6 +209,12 @@ int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats) 
   return PARCEL_HIP_OK;
 }
 
+int parcel_hip_plan_set_profile(parcel_hip_plan* plan, int on) {
+  if (!plan) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  plan->P.prm.profile = on ? 1 : 0;
+  return PARCEL_HIP_OK;
+}
+
 int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb, int* itrmax,
                           double rtolmax, double* reshistory, int* iret) {
   PARCEL_API_BEGIN
@@ -231,7 +237,8 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
     b = P.b.p;
   }
   int niter = 0, ret = 0;
-  int rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret);
+  int rc = P.prm.solver == PARCEL_SOLVER_BICG ? bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret)
+                                              : cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret);
   if (rc) return rc;
   if (!xdev) {
     PARCEL_HIP_CHECK(hipMemcpyAsync(vecx, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, P.stream));
@@ -241,6 +248,44 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
   *iret = ret;
   return PARCEL_HIP_OK;
   PARCEL_API_END
+}
+
+static int one_shot(parcel_hip_param& prm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                    const double* crsA, const int* crsRow_ptr, const int* crsCol, int* itrmax, double rtolmax,
+                    double* reshistory, int* iret) {
+  parcel_hip_plan* plan = nullptr;
+  int rc = parcel_hip_plan_create(&plan, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &prm);
+  if (rc) return rc;
+  rc = parcel_hip_plan_solve(plan, vecx, vecb, itrmax, rtolmax, reshistory, iret);
+  parcel_hip_plan_destroy(plan);
+  return rc;
+}
+
+static void unset_param(parcel_hip_param& prm) {
+  for (size_t k = 0; k < sizeof(prm) / sizeof(int); ++k) reinterpret_cast<int*>(&prm)[k] = -1;
+  prm.profile = 0;
+}
+
+int parcel_hip_dbicgstab(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                         const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                         int ILU_method, int addL, int iflagAS, int* itrmax, double rtolmax,
+                         double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_BICG;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
+                  iret);
 }
 
 int parcel_hip_dcg(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,

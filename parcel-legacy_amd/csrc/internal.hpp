@@ -137,6 +137,8 @@ struct Plan {
   Precon M;
   // workspace
   DevBuf<double> r, q, s, p0, p1, x, b, t0;
+  std::vector<DevBuf<double>> vec;  // extra work vectors of the other solvers
+  DevBuf<char> solver_state;        // device-resident scalars of the other solvers
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
   DevBuf<double> hist;      // residual history
@@ -161,13 +163,21 @@ struct CGFuse {
 
 // setup helpers (precon.hip)
 void precon_setup(Plan& P);
-void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse);
+// skip: if non-null, the kernels return at once when *skip != 0 (a solver
+// whose loop has already ended on the device).
+void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip = nullptr);
 void sell_build(Plan& P);
 
 // solvers and kernels (krylov.hip)
 void plan_workspace(Plan& P, int itrmax);
 int cg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
              double* hist_host, int* niter, int* iret);
+int bicg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
+               double* hist_host, int* niter, int* iret);
+// n-vector work buffers 0..k-1 of P.vec
+void ensure_vecs(Plan& P, int k);
+// launch helpers shared by the solvers (krylov.hip)
+void launch_residual(Plan& P, const double* x, const double* b, double* r, double* part);
 void spmv(Plan& P, const double* x, double* y);
 double dot_device(int n, const double* x, const double* y, hipStream_t st);
 double time_kernel(Plan& P, int which, int reps);

@@ -1,0 +1,54 @@
+// kernels.hpp -- device helpers shared by the solver translation units:
+// SELL-64 row products and the fixed-order reduction of partial sums.
+#pragma once
+
+#include "internal.hpp"
+
+namespace parcel {
+
+struct SellArgs {
+  const long* sptr;
+  const int* sk;
+  const int* col;
+  const double* val;
+};
+
+inline SellArgs sell_args(const DevSELL& S) { return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p}; }
+
+// Row i of the SELL matrix times the vector produced by `g(c)`, entries in
+// stored order.
+template <class G>
+__device__ __forceinline__ double sell_row(const SellArgs& A, long i, G g) {
+  const long s = i >> 6;
+  const long base = A.sptr[s] + (i & 63);
+  const int k = A.sk[s];
+  double t = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const int c = A.col[base + (long)j * 64];
+    const double v = A.val[base + (long)j * 64];
+    if (c >= 0) t = fma(v, g(c), t);
+  }
+  return t;
+}
+
+// Reduce the nb partials of a producer kernel (reference order of
+// common.hpp) in a workgroup of kNT threads.
+__device__ __forceinline__ double reduce_part(const double* part, int nb, double* sh) {
+  double a = 0.0;
+  for (int j = threadIdx.x; j < nb; j += kNT) a = a + part[j];
+  return block_sum(a, sh);
+}
+
+
+// Partials of up to three dot products over the tiled layout, written to
+// part[slot0], part[slot1], part[slot2] (each kNBMax doubles apart).
+template <int ND>
+__device__ __forceinline__ void store_partials(double* part, const int* slots, const double* acc, double* sh) {
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double v = block_sum(acc[d], sh);
+    if (threadIdx.x == 0) part[slots[d] * kNBMax + blockIdx.x] = v;
+  }
+}
+
+}  // namespace parcel

@@ -29,7 +29,7 @@
 #include <cmath>
 #include <vector>
 
-#include "internal.hpp"
+#include "kernels.hpp"
 
 namespace parcel {
 
@@ -82,39 +82,6 @@ void sell_build(Plan& P) {
 }
 
 namespace {
-
-struct SellArgs {
-  const long* sptr;
-  const int* sk;
-  const int* col;
-  const double* val;
-};
-
-SellArgs sell_args(const DevSELL& S) { return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p}; }
-
-// Row i of the SELL matrix times the vector produced by `g(c)`, entries in
-// stored order.
-template <class G>
-__device__ __forceinline__ double sell_row(const SellArgs& A, long i, G g) {
-  const long s = i >> 6;
-  const long base = A.sptr[s] + (i & 63);
-  const int k = A.sk[s];
-  double t = 0.0;
-  for (int j = 0; j < k; ++j) {
-    const int c = A.col[base + (long)j * 64];
-    const double v = A.val[base + (long)j * 64];
-    if (c >= 0) t = fma(v, g(c), t);
-  }
-  return t;
-}
-
-// Reduce the nb partials of a producer kernel (reference order of
-// common.hpp) in a workgroup of kNT threads.
-__device__ __forceinline__ double reduce_part(const double* part, int nb, double* sh) {
-  double a = 0.0;
-  for (int j = threadIdx.x; j < nb; j += kNT) a = a + part[j];
-  return block_sum(a, sh);
-}
 
 // r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,
 // src/krylov_h.f90:343; pcg.f90:456-470)
@@ -335,6 +302,16 @@ __global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgs A, const double* _
 }
 
 }  // namespace
+
+void ensure_vecs(Plan& P, int k) {
+  const int n = P.A.n;
+  while ((int)P.vec.size() < k) P.vec.emplace_back((size_t)n);
+}
+
+void launch_residual(Plan& P, const double* x, const double* b, double* r, double* part) {
+  const int n = P.A.n;
+  hipLaunchKernelGGL(k_residual, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, b, r, part);
+}
 
 void plan_workspace(Plan& P, int itrmax) {
   const int n = P.A.n;

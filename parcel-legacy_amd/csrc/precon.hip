@@ -357,7 +357,8 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
 // --- kernels -----------------------------------------------------------------
 
 __global__ void k_scale_pj(int n, const double* __restrict__ r, const double* __restrict__ dinv,
-                           double* __restrict__ s) {
+                           double* __restrict__ s, const int* skip) {
+  if (skip && *skip) return;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     s[i] = dinv ? r[i] * dinv[i] : r[i];
 }
@@ -591,7 +592,8 @@ __device__ void run_sweep_generic(const SweepArgs& A, const double* rin, double*
 // every level fits the workgroup (register-pipelined path); K = 0: generic.
 template <int MODE, int NTH, int K>
 __global__ void __launch_bounds__(NTH) k_bj_apply(SweepArgs F, SweepArgs B, const double* rin, double* s,
-                                                  CGFuse fuse) {
+                                                  CGFuse fuse, const int* skip) {
+  if (skip && *skip) return;
   extern __shared__ double ring[];
   __shared__ int4 smeta[kMetaLds];
   __shared__ double sh[NTH / 64];
@@ -711,14 +713,14 @@ void precon_setup(Plan& P) {
   M.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse) {
+void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
   Precon& M = P.M;
   const int n = P.A.n;
   if (M.kind == PRECON_NONE || M.kind == PRECON_PJ) {
     if (fuse) throw hip_error("fused update needs the block-Jacobi preconditioner", PARCEL_HIP_ERR_ARG);
     int grid = tiled_nb(n);
     hipLaunchKernelGGL(k_scale_pj, dim3(grid), dim3(kNT), 0, P.stream, n, r,
-                       M.kind == PRECON_PJ ? (const double*)M.dinv.p : nullptr, s);
+                       M.kind == PRECON_PJ ? (const double*)M.dinv.p : nullptr, s, skip);
   } else {
     const int nb = M.nblocks;
     const int nth = M.nth;  // >= kNT: the fused gamma reduction uses kNT threads
@@ -735,9 +737,9 @@ void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse) {
 #define PARCEL_BJ_K(NTH, KK)                                                                            \
   do {                                                                                                  \
     if (M.mode == BJ_ILU0)                                                                              \
-      hipLaunchKernelGGL((k_bj_apply<BJ_ILU0, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f); \
+      hipLaunchKernelGGL((k_bj_apply<BJ_ILU0, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
     else                                                                                                \
-      hipLaunchKernelGGL((k_bj_apply<BJ_DILU, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f); \
+      hipLaunchKernelGGL((k_bj_apply<BJ_DILU, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
   } while (0)
 #define PARCEL_BJ_LAUNCH(NTH)                                                     \
   do {                                                                            \

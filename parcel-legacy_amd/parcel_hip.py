@@ -79,11 +79,14 @@ SYMBOLS = {
     "parcel_hip_default_param": (None, [ctypes.POINTER(Param)]),
     "parcel_hip_dcg": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                             _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dbicgstab": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                                  _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
                                     ctypes.POINTER(Param)]),
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
     "parcel_hip_plan_stats": (_I, [_P, ctypes.POINTER(Stats)]),
     "parcel_hip_plan_destroy": (_I, [_P]),
+    "parcel_hip_plan_set_profile": (_I, [_P, _I]),
     "parcel_hip_plan_spmv": (_I, [_P, _P, _P]),
     "parcel_hip_plan_precon": (_I, [_P, _P, _P]),
     "parcel_hip_dot": (_I, [_I, _P, _P, _DP]),
@@ -265,7 +268,8 @@ def init(myrank=0):
 
 
 def dcg(x, b, rowptr, col, val, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=1000, rtolmax=1e-8,
-        precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000):
+        precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000,
+        _entry="parcel_hip_dcg"):
     """parcel_dcg: solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
     n = int(rowptr.shape[0]) - 1
@@ -273,12 +277,17 @@ def dcg(x, b, rowptr, col, val, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=10
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
-    rc = lib.parcel_hip_dcg(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+    rc = getattr(lib, _entry)(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
                             ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
                             iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock,
                             ctypes.byref(iret))
     _check(rc)
     return hist[: it.value].copy(), it.value, iret.value
+
+
+def dbicgstab(x, b, rowptr, col, val, **kw):
+    """parcel_dbicgstab: same arguments and results as dcg."""
+    return dcg(x, b, rowptr, col, val, _entry="parcel_hip_dbicgstab", **kw)
 
 
 class Plan:
@@ -302,6 +311,9 @@ class Plan:
         _check(self.lib.parcel_hip_plan_solve(self.h, _ptr(x), _ptr(b), ctypes.byref(it), rtolmax, _ptr(hist),
                                               ctypes.byref(iret)))
         return hist[: it.value].copy(), it.value, iret.value
+
+    def set_profile(self, on):
+        _check(self.lib.parcel_hip_plan_set_profile(self.h, 1 if on else 0))
 
     def spmv(self, x, y):
         _check(self.lib.parcel_hip_plan_spmv(self.h, _ptr(x), _ptr(y)))

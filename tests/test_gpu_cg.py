@@ -207,3 +207,30 @@ def test_pcg_matches_reference_golden_160(ilu, nblk, gname):
         assert np.max(np.abs(hist - gold[:, c]) / gold[:, c]) <= 1e-12
     err = np.abs(x - xe).max()
     assert abs(err - GOLD[gname]["extras"]["max_abs_err"][0]) <= 1e-6 * err
+
+
+@pytest.mark.parametrize("case", ["poisson2d_64", "parcel3d_20", "random_5000"])
+@pytest.mark.parametrize("name,kw", [PRECONS[0], PRECONS[2], PRECONS[3], ("none", dict(ipreflag=0))],
+                         ids=["pj", "ilu0_7", "ilud_13", "none"])
+def test_bicgstab_history_bit_exact(case, name, kw):
+    say("bicgstab", case, name)
+    rp, col, val = CASES[case]()
+    n = rp.shape[0] - 1
+    b = ph.uniform(n, 22)
+    x = np.zeros(n)
+    hist, niter, iret = ph.dbicgstab(x, b, rp, col, val, itrmax=3000, rtolmax=1e-10, ipreflag=kw["ipreflag"],
+                                     ILU_method=kw.get("ILU_method", 1), precon_thblock=kw.get("precon_thblock", 1))
+    xo, ho, no, io = orc.bicgstab(rp, col, val, b, ipreflag=kw["ipreflag"], ilu_method=kw.get("ILU_method", 1),
+                                  nblocks=kw.get("precon_thblock", 1), itrmax=3000, rtol=1e-10)
+    assert (niter, iret) == (no, io)
+    assert np.array_equal(hist, ho) and np.array_equal(x, xo)
+
+
+def test_bicgstab_itrmax_and_edge():
+    rp, col, val = ph.poisson2d(16, 16, -1.0)
+    b = ph.uniform(256, 3)
+    for itrmax in (0, 1, 4):
+        x = np.zeros(256)
+        hist, niter, iret = ph.dbicgstab(x, b, rp, col, val, ipreflag=1, itrmax=itrmax, rtolmax=1e-15)
+        xo, ho, no, io = orc.bicgstab(rp, col, val, b, ipreflag=1, itrmax=itrmax, rtol=1e-15)
+        assert (niter, iret) == (no, io) and np.array_equal(hist, ho) and np.array_equal(x, xo)

@@ -257,3 +257,69 @@ def test_pcg_exit_semantics():
     assert iret == 0 and hist[-1] >= 1e-10
     A = csr_to_scipy(rp, col, val)
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
+
+
+# --- BiCGStab (src/pbicg.f90:280-698) --------------------------------------------
+
+def convdiff3d(g, peclet=0.5, seed=None):
+    """Small nonsymmetric 7-point upwind convection-diffusion operator (1-based CSR)."""
+    n = g ** 3
+    h = 1.0 / (g + 1)
+    vel = np.array([1.0, 0.5, 0.25])
+    vel = vel * (2.0 * peclet / (h * np.abs(vel).max()))
+    rows, cols, vals = [], [], []
+    idx = lambda i, j, k: i + g * (j + g * k)
+    for k in range(g):
+        for j in range(g):
+            for i in range(g):
+                r = idx(i, j, k)
+                diag = 6.0 / h ** 2 + np.abs(vel).sum() / h
+                for d, (di, dj, dk) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    for sgn in (-1, 1):
+                        ii, jj, kk = i + sgn * di, j + sgn * dj, k + sgn * dk
+                        if 0 <= ii < g and 0 <= jj < g and 0 <= kk < g:
+                            c = -1.0 / h ** 2
+                            if sgn == -1:
+                                c -= vel[d] / h  # upwind for positive velocity
+                            rows.append(r); cols.append(idx(ii, jj, kk)); vals.append(c)
+                rows.append(r); cols.append(r); vals.append(diag)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.astype(np.float64), A
+
+
+def test_bicgstab_diagonal_one_iteration():
+    # A = diag(d), no preconditioner: r0 = b, q = A b, alpha = <b,b>/<b,Ab>,
+    # e = b - alpha A b, v = A e, c3 = <e,v>/<v,v>; with a point-Jacobi
+    # preconditioner M^-1 A = I and the first iteration is exact
+    n = 50
+    d = np.linspace(1.0, 3.0, n)
+    A = sp.diags(d).tocsr()
+    rp, col, val = A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.copy()
+    b = np.random.default_rng(3).standard_normal(n)
+    x, hist, niter, iret = orc.bicgstab(rp, col, val, b, ipreflag=1, itrmax=10, rtol=1e-14)
+    assert niter == 1 and iret == 0
+    assert np.allclose(x, b / d, rtol=1e-15, atol=0)
+
+
+def test_bicgstab_matches_scipy_nonsymmetric():
+    import scipy.sparse.linalg as spla
+    rp, col, val, A = convdiff3d(10)
+    b = np.random.default_rng(4).standard_normal(A.shape[0])
+    x, hist, niter, iret = orc.bicgstab(rp, col, val, b, ipreflag=0, itrmax=500, rtol=1e-10)
+    assert iret == 0
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
+    its = []
+    xs, info = spla.bicgstab(A, b, rtol=1e-10, atol=0, maxiter=500, callback=lambda xk: its.append(1))
+    assert info == 0
+    assert abs(len(its) - niter) <= 2
+    assert np.allclose(x, xs, rtol=1e-6, atol=1e-9 * np.abs(xs).max())
+
+
+def test_bicgstab_block_jacobi_converges_faster():
+    rp, col, val, A = convdiff3d(12)
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    _, _, n0, _ = orc.bicgstab(rp, col, val, b, ipreflag=0, itrmax=500, rtol=1e-10)
+    x, _, n2, iret = orc.bicgstab(rp, col, val, b, ipreflag=2, ilu_method=0, nblocks=4, itrmax=500, rtol=1e-10)
+    assert iret == 0 and n2 < n0
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
