@@ -45,6 +45,14 @@ extern "C" {
 #define PARCEL_ILUD 1
 #define PARCEL_MLUD 2
 
+/* Gram-Schmidt / QR variants, src/define.f90:56-68 (GMRES_GSflag: 1 MGS,
+ * any other value CGS, as src/pgmres.f90:336 tests only for MGS) */
+#define PARCEL_QR_MGS 1
+#define PARCEL_QR_CGS 2
+#define PARCEL_QR_TSQR 3
+#define PARCEL_QR_CHOLESKYQR 4
+#define PARCEL_QR_CHOLESKYQR2 5
+
 /* return codes of the solvers (iret), src/define.f90:101-102 */
 #define PARCEL_RETURNCODE_NORMAL 0
 #define PARCEL_RETURNCODE_MAXITERATION 1
@@ -135,6 +143,19 @@ int parcel_hip_dbicgstab(int icomm, double* vecx, const double* vecb, int n, int
                          int iflagAS, int* itrmax, double rtolmax, double* reshistory,
                          int iovlflag, int precon_thblock, int Neuman_itr,
                          int independ_nvec, int nBlock, int* iret);
+
+/* parcel_dgmres (src/wrapper.f90:727-818 GPU build, :2179-2270 CPU build):
+ * parcel_dcg's arguments plus the restart length GMRES_m and the
+ * orthogonalisation GMRES_GSflag; restarted GMRES of src/pgmres.f90:62-525
+ * with right preconditioning.  The optional machineflag argument of the
+ * reference selects CPU or GPU and has no counterpart here. */
+int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn,
+                      int nnz, int istart, const double* crsA, const int* crsRow_ptr,
+                      const int* crsCol, int ipreflag, int ILU_method, int addL,
+                      int iflagAS, int* itrmax, double rtolmax, double* reshistory,
+                      int iovlflag, int precon_thblock, int Neuman_itr,
+                      int independ_nvec, int nBlock, int GMRES_m, int GMRES_GSflag,
+                      int* iret);
 
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,

@@ -50,6 +50,7 @@ def load():
         "oracle_pcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_last_times": (None, [_P]),
         "oracle_bicgstab": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
+        "oracle_gmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _RP, _P, _IP, _IP]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -122,6 +123,21 @@ def pcg(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, indep
     if rc != 0:
         raise RuntimeError("oracle_pcg failed: %d" % rc)
     return x, hist[: niter.value].copy(), niter.value, iret.value
+
+
+def gmres(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=1000,
+          rtol=1e-8, m=4, gsflag=1, red=TILED):
+    """Restarted GMRES(m); returns (x, hist[:niter], niter, iret)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, m + 1, 1))
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    rc = load().oracle_gmres(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+                             independ_nvec, itrmax, rtol, m, gsflag, ctypes.byref(red), _p(hist),
+                             ctypes.byref(niter), ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_gmres failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
 
 
 def last_times():

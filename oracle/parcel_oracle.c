@@ -594,3 +594,131 @@ done:
   oracle_precon_free(M);
   return 0;
 }
+
+/* Givens rotation of Hessenberg column i (1-based) and of e
+ * (GivensRotations, src/pcagmres.f90:190-242); plain arithmetic. */
+static void givens(int m, double* H, int i, double* c, double* s, double* e) {
+  double* hc = H + (size_t)(i - 1) * (m + 1);
+  for (int k = 0; k < i - 1; ++k) {
+    double t1 = hc[k] * c[k] - hc[k + 1] * s[k];
+    double t2 = hc[k] * s[k] + hc[k + 1] * c[k];
+    hc[k] = t1;
+    hc[k + 1] = t2;
+  }
+  double t;
+  if (hc[i - 1] == 0.0) t = hc[i] / 1.0 - 300; /* the reference's `/1.0d0-300` */
+  else t = hc[i] / hc[i - 1];
+  double ci = 1.0 / sqrt(1.0 + t * t);
+  double si = -ci * t;
+  c[i - 1] = ci;
+  s[i - 1] = si;
+  double e1 = ci * e[i - 1] - si * e[i];
+  double e2 = si * e[i - 1] + ci * e[i];
+  e[i - 1] = e1;
+  e[i] = e2;
+  hc[i - 1] = ci * hc[i - 1] - si * hc[i];
+  hc[i] = 0.0;
+}
+
+int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
+                 const double* b, double* x, int ipreflag, int ilu_method,
+                 int nblocks, int independ_nvec, int itrmax, double rtol, int m,
+                 int gsflag, const oracle_red* red, double* hist, int* niter, int* iret) {
+  if (m < 1) return -1;
+  int err = 0;
+  oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec,
+                                          n, rowptr, col, val, &err);
+  if (!M) return err;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  double* r = (double*)calloc(nn, sizeof(double));
+  double* V = (double*)calloc(nn * (size_t)(m + 1), sizeof(double));
+  double* pv = (double*)calloc(nn, sizeof(double));
+  double* w = (double*)calloc(nn, sizeof(double));
+  double* x0 = (double*)calloc(nn, sizeof(double));
+  double* v0 = (double*)calloc(nn, sizeof(double));
+  double* H = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
+  double* c = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* s = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* e = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* y = (double*)calloc((size_t)m + 1, sizeof(double));
+  double* h = (double*)calloc((size_t)m + 1, sizeof(double));
+  const int ld = m + 1;
+  int itr = 1, inner = 1, mm = 0;                  /* pgmres.f90:248 inner = 1 */
+  double bnorm = sqrt(oracle_dot(red, n, b, b));
+  double rnorm = 0.0;
+  for (itr = 1; itr <= itrmax; itr += m) {         /* do itr = 1,itrmax,m */
+    memset(H, 0, sizeof(double) * (size_t)(m + 1) * m);
+    memset(c, 0, sizeof(double) * (size_t)(m + 1));
+    memset(s, 0, sizeof(double) * (size_t)(m + 1));
+    memset(e, 0, sizeof(double) * (size_t)(m + 1));
+    oracle_spmv(n, rowptr, col, val, x, r);        /* r = b - A x */
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    rnorm = sqrt(oracle_dot(red, n, r, r));
+    double a = -1.0 / rnorm;                        /* v1 = (-1/||r||) r */
+    for (int i = 0; i < n; ++i) V[i] = a * r[i];
+    e[0] = -rnorm;
+    if (rnorm / bnorm < rtol) break;                /* pgmres.f90:293 (abstol = 0) */
+    if (itr == 1) hist[0] = rnorm / bnorm;
+    int stop = 0;
+    for (inner = 1; inner <= m; ++inner) {
+      const double* vj = V + (size_t)(inner - 1) * nn;
+      oracle_precon_apply(M, vj, pv);               /* pv = M^-1 v_j */
+      oracle_spmv(n, rowptr, col, val, pv, w);      /* w = A pv */
+      double* hc = H + (size_t)(inner - 1) * ld;
+      if (gsflag == 1) {                            /* MGS, pgmres.f90:336-360 */
+        for (int k = 1; k <= inner; ++k) {
+          const double* vk = V + (size_t)(k - 1) * nn;
+          double hk = oracle_dot(red, n, w, vk);
+          hc[k - 1] = hk;
+          for (int i = 0; i < n; ++i) w[i] = fma(-hk, vk[i], w[i]);
+        }
+      } else {                                      /* CGS, pcagmres.f90:5-51 */
+        for (int k = 1; k <= inner; ++k) h[k - 1] = oracle_dot(red, n, w, V + (size_t)(k - 1) * nn);
+        for (int i = 0; i < n; ++i) {
+          double wi = w[i];
+          for (int k = 1; k <= inner; ++k) wi = fma(-h[k - 1], V[(size_t)(k - 1) * nn + i], wi);
+          w[i] = wi;
+        }
+        for (int k = 1; k <= inner; ++k) hc[k - 1] = h[k - 1];
+      }
+      double hn = sqrt(oracle_dot(red, n, w, w));   /* h(j+1,j) = ||w|| */
+      hc[inner] = hn;
+      double sc = 1.0 / hn;
+      double* vn = V + (size_t)inner * nn;
+      for (int i = 0; i < n; ++i) vn[i] = sc * w[i];
+      givens(m, H, inner, c, s, e);
+      rnorm = fabs(e[inner]);
+      mm = inner;
+      hist[itr + inner - 1] = rnorm / bnorm;       /* reshistory(itr+inner) */
+      if (rnorm / bnorm < rtol) { stop = 1; break; }
+      if (itr + inner == itrmax) { stop = 1; break; }
+    }
+    (void)stop;                                     /* a completed DO leaves inner = m+1 */
+    if (m < mm) mm = m;
+    /* update_xVec: y = H^-1 e (back substitution), v0 = V y, x += M^-1 v0 */
+    for (int i = mm; i >= 1; --i) {
+      double yi = e[i - 1];
+      for (int j = mm; j >= i + 1; --j) yi = yi - H[(i - 1) + (size_t)(j - 1) * ld] * y[j - 1];
+      double d = H[(i - 1) + (size_t)(i - 1) * ld];
+      y[i - 1] = (d == 0.0) ? yi / 1.0e-300 : yi / d;
+    }
+    for (int i = 0; i < n; ++i) {                   /* sum_vec_FP64_CPU, pcagmres.f90:54-90 */
+      double acc = 0.0;
+      for (int k = 0; k < mm; ++k) acc = fma(y[k], V[(size_t)k * nn + i], acc);
+      v0[i] = acc;
+    }
+    oracle_precon_apply(M, v0, x0);
+    for (int i = 0; i < n; ++i) x[i] = x0[i] + x[i];
+    if (rnorm / bnorm < rtol) break;                /* pgmres.f90:457-462 */
+    if (itr + inner == itrmax) break;
+  }
+  {
+    int it = itr + inner;                           /* pgmres.f90:500 */
+    *niter = it < itrmax ? it : itrmax;
+    *iret = (itrmax <= itr) ? 1 : 0;
+  }
+  free(r); free(V); free(pv); free(w); free(x0); free(v0);
+  free(H); free(c); free(s); free(e); free(y); free(h);
+  oracle_precon_free(M);
+  return 0;
+}

@@ -88,6 +88,17 @@ int oracle_bicgstab(int n, const int* rowptr, const int* col, const double* val,
                     int nblocks, int independ_nvec, int itrmax, double rtol,
                     const oracle_red* red, double* hist, int* niter, int* iret);
 
+/* KSP_pGMRES (src/pgmres.f90:62-525), restarted GMRES(m) with right
+ * preconditioning.  gsflag 1: modified Gram-Schmidt; any other value:
+ * classical Gram-Schmidt (CGS_Orth_FP64_CPU, src/pcagmres.f90:5-51).
+ * Givens rotations: src/pcagmres.f90:190-242; solution update: update_xVec,
+ * src/pcagmres.f90:1136-1236.  hist must hold max(itrmax, m+1) entries (the
+ * reference writes reshistory(itr+inner) past itrmax when itrmax < m+1). */
+int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
+                 const double* b, double* x, int ipreflag, int ilu_method,
+                 int nblocks, int independ_nvec, int itrmax, double rtol, int m,
+                 int gsflag, const oracle_red* red, double* hist, int* niter, int* iret);
+
 /* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
  * [1] iterations (everything after setup). */
 void oracle_last_times(double* t);

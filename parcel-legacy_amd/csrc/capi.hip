@@ -168,8 +168,11 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
       A.col[j] = c;
     }
     sell_build(P);
-    if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG)
+    if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
+        P.prm.solver != PARCEL_SOLVER_GMRES)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
+    if (P.prm.solver == PARCEL_SOLVER_GMRES && (P.prm.GMRES_m < 1 || P.prm.GMRES_m > 256))
+      throw hip_error("GMRES_m must be in 1..256", PARCEL_HIP_ERR_ARG);
     precon_setup(P);
     P.stats.precon_setup_s = P.M.setup_seconds;
     P.stats.bj_blocks = P.M.nblocks;
@@ -237,8 +240,12 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
     b = P.b.p;
   }
   int niter = 0, ret = 0;
-  int rc = P.prm.solver == PARCEL_SOLVER_BICG ? bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret)
-                                              : cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret);
+  int rc;
+  switch (P.prm.solver) {
+    case PARCEL_SOLVER_BICG: rc = bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+    case PARCEL_SOLVER_GMRES: rc = gmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+    default: rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+  }
   if (rc) return rc;
   if (!xdev) {
     PARCEL_HIP_CHECK(hipMemcpyAsync(vecx, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, P.stream));
@@ -313,6 +320,30 @@ int parcel_hip_dcg(int icomm, double* vecx, const double* vecb, int n, int gn, i
   rc = parcel_hip_plan_solve(plan, vecx, vecb, itrmax, rtolmax, reshistory, iret);
   parcel_hip_plan_destroy(plan);
   return rc;
+}
+
+int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                      const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                      int ILU_method, int addL, int iflagAS, int* itrmax, double rtolmax,
+                      double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                      int independ_nvec, int nBlock, int GMRES_m, int GMRES_GSflag, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_GMRES;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  prm.GMRES_m = GMRES_m;
+  prm.GMRES_GSflag = GMRES_GSflag;
+  return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
+                  iret);
 }
 
 int parcel_hip_plan_spmv(parcel_hip_plan* plan, const double* x, double* y) {

@@ -138,6 +138,8 @@ struct Plan {
   // workspace
   DevBuf<double> r, q, s, p0, p1, x, b, t0;
   std::vector<DevBuf<double>> vec;  // extra work vectors of the other solvers
+  DevBuf<double> basis;             // Krylov basis columns (GMRES family), n-strided
+  DevBuf<double> wpart;             // partial-sum slots of the other solvers
   DevBuf<char> solver_state;        // device-resident scalars of the other solvers
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
@@ -174,6 +176,8 @@ int cg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rto
              double* hist_host, int* niter, int* iret);
 int bicg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                double* hist_host, int* niter, int* iret);
+int gmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
+                double* hist_host, int* niter, int* iret);
 // n-vector work buffers 0..k-1 of P.vec
 void ensure_vecs(Plan& P, int k);
 // launch helpers shared by the solvers (krylov.hip)

@@ -81,6 +81,8 @@ SYMBOLS = {
                             _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dbicgstab": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                   _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dgmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                               _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
                                     ctypes.POINTER(Param)]),
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
@@ -288,6 +290,23 @@ def dcg(x, b, rowptr, col, val, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=10
 def dbicgstab(x, b, rowptr, col, val, **kw):
     """parcel_dbicgstab: same arguments and results as dcg."""
     return dcg(x, b, rowptr, col, val, _entry="parcel_hip_dbicgstab", **kw)
+
+
+def dgmres(x, b, rowptr, col, val, GMRES_m=4, GMRES_GSflag=1, ipreflag=PRECON_NONE, ILU_method=ILUD,
+           itrmax=1000, rtolmax=1e-8, precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0,
+           Neuman_itr=1, nBlock=2000):
+    """parcel_dgmres: restarted GMRES(m); solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1), np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(lib.parcel_hip_dgmres(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+                                 ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                                 iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, GMRES_m,
+                                 GMRES_GSflag, ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
 class Plan:
