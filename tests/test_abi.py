@@ -21,6 +21,26 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(parcel_(?:hip|gen)_\w+)\s*\(", text)))
 
 
+FORTRAN_HEADER = os.path.join(ROOT, "include", "parcel_fortran.h")
+
+
+def test_by_reference_symbols_exported():
+    text = re.sub(r"/\*.*?\*/", "", open(FORTRAN_HEADER).read(), flags=re.S)
+    syms = sorted(set(re.findall(r"\b(parcel_\w+_c_)\s*\(", text)))
+    assert syms == ["parcel_dbicgstab_c_", "parcel_dcg_c_", "parcel_dgmres_c_", "parcel_free_c_", "parcel_init_c_"]
+    lib = ph.load()
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_fortran_module_built():
+    lib_dir = os.path.join(ROOT, "parcel-legacy_amd", "lib")
+    assert os.path.exists(os.path.join(lib_dir, "libparcel_fortran.so"))
+    mod = open(os.path.join(lib_dir, "parcel_wrapper.mod"), "rb").read().lower()
+    for name in (b"parcel_dcg", b"parcel_dbicgstab", b"parcel_dgmres", b"parcel_init", b"parcel_free"):
+        assert name in mod
+
+
 def test_header_symbols_exported_and_bound():
     lib = ph.load()
     syms = declared_symbols()
