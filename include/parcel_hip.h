@@ -157,6 +157,20 @@ int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn
                       int independ_nvec, int nBlock, int GMRES_m, int GMRES_GSflag,
                       int* iret);
 
+/* parcel_dcagmres (src/wrapper.f90:1355-1454 GPU build, :2807-2906 CPU build):
+ * parcel_dcg's arguments plus CAGMRES_sstep (s, 1..15), CAGMRES_tstep (t;
+ * t = 1 only), CAGMRES_basis (0 monomial, 1 Newton with Leja-ordered Ritz
+ * shifts) and CAGMRES_QRflag (PARCEL_QR_MGS, _CGS, _CHOLESKYQR,
+ * _CHOLESKYQR2; TSQR is not available).  CA-GMRES of
+ * src/pcagmres.f90:1557-2517. */
+int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int gn,
+                        int nnz, int istart, const double* crsA, const int* crsRow_ptr,
+                        const int* crsCol, int ipreflag, int ILU_method, int addL,
+                        int iflagAS, int* itrmax, double rtolmax, double* reshistory,
+                        int iovlflag, int precon_thblock, int Neuman_itr,
+                        int independ_nvec, int nBlock, int CAGMRES_sstep, int CAGMRES_tstep,
+                        int CAGMRES_basis, int CAGMRES_QRflag, int* iret);
+
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
  * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident

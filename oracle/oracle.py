@@ -51,6 +51,14 @@ def load():
         "oracle_last_times": (None, [_P]),
         "oracle_bicgstab": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_gmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _RP, _P, _IP, _IP]),
+        "oracle_cagmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _I, _RP, _P, _IP,
+                                _IP]),
+        "oracle_gemm_nn": (None, [_I, _I, _I, _P, _I, _P, _I, _P, _I]),
+        "oracle_trmm_lunn": (None, [_I, _P, _I, _P, _I]),
+        "oracle_trinv_upper": (None, [_I, _P, _I]),
+        "oracle_chol_upper": (_I, [_I, _P, _I]),
+        "oracle_hqr_eigs": (_I, [_I, _P, _I, _P, _P]),
+        "oracle_leja": (None, [_I, _P, _P]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -138,6 +146,70 @@ def gmres(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, ind
     if rc != 0:
         raise RuntimeError("oracle_gmres failed: %d" % rc)
     return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
+
+
+def cagmres(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=1000,
+            rtol=1e-8, s=4, t=1, basis=0, qrflag=4, red=TILED):
+    """CA-GMRES(s,t); returns (x, hist[:niter], niter, iret)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 0) + s * t + 1)
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    rc = load().oracle_cagmres(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+                               independ_nvec, itrmax, rtol, s, t, basis, qrflag, ctypes.byref(red), _p(hist),
+                               ctypes.byref(niter), ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_cagmres failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
+
+
+def _fortran(a):
+    return np.asfortranarray(np.array(a, dtype=np.float64))
+
+
+def hqr_eigs(H):
+    """Eigenvalues (wr, wi) of an upper Hessenberg matrix (DLAHQR order)."""
+    A = _fortran(H)
+    n = A.shape[0]
+    wr, wi = np.zeros(n), np.zeros(n)
+    rc = load().oracle_hqr_eigs(n, ctypes.c_void_p(A.ctypes.data), n, _p(wr), _p(wi))
+    if rc != 0:
+        raise RuntimeError("hqr did not converge: %d" % rc)
+    return wr, wi
+
+
+def leja(re, im):
+    re = np.array(re, dtype=np.float64)
+    im = np.array(im, dtype=np.float64)
+    load().oracle_leja(int(re.shape[0]), _p(re), _p(im))
+    return re, im
+
+
+def chol_upper(A):
+    R = _fortran(A)
+    info = load().oracle_chol_upper(R.shape[0], ctypes.c_void_p(R.ctypes.data), R.shape[0])
+    return np.triu(R), info
+
+
+def trinv_upper(R):
+    A = _fortran(R)
+    load().oracle_trinv_upper(A.shape[0], ctypes.c_void_p(A.ctypes.data), A.shape[0])
+    return np.triu(A)
+
+
+def gemm_nn(A, B):
+    A, B = _fortran(A), _fortran(B)
+    C = np.zeros((A.shape[0], B.shape[1]), order="F")
+    load().oracle_gemm_nn(A.shape[0], B.shape[1], A.shape[1], ctypes.c_void_p(A.ctypes.data), A.shape[0],
+                          ctypes.c_void_p(B.ctypes.data), B.shape[0], ctypes.c_void_p(C.ctypes.data), A.shape[0])
+    return C
+
+
+def trmm_lunn(A, B):
+    A, B = _fortran(A), _fortran(B)
+    load().oracle_trmm_lunn(A.shape[0], ctypes.c_void_p(A.ctypes.data), A.shape[0], ctypes.c_void_p(B.ctypes.data),
+                            B.shape[0])
+    return B
 
 
 def last_times():

@@ -99,6 +99,27 @@ int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
                  int nblocks, int independ_nvec, int itrmax, double rtol, int m,
                  int gsflag, const oracle_red* red, double* hist, int* niter, int* iret);
 
+/* KSP_pCAGMRES (src/pcagmres.f90:1557-2517), CA-GMRES(s,t) with right
+ * preconditioning; t = 1.  basis: 0 monomial, 1 Newton with Leja-ordered
+ * Ritz shifts.  qrflag: 1 MGS, 2 CGS, 4 CholeskyQR, 5 CholeskyQR2.  hist must
+ * hold itrmax + s entries (the reference writes reshistory(itr+i) up to
+ * itrmax + s when a cycle starts at itr = itrmax).  Returns -4 for t != 1
+ * or another qrflag. */
+int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val,
+                   const double* b, double* x, int ipreflag, int ilu_method,
+                   int nblocks, int independ_nvec, int itrmax, double rtol, int s,
+                   int t, int basis, int qrflag, const oracle_red* red, double* hist,
+                   int* niter, int* iret);
+
+/* Dense kernels used by CA-GMRES (column-major, 0-based); exposed for tests. */
+void oracle_gemm_nn(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc);
+void oracle_trmm_lunn(int n, const double* A, int lda, double* B, int ldb);
+void oracle_trinv_upper(int n, double* A, int lda);
+int oracle_chol_upper(int n, double* A, int lda);
+int oracle_hqr_eigs(int n, double* H, int ldh, double* wr, double* wi);
+void oracle_leja(int n, double* re, double* im);
+
 /* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
  * [1] iterations (everything after setup). */
 void oracle_last_times(double* t);

@@ -169,10 +169,19 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
     }
     sell_build(P);
     if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
-        P.prm.solver != PARCEL_SOLVER_GMRES)
+        P.prm.solver != PARCEL_SOLVER_GMRES && P.prm.solver != PARCEL_SOLVER_CAGMRES)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver == PARCEL_SOLVER_GMRES && (P.prm.GMRES_m < 1 || P.prm.GMRES_m > 256))
       throw hip_error("GMRES_m must be in 1..256", PARCEL_HIP_ERR_ARG);
+    if (P.prm.solver == PARCEL_SOLVER_CAGMRES) {
+      if (P.prm.CAGMRES_sstep < 1 || P.prm.CAGMRES_sstep > 15)
+        throw hip_error("CAGMRES_sstep must be in 1..15", PARCEL_HIP_ERR_ARG);
+      if (P.prm.CAGMRES_tstep != 1)
+        throw hip_error("CAGMRES_tstep > 1 is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
+      const int q = P.prm.CAGMRES_QRflag;
+      if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
+        throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
+    }
     precon_setup(P);
     P.stats.precon_setup_s = P.M.setup_seconds;
     P.stats.bj_blocks = P.M.nblocks;
@@ -244,6 +253,7 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
   switch (P.prm.solver) {
     case PARCEL_SOLVER_BICG: rc = bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_GMRES: rc = gmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+    case PARCEL_SOLVER_CAGMRES: rc = cagmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     default: rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
   }
   if (rc) return rc;
@@ -342,6 +352,33 @@ int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn
   prm.nBlock = nBlock;
   prm.GMRES_m = GMRES_m;
   prm.GMRES_GSflag = GMRES_GSflag;
+  return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
+                  iret);
+}
+
+int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                        const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                        int ILU_method, int addL, int iflagAS, int* itrmax, double rtolmax,
+                        double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                        int independ_nvec, int nBlock, int CAGMRES_sstep, int CAGMRES_tstep,
+                        int CAGMRES_basis, int CAGMRES_QRflag, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CAGMRES;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  prm.CAGMRES_sstep = CAGMRES_sstep;
+  prm.CAGMRES_tstep = CAGMRES_tstep;
+  prm.CAGMRES_basis = CAGMRES_basis;
+  prm.CAGMRES_QRflag = CAGMRES_QRflag;
   return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
                   iret);
 }

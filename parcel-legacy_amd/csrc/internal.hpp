@@ -139,6 +139,7 @@ struct Plan {
   DevBuf<double> r, q, s, p0, p1, x, b, t0;
   std::vector<DevBuf<double>> vec;  // extra work vectors of the other solvers
   DevBuf<double> basis;             // Krylov basis columns (GMRES family), n-strided
+  DevBuf<double> basis2;            // second basis (orthonormal Q of CA-GMRES with MGS/CGS)
   DevBuf<double> wpart;             // partial-sum slots of the other solvers
   DevBuf<char> solver_state;        // device-resident scalars of the other solvers
   DevBuf<CGState> cg;       // device-resident solver scalars
@@ -178,6 +179,8 @@ int bicg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double r
                double* hist_host, int* niter, int* iret);
 int gmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                 double* hist_host, int* niter, int* iret);
+int cagmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
+                  double* hist_host, int* niter, int* iret);
 // n-vector work buffers 0..k-1 of P.vec
 void ensure_vecs(Plan& P, int k);
 // launch helpers shared by the solvers (krylov.hip)

@@ -83,6 +83,8 @@ SYMBOLS = {
                                   _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dgmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dcagmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                                 _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
                                     ctypes.POINTER(Param)]),
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
@@ -306,6 +308,23 @@ def dgmres(x, b, rowptr, col, val, GMRES_m=4, GMRES_GSflag=1, ipreflag=PRECON_NO
                                  ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
                                  iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, GMRES_m,
                                  GMRES_GSflag, ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
+
+
+def dcagmres(x, b, rowptr, col, val, CAGMRES_sstep=4, CAGMRES_tstep=1, CAGMRES_basis=0, CAGMRES_QRflag=4,
+             ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=1000, rtolmax=1e-8, precon_thblock=-1, independ_nvec=1,
+             addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000):
+    """parcel_dcagmres: CA-GMRES(s,t); solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1), np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(lib.parcel_hip_dcagmres(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+                                   ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                                   iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep,
+                                   CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, ctypes.byref(iret)))
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
