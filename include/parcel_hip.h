@@ -33,6 +33,8 @@ extern "C" {
 #define PARCEL_SOLVER_GMRES 3
 #define PARCEL_SOLVER_CAGMRES 4
 #define PARCEL_SOLVER_CBCG 5
+/* FP64 iterative refinement around an FP32 PCG (no reference counterpart) */
+#define PARCEL_SOLVER_CG_MIXED 11
 
 /* preconditioner ids, src/define.f90:30-35 */
 #define PARCEL_PRECON_NONE 0
@@ -112,6 +114,9 @@ typedef struct parcel_hip_stats {
   long bj_entries;         /* stored factor entries (both sweeps, with padding) */
   long bj_nnz_l;           /* off-diagonal entries read by the forward sweep */
   long bj_nnz_u;           /* off-diagonal entries read by the backward sweep */
+  /* mixed-precision refinement (PARCEL_SOLVER_CG_MIXED) */
+  int inner_niter;         /* FP32 PCG iterations summed over the corrections */
+  double final_rel_res;    /* ||b - A x|| / ||b|| after the last correction (FP64) */
 } parcel_hip_stats;
 
 /* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */
@@ -171,6 +176,20 @@ int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int 
                         int independ_nvec, int nBlock, int CAGMRES_sstep, int CAGMRES_tstep,
                         int CAGMRES_basis, int CAGMRES_QRflag, int* iret);
 
+/* Mixed-precision PCG (no counterpart in the reference; its FP64 PCG is
+ * KSP_pCG, src/pcg.f90:224-892): parcel_dcg's arguments plus the inner
+ * tolerance and iteration limit.  FP64 refinement x += d, r = b - A x, each
+ * d from an FP32 PCG on A d = r (FP32 matrix, vectors and preconditioner
+ * factors; dots in double) to inner_rtol.  reshistory(k) = ||b - A x|| / ||b||
+ * before correction k; itrmax/niter count corrections. */
+int parcel_hip_dcg_mixed(int icomm, double* vecx, const double* vecb, int n, int gn,
+                         int nnz, int istart, const double* crsA, const int* crsRow_ptr,
+                         const int* crsCol, int ipreflag, int ILU_method, int addL,
+                         int iflagAS, int* itrmax, double rtolmax, double* reshistory,
+                         int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, double inner_rtol, int inner_itrmax,
+                         int* iret);
+
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
  * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident
@@ -183,6 +202,9 @@ int parcel_hip_plan_create(parcel_hip_plan** plan, int n, int gn, int nnz, int i
 int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb,
                           int* itrmax, double rtolmax, double* reshistory, int* iret);
 int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats);
+/* Inner tolerance and iteration limit of PARCEL_SOLVER_CG_MIXED plans
+ * (defaults 1e-4 and 10000). */
+int parcel_hip_plan_set_inner(parcel_hip_plan* plan, double inner_rtol, int inner_itrmax);
 /* Switch per-kernel HIP-event timing of later solves on (1) or off (0). */
 int parcel_hip_plan_set_profile(parcel_hip_plan* plan, int on);
 int parcel_hip_plan_destroy(parcel_hip_plan* plan);

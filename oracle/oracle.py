@@ -53,6 +53,7 @@ def load():
         "oracle_gmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _RP, _P, _IP, _IP]),
         "oracle_cagmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _I, _RP, _P, _IP,
                                 _IP]),
+        "oracle_pcg_mixed": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _D, _RP, _P, _IP, _IP, _IP]),
         "oracle_gemm_nn": (None, [_I, _I, _I, _P, _I, _P, _I, _P, _I]),
         "oracle_trmm_lunn": (None, [_I, _P, _I, _P, _I]),
         "oracle_trinv_upper": (None, [_I, _P, _I]),
@@ -161,6 +162,21 @@ def cagmres(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, i
     if rc != 0:
         raise RuntimeError("oracle_cagmres failed: %d" % rc)
     return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
+
+
+def pcg_mixed(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=50,
+              rtol=1e-10, inner_itrmax=5000, inner_rtol=1e-4, red=TILED):
+    """FP64 iterative refinement around FP32 PCG; returns (x, hist[:niter], niter, iret, inner_total)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 1))
+    niter, iret, tot = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    rc = load().oracle_pcg_mixed(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+                                 independ_nvec, itrmax, rtol, inner_itrmax, inner_rtol, ctypes.byref(red),
+                                 _p(hist), ctypes.byref(niter), ctypes.byref(iret), ctypes.byref(tot))
+    if rc != 0:
+        raise RuntimeError("oracle_pcg_mixed failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value, tot.value
 
 
 def _fortran(a):

@@ -455,6 +455,56 @@ void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   }
 }
 
+/* FP32 apply: the same sweeps in float, with the FP64 factors rounded to
+ * float where they are read (the mixed-precision inner solve). */
+static void dilu_apply_block32(const oracle_precon* p, int lo, int hi, const float* r, float* s) {
+  const int* cl = p->col; const double* a = p->val;
+  for (int k = lo; k <= hi; ++k) {
+    float zz = r[k - 1];
+    for (int j = p->lstart[k - 1]; j <= p->lend[k - 1]; ++j) zz = fmaf(-s[cl[j] - 1], (float)a[j], zz);
+    s[k - 1] = (float)p->dinv[k - 1] * zz;
+  }
+  for (int k = hi; k >= lo; --k) {
+    float zz = 0.0f;
+    for (int j = p->ustart[k - 1]; j >= p->uend[k - 1]; --j) zz = fmaf(s[cl[j] - 1], (float)a[j], zz);
+    s[k - 1] = fmaf(-(float)p->dinv[k - 1], zz, s[k - 1]);
+  }
+}
+
+static void ilu0_apply_block32(const oracle_precon* p, int b, int lo, int hi, const float* r, float* s) {
+  int nb = hi - lo + 1;
+  const int* lp = p->lptr[b]; const int* lc = p->lcol[b]; const double* lv = p->lval[b];
+  const int* up = p->uptr[b]; const int* uc = p->ucol[b]; const double* uv = p->uval[b];
+  const float* rr = r + (lo - 1);
+  float* ss = s + (lo - 1);
+  for (int i = 0; i < nb; ++i) {
+    float z = rr[i];
+    for (int j = lp[i]; j < lp[i + 1] - 1; ++j) z = fmaf(-ss[lc[j]], (float)lv[j], z);
+    ss[i] = z / (float)lv[lp[i + 1] - 1];
+  }
+  for (int i = nb - 1; i >= 0; --i) {
+    float z = ss[i];
+    for (int j = up[i] + 1; j < up[i + 1]; ++j) z = fmaf(-ss[uc[j]], (float)uv[j], z);
+    ss[i] = z / (float)uv[up[i]];
+  }
+}
+
+void oracle_precon_apply32(const oracle_precon* p, const float* r, float* s) {
+  int n = p->n;
+  if (p->ipreflag == 0) {
+    memcpy(s, r, sizeof(float) * (size_t)n);
+  } else if (p->ipreflag == 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) s[i] = r[i] * (float)p->dinv[i];
+  } else if (p->ilu_method == 0) {
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < p->nblocks; ++b) ilu0_apply_block32(p, b, p->blo[b], p->bhi[b], r, s);
+  } else {
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < p->nblocks; ++b) dilu_apply_block32(p, p->blo[b], p->bhi[b], r, s);
+  }
+}
+
 int oracle_precon_nblocks(const oracle_precon* p) { return p->nblocks; }
 const double* oracle_precon_dinv(const oracle_precon* p) { return p->dinv; }
 
@@ -719,6 +769,101 @@ int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
   }
   free(r); free(V); free(pv); free(w); free(x0); free(v0);
   free(H); free(c); free(s); free(e); free(y); free(h);
+  oracle_precon_free(M);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixed-precision iterative refinement (no reference counterpart): FP64     */
+/* outer loop r = b - A x, x += d with d from an FP32 PCG on A d = r.        */
+/* ------------------------------------------------------------------------ */
+static void spmv32(int n, const int* rowptr, const int* col, const double* val, const float* x, float* y) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k) {
+    float t = 0.0f;
+    for (int j = rowptr[k] - 1; j < rowptr[k + 1] - 1; ++j) t = fmaf((float)val[j], x[col[j] - 1], t);
+    y[k] = t;
+  }
+}
+
+/* <x,y> of float vectors, products and sums in double, tiled order */
+static double dot32(const oracle_red* red, int n, const float* x, const float* y, double* wx, double* wy) {
+  for (int i = 0; i < n; ++i) { wx[i] = (double)x[i]; wy[i] = (double)y[i]; }
+  return oracle_dot(red, n, wx, wy);
+}
+
+/* FP32 PCG on A d = b32 from d = 0 (the GPU's cg_run<float>); returns
+ * the iterations it counts the way oracle_pcg does. */
+static int pcg32(int n, const int* rowptr, const int* col, const double* val, const oracle_precon* M,
+                 const float* b, float* x, int itrmax, double rtol, const oracle_red* red, float* r, float* p,
+                 float* q, float* s, double* wx, double* wy) {
+  for (int i = 0; i < n; ++i) { x[i] = 0.0f; p[i] = 0.0f; }
+  spmv32(n, rowptr, col, val, x, r);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  const double bnorm = sqrt(dot32(red, n, b, b, wx, wy));
+  double rnorm = sqrt(dot32(red, n, r, r, wx, wy));
+  (void)rnorm;
+  double alpha = 0.0, beta = 0.0;
+  oracle_precon_apply32(M, r, s);
+  double rho0 = dot32(red, n, s, r, wx, wy);
+  int itr;
+  for (itr = 1; itr <= itrmax; ++itr) {
+    for (int i = 0; i < n; ++i) {
+      x[i] = fmaf((float)alpha, p[i], x[i]);
+      p[i] = fmaf((float)beta, p[i], s[i]);
+    }
+    spmv32(n, rowptr, col, val, p, q);
+    const double gamma = dot32(red, n, p, q, wx, wy);
+    alpha = rho0 / gamma;
+    for (int i = 0; i < n; ++i) r[i] = fmaf(-(float)alpha, q[i], r[i]);
+    oracle_precon_apply32(M, r, s);
+    const double rho1 = dot32(red, n, s, r, wx, wy);
+    rnorm = sqrt(dot32(red, n, r, r, wx, wy));
+    if (rnorm / bnorm < rtol) break;
+    if (rnorm / bnorm > 1.0e8) break;
+    beta = rho1 / rho0;
+    rho0 = rho1;
+  }
+  for (int i = 0; i < n; ++i) x[i] = fmaf((float)alpha, p[i], x[i]);
+  return itr < itrmax ? itr : itrmax;
+}
+
+int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
+                     int ipreflag, int ilu_method, int nblocks, int independ_nvec, int itrmax, double rtol,
+                     int inner_itrmax, double inner_rtol, const oracle_red* red, double* hist, int* niter,
+                     int* iret, int* inner_total) {
+  int err = 0;
+  oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
+  if (!M) return err;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  double* r = (double*)calloc(nn, sizeof(double));
+  double* wx = (double*)calloc(nn, sizeof(double));
+  double* wy = (double*)calloc(nn, sizeof(double));
+  float* r32 = (float*)calloc(nn, sizeof(float));
+  float* d32 = (float*)calloc(nn, sizeof(float));
+  float* ri = (float*)calloc(nn, sizeof(float));
+  float* pi = (float*)calloc(nn, sizeof(float));
+  float* qi = (float*)calloc(nn, sizeof(float));
+  float* si = (float*)calloc(nn, sizeof(float));
+  int total = 0, it = 0;
+  oracle_spmv(n, rowptr, col, val, x, r);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  const double bnorm = sqrt(oracle_dot(red, n, b, b));
+  double rnorm = sqrt(oracle_dot(red, n, r, r));
+  for (it = 1; it <= itrmax; ++it) {
+    hist[it - 1] = rnorm / bnorm;
+    for (int i = 0; i < n; ++i) r32[i] = (float)r[i];
+    total += pcg32(n, rowptr, col, val, M, r32, d32, inner_itrmax, inner_rtol, red, ri, pi, qi, si, wx, wy);
+    for (int i = 0; i < n; ++i) x[i] = x[i] + (double)d32[i];
+    oracle_spmv(n, rowptr, col, val, x, r);
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    rnorm = sqrt(oracle_dot(red, n, r, r));
+    if (rnorm / bnorm < rtol) break;
+  }
+  *niter = it < itrmax ? it : itrmax;
+  *iret = (itrmax <= it) ? 1 : 0;
+  *inner_total = total;
+  free(r); free(wx); free(wy); free(r32); free(d32); free(ri); free(pi); free(qi); free(si);
   oracle_precon_free(M);
   return 0;
 }

@@ -68,6 +68,9 @@ oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
                                     const int* col, const double* val, int* err);
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s);
 void oracle_precon_free(oracle_precon* p);
+/* The same preconditioner in FP32: sweeps in float with the factors rounded
+ * to float where they are read. */
+void oracle_precon_apply32(const oracle_precon* p, const float* r, float* s);
 /* Inspect the factor data (for tests). */
 int oracle_precon_nblocks(const oracle_precon* p);
 const double* oracle_precon_dinv(const oracle_precon* p);
@@ -119,6 +122,18 @@ void oracle_trinv_upper(int n, double* A, int lda);
 int oracle_chol_upper(int n, double* A, int lda);
 int oracle_hqr_eigs(int n, double* H, int ldh, double* wr, double* wi);
 void oracle_leja(int n, double* re, double* im);
+
+/* Mixed-precision iterative refinement (no reference counterpart): FP64
+ * outer loop (PCG-style history: hist[k-1] = ||b - A x_{k-1}|| / ||b|| before
+ * correction k; exit after the correction whose new residual is below
+ * rtol; niter = corrections, iret = 1 at the limit), each correction an
+ * FP32 PCG from 0 on A d = r to inner_rtol with the FP32 preconditioner
+ * (dots and scalars in double, in the tiled order).  *inner_total = FP32
+ * iterations summed over the corrections. */
+int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val, const double* b,
+                     double* x, int ipreflag, int ilu_method, int nblocks, int independ_nvec,
+                     int itrmax, double rtol, int inner_itrmax, double inner_rtol,
+                     const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total);
 
 /* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
  * [1] iterations (everything after setup). */

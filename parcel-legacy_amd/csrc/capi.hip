@@ -169,7 +169,8 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
     }
     sell_build(P);
     if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
-        P.prm.solver != PARCEL_SOLVER_GMRES && P.prm.solver != PARCEL_SOLVER_CAGMRES)
+        P.prm.solver != PARCEL_SOLVER_GMRES && P.prm.solver != PARCEL_SOLVER_CAGMRES &&
+        P.prm.solver != PARCEL_SOLVER_CG_MIXED)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver == PARCEL_SOLVER_GMRES && (P.prm.GMRES_m < 1 || P.prm.GMRES_m > 256))
       throw hip_error("GMRES_m must be in 1..256", PARCEL_HIP_ERR_ARG);
@@ -221,6 +222,14 @@ int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats) 
   return PARCEL_HIP_OK;
 }
 
+int parcel_hip_plan_set_inner(parcel_hip_plan* plan, double inner_rtol, int inner_itrmax) {
+  if (!plan) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  if (!(inner_rtol > 0.0) || inner_itrmax < 1) return fail(PARCEL_HIP_ERR_ARG, "inner_rtol > 0 and inner_itrmax >= 1");
+  plan->P.inner_rtol = inner_rtol;
+  plan->P.inner_itrmax = inner_itrmax;
+  return PARCEL_HIP_OK;
+}
+
 int parcel_hip_plan_set_profile(parcel_hip_plan* plan, int on) {
   if (!plan) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
   plan->P.prm.profile = on ? 1 : 0;
@@ -254,6 +263,7 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
     case PARCEL_SOLVER_BICG: rc = bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_GMRES: rc = gmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_CAGMRES: rc = cagmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+    case PARCEL_SOLVER_CG_MIXED: rc = mixed_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     default: rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
   }
   if (rc) return rc;
@@ -381,6 +391,34 @@ int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int 
   prm.CAGMRES_QRflag = CAGMRES_QRflag;
   return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
                   iret);
+}
+
+int parcel_hip_dcg_mixed(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                         const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                         int ILU_method, int addL, int iflagAS, int* itrmax, double rtolmax,
+                         double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, double inner_rtol, int inner_itrmax, int* iret) {
+  (void)icomm;
+  if (!(inner_rtol > 0.0) || inner_itrmax < 1) return fail(PARCEL_HIP_ERR_ARG, "inner_rtol > 0 and inner_itrmax >= 1");
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CG_MIXED;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  parcel_hip_plan* plan = nullptr;
+  int rc = parcel_hip_plan_create(&plan, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &prm);
+  if (rc) return rc;
+  rc = parcel_hip_plan_set_inner(plan, inner_rtol, inner_itrmax);
+  if (!rc) rc = parcel_hip_plan_solve(plan, vecx, vecb, itrmax, rtolmax, reshistory, iret);
+  parcel_hip_plan_destroy(plan);
+  return rc;
 }
 
 int parcel_hip_plan_spmv(parcel_hip_plan* plan, const double* x, double* y) {

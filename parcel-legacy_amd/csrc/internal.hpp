@@ -57,6 +57,7 @@ struct DevSELL {
   DevBuf<int> sk;        // nslices: width of each slice
   DevBuf<int> col;       // entries, 0-based, -1 pads
   DevBuf<double> val;    // entries
+  DevBuf<float> val32;   // FP32 copy (mixed-precision inner solve)
 };
 
 // 0-based CSR copy on the host (setup work: factorisations, level analysis).
@@ -84,6 +85,8 @@ struct Sweep {
   DevBuf<int> dep;       // encoded dependencies
   DevBuf<double> eval;   // factor values in accumulation order
   DevBuf<double> diag;   // per level-ordered row: scale (see mode)
+  DevBuf<float> eval32;  // FP32 copies for the mixed-precision inner solve
+  DevBuf<float> diag32;
   int max_width = 0;
   int max_k = 0;
   int ring = 0;          // levels kept in the LDS ring (0: none)
@@ -124,7 +127,9 @@ struct Precon {
   int nblocks = 0;
   int nth = 256;        // workgroup size of the block-Jacobi apply kernel
   DevBuf<double> dinv;  // point Jacobi
+  DevBuf<float> dinv32;
   Sweep fwd, bwd;       // block Jacobi
+  bool has32 = false;   // FP32 copies of the factors exist
   double setup_seconds = 0.0;
 };
 
@@ -141,6 +146,9 @@ struct Plan {
   DevBuf<double> basis;             // Krylov basis columns (GMRES family), n-strided
   DevBuf<double> basis2;            // second basis (orthonormal Q of CA-GMRES with MGS/CGS)
   DevBuf<double> wpart;             // partial-sum slots of the other solvers
+  std::vector<DevBuf<float>> vec32; // FP32 work vectors (mixed-precision solve)
+  double inner_rtol = 1e-4;         // mixed-precision inner PCG tolerance
+  int inner_itrmax = 10000;
   DevBuf<char> solver_state;        // device-resident scalars of the other solvers
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
@@ -155,20 +163,43 @@ enum PartSlot { PART_BB = 0, PART_RR0 = 1, PART_RHO0 = 2, PART_GAMMA = 3, PART_R
 // Fused residual update for the block-Jacobi apply inside CG iteration `it`:
 // r -= alpha*q with alpha = slot.rho0 / gamma, gamma reduced from the
 // partials of the preceding SpMV kernel.
-struct CGFuse {
-  const double* q = nullptr;
-  double* r = nullptr;
+template <class T>
+struct CGFuseT {
+  const T* q = nullptr;
+  T* r = nullptr;
   CGState* st = nullptr;
   const double* part_gamma = nullptr;
   int nb_prod = 0;
   int it = 0;
 };
+using CGFuse = CGFuseT<double>;
+using CGFuse32 = CGFuseT<float>;
+
+// Vectors, matrix view and preconditioner of one CG run (T = double for the
+// FP64 solver, float for the mixed-precision inner solve).
+template <class T>
+struct SellArgsT;
+template <class T>
+struct CGVecs {
+  SellArgsT<T> A;
+  T *r = nullptr, *q = nullptr, *s = nullptr, *p0 = nullptr, *p1 = nullptr;
+  const T* dinv = nullptr;   // point Jacobi scale, or null
+  double* hist = nullptr;    // device residual history (hist_cap entries)
+  int hist_cap = 0;
+  void (*precon)(Plan&, const T*, T*, const CGFuseT<T>*) = nullptr;
+};
+template <class T>
+int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rtol, double* hist_host, int* niter,
+           int* iret);
 
 // setup helpers (precon.hip)
 void precon_setup(Plan& P);
 // skip: if non-null, the kernels return at once when *skip != 0 (a solver
 // whose loop has already ended on the device).
 void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip = nullptr);
+// FP32 apply with the factors rounded to float (precon_setup32 first)
+void precon_setup32(Plan& P);
+void precon_apply32(Plan& P, const float* r, float* s, const CGFuse32* fuse, const int* skip = nullptr);
 void sell_build(Plan& P);
 
 // solvers and kernels (krylov.hip)
@@ -181,6 +212,9 @@ int gmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double 
                 double* hist_host, int* niter, int* iret);
 int cagmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                   double* hist_host, int* niter, int* iret);
+int mixed_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
+                double* hist_host, int* niter, int* iret);
+void mixed_setup(Plan& P);
 // n-vector work buffers 0..k-1 of P.vec
 void ensure_vecs(Plan& P, int k);
 // launch helpers shared by the solvers (krylov.hip)

@@ -6,26 +6,33 @@
 
 namespace parcel {
 
-struct SellArgs {
+// SELL-64 arrays as the kernels see them; T is the value type (double, or
+// float for the FP32 copy used by the mixed-precision inner solve).
+template <class T>
+struct SellArgsT {
   const long* sptr;
   const int* sk;
   const int* col;
-  const double* val;
+  const T* val;
 };
+using SellArgs = SellArgsT<double>;
 
 inline SellArgs sell_args(const DevSELL& S) { return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p}; }
+inline SellArgsT<float> sell_args32(const DevSELL& S) {
+  return SellArgsT<float>{S.sptr.p, S.sk.p, S.col.p, S.val32.p};
+}
 
 // Row i of the SELL matrix times the vector produced by `g(c)`, entries in
-// stored order.
-template <class G>
-__device__ __forceinline__ double sell_row(const SellArgs& A, long i, G g) {
+// stored order, accumulated in T.
+template <class T, class G>
+__device__ __forceinline__ T sell_row(const SellArgsT<T>& A, long i, G g) {
   const long s = i >> 6;
   const long base = A.sptr[s] + (i & 63);
   const int k = A.sk[s];
-  double t = 0.0;
+  T t = 0;
   for (int j = 0; j < k; ++j) {
     const int c = A.col[base + (long)j * 64];
-    const double v = A.val[base + (long)j * 64];
+    const T v = A.val[base + (long)j * 64];
     if (c >= 0) t = fma(v, g(c), t);
   }
   return t;

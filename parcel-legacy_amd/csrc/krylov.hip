@@ -85,9 +85,10 @@ namespace {
 
 // r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,
 // src/krylov_h.f90:343; pcg.f90:456-470)
-__global__ void __launch_bounds__(kNT) k_residual(int n, SellArgs A, const double* __restrict__ x,
-                                                  const double* __restrict__ b, double* __restrict__ r,
-                                                  double* part) {
+// (T: vector and matrix value type; sums of products accumulate in double)
+template <class T>
+__global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T* __restrict__ x,
+                                                  const T* __restrict__ b, T* __restrict__ r, double* part) {
   __shared__ double sh[kWaves];
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
@@ -95,12 +96,12 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, SellArgs A, const doubl
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
-      const double t = sell_row(A, i, [&](int c) { return x[c]; });
-      const double bi = b[i];
-      const double ri = bi - t;
+      const T t = sell_row(A, i, [&](int c) { return x[c]; });
+      const T bi = b[i];
+      const T ri = bi - t;
       r[i] = ri;
-      a0 = fma(bi, bi, a0);
-      a1 = fma(ri, ri, a1);
+      a0 = fma((double)bi, (double)bi, a0);
+      a1 = fma((double)ri, (double)ri, a1);
     }
   }
   double s0 = block_sum(a0, sh);
@@ -112,15 +113,16 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, SellArgs A, const doubl
 }
 
 // partials of <x,y> into part[slot]
-__global__ void __launch_bounds__(kNT) k_dot_part(int n, const double* __restrict__ x,
-                                                  const double* __restrict__ y, double* part) {
+template <class T>
+__global__ void __launch_bounds__(kNT) k_dot_part(int n, const T* __restrict__ x, const T* __restrict__ y,
+                                                  double* part) {
   __shared__ double sh[kWaves];
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0;
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
-    if (i < n) a0 = fma(x[i], y[i], a0);
+    if (i < n) a0 = fma((double)x[i], (double)y[i], a0);
   }
   double s0 = block_sum(a0, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s0;
@@ -153,10 +155,11 @@ __global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGS
 }
 
 // K1 of iteration `it`
-__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgs A,
-                                                 double* __restrict__ x, const double* __restrict__ p_old,
-                                                 const double* __restrict__ s, double* __restrict__ p_new,
-                                                 double* __restrict__ q, double* part, CGState* st,
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgsT<T> A,
+                                                 T* __restrict__ x, const T* __restrict__ p_old,
+                                                 const T* __restrict__ s, T* __restrict__ p_new,
+                                                 T* __restrict__ q, double* part, CGState* st,
                                                  double* hist, int hist_cap) {
   __shared__ double sh[kWaves];
   {
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
     // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
     for (long k = blockIdx.x; k < ntiles; k += nb) {
       const long i = k * kNT + threadIdx.x;
-      if (i < n) x[i] = fma(alpha_prev, p_old[i], x[i]);
+      if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done_at = it;
@@ -204,13 +207,13 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
-      const double po = p_old[i];
-      x[i] = fma(alpha_prev, po, x[i]);
-      const double pn = fma(beta, po, s[i]);
+      const T po = p_old[i];
+      x[i] = fma((T)alpha_prev, po, x[i]);
+      const T pn = fma((T)beta, po, s[i]);
       p_new[i] = pn;
-      const double t = sell_row(A, i, [&](int c) { return fma(beta, p_old[c], s[c]); });
+      const T t = sell_row(A, i, [&](int c) { return fma((T)beta, p_old[c], s[c]); });
       q[i] = t;
-      acc = fma(pn, t, acc);
+      acc = fma((double)pn, (double)t, acc);
     }
   }
   double s0 = block_sum(acc, sh);
@@ -227,11 +230,10 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
 
 // K2 of iteration `it` (point Jacobi or none): alpha; r -= alpha q;
 // s = Dinv r; (rho1, rr) partials
-__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, double* __restrict__ r,
-                                                      const double* __restrict__ q,
-                                                      const double* __restrict__ dinv,
-                                                      double* __restrict__ s, double* part, CGState* st,
-                                                      int nb_prod) {
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restrict__ r, const T* __restrict__ q,
+                                                      const T* __restrict__ dinv, T* __restrict__ s, double* part,
+                                                      CGState* st, int nb_prod) {
   __shared__ double sh[kWaves];
   {
     const int d = st->done_at;
@@ -247,12 +249,12 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, double* __r
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
-      const double ri = fma(-alpha, q[i], r[i]);
+      const T ri = fma(-(T)alpha, q[i], r[i]);
       r[i] = ri;
-      const double si = dinv ? ri * dinv[i] : ri;
+      const T si = dinv ? ri * dinv[i] : ri;
       s[i] = si;
-      a0 = fma(si, ri, a0);
-      a1 = fma(ri, ri, a1);
+      a0 = fma((double)si, (double)ri, a0);
+      a1 = fma((double)ri, (double)ri, a1);
     }
   }
   double s0 = block_sum(a0, sh);
@@ -264,9 +266,9 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, double* __r
 }
 
 // rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30)
-__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const double* __restrict__ s,
-                                                 const double* __restrict__ r, double* part,
-                                                 const CGState* st) {
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restrict__ s, const T* __restrict__ r,
+                                                 double* part, const CGState* st) {
   __shared__ double sh[kWaves];
   {
     const int d = st->done_at;
@@ -278,9 +280,9 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const double* __
   for (long k = blockIdx.x; k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
-      const double ri = r[i];
-      a0 = fma(s[i], ri, a0);
-      a1 = fma(ri, ri, a1);
+      const T ri = r[i];
+      a0 = fma((double)s[i], (double)ri, a0);
+      a1 = fma((double)ri, (double)ri, a1);
     }
   }
   double s0 = block_sum(a0, sh);
@@ -310,7 +312,7 @@ void ensure_vecs(Plan& P, int k) {
 
 void launch_residual(Plan& P, const double* x, const double* b, double* r, double* part) {
   const int n = P.A.n;
-  hipLaunchKernelGGL(k_residual, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, b, r, part);
+  hipLaunchKernelGGL(k_residual<double>, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, b, r, part);
 }
 
 void plan_workspace(Plan& P, int itrmax) {
@@ -335,7 +337,7 @@ void spmv(Plan& P, const double* x, double* y) {
 double dot_device(int n, const double* x, const double* y, hipStream_t st) {
   DevBuf<double> part(kNBMax + 1);
   const int nb = tiled_nb(n);
-  hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kNT), 0, st, n, x, y, part.p);
+  hipLaunchKernelGGL(k_dot_part<double>, dim3(nb), dim3(kNT), 0, st, n, x, y, part.p);
   hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kNT), 0, st, (const double*)part.p, nb, part.p + kNBMax);
   PARCEL_HIP_CHECK(hipGetLastError());
   double h = 0.0;
@@ -373,8 +375,9 @@ struct Timer {
 
 }  // namespace
 
-int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, double* hist_host,
-             int* niter, int* iret) {
+template <class T>
+int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rtol, double* hist_host, int* niter,
+           int* iret) {
   const int n = P.A.n;
   const hipStream_t st = P.stream;
   plan_workspace(P, itrmax);
@@ -383,10 +386,10 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   const dim3 grid(nb), block(kNT);
   CGState* S = P.cg.p;
   double* part = P.part.p;
-  const SellArgs A = sell_args(P.A);
+  const SellArgsT<T> A = V.A;
   int launches = 0;
-  Timer T;
-  T.on = P.prm.profile == 1;
+  Timer Tm;
+  Tm.on = P.prm.profile == 1;
   hipEvent_t ev0, ev1;
   PARCEL_HIP_CHECK(hipEventCreate(&ev0));
   PARCEL_HIP_CHECK(hipEventCreate(&ev1));
@@ -404,10 +407,10 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
     return PARCEL_HIP_OK;
   }
 
-  PARCEL_HIP_CHECK(hipMemsetAsync(P.p0.p, 0, sizeof(double) * (size_t)n, st));
-  hipLaunchKernelGGL(k_residual, grid, block, 0, st, n, A, x, b, P.r.p, part);
-  precon_apply(P, P.r.p, P.s.p, nullptr);
-  hipLaunchKernelGGL(k_dot_part, grid, block, 0, st, n, P.s.p, P.r.p, part + PART_RHO0 * kNBMax);
+  PARCEL_HIP_CHECK(hipMemsetAsync(V.p0, 0, sizeof(T) * (size_t)n, st));
+  hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, A, (const T*)x, b, V.r, part);
+  V.precon(P, V.r, V.s, nullptr);
+  hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax);
   hipLaunchKernelGGL(k_cg_init, dim3(1), block, 0, st, (const double*)part, nb, S);
   launches += 4;
   PARCEL_HIP_CHECK(hipGetLastError());
@@ -418,38 +421,40 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   int enq = 0;  // K1 launches enqueued
   int chunk = 8;
   const bool bj = P.M.kind == PRECON_BJ;
-  const double* dinv = P.M.kind == PRECON_PJ ? P.M.dinv.p : nullptr;
+  const T* dinv = V.dinv;
   while (enq < itrmax + 1) {
     const int k = std::min(chunk, itrmax + 1 - enq);
     for (int j = 1; j <= k; ++j) {
       const int it = enq + j;
-      double* pold = (it - 1) & 1 ? P.p1.p : P.p0.p;
-      double* pnew = it & 1 ? P.p1.p : P.p0.p;
-      hipEvent_t ea = T.on ? T.rec(st) : nullptr;
-      hipLaunchKernelGGL(k_cg_spmv, grid, block, 0, st, n, it, itrmax, rtol, A, x, pold, P.s.p, pnew, P.q.p,
-                         part, S, P.hist.p, P.hist_cap);
+      T* pold = (it - 1) & 1 ? V.p1 : V.p0;
+      T* pnew = it & 1 ? V.p1 : V.p0;
+      hipEvent_t ea = Tm.on ? Tm.rec(st) : nullptr;
+      hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, A, x, (const T*)pold,
+                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap);
       launches++;
-      hipEvent_t eb = T.on ? T.rec(st) : nullptr;
-      if (T.on) T.spmv.push_back({ea, eb, it});
+      hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
+      if (Tm.on) Tm.spmv.push_back({ea, eb, it});
       if (it > itrmax) break;  // the last K1 only closes the loop
       if (!bj) {
-        hipLaunchKernelGGL(k_cg_update_pj, grid, block, 0, st, n, it, P.r.p, P.q.p, dinv, P.s.p, part, S, nb);
+        hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, dinv, V.s, part, S,
+                           nb);
         launches++;
-        if (T.on) T.vec.push_back({eb, T.rec(st), it});
+        if (Tm.on) Tm.vec.push_back({eb, Tm.rec(st), it});
       } else {
-        CGFuse f;
-        f.q = P.q.p;
-        f.r = P.r.p;
+        CGFuseT<T> f;
+        f.q = V.q;
+        f.r = V.r;
         f.st = S;
         f.part_gamma = part + PART_GAMMA * kNBMax;
         f.nb_prod = nb;
         f.it = it;
-        precon_apply(P, P.r.p, P.s.p, &f);
-        hipEvent_t ec = T.on ? T.rec(st) : nullptr;
-        if (T.on) T.prec.push_back({eb, ec, it});
-        hipLaunchKernelGGL(k_cg_dot2, grid, block, 0, st, n, it, P.s.p, P.r.p, part, S);
+        V.precon(P, V.r, V.s, &f);
+        hipEvent_t ec = Tm.on ? Tm.rec(st) : nullptr;
+        if (Tm.on) Tm.prec.push_back({eb, ec, it});
+        hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                           (const CGState*)S);
         launches += 2;
-        if (T.on) T.vec.push_back({ec, T.rec(st), it});
+        if (Tm.on) Tm.vec.push_back({ec, Tm.rec(st), it});
       }
     }
     PARCEL_HIP_CHECK(hipGetLastError());
@@ -469,14 +474,14 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   const int nit = h->niter;
   const int ret = h->iret;
   if (nit > 0 && hist_host)
-    PARCEL_HIP_CHECK(hipMemcpy(hist_host, P.hist.p, sizeof(double) * (size_t)nit, hipMemcpyDeviceToHost));
+    PARCEL_HIP_CHECK(hipMemcpy(hist_host, V.hist, sizeof(double) * (size_t)nit, hipMemcpyDeviceToHost));
   float ms = 0.f;
   PARCEL_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
   P.stats.solve_device_ms = ms;
   P.stats.solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   P.stats.niter = nit;
   P.stats.launches = launches;
-  if (T.on) {
+  if (Tm.on) {
     auto sum = [&](std::vector<EventPair>& v, double& out, int& calls) {
       out = 0.0; calls = 0;
       for (size_t i = 0; i < v.size(); ++i) {
@@ -486,9 +491,9 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
         out += t; calls++;
       }
     };
-    sum(T.spmv, P.stats.spmv_ms, P.stats.spmv_calls);
-    sum(T.prec, P.stats.precon_ms, P.stats.precon_calls);
-    sum(T.vec, P.stats.vec_ms, P.stats.vec_calls);
+    sum(Tm.spmv, P.stats.spmv_ms, P.stats.spmv_calls);
+    sum(Tm.prec, P.stats.precon_ms, P.stats.precon_calls);
+    sum(Tm.vec, P.stats.vec_ms, P.stats.vec_calls);
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
@@ -537,6 +542,25 @@ double time_kernel(Plan& P, int which, int reps) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return ms / reps;
+}
+
+template int cg_run<float>(Plan&, const CGVecs<float>&, float*, const float*, int, double, double*, int*, int*);
+
+int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, double* hist_host, int* niter,
+             int* iret) {
+  plan_workspace(P, itrmax);
+  CGVecs<double> V;
+  V.A = sell_args(P.A);
+  V.r = P.r.p;
+  V.q = P.q.p;
+  V.s = P.s.p;
+  V.p0 = P.p0.p;
+  V.p1 = P.p1.p;
+  V.dinv = P.M.kind == PRECON_PJ ? P.M.dinv.p : nullptr;
+  V.hist = P.hist.p;
+  V.hist_cap = P.hist_cap;
+  V.precon = [](Plan& Q, const double* r, double* s, const CGFuse* f) { precon_apply(Q, r, s, f); };
+  return cg_run<double>(P, V, x, b, itrmax, rtol, hist_host, niter, iret);
 }
 
 }  // namespace parcel

@@ -356,20 +356,24 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
 
 // --- kernels -----------------------------------------------------------------
 
-__global__ void k_scale_pj(int n, const double* __restrict__ r, const double* __restrict__ dinv,
-                           double* __restrict__ s, const int* skip) {
+template <class T>
+__global__ void k_scale_pj(int n, const T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ s,
+                           const int* skip) {
   if (skip && *skip) return;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     s[i] = dinv ? r[i] * dinv[i] : r[i];
 }
 
+// Sweep arrays as the kernels see them; T is the arithmetic type of the
+// factor values and vectors (double, or float for the FP32 apply).
+template <class T>
 struct SweepArgs {
   const int* blk_lev;
   const int4* meta;
   const int* rows;
   const int* dep;
-  const double* eval;
-  const double* diag;
+  const T* eval;
+  const T* diag;
   int ring;
   int stride;
   int ldsbar;  // every dependency is in the LDS ring: level barriers need no HBM wait
@@ -389,15 +393,15 @@ constexpr int kKR = 8;  // widest rows handled by the register-pipelined path
 // Level data of one thread, filled in two pipeline stages: structure
 // (row id, dependencies, values, scale) two levels ahead, vector operands
 // (r and q, or the forward value s) one level ahead.
-template <int K>
+template <int K, class T>
 struct LevelRegs {
   int row;
   int m;     // rows in the level (threads >= m only shadow the last row)
   int dep[K];
-  double val[K];
-  double d;
-  double a;  // forward: r[row]; backward: s[row] (the forward value)
-  double b;  // forward with the fused update: q[row]
+  T val[K];
+  T d;
+  T a;  // forward: r[row]; backward: s[row] (the forward value)
+  T b;  // forward with the fused update: q[row]
 };
 
 constexpr int kMetaLds = 2048;  // levels of metadata held in LDS at a time
@@ -408,9 +412,9 @@ constexpr int kMetaLds = 2048;  // levels of metadata held in LDS at a time
 // owning thread (t < m) stores to HBM: a shadow's inputs may be read after
 // the owner has already updated them (r in the fused update, s in the
 // backward sweep), so its result must never be written.
-template <bool FWD, int MODE, int K>
-__device__ __forceinline__ void stage_struct(const SweepArgs& A, const int4* sm, int base, int l, int l1,
-                                             LevelRegs<K>& R) {
+template <bool FWD, int MODE, int K, class T>
+__device__ __forceinline__ void stage_struct(const SweepArgs<T>& A, const int4* sm, int base, int l, int l1,
+                                             LevelRegs<K, T>& R) {
   const int4 mt = sm[min(l, l1 - 1) - base];
   const int t = min((int)threadIdx.x, mt.x - 1);
   R.m = mt.x;
@@ -423,8 +427,8 @@ __device__ __forceinline__ void stage_struct(const SweepArgs& A, const int4* sm,
   if (!(FWD && MODE == BJ_ILU0)) R.d = A.diag[mt.w + t];
 }
 
-template <bool FWD, bool FUSED, int K>
-__device__ __forceinline__ void stage_vec(LevelRegs<K>& R, const double* rin, const double* q, const double* s) {
+template <bool FWD, bool FUSED, int K, class T>
+__device__ __forceinline__ void stage_vec(LevelRegs<K, T>& R, const T* rin, const T* q, const T* s) {
   if (FWD) {
     R.a = rin[R.row];
     if (FUSED) R.b = q[R.row];
@@ -433,15 +437,14 @@ __device__ __forceinline__ void stage_vec(LevelRegs<K>& R, const double* rin, co
   }
 }
 
-template <bool FWD, int MODE, int K, bool RING, bool FUSED>
-__device__ __forceinline__ void compute_level(const SweepArgs& A, const int4* sm, int base, int l0, int L,
-                                              const LevelRegs<K>& C, double* s, double* rout, double alpha,
-                                              double* ring) {
+template <bool FWD, int MODE, int K, bool RING, bool FUSED, class T>
+__device__ __forceinline__ void compute_level(const SweepArgs<T>& A, const int4* sm, int base, int l0, int L,
+                                              const LevelRegs<K, T>& C, T* s, T* rout, double alpha, T* ring) {
   const int t = threadIdx.x;
   const bool own = t < C.m;
-  double z0 = C.a;
+  T z0 = C.a;
   if (FWD && FUSED) {
-    z0 = fma(-alpha, C.b, C.a);
+    z0 = fma(-(T)alpha, C.b, C.a);
     if (own) rout[C.row] = z0;
   }
   // FWD:      z = z0 - sum dep*val ; ILU0: z ; DILU: d*z
@@ -449,11 +452,11 @@ __device__ __forceinline__ void compute_level(const SweepArgs& A, const int4* sm
   // BWD DILU: z = sum dep*val ; z0 - d*z
   // Padding entries only follow a row's real entries, so skipping them
   // keeps the stored order.
-  double z = (!FWD && MODE == BJ_DILU) ? 0.0 : z0;
+  T z = (!FWD && MODE == BJ_DILU) ? (T)0 : z0;
 #pragma unroll
   for (int e = 0; e < K; ++e) {
     const int c = C.dep[e];
-    double v;
+    T v;
     if (RING) {
       // padding points at a slot holding 0.0 with value 0.0:
       // fma(+-0, 0, z) == z exactly, so no branch is needed
@@ -465,7 +468,7 @@ __device__ __forceinline__ void compute_level(const SweepArgs& A, const int4* sm
     if (!FWD && MODE == BJ_DILU) z = fma(v, C.val[e], z);
     else z = fma(-v, C.val[e], z);
   }
-  double v;
+  T v;
   if (FWD) v = MODE == BJ_ILU0 ? z : C.d * z;
   else v = MODE == BJ_ILU0 ? z / C.d : fma(-C.d, z, z0);
   if (own) s[C.row] = v;
@@ -474,25 +477,25 @@ __device__ __forceinline__ void compute_level(const SweepArgs& A, const int4* sm
 }
 
 // Generic level (wide levels, long rows): direct loads, no pipelining.
-template <bool FWD, int MODE>
-__device__ void compute_level_generic(const SweepArgs& A, const int4 mt, int l0, int L, const double* rin,
-                                      double* s, const double* q, double* rout, double alpha, double* ring) {
+template <bool FWD, int MODE, class T>
+__device__ void compute_level_generic(const SweepArgs<T>& A, const int4 mt, int l0, int L, const T* rin, T* s,
+                                      const T* q, T* rout, double alpha, T* ring) {
   const int m = mt.x, k = mt.y;
   const int W = A.ring;
   for (int tt = threadIdx.x; tt < m; tt += blockDim.x) {
     const int row = A.rows[mt.w + tt];
-    double z0;
+    T z0;
     if (FWD) {
       z0 = rin[row];
-      if (q) { z0 = fma(-alpha, q[row], z0); rout[row] = z0; }
+      if (q) { z0 = fma(-(T)alpha, q[row], z0); rout[row] = z0; }
     } else {
       z0 = s[row];
     }
-    double z = (!FWD && MODE == BJ_DILU) ? 0.0 : z0;
+    T z = (!FWD && MODE == BJ_DILU) ? (T)0 : z0;
     for (int e = 0; e < k; ++e) {
       const int c = A.dep[mt.z + e * m + tt];
-      const double a = A.eval[mt.z + e * m + tt];
-      double v;
+      const T a = A.eval[mt.z + e * m + tt];
+      T v;
       if (W) {
         v = ring[-c - 1];
       } else {
@@ -502,7 +505,7 @@ __device__ void compute_level_generic(const SweepArgs& A, const int4 mt, int l0,
       if (!FWD && MODE == BJ_DILU) z = fma(v, a, z);
       else z = fma(-v, a, z);
     }
-    double v;
+    T v;
     if (FWD) v = MODE == BJ_ILU0 ? z : A.diag[mt.w + tt] * z;
     else v = MODE == BJ_ILU0 ? z / A.diag[mt.w + tt] : fma(-A.diag[mt.w + tt], z, z0);
     s[row] = v;
@@ -510,13 +513,15 @@ __device__ void compute_level_generic(const SweepArgs& A, const int4 mt, int l0,
   }
 }
 
-__device__ __forceinline__ void level_barrier(const SweepArgs& A) {
+template <class T>
+__device__ __forceinline__ void level_barrier(const SweepArgs<T>& A) {
   if (A.ldsbar) lds_barrier();
   else __syncthreads();
 }
 
 // Load the metadata of levels [c0, min(c0 + kMetaLds, l1)) into LDS.
-__device__ __forceinline__ void load_meta(const SweepArgs& A, int4* sm, int c0, int l1) {
+template <class T>
+__device__ __forceinline__ void load_meta(const SweepArgs<T>& A, int4* sm, int c0, int l1) {
   __syncthreads();
   const int cnt = min(kMetaLds, l1 - c0);
   for (int j = threadIdx.x; j < cnt; j += blockDim.x) sm[j] = A.meta[c0 + j];
@@ -526,59 +531,59 @@ __device__ __forceinline__ void load_meta(const SweepArgs& A, int4* sm, int c0, 
 // Register-pipelined sweep: every row of the sweep has K entries (padded)
 // and every level fits the workgroup.  Level l computes with data loaded
 // while levels l-2 (structure) and l-1 (vector operands) were computed.
-template <bool FWD, int MODE, int K, bool RING, bool FUSED>
-__device__ void run_sweep_fast(const SweepArgs& A, const double* rin, double* s, const double* q,
-                               double* rout, double alpha, double* ring, int4* sm) {
+template <bool FWD, int MODE, int K, bool RING, bool FUSED, class T>
+__device__ void run_sweep_fast(const SweepArgs<T>& A, const T* rin, T* s, const T* q, T* rout, double alpha,
+                               T* ring, int4* sm) {
   const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
   if (l1 <= l0) return;
   // metadata window of kMetaLds levels starting at `base`; it is moved
   // forward before the look-ahead would leave it
   int base = l0;
-  if (RING && threadIdx.x == 0) ring[A.ring * A.stride] = 0.0;  // zero slot for padding
+  if (RING && threadIdx.x == 0) ring[A.ring * A.stride] = (T)0;  // zero slot for padding
   load_meta(A, sm, base, l1);  // its barriers also publish the zero slot
-  LevelRegs<K> R0, R1, R2;
-  stage_struct<FWD, MODE, K>(A, sm, base, l0, l1, R0);
-  stage_struct<FWD, MODE, K>(A, sm, base, l0 + 1, l1, R1);
-  stage_vec<FWD, FUSED, K>(R0, rin, q, s);
+  LevelRegs<K, T> R0, R1, R2;
+  stage_struct<FWD, MODE, K, T>(A, sm, base, l0, l1, R0);
+  stage_struct<FWD, MODE, K, T>(A, sm, base, l0 + 1, l1, R1);
+  stage_vec<FWD, FUSED, K, T>(R0, rin, q, s);
   int l = l0;
   for (; l + 2 < l1; l += 3) {
     if (l + 4 >= base + kMetaLds) {
       base = l;
       load_meta(A, sm, base, l1);
     }
-    stage_struct<FWD, MODE, K>(A, sm, base, l + 2, l1, R2);
-    stage_vec<FWD, FUSED, K>(R1, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
+    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 2, l1, R2);
+    stage_vec<FWD, FUSED, K, T>(R1, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
     level_barrier(A);
-    stage_struct<FWD, MODE, K>(A, sm, base, l + 3, l1, R0);
-    stage_vec<FWD, FUSED, K>(R2, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
+    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 3, l1, R0);
+    stage_vec<FWD, FUSED, K, T>(R2, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
     level_barrier(A);
-    stage_struct<FWD, MODE, K>(A, sm, base, l + 4, l1, R1);
-    stage_vec<FWD, FUSED, K>(R0, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 2, R2, s, rout, alpha, ring);
+    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 4, l1, R1);
+    stage_vec<FWD, FUSED, K, T>(R0, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 2, R2, s, rout, alpha, ring);
     level_barrier(A);
   }
   // tail: at most two levels; R0 holds level l, R1 the structure of l+1
   if (l < l1) {
-    if (l + 1 < l1) stage_vec<FWD, FUSED, K>(R1, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
+    if (l + 1 < l1) stage_vec<FWD, FUSED, K, T>(R1, rin, q, s);
+    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
     level_barrier(A);
     if (l + 1 < l1) {
-      compute_level<FWD, MODE, K, RING, FUSED>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
+      compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
       level_barrier(A);
     }
   }
 }
 
-template <bool FWD, int MODE>
-__device__ void run_sweep_generic(const SweepArgs& A, const double* rin, double* s, const double* q,
-                                  double* rout, double alpha, double* ring) {
+template <bool FWD, int MODE, class T>
+__device__ void run_sweep_generic(const SweepArgs<T>& A, const T* rin, T* s, const T* q, T* rout, double alpha,
+                                  T* ring) {
   const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
-  if (A.ring && threadIdx.x == 0) ring[A.ring * A.stride] = 0.0;  // zero slot for padding
+  if (A.ring && threadIdx.x == 0) ring[A.ring * A.stride] = (T)0;  // zero slot for padding
   __syncthreads();
   for (int l = l0; l < l1; ++l) {
-    compute_level_generic<FWD, MODE>(A, A.meta[l], l0, l, rin, s, q, rout, alpha, ring);
+    compute_level_generic<FWD, MODE, T>(A, A.meta[l], l0, l, rin, s, q, rout, alpha, ring);
     __syncthreads();
   }
 }
@@ -590,16 +595,17 @@ __device__ void run_sweep_generic(const SweepArgs& A, const double* rin, double*
 // the same value every workgroup computes) and returns at once if the
 // solve has already ended.  K > 0: every sweep row has at most K entries and
 // every level fits the workgroup (register-pipelined path); K = 0: generic.
-template <int MODE, int NTH, int K>
-__global__ void __launch_bounds__(NTH) k_bj_apply(SweepArgs F, SweepArgs B, const double* rin, double* s,
-                                                  CGFuse fuse, const int* skip) {
+template <int MODE, int NTH, int K, class T>
+__global__ void __launch_bounds__(NTH) k_bj_apply(SweepArgs<T> F, SweepArgs<T> B, const T* rin, T* s,
+                                                  CGFuseT<T> fuse, const int* skip) {
   if (skip && *skip) return;
-  extern __shared__ double ring[];
+  extern __shared__ double ring_lds[];
+  T* ring = reinterpret_cast<T*>(ring_lds);
   __shared__ int4 smeta[kMetaLds];
   __shared__ double sh[NTH / 64];
   double alpha = 0.0;
-  const double* q = nullptr;
-  double* rout = nullptr;
+  const T* q = nullptr;
+  T* rout = nullptr;
   if (fuse.st) {
     const int d = fuse.st->done_at;
     if (d != 0 && d <= fuse.it) return;
@@ -622,30 +628,49 @@ __global__ void __launch_bounds__(NTH) k_bj_apply(SweepArgs F, SweepArgs B, cons
   if (K > 0) {
     constexpr int KK = K > 0 ? K : 1;
     if (q) {
-      if (F.ring) run_sweep_fast<true, MODE, KK, true, true>(F, rin, s, q, rout, alpha, ring, smeta);
-      else run_sweep_fast<true, MODE, KK, false, true>(F, rin, s, q, rout, alpha, ring, smeta);
+      if (F.ring) run_sweep_fast<true, MODE, KK, true, true, T>(F, rin, s, q, rout, alpha, ring, smeta);
+      else run_sweep_fast<true, MODE, KK, false, true, T>(F, rin, s, q, rout, alpha, ring, smeta);
     } else {
-      if (F.ring) run_sweep_fast<true, MODE, KK, true, false>(F, rin, s, q, rout, alpha, ring, smeta);
-      else run_sweep_fast<true, MODE, KK, false, false>(F, rin, s, q, rout, alpha, ring, smeta);
+      if (F.ring) run_sweep_fast<true, MODE, KK, true, false, T>(F, rin, s, q, rout, alpha, ring, smeta);
+      else run_sweep_fast<true, MODE, KK, false, false, T>(F, rin, s, q, rout, alpha, ring, smeta);
     }
     __syncthreads();  // forward values in HBM before the backward sweep reads them
-    if (B.ring) run_sweep_fast<false, MODE, KK, true, false>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
-    else run_sweep_fast<false, MODE, KK, false, false>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
+    if (B.ring) run_sweep_fast<false, MODE, KK, true, false, T>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
+    else run_sweep_fast<false, MODE, KK, false, false, T>(B, nullptr, s, nullptr, nullptr, 0.0, ring, smeta);
   } else {
-    run_sweep_generic<true, MODE>(F, rin, s, q, rout, alpha, ring);
+    run_sweep_generic<true, MODE, T>(F, rin, s, q, rout, alpha, ring);
     __syncthreads();
-    run_sweep_generic<false, MODE>(B, nullptr, s, nullptr, nullptr, 0.0, ring);
+    run_sweep_generic<false, MODE, T>(B, nullptr, s, nullptr, nullptr, 0.0, ring);
   }
 }
 
-SweepArgs sweep_args(const Sweep& S) {
-  SweepArgs a;
+template <class T>
+SweepArgs<T> sweep_args(const Sweep& S);
+
+template <>
+SweepArgs<double> sweep_args<double>(const Sweep& S) {
+  SweepArgs<double> a;
+  a.eval = S.eval.p;
+  a.diag = S.diag.p;
   a.blk_lev = S.blk_lev.p;
   a.meta = S.meta.p;
   a.rows = S.rows.p;
   a.dep = S.dep.p;
-  a.eval = S.eval.p;
-  a.diag = S.diag.p;
+  a.ring = S.ring;
+  a.stride = S.ring_stride;
+  a.ldsbar = (S.ring > 0 && S.global_deps == 0) ? 1 : 0;
+  return a;
+}
+
+template <>
+SweepArgs<float> sweep_args<float>(const Sweep& S) {
+  SweepArgs<float> a;
+  a.eval = S.eval32.p;
+  a.diag = S.diag32.p;
+  a.blk_lev = S.blk_lev.p;
+  a.meta = S.meta.p;
+  a.rows = S.rows.p;
+  a.dep = S.dep.p;
   a.ring = S.ring;
   a.stride = S.ring_stride;
   a.ldsbar = (S.ring > 0 && S.global_deps == 0) ? 1 : 0;
@@ -713,21 +738,31 @@ void precon_setup(Plan& P) {
   M.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
+namespace {
+
+template <class T>
+const T* dinv_of(const Precon& M);
+template <>
+const double* dinv_of<double>(const Precon& M) { return M.dinv.p; }
+template <>
+const float* dinv_of<float>(const Precon& M) { return M.dinv32.p; }
+
+template <class T>
+void precon_apply_t(Plan& P, const T* r, T* s, const CGFuseT<T>* fuse, const int* skip) {
   Precon& M = P.M;
   const int n = P.A.n;
   if (M.kind == PRECON_NONE || M.kind == PRECON_PJ) {
     if (fuse) throw hip_error("fused update needs the block-Jacobi preconditioner", PARCEL_HIP_ERR_ARG);
     int grid = tiled_nb(n);
-    hipLaunchKernelGGL(k_scale_pj, dim3(grid), dim3(kNT), 0, P.stream, n, r,
-                       M.kind == PRECON_PJ ? (const double*)M.dinv.p : nullptr, s, skip);
+    hipLaunchKernelGGL(k_scale_pj<T>, dim3(grid), dim3(kNT), 0, P.stream, n, r,
+                       M.kind == PRECON_PJ ? dinv_of<T>(M) : nullptr, s, skip);
   } else {
     const int nb = M.nblocks;
     const int nth = M.nth;  // >= kNT: the fused gamma reduction uses kNT threads
     size_t shm = sizeof(double) * (size_t)(std::max(M.fwd.ring * M.fwd.ring_stride, M.bwd.ring * M.bwd.ring_stride) + 1);
-    CGFuse f;
+    CGFuseT<T> f;
     if (fuse) f = *fuse;
-    SweepArgs F = sweep_args(M.fwd), B = sweep_args(M.bwd);
+    SweepArgs<T> F = sweep_args<T>(M.fwd), B = sweep_args<T>(M.bwd);
     // register-pipelined path when every level fits the workgroup and rows
     // are at most kKR long; the generic kernel otherwise
     int kk = std::max(M.fwd.max_k, M.bwd.max_k);
@@ -737,9 +772,9 @@ void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const
 #define PARCEL_BJ_K(NTH, KK)                                                                            \
   do {                                                                                                  \
     if (M.mode == BJ_ILU0)                                                                              \
-      hipLaunchKernelGGL((k_bj_apply<BJ_ILU0, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
+      hipLaunchKernelGGL((k_bj_apply<BJ_ILU0, NTH, KK, T>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
     else                                                                                                \
-      hipLaunchKernelGGL((k_bj_apply<BJ_DILU, NTH, KK>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
+      hipLaunchKernelGGL((k_bj_apply<BJ_DILU, NTH, KK, T>), dim3(nb), dim3(NTH), shm, P.stream, F, B, r, s, f, skip); \
   } while (0)
 #define PARCEL_BJ_LAUNCH(NTH)                                                     \
   do {                                                                            \
@@ -760,6 +795,43 @@ void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const
 #undef PARCEL_BJ_K
   }
   PARCEL_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_to_float(long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+void to_float(const DevBuf<double>& a, DevBuf<float>& b, hipStream_t st) {
+  if (!a.n) return;
+  if (b.n != a.n) b.alloc(a.n);
+  const long blocks = std::min<long>(((long)a.n + kNT - 1) / kNT, 4096);
+  hipLaunchKernelGGL(k_to_float, dim3((unsigned)blocks), dim3(kNT), 0, st, (long)a.n, (const double*)a.p, b.p);
+}
+
+}  // namespace
+
+void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
+  precon_apply_t<double>(P, r, s, fuse, skip);
+}
+
+void precon_apply32(Plan& P, const float* r, float* s, const CGFuse32* fuse, const int* skip) {
+  if (!P.M.has32) throw hip_error("FP32 preconditioner not set up", PARCEL_HIP_ERR_ARG);
+  precon_apply_t<float>(P, r, s, fuse, skip);
+}
+
+void precon_setup32(Plan& P) {
+  Precon& M = P.M;
+  if (M.has32) return;
+  // FP32 copies of the FP64 factors (rounded to nearest)
+  to_float(M.dinv, M.dinv32, P.stream);
+  to_float(M.fwd.eval, M.fwd.eval32, P.stream);
+  to_float(M.fwd.diag, M.fwd.diag32, P.stream);
+  to_float(M.bwd.eval, M.bwd.eval32, P.stream);
+  to_float(M.bwd.diag, M.bwd.diag32, P.stream);
+  PARCEL_HIP_CHECK(hipGetLastError());
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  M.has32 = true;
 }
 
 }  // namespace parcel

@@ -60,6 +60,8 @@ class Stats(ctypes.Structure):
         ("bj_entries", ctypes.c_long),
         ("bj_nnz_l", ctypes.c_long),
         ("bj_nnz_u", ctypes.c_long),
+        ("inner_niter", ctypes.c_int),
+        ("final_rel_res", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -85,6 +87,9 @@ SYMBOLS = {
                                _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcagmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                  _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dcg_mixed": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                                  _I, _I, _I, _I, _I, _D, _I, _IP]),
+    "parcel_hip_plan_set_inner": (_I, [_P, _D, _I]),
     "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
                                     ctypes.POINTER(Param)]),
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
@@ -328,6 +333,23 @@ def dcagmres(x, b, rowptr, col, val, CAGMRES_sstep=4, CAGMRES_tstep=1, CAGMRES_b
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
+def dcg_mixed(x, b, rowptr, col, val, inner_rtol=1e-4, inner_itrmax=10000, ipreflag=PRECON_NONE,
+              ILU_method=ILUD, itrmax=50, rtolmax=1e-10, precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4,
+              iovlflag=0, Neuman_itr=1, nBlock=2000):
+    """FP64 refinement around FP32 PCG; solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1), np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(lib.parcel_hip_dcg_mixed(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+                                    ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                                    iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, inner_rtol,
+                                    inner_itrmax, ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
+
+
 class Plan:
     """Setup once, solve many (parcel_hip_plan_*)."""
 
@@ -349,6 +371,9 @@ class Plan:
         _check(self.lib.parcel_hip_plan_solve(self.h, _ptr(x), _ptr(b), ctypes.byref(it), rtolmax, _ptr(hist),
                                               ctypes.byref(iret)))
         return hist[: it.value].copy(), it.value, iret.value
+
+    def set_inner(self, inner_rtol, inner_itrmax):
+        _check(self.lib.parcel_hip_plan_set_inner(self.h, inner_rtol, inner_itrmax))
 
     def set_profile(self, on):
         _check(self.lib.parcel_hip_plan_set_profile(self.h, 1 if on else 0))
