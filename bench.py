@@ -4,11 +4,16 @@ on one MI355X, with HBM GB/s against the roofline.
 
 Workload (default): 3D 7-point constant-coefficient Poisson 256^3
 (16,777,216 unknowns, 117,047,296 nonzeros, diag 6 / off -1), x_true ~ U[-1,1]
-from SplitMix64 seed 3, b = A x_true, x0 = 0, rtol 1e-10.  Methods:
-  bj  FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks
-  pj  FP64 PCG, point Jacobi (the SpMV/CG path)
+from SplitMix64 seed 3, b = A x_true, x0 = 0, rtol 1e-10 (FP64 residual).
+Methods:
+  bj        FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks
+  pj        FP64 PCG, point Jacobi (the SpMV/CG path)
+  mixed_bj  FP64 iterative refinement around FP32 PCG with FP32 block-Jacobi
+            ILU(0) (256 blocks), inner tolerance 1e-4
+  mixed_pj  the same with FP32 point Jacobi
 A "step" is one full solve from x0 = 0 with the matrix, b and x already in
-HBM; `value` is the mean time-to-solution of the fastest method.  The timed
+HBM; `value` is the mean time-to-solution of the fastest method, and the
+config reports its speed-up over the fastest plain FP64 PCG.  The timed
 solves run without per-kernel events; one more solve with HIP events around
 every kernel gives the per-kernel means behind `roofline`.
 
@@ -44,13 +49,25 @@ def ilu0_apply_bytes(n, nnz_l, nnz_u):
     return 12 * (nnz_l + nnz_u) + 8 * n + 8 * n + 8 * n + 16 * n + 8 * n
 
 
+def k1_bytes32(n, nnz):
+    """FP32 fused CG kernel: values+columns 8 B/nnz, row pointers 4(n+1),
+    x read/write 8n, p_old 4n, s 4n, p_new 4n, q 4n."""
+    return 8 * nnz + 4 * (n + 1) + 24 * n
+
+
+def ilu0_apply_bytes32(n, nnz_l, nnz_u):
+    """FP32 block-Jacobi ILU(0) apply: 8 B per factor entry, row ids 4n per
+    sweep, r 4n, s written 4n (forward), s read+write 8n and u_ii 4n."""
+    return 8 * (nnz_l + nnz_u) + 8 * n + 4 * n + 4 * n + 8 * n + 4 * n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--methods", default="bj,pj")
+    ap.add_argument("--methods", default="bj,pj,mixed_bj,mixed_pj")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--calib", action="store_true",
@@ -89,19 +106,26 @@ def main():
 
     methods = {}
     for m in args.methods.split(","):
-        if m == "bj":
+        mixed = m.startswith("mixed_")
+        base = m[len("mixed_"):] if mixed else m
+        if base == "bj":
             kw = dict(ipreflag=2, ILU_method=0, precon_thblock=256)
-        elif m == "pj":
+        elif base == "pj":
             kw = dict(ipreflag=1)
         else:
             raise SystemExit("unknown method " + m)
+        if mixed:
+            kw["solver"] = 11
+        itrmax = 50 if mixed else 20000
         t0 = time.perf_counter()
         plan = ph.Plan(rp, col, val, profile=0, **kw)
+        if mixed:
+            plan.set_inner(1e-4, 20000)
         setup = time.perf_counter() - t0
         plan.spmv(xt_d, b_d)
         for _ in range(args.warmup):
             x_d.zero()
-            plan.solve(x_d, b_d, 20000, 1e-10)
+            plan.solve(x_d, b_d, itrmax, 1e-10)
         times = []
         dev_ms = []
         st = None
@@ -109,7 +133,7 @@ def main():
             x_d.zero()
             ph.load().parcel_hip_device_synchronize()
             t1 = time.perf_counter()
-            hist, niter, iret = plan.solve(x_d, b_d, 20000, 1e-10)
+            hist, niter, iret = plan.solve(x_d, b_d, itrmax, 1e-10)
             times.append(time.perf_counter() - t1)
             st = plan.stats()
             dev_ms.append(st["solve_device_ms"])
@@ -117,7 +141,7 @@ def main():
         # kernel timing pass: the same solve with HIP events around each kernel
         plan.set_profile(True)
         x_d.zero()
-        plan.solve(x_d, b_d, 20000, 1e-10)
+        plan.solve(x_d, b_d, itrmax, 1e-10)
         st = plan.stats()
         plan.set_profile(False)
         # true residual of the last solve
@@ -127,23 +151,27 @@ def main():
         k1_ms = st["spmv_ms"] / max(st["spmv_calls"], 1)
         pre_ms = st["precon_ms"] / max(st["precon_calls"], 1) if st["precon_calls"] else 0.0
         vec_ms = st["vec_ms"] / max(st["vec_calls"], 1) if st["vec_calls"] else 0.0
-        kernels = {"cg_spmv_fused": {"mean_ms": k1_ms, "calls_per_solve": st["spmv_calls"],
-                                     "bytes": k1_bytes(n, nnz),
-                                     "GBs": k1_bytes(n, nnz) / (k1_ms * 1e-3) / 1e9}}
-        if m == "bj":
-            pb = ilu0_apply_bytes(n, st["bj_nnz_l"], st["bj_nnz_u"])
-            kernels["bj_ilu0_apply"] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"], "bytes": pb,
-                                        "GBs": pb / (pre_ms * 1e-3) / 1e9, "levels_fwd": st["bj_levels_fwd"],
-                                        "levels_bwd": st["bj_levels_bwd"]}
+        sfx = "_f32" if mixed else ""
+        kb = k1_bytes32(n, nnz) if mixed else k1_bytes(n, nnz)
+        kernels = {"cg_spmv_fused" + sfx: {"mean_ms": k1_ms, "calls_per_solve": st["spmv_calls"], "bytes": kb,
+                                           "GBs": kb / (k1_ms * 1e-3) / 1e9}}
+        if base == "bj":
+            pb = (ilu0_apply_bytes32 if mixed else ilu0_apply_bytes)(n, st["bj_nnz_l"], st["bj_nnz_u"])
+            kernels["bj_ilu0_apply" + sfx] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"],
+                                              "bytes": pb, "GBs": pb / (pre_ms * 1e-3) / 1e9,
+                                              "levels_fwd": st["bj_levels_fwd"], "levels_bwd": st["bj_levels_bwd"]}
         kernels["vector_reductions"] = {"mean_ms": vec_ms, "calls_per_solve": st["vec_calls"]}
         methods[m] = {"time_to_solution_s": float(np.mean(times)), "device_ms": float(np.mean(dev_ms)),
                       "niter": niter, "iret": iret, "final_recurrence_res": float(hist[-1]),
+                      "inner_fp32_iters": st["inner_niter"] if mixed else None,
                       "true_rel_res": true_res, "setup_s": setup, "precon_setup_s": st["precon_setup_s"],
                       "kernels": kernels}
         plan.close()
 
     head = min(methods, key=lambda k: methods[k]["time_to_solution_s"])
     H = methods[head]
+    plain = [k for k in methods if not k.startswith("mixed_")]
+    best_plain = min(plain, key=lambda k: methods[k]["time_to_solution_s"]) if plain else None
     # dominant kernel of the headline method: the one with most time per solve
     dom_name, dom = max(((k, v) for k, v in H["kernels"].items() if "bytes" in v),
                         key=lambda kv: kv[1]["mean_ms"] * kv[1]["calls_per_solve"])
@@ -165,16 +193,35 @@ def main():
         import oracle as orc
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
         hist_len = args.cpu_iters
-        kwm = dict(ipreflag=2, ilu_method=0, nblocks=256) if head == "bj" else dict(ipreflag=1)
+        hbase = head[len("mixed_"):] if head.startswith("mixed_") else head
+        kwm = dict(ipreflag=2, ilu_method=0, nblocks=256) if hbase == "bj" else dict(ipreflag=1)
         b_h = b_d.numpy()
-        t0 = time.perf_counter()
-        _, ho, no, _ = orc.pcg(rp, col, val, b_h, itrmax=hist_len, rtol=1e-10, **kwm)
-        tset, titer = orc.last_times()
-        per_it = titer / no
-        cpu = {"value": round(per_it * H["niter"], 3), "unit": "s", "cores": threads, "kind": "port",
-               "sample": "oracle PCG (%s) on the same 256^3 system for %d iterations (%.3f s/iteration, "
-                         "setup %.1f s excluded), scaled to the GPU's %d iterations" %
-                         (head, no, per_it, tset, H["niter"])}
+        if head.startswith("mixed_"):
+            # one FP64 correction with its FP32 inner PCG cut to hist_len iterations
+            t0 = time.perf_counter()
+            orc.pcg_mixed(rp, col, val, b_h, itrmax=1, rtol=1e-30, inner_itrmax=hist_len, inner_rtol=1e-30,
+                          **kwm)
+            tot = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            pre = orc.Precon(rp, col, val, kwm["ipreflag"], **{k: v for k, v in kwm.items() if k != "ipreflag"})
+            tset = time.perf_counter() - t0
+            del pre
+            per_it = max(tot - tset, 1e-9) / hist_len
+            total_its = H["inner_fp32_iters"]
+            cpu = {"value": round(per_it * total_its, 3), "unit": "s", "cores": threads, "kind": "port",
+                   "sample": "oracle mixed refinement (%s) on the same 256^3 system: one correction with %d FP32 "
+                             "PCG iterations (%.3f s/iteration, preconditioner setup %.1f s excluded), scaled to "
+                             "the GPU's %d FP32 iterations" % (head, hist_len, per_it, tset, total_its)}
+            no = hist_len
+        else:
+            t0 = time.perf_counter()
+            _, ho, no, _ = orc.pcg(rp, col, val, b_h, itrmax=hist_len, rtol=1e-10, **kwm)
+            tset, titer = orc.last_times()
+            per_it = titer / no
+            cpu = {"value": round(per_it * H["niter"], 3), "unit": "s", "cores": threads, "kind": "port",
+                   "sample": "oracle PCG (%s) on the same 256^3 system for %d iterations (%.3f s/iteration, "
+                             "setup %.1f s excluded), scaled to the GPU's %d iterations" %
+                             (head, no, per_it, tset, H["niter"])}
         log("cpu baseline: %.3f s/it over %d its" % (per_it, no))
 
     line = {
@@ -188,14 +235,20 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32/f64" if head.startswith("mixed_") else "f64",
         "data": "synthetic",
         "config": {
             "workload": "3D 7-point Poisson %d^3 (n=%d, nnz=%d), x_true~U[-1,1] seed 3, b=A x_true, x0=0, "
                         "rtol 1e-10" % (g, n, nnz),
             "method": {"bj": "FP64 PCG + block-Jacobi ILU(0), 256 row blocks",
-                       "pj": "FP64 PCG + point Jacobi"}[head],
+                       "pj": "FP64 PCG + point Jacobi",
+                       "mixed_bj": "FP64 refinement around FP32 PCG + FP32 block-Jacobi ILU(0), 256 blocks, "
+                                   "inner rtol 1e-4",
+                       "mixed_pj": "FP64 refinement around FP32 PCG + FP32 point Jacobi, inner rtol 1e-4"}[head],
             "headline": head,
+            "speedup_vs_plain_fp64": (round(methods[best_plain]["time_to_solution_s"] / H["time_to_solution_s"], 3)
+                                      if best_plain else None),
+            "plain_fp64_reference": best_plain,
             "methods": methods,
             "parallelism": "1 GPU",
         },

@@ -190,6 +190,20 @@ int parcel_hip_dcg_mixed(int icomm, double* vecx, const double* vecb, int n, int
                          int independ_nvec, int nBlock, double inner_rtol, int inner_itrmax,
                          int* iret);
 
+/* parcel_ddcg (src/ddwrapper.f90:184-282): double-double PCG of
+ * src/ddpcg.f90:247-563.  Vectors and the matrix as hi / lo pairs;
+ * PRECISION_A / _b / _x / _PRECONDITION: 1 double (the lo array is ignored
+ * and may be NULL), 2 double-double.  The preconditioner factors are
+ * always computed in double-double; precision 1 applies their hi parts.
+ * ipreflag 0, 1 or 2 (ILU_method 0, 1, 2). */
+int parcel_hip_ddcg(int icomm, double* vecx_hi, double* vecx_lo, const double* vecb_hi,
+                    const double* vecb_lo, int n, int gn, int nnz, int istart,
+                    const double* crsA_hi, const double* crsA_lo, const int* crsRow_ptr,
+                    const int* crsCol, int ipreflag, int ILU_method, int addL, int iflagAS,
+                    int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                    int precon_thblock, int independ_nvec, int nBlock, int PRECISION_A,
+                    int PRECISION_b, int PRECISION_x, int PRECISION_PRECONDITION, int* iret);
+
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
  * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident
@@ -202,6 +216,16 @@ int parcel_hip_plan_create(parcel_hip_plan** plan, int n, int gn, int nnz, int i
 int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb,
                           int* itrmax, double rtolmax, double* reshistory, int* iret);
 int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats);
+/* double-double plans (the split of parcel_ddcg): the solver must be CG */
+int parcel_hip_plan_create_dd(parcel_hip_plan** plan, int n, int gn, int nnz, int istart,
+                              const double* crsA_hi, const double* crsA_lo,
+                              const int* crsRow_ptr, const int* crsCol,
+                              const parcel_hip_param* prm, int PRECISION_A,
+                              int PRECISION_PRECONDITION);
+int parcel_hip_plan_solve_dd(parcel_hip_plan* plan, double* vecx_hi, double* vecx_lo,
+                             const double* vecb_hi, const double* vecb_lo, int* itrmax,
+                             double rtolmax, double* reshistory, int PRECISION_b,
+                             int PRECISION_x, int* iret);
 /* Inner tolerance and iteration limit of PARCEL_SOLVER_CG_MIXED plans
  * (defaults 1e-4 and 10000). */
 int parcel_hip_plan_set_inner(parcel_hip_plan* plan, double inner_rtol, int inner_itrmax);

@@ -54,6 +54,13 @@ def load():
         "oracle_cagmres": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _I, _RP, _P, _IP,
                                 _IP]),
         "oracle_pcg_mixed": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _D, _RP, _P, _IP, _IP, _IP]),
+        "oracle_dd_op": (None, [_I, _I, _P, _P, _P, _P, _P, _P]),
+        "oracle_dd_dot": (None, [_RP, _I, _P, _P, _P, _P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+        "oracle_dd_spmv": (None, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "oracle_ddprecon_create": (_P, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _IP]),
+        "oracle_ddprecon_apply": (None, [_P, _P, _P, _P, _P]),
+        "oracle_ddprecon_free": (None, [_P]),
+        "oracle_ddcg": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_gemm_nn": (None, [_I, _I, _I, _P, _I, _P, _I, _P, _I]),
         "oracle_trmm_lunn": (None, [_I, _P, _I, _P, _I]),
         "oracle_trinv_upper": (None, [_I, _P, _I]),
@@ -177,6 +184,73 @@ def pcg_mixed(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1,
     if rc != 0:
         raise RuntimeError("oracle_pcg_mixed failed: %d" % rc)
     return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value, tot.value
+
+
+DD_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}
+
+
+def dd_op(op, ah, al, bh=None, bl=None):
+    ah = np.ascontiguousarray(ah, dtype=np.float64)
+    al = np.ascontiguousarray(al, dtype=np.float64)
+    bh = np.zeros_like(ah) if bh is None else np.ascontiguousarray(bh, dtype=np.float64)
+    bl = np.zeros_like(ah) if bl is None else np.ascontiguousarray(bl, dtype=np.float64)
+    ch, cl = np.empty_like(ah), np.empty_like(ah)
+    load().oracle_dd_op(DD_OPS[op], int(ah.shape[0]), _p(ah), _p(al), _p(bh), _p(bl), _p(ch), _p(cl))
+    return ch, cl
+
+
+def dd_dot(xh, xl, yh, yl, red=TILED):
+    rh, rl = ctypes.c_double(), ctypes.c_double()
+    load().oracle_dd_dot(ctypes.byref(red), int(xh.shape[0]), _p(xh), _p(xl), _p(yh), _p(yl), ctypes.byref(rh),
+                         ctypes.byref(rl))
+    return rh.value, rl.value
+
+
+def dd_spmv(rowptr, col, vh, vl, xh, xl):
+    n = rowptr.shape[0] - 1
+    yh, yl = np.empty(n), np.empty(n)
+    load().oracle_dd_spmv(n, _p(rowptr), _p(col), _p(vh), None if vl is None else _p(vl), _p(xh), _p(xl), _p(yh),
+                          _p(yl))
+    return yh, yl
+
+
+class DDPrecon:
+    def __init__(self, rowptr, col, vh, vl, ipreflag, ilu_method=1, nblocks=1, independ_nvec=1, prec=2):
+        self._keep = (rowptr, col, vh, vl)
+        self.n = rowptr.shape[0] - 1
+        err = ctypes.c_int(0)
+        self.h = load().oracle_ddprecon_create(ipreflag, ilu_method, nblocks, independ_nvec, prec, self.n,
+                                               _p(rowptr), _p(col), _p(vh), None if vl is None else _p(vl),
+                                               ctypes.byref(err))
+        if not self.h:
+            raise RuntimeError("oracle dd precon setup failed: %d" % err.value)
+
+    def apply(self, rh, rl):
+        sh, sl = np.zeros(self.n), np.zeros(self.n)
+        load().oracle_ddprecon_apply(self.h, _p(rh), _p(rl), _p(sh), _p(sl))
+        return sh, sl
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_ddprecon_free(self.h)
+            self.h = None
+
+
+def ddcg(rowptr, col, vh, vl, bh, bl, x0h=None, x0l=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1,
+         prec=2, itrmax=1000, rtol=1e-20, red=TILED):
+    """dd PCG; returns (xh, xl, hist[:niter], niter, iret)."""
+    n = rowptr.shape[0] - 1
+    xh = np.zeros(n) if x0h is None else np.array(x0h, dtype=np.float64, copy=True)
+    xl = np.zeros(n) if x0l is None else np.array(x0l, dtype=np.float64, copy=True)
+    bl = np.zeros(n) if bl is None else bl
+    hist = np.zeros(max(itrmax, 1))
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    rc = load().oracle_ddcg(n, _p(rowptr), _p(col), _p(vh), None if vl is None else _p(vl), _p(bh), _p(bl), _p(xh),
+                            _p(xl), ipreflag, ilu_method, nblocks, independ_nvec, prec, itrmax, rtol,
+                            ctypes.byref(red), _p(hist), ctypes.byref(niter), ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_ddcg failed: %d" % rc)
+    return xh, xl, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
 
 
 def _fortran(a):

@@ -1,0 +1,27 @@
+/* oracle_internal.h -- preconditioner internals shared by the oracle's
+ * translation units.  TEST INFRASTRUCTURE ONLY (see parcel_oracle.h). */
+#ifndef PARCEL_ORACLE_INTERNAL_H
+#define PARCEL_ORACLE_INTERNAL_H
+
+#include "parcel_oracle.h"
+
+struct oracle_precon {
+  int ipreflag, ilu_method, nblocks, independ_nvec, n;
+  int* blo; /* 1-based first row of each block */
+  int* bhi; /* 1-based last row of each block */
+  /* point Jacobi and D-ILU */
+  double* dinv;
+  /* D-ILU uses the original matrix entries and per-row L/U ranges */
+  const int* rowptr;
+  const int* col;
+  const double* val;
+  int* lstart; int* lend; int* ustart; int* uend; /* 0-based entry ids */
+  /* ILU(0): per block split factors, 0-based local CSR */
+  int** lptr; int** lcol; double** lval;
+  int** uptr; int** ucol; double** uval;
+};
+
+/* D-ILU L/U entry ranges of rows lo..hi (1-based), init_solve_ILU_crs order */
+void oracle_dilu_ranges(oracle_precon* p, int lo, int hi);
+
+#endif

@@ -138,21 +138,7 @@ void oracle_spmv(int n, const int* rowptr, const int* col, const double* val,
 /* ------------------------------------------------------------------------ */
 /* Preconditioners.                                                          */
 /* ------------------------------------------------------------------------ */
-struct oracle_precon {
-  int ipreflag, ilu_method, nblocks, independ_nvec, n;
-  int* blo; /* 1-based first row of each block */
-  int* bhi; /* 1-based last row of each block */
-  /* point Jacobi and D-ILU */
-  double* dinv;
-  /* D-ILU uses the original matrix entries and per-row L/U ranges */
-  const int* rowptr;
-  const int* col;
-  const double* val;
-  int* lstart; int* lend; int* ustart; int* uend; /* 0-based entry ids */
-  /* ILU(0): per block split factors, 0-based local CSR */
-  int** lptr; int** lcol; double** lval;
-  int** uptr; int** ucol; double** uval;
-};
+#include "oracle_internal.h"
 
 /* get_DInv_crs (src/matrix.f90:3104-3145): Dinv = 1/a_kk, 1/1e-300 for an
  * explicit zero diagonal. */
@@ -218,7 +204,7 @@ static int dilu_block(oracle_precon* p, int lo, int hi) {
  * entries left of its independ_nvec chunk, U = in-block entries right of it.
  * lstart..lend ascending; ustart..uend is walked DOWNWARD (ustart is the last
  * in-block entry, uend the first entry right of the chunk). */
-static void dilu_ranges(oracle_precon* p, int lo, int hi) {
+void oracle_dilu_ranges(oracle_precon* p, int lo, int hi) {
   const int* rp = p->rowptr; const int* cl = p->col;
   int nvec = p->independ_nvec;
   for (int k = lo; k <= hi; ++k) {
@@ -426,7 +412,7 @@ oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
 #pragma omp parallel for schedule(dynamic) reduction(|:bad)
       for (int b = 0; b < nblocks; ++b) {
         if (dilu_block(p, p->blo[b], p->bhi[b]) != 0) bad |= 1;
-        else dilu_ranges(p, p->blo[b], p->bhi[b]);
+        else oracle_dilu_ranges(p, p->blo[b], p->bhi[b]);
       }
       if (bad) { *err = -2; oracle_precon_free(p); return NULL; }
     }

@@ -135,6 +135,31 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
                      int itrmax, double rtol, int inner_itrmax, double inner_rtol,
                      const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total);
 
+/* --- double-double (src/dd_func.f90, src/ddpcg.f90, src/ddmatrix.f90) ---- */
+/* op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (b unused), elementwise over n */
+void oracle_dd_op(int op, int n, const double* ah, const double* al, const double* bh,
+                  const double* bl, double* ch, double* cl);
+void oracle_dd_dot(const oracle_red* red, int n, const double* xh, const double* xl,
+                   const double* yh, const double* yl, double* rh, double* rl);
+/* vl may be NULL (matrix without lo part, PRECISION_A = 1) */
+void oracle_dd_spmv(int n, const int* rowptr, const int* col, const double* vh,
+                    const double* vl, const double* xh, const double* xl, double* yh,
+                    double* yl);
+typedef struct oracle_ddprecon oracle_ddprecon;
+/* prec: PRECISION_PRECONDITION (1: apply with the hi parts only, 2: dd) */
+oracle_ddprecon* oracle_ddprecon_create(int ipreflag, int ilu_method, int nblocks,
+                                        int independ_nvec, int prec, int n, const int* rowptr,
+                                        const int* col, const double* vh, const double* vl,
+                                        int* err);
+void oracle_ddprecon_apply(const oracle_ddprecon* p, const double* rh, const double* rl,
+                           double* sh, double* sl);
+void oracle_ddprecon_free(oracle_ddprecon* p);
+/* ddKSP_pCG (src/ddpcg.f90:247-563); x (hi, lo) in/out; hist[i] = rnorm_hi/bnorm_hi */
+int oracle_ddcg(int n, const int* rowptr, const int* col, const double* vh, const double* vl,
+                const double* bh, const double* bl, double* xh, double* xl, int ipreflag,
+                int ilu_method, int nblocks, int independ_nvec, int prec, int itrmax,
+                double rtol, const oracle_red* red, double* hist, int* niter, int* iret);
+
 /* Wall seconds of the last oracle_pcg call: [0] preconditioner setup,
  * [1] iterations (everything after setup). */
 void oracle_last_times(double* t);

@@ -103,6 +103,12 @@ parcel_hip_param merged(const parcel_hip_param* user) {
   return r;
 }
 
+// every field -1 ("unset, use the default"), as INIT_PARCEL_INPUT marks them
+void unset_param(parcel_hip_param& prm) {
+  for (size_t k = 0; k < sizeof(prm) / sizeof(int); ++k) reinterpret_cast<int*>(&prm)[k] = -1;
+  prm.profile = 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -130,9 +136,16 @@ void parcel_hip_default_param(parcel_hip_param* prm) {
   if (prm) fill_defaults(prm);
 }
 
-int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int istart,
-                           const double* crsA, const int* crsRow_ptr, const int* crsCol,
-                           const parcel_hip_param* prm) {
+}  // extern "C"
+
+namespace {
+
+// Plan set-up shared by the FP64 and the double-double entry points:
+// dd_prec = 0 for FP64, else PRECISION_PRECONDITION of a dd plan; crsA_lo
+// (dd only) may be null (PRECISION_A = 1).
+int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const double* crsA,
+                const double* crsA_lo, const int* crsRow_ptr, const int* crsCol, const parcel_hip_param* prm,
+                int dd_prec) {
   PARCEL_API_BEGIN
   if (!out) return fail(PARCEL_HIP_ERR_ARG, "plan pointer is NULL");
   *out = nullptr;
@@ -155,6 +168,7 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
     fetch(crsRow_ptr, (size_t)n + 1, A.rowptr);
     fetch(crsCol, (size_t)nnz, A.col);
     fetch(crsA, (size_t)nnz, A.val);
+    if (crsA_lo) fetch(crsA_lo, (size_t)nnz, A.val_lo);
     if (A.rowptr[0] != 1 || A.rowptr[n] != nnz + 1)
       throw hip_error("crsRow_ptr must be 1-based with crsRow_ptr(n+1) = nnz+1", PARCEL_HIP_ERR_ARG);
     for (int i = 0; i <= n; ++i) A.rowptr[i] -= 1;
@@ -168,6 +182,8 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
       A.col[j] = c;
     }
     sell_build(P);
+    if (dd_prec && P.prm.solver != PARCEL_SOLVER_CG)
+      throw hip_error("the double-double path has CG only", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
         P.prm.solver != PARCEL_SOLVER_GMRES && P.prm.solver != PARCEL_SOLVER_CAGMRES &&
         P.prm.solver != PARCEL_SOLVER_CG_MIXED)
@@ -183,7 +199,8 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
       if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
         throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
     }
-    precon_setup(P);
+    if (dd_prec) precon_setup_dd(P, dd_prec);
+    else precon_setup(P);
     P.stats.precon_setup_s = P.M.setup_seconds;
     P.stats.bj_blocks = P.M.nblocks;
     P.stats.bj_levels_fwd = P.M.fwd.nlev;
@@ -203,6 +220,96 @@ int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int is
   *out = h;
   return PARCEL_HIP_OK;
   PARCEL_API_END
+}
+
+}  // namespace
+
+extern "C" {
+
+int parcel_hip_plan_create(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const double* crsA,
+                           const int* crsRow_ptr, const int* crsCol, const parcel_hip_param* prm) {
+  return create_impl(out, n, gn, nnz, istart, crsA, nullptr, crsRow_ptr, crsCol, prm, 0);
+}
+
+int parcel_hip_plan_create_dd(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const double* crsA_hi,
+                              const double* crsA_lo, const int* crsRow_ptr, const int* crsCol,
+                              const parcel_hip_param* prm, int PRECISION_A, int PRECISION_PRECONDITION) {
+  if ((PRECISION_A != 1 && PRECISION_A != 2) || (PRECISION_PRECONDITION != 1 && PRECISION_PRECONDITION != 2))
+    return fail(PARCEL_HIP_ERR_ARG, "PRECISION_A and PRECISION_PRECONDITION must be 1 or 2");
+  if (PRECISION_A == 2 && !crsA_lo) return fail(PARCEL_HIP_ERR_ARG, "PRECISION_A = 2 needs crsA_lo");
+  return create_impl(out, n, gn, nnz, istart, crsA_hi, PRECISION_A == 2 ? crsA_lo : nullptr, crsRow_ptr, crsCol,
+                     prm, PRECISION_PRECONDITION);
+}
+
+int parcel_hip_plan_solve_dd(parcel_hip_plan* plan, double* vecx_hi, double* vecx_lo, const double* vecb_hi,
+                             const double* vecb_lo, int* itrmax, double rtolmax, double* reshistory,
+                             int PRECISION_b, int PRECISION_x, int* iret) {
+  PARCEL_API_BEGIN
+  if (!plan || !vecx_hi || !vecb_hi || !itrmax || !iret) return fail(PARCEL_HIP_ERR_ARG, "NULL argument");
+  if ((PRECISION_b != 1 && PRECISION_b != 2) || (PRECISION_x != 1 && PRECISION_x != 2))
+    return fail(PARCEL_HIP_ERR_ARG, "PRECISION_b and PRECISION_x must be 1 or 2");
+  if ((PRECISION_b == 2 && !vecb_lo) || (PRECISION_x == 2 && !vecx_lo))
+    return fail(PARCEL_HIP_ERR_ARG, "lo vectors required for precision 2");
+  Plan& P = plan->P;
+  if (!P.M.dd_prec) return fail(PARCEL_HIP_ERR_ARG, "plan was not created with parcel_hip_plan_create_dd");
+  PARCEL_HIP_CHECK(hipSetDevice(P.device));
+  const int n = P.A.n;
+  // device copies: x (hi, lo) and b (hi, lo); lo parts zero at precision 1 (ddpcg.f90:330-345)
+  DevBuf<double> d(4 * (size_t)n);
+  double *xh = d.p, *xl = d.p + n, *bh = d.p + 2 * (size_t)n, *bl = d.p + 3 * (size_t)n;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  auto put = [&](double* dst, const double* src) {
+    PARCEL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, is_device_ptr(src) ? hipMemcpyDeviceToDevice
+                                                                         : hipMemcpyHostToDevice, P.stream));
+  };
+  put(xh, vecx_hi);
+  put(bh, vecb_hi);
+  if (PRECISION_x == 2) put(xl, vecx_lo);
+  else PARCEL_HIP_CHECK(hipMemsetAsync(xl, 0, bytes, P.stream));
+  if (PRECISION_b == 2) put(bl, vecb_lo);
+  else PARCEL_HIP_CHECK(hipMemsetAsync(bl, 0, bytes, P.stream));
+  int niter = 0, ret = 0;
+  int rc = ddcg_solve(P, xh, xl, bh, bl, *itrmax, rtolmax, reshistory, &niter, &ret);
+  if (rc) return rc;
+  auto get = [&](double* dst, const double* src) {
+    PARCEL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, is_device_ptr(dst) ? hipMemcpyDeviceToDevice
+                                                                         : hipMemcpyDeviceToHost, P.stream));
+  };
+  get(vecx_hi, xh);
+  if (PRECISION_x == 2) get(vecx_lo, xl);
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  *itrmax = niter;
+  *iret = ret;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_ddcg(int icomm, double* vecx_hi, double* vecx_lo, const double* vecb_hi, const double* vecb_lo,
+                    int n, int gn, int nnz, int istart, const double* crsA_hi, const double* crsA_lo,
+                    const int* crsRow_ptr, const int* crsCol, int ipreflag, int ILU_method, int addL, int iflagAS,
+                    int* itrmax, double rtolmax, double* reshistory, int iovlflag, int precon_thblock,
+                    int independ_nvec, int nBlock, int PRECISION_A, int PRECISION_b, int PRECISION_x,
+                    int PRECISION_PRECONDITION, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  parcel_hip_plan* plan = nullptr;
+  int rc = parcel_hip_plan_create_dd(&plan, n, gn, nnz, istart, crsA_hi, crsA_lo, crsRow_ptr, crsCol, &prm,
+                                     PRECISION_A, PRECISION_PRECONDITION);
+  if (rc) return rc;
+  rc = parcel_hip_plan_solve_dd(plan, vecx_hi, vecx_lo, vecb_hi, vecb_lo, itrmax, rtolmax, reshistory, PRECISION_b,
+                                PRECISION_x, iret);
+  parcel_hip_plan_destroy(plan);
+  return rc;
 }
 
 int parcel_hip_plan_destroy(parcel_hip_plan* plan) {
@@ -288,10 +395,6 @@ static int one_shot(parcel_hip_param& prm, double* vecx, const double* vecb, int
   return rc;
 }
 
-static void unset_param(parcel_hip_param& prm) {
-  for (size_t k = 0; k < sizeof(prm) / sizeof(int); ++k) reinterpret_cast<int*>(&prm)[k] = -1;
-  prm.profile = 0;
-}
 
 int parcel_hip_dbicgstab(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
                          const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,

@@ -58,6 +58,7 @@ struct DevSELL {
   DevBuf<int> col;       // entries, 0-based, -1 pads
   DevBuf<double> val;    // entries
   DevBuf<float> val32;   // FP32 copy (mixed-precision inner solve)
+  DevBuf<double> val_lo; // lo parts of a double-double matrix (empty: none)
 };
 
 // 0-based CSR copy on the host (setup work: factorisations, level analysis).
@@ -65,6 +66,7 @@ struct HostCSR {
   int n = 0;
   std::vector<int> rowptr, col;
   std::vector<double> val;
+  std::vector<double> val_lo;  // lo parts of a double-double matrix (empty: none)
 };
 
 // Level-scheduled triangular sweep of one block-Jacobi factor.
@@ -87,6 +89,8 @@ struct Sweep {
   DevBuf<double> diag;   // per level-ordered row: scale (see mode)
   DevBuf<float> eval32;  // FP32 copies for the mixed-precision inner solve
   DevBuf<float> diag32;
+  DevBuf<double> eval_lo;  // lo parts of double-double factors
+  DevBuf<double> diag_lo;
   int max_width = 0;
   int max_k = 0;
   int ring = 0;          // levels kept in the LDS ring (0: none)
@@ -130,6 +134,9 @@ struct Precon {
   DevBuf<float> dinv32;
   Sweep fwd, bwd;       // block Jacobi
   bool has32 = false;   // FP32 copies of the factors exist
+  // double-double factors (precon_setup_dd): Dinv lo parts, apply precision
+  DevBuf<double> dinv_lo;
+  int dd_prec = 0;      // 0: not set up; 1: apply with hi parts; 2: double-double
   double setup_seconds = 0.0;
 };
 
@@ -197,6 +204,10 @@ void precon_setup(Plan& P);
 // skip: if non-null, the kernels return at once when *skip != 0 (a solver
 // whose loop has already ended on the device).
 void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip = nullptr);
+// double-double preconditioner (factorisations in dd on the host; apply with
+// the factors' lo parts when prec == 2); vectors as separate hi / lo arrays
+void precon_setup_dd(Plan& P, int prec);
+void precon_apply_dd(Plan& P, const double* rh, const double* rl, double* sh, double* sl, const int* skip = nullptr);
 // FP32 apply with the factors rounded to float (precon_setup32 first)
 void precon_setup32(Plan& P);
 void precon_apply32(Plan& P, const float* r, float* s, const CGFuse32* fuse, const int* skip = nullptr);
@@ -214,6 +225,8 @@ int cagmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, doubl
                   double* hist_host, int* niter, int* iret);
 int mixed_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                 double* hist_host, int* niter, int* iret);
+int ddcg_solve(Plan& P, double* xh, double* xl, const double* bh, const double* bl, int itrmax, double rtol,
+               double* hist_host, int* niter, int* iret);
 void mixed_setup(Plan& P);
 // n-vector work buffers 0..k-1 of P.vec
 void ensure_vecs(Plan& P, int k);

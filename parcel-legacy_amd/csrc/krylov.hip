@@ -55,6 +55,8 @@ void sell_build(Plan& P) {
   S.entries = tot;
   std::vector<int> col((size_t)tot);
   std::vector<double> val((size_t)tot);
+  const bool lo = !A.val_lo.empty();
+  std::vector<double> vlo(lo ? (size_t)tot : 0);
 #pragma omp parallel for schedule(static)
   for (long s = 0; s < ns; ++s) {
     const long base = sptr[s];
@@ -67,9 +69,11 @@ void sell_build(Plan& P) {
         if (j < len) {
           col[at] = A.col[A.rowptr[r] + j];
           val[at] = A.val[A.rowptr[r] + j];
+          if (lo) vlo[at] = A.val_lo[A.rowptr[r] + j];
         } else {
           col[at] = -1;
           val[at] = 0.0;
+          if (lo) vlo[at] = 0.0;
         }
       }
     }
@@ -78,6 +82,7 @@ void sell_build(Plan& P) {
   S.sk.upload(sk, P.stream);
   S.col.upload(col, P.stream);
   S.val.upload(val, P.stream);
+  if (lo) S.val_lo.upload(vlo, P.stream);
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 

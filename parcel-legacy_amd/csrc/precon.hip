@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "dd.hpp"
 #include "internal.hpp"
 
 namespace parcel {
@@ -56,6 +57,9 @@ struct BlockSweep {
   std::vector<int> dep;         // local dependency rows
   std::vector<double> val;
   std::vector<double> diag;     // per local row
+  // double-double factors: lo parts of val and diag (empty otherwise)
+  std::vector<double> val_lo;
+  std::vector<double> diag_lo;
 };
 
 // Level analysis.  forward: deps of row i are < i; backward: > i.
@@ -87,10 +91,13 @@ void build_levels(BlockSweep& S, int nrow, bool forward) {
 
 // ILU(0) of one block, reference semantics (see file header).
 // Returns false if a row of the block has no diagonal.
-bool ilu0_block(const HostCSR& A, int lo, int hi, int nvec, BlockSweep& F, BlockSweep& B) {
+// ddm: factorise in double-double (ddILU_LU_crs, src/ddmatrix.f90:5686-5830,
+// with A's lo parts from A.val_lo when present) and fill val_lo / diag_lo.
+bool ilu0_block(const HostCSR& A, int lo, int hi, int nvec, BlockSweep& F, BlockSweep& B, bool ddm = false) {
   const int nb = hi - lo;
   std::vector<int> ptr(nb + 1), c2;
-  std::vector<double> v2;
+  std::vector<double> v2, w2;  // w2: lo parts in dd mode
+  const bool alo = ddm && !A.val_lo.empty();
   c2.reserve((size_t)(A.rowptr[hi] - A.rowptr[lo]));
   v2.reserve(c2.capacity());
   for (int ii = lo; ii < hi; ii += nvec) {
@@ -100,7 +107,11 @@ bool ilu0_block(const HostCSR& A, int lo, int hi, int nvec, BlockSweep& F, Block
       for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; ++j) {
         int c = A.col[j];
         if (c < lo || c >= hi) continue;
-        if (c < ii || c >= iimax || c == i) { c2.push_back(c - lo); v2.push_back(A.val[j]); }
+        if (c < ii || c >= iimax || c == i) {
+          c2.push_back(c - lo);
+          v2.push_back(A.val[j]);
+          if (ddm) w2.push_back(alo ? A.val_lo[j] : 0.0);
+        }
       }
     }
   }
@@ -116,30 +127,56 @@ bool ilu0_block(const HostCSR& A, int lo, int hi, int nvec, BlockSweep& F, Block
       int k = c2[kc];
       if (k > i - 1) break;
       int kd = dptr[k];
-      v2[kc] = v2[kc] / v2[kd];
+      if (ddm) {
+        const dd l = dd_div(mkdd(v2[kc], w2[kc]), mkdd(v2[kd], w2[kd]));
+        v2[kc] = l.hi;
+        w2[kc] = l.lo;
+      } else {
+        v2[kc] = v2[kc] / v2[kd];
+      }
       for (int jc = ptr[i]; jc < ptr[i + 1]; ++jc) {
         int j = c2[jc];
         if (j < k + 1) continue;
         for (int q = kd; q < ptr[k + 1]; ++q)
-          if (c2[q] == j) { v2[jc] = std::fma(-v2[kc], v2[q], v2[jc]); break; }
+          if (c2[q] == j) {
+            if (ddm) {
+              const dd u = dd_sub(mkdd(v2[jc], w2[jc]), dd_mul(mkdd(v2[kc], w2[kc]), mkdd(v2[q], w2[q])));
+              v2[jc] = u.hi;
+              w2[jc] = u.lo;
+            } else {
+              v2[jc] = std::fma(-v2[kc], v2[q], v2[jc]);
+            }
+            break;
+          }
       }
     }
   }
   // forward: strictly-lower entries in stored order, unit diagonal
   F.ptr.assign(nb + 1, 0); F.dep.clear(); F.val.clear(); F.diag.assign(nb, 1.0);
   B.ptr.assign(nb + 1, 0); B.dep.clear(); B.val.clear(); B.diag.assign(nb, 1.0);
+  F.val_lo.clear(); B.val_lo.clear(); F.diag_lo.clear(); B.diag_lo.clear();
+  if (ddm) { F.diag_lo.assign(nb, 0.0); B.diag_lo.assign(nb, 0.0); }
   for (int i = 0; i < nb; ++i) {
     F.ptr[i] = (int)F.dep.size();
     for (int q = ptr[i]; q < ptr[i + 1]; ++q)
-      if (c2[q] < i) { F.dep.push_back(c2[q]); F.val.push_back(v2[q]); }
+      if (c2[q] < i) {
+        F.dep.push_back(c2[q]);
+        F.val.push_back(v2[q]);
+        if (ddm) F.val_lo.push_back(w2[q]);
+      }
   }
   F.ptr[nb] = (int)F.dep.size();
   // backward: strictly-upper entries in stored order, divide by u_ii
   for (int i = 0; i < nb; ++i) {
     B.ptr[i] = (int)B.dep.size();
     for (int q = ptr[i]; q < ptr[i + 1]; ++q)
-      if (c2[q] > i) { B.dep.push_back(c2[q]); B.val.push_back(v2[q]); }
+      if (c2[q] > i) {
+        B.dep.push_back(c2[q]);
+        B.val.push_back(v2[q]);
+        if (ddm) B.val_lo.push_back(w2[q]);
+      }
     B.diag[i] = v2[dptr[i]];
+    if (ddm) B.diag_lo[i] = w2[dptr[i]];
   }
   B.ptr[nb] = (int)B.dep.size();
   return true;
@@ -147,10 +184,14 @@ bool ilu0_block(const HostCSR& A, int lo, int hi, int nvec, BlockSweep& F, Block
 
 // D-ILU of one block (DILU_DInv_crs + init_solve_ILU_crs), ilu_method 1
 // (diagonal match, ILUD) or 2 (row-sum match, MLUD).
+// ddm: D and Dinv in double-double (ddDILU_DInv_crs, src/ddmatrix.f90:1451-1662,
+// zero pivots replaced by 1e-100), val_lo / diag_lo filled.
 bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSweep& F,
-                BlockSweep& B) {
+                BlockSweep& B, bool ddm = false) {
   const int nb = hi - lo;
   const int* rp = A.rowptr.data(); const int* cl = A.col.data(); const double* a = A.val.data();
+  const double* al = (ddm && !A.val_lo.empty()) ? A.val_lo.data() : nullptr;
+  auto ad = [&](int j) { return mkdd(a[j], al ? al[j] : 0.0); };
   for (int k = lo; k < hi; ++k) {
     bool found = false;
     for (int j = rp[k]; j < rp[k + 1]; ++j) {
@@ -160,6 +201,7 @@ bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSwe
     if (!found) return false;
   }
   std::vector<double> D(nb);
+  std::vector<dd> Ddd(ddm ? nb : 0);
   for (int k = lo; k < hi; ++k) {
     int ist = lo + ((k - lo) / nvec) * nvec;
     int ien = std::min(ist + nvec - 1, hi - 1);
@@ -170,27 +212,45 @@ bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSwe
       if (jstart == -1) jstart = j;
       if (c != k) continue;
       double d = a[j];
+      dd d2 = ad(j);
       for (int jr = jstart; jr <= j - 1; ++jr) {
         int kr = cl[jr];
         if (ist <= kr && kr <= ien && kr != k) continue;
         for (int i = rp[kr]; i < rp[kr + 1]; ++i) {
           int rc = cl[i];
           if (rc < lo || rc >= hi) continue;
-          if (method == PARCEL_ILUD) {
-            if (rc == k) { d = d - a[jr] * a[i] / D[kr - lo]; break; }
+          const bool upd = (method == PARCEL_ILUD) ? (rc == k) : (kr < rc);
+          if (!upd) continue;
+          if (ddm) {
+            const dd dk = Ddd[kr - lo];
+            dd t = dd_mul(ad(jr), ad(i));
+            t = (dk.hi == 0.0 && dk.lo == 0.0) ? dd_div(t, mkdd(1.0e-100, 0.0)) : dd_div(t, dk);
+            d2 = dd_sub(d2, t);
           } else {
-            if (kr < rc) d = d - a[jr] * a[i] / D[kr - lo];
+            d = d - a[jr] * a[i] / D[kr - lo];
           }
+          if (method == PARCEL_ILUD) break;
         }
       }
       D[k - lo] = d;
+      if (ddm) Ddd[k - lo] = d2;
       break;
     }
   }
   F.ptr.assign(nb + 1, 0); F.dep.clear(); F.val.clear(); F.diag.assign(nb, 0.0);
   B.ptr.assign(nb + 1, 0); B.dep.clear(); B.val.clear(); B.diag.assign(nb, 0.0);
+  F.val_lo.clear(); B.val_lo.clear(); F.diag_lo.clear(); B.diag_lo.clear();
+  if (ddm) { F.diag_lo.assign(nb, 0.0); B.diag_lo.assign(nb, 0.0); }
   for (int k = lo; k < hi; ++k) {
     double dinv = 1.0 / D[k - lo];
+    if (ddm) {
+      const dd dk = Ddd[k - lo];
+      const dd v = (dk.hi == 0.0 && dk.lo == 0.0) ? dd_div(mkdd(1.0, 0.0), mkdd(1.0e-100, 0.0))
+                                                  : dd_div(mkdd(1.0, 0.0), dk);
+      dinv = v.hi;
+      F.diag_lo[k - lo] = v.lo;
+      B.diag_lo[k - lo] = v.lo;
+    }
     F.diag[k - lo] = dinv;
     B.diag[k - lo] = dinv;
     int ist = lo + ((k - lo) / nvec) * nvec;
@@ -203,7 +263,11 @@ bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSwe
       if (ist <= cl[j]) { le = j - 1; break; }
     }
     F.ptr[k - lo] = (int)F.dep.size();
-    for (int j = ls; j <= le; ++j) { F.dep.push_back(cl[j] - lo); F.val.push_back(a[j]); }
+    for (int j = ls; j <= le; ++j) {
+      F.dep.push_back(cl[j] - lo);
+      F.val.push_back(a[j]);
+      if (ddm) F.val_lo.push_back(al ? al[j] : 0.0);
+    }
     // U range walked downward: last in-block entry .. first entry right of the chunk
     int us = rp[k], ue = rp[k + 1] - 1;
     for (int j = rp[k + 1] - 1; j >= rp[k]; --j) { if (hi - 1 < cl[j]) continue; us = j; break; }
@@ -212,7 +276,11 @@ bool dilu_block(const HostCSR& A, int lo, int hi, int nvec, int method, BlockSwe
       ue = j + 1; break;
     }
     B.ptr[k - lo] = (int)B.dep.size();
-    for (int j = us; j >= ue; --j) { B.dep.push_back(cl[j] - lo); B.val.push_back(a[j]); }
+    for (int j = us; j >= ue; --j) {
+      B.dep.push_back(cl[j] - lo);
+      B.val.push_back(a[j]);
+      if (ddm) B.val_lo.push_back(al ? al[j] : 0.0);
+    }
   }
   F.ptr[nb] = (int)F.dep.size();
   B.ptr[nb] = (int)B.dep.size();
@@ -301,6 +369,8 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
   const int pad = W ? -(1 + zero_slot) : kPad;
   std::vector<int> rows(nrows), dep((size_t)nent, pad);
   std::vector<double> eval((size_t)nent, 0.0), diag(nrows);
+  const bool lo_parts = !blocks.empty() && !blocks[0].diag_lo.empty();
+  std::vector<double> eval_lo(lo_parts ? (size_t)nent : 0, 0.0), diag_lo(lo_parts ? nrows : 0);
   long real = 0, gdeps = 0;
 #pragma omp parallel for schedule(dynamic) reduction(+ : real, gdeps)
   for (int b = 0; b < nbk; ++b) {
@@ -321,6 +391,7 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
         int i = B.order[B.lev_start[l] + t];
         rows[ro[g] + t] = blo[b] + i;
         diag[ro[g] + t] = B.diag[i];
+        if (lo_parts) diag_lo[ro[g] + t] = B.diag_lo[i];
         int e = 0;
         for (int q = B.ptr[i]; q < B.ptr[i + 1]; ++q, ++e) {
           size_t at = (size_t)eo[g] + (size_t)e * m + t;
@@ -332,6 +403,7 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
             gdeps++;
           }
           eval[at] = B.val[q];
+          if (lo_parts) eval_lo[at] = B.val_lo[q];
           real++;
         }
       }
@@ -351,6 +423,10 @@ void pack_sweep(const std::vector<BlockSweep>& blocks, const std::vector<int>& b
   S.dep.upload(dep, st);
   S.eval.upload(eval, st);
   S.diag.upload(diag, st);
+  if (lo_parts) {
+    S.eval_lo.upload(eval_lo, st);
+    S.diag_lo.upload(diag_lo, st);
+  }
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -810,6 +886,162 @@ void to_float(const DevBuf<double>& a, DevBuf<float>& b, hipStream_t st) {
 }
 
 }  // namespace
+
+// ---- double-double preconditioners ------------------------------------------
+namespace {
+
+struct DDSweep {
+  const int* blk_lev;
+  const int4* meta;
+  const int* rows;
+  const int* dep;
+  const double* eh;
+  const double* el;
+  const double* dh;
+  const double* dl;
+};
+
+DDSweep dd_sweep(const Sweep& S) {
+  return DDSweep{S.blk_lev.p, S.meta.p, S.rows.p, S.dep.p, S.eval.p, S.eval_lo.p, S.diag.p, S.diag_lo.p};
+}
+
+// One workgroup per block; level by level, rows of a level in parallel.
+// Every row accumulates its entries in the reference's order (ddsolve_LU_crs,
+// src/ddmatrix.f90:6580-7132; ddsolve_ILU_crs, :3737-4230).  two == 0
+// applies the factors' hi parts only (PRECISION_PRECONDITION = 1).
+template <bool FWD, int MODE>
+__device__ void dd_sweep_run(const DDSweep& A, const double* rh, const double* rl, double* sh, double* sl,
+                             int two) {
+  const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
+  for (int l = l0; l < l1; ++l) {
+    const int4 mt = A.meta[l];
+    const int m = mt.x, k = mt.y;
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      const int row = A.rows[mt.w + t];
+      dd z;
+      if (FWD) z = mkdd(rh[row], rl[row]);
+      else z = (MODE == BJ_DILU) ? mkdd(0.0, 0.0) : mkdd(sh[row], sl[row]);
+      for (int e = 0; e < k; ++e) {
+        const int c = A.dep[mt.z + e * m + t];
+        if (c == kPad) break;
+        const dd a = mkdd(A.eh[mt.z + e * m + t], two ? A.el[mt.z + e * m + t] : 0.0);
+        const dd prod = dd_mul(mkdd(sh[c], sl[c]), a);
+        z = (!FWD && MODE == BJ_DILU) ? dd_add(z, prod) : dd_sub(z, prod);
+      }
+      const dd d = mkdd(A.dh[mt.w + t], two ? A.dl[mt.w + t] : 0.0);
+      dd v;
+      if (MODE == BJ_ILU0) v = dd_div(z, d);
+      else if (FWD) v = dd_mul(z, d);
+      else v = dd_sub(mkdd(sh[row], sl[row]), dd_mul(d, z));
+      sh[row] = v.hi;
+      sl[row] = v.lo;
+    }
+    __syncthreads();
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dd_bj(DDSweep F, DDSweep B, const double* rh, const double* rl, double* sh,
+                                               double* sl, int two, const int* skip) {
+  if (skip && *skip) return;
+  dd_sweep_run<true, MODE>(F, rh, rl, sh, sl, two);
+  dd_sweep_run<false, MODE>(B, nullptr, nullptr, sh, sl, two);
+}
+
+__global__ void k_dd_pj(int n, const double* __restrict__ rh, const double* __restrict__ rl,
+                        const double* __restrict__ dh, const double* __restrict__ dl, double* __restrict__ sh,
+                        double* __restrict__ sl, const int* skip) {
+  if (skip && *skip) return;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    dd v;
+    if (dh) v = dd_mul(mkdd(dh[i], dl ? dl[i] : 0.0), mkdd(rh[i], rl[i]));
+    else v = mkdd(rh[i], rl[i]);
+    sh[i] = v.hi;
+    sl[i] = v.lo;
+  }
+}
+
+}  // namespace
+
+void precon_setup_dd(Plan& P, int prec) {
+  auto t0 = std::chrono::steady_clock::now();
+  Precon& M = P.M;
+  const HostCSR& A = P.hA;
+  const int n = A.n;
+  M.kind = P.prm.ipreflag;
+  M.dd_prec = prec;
+  if (M.kind == PRECON_PJ) {
+    // ddget_DInv_crs (src/ddmatrix.f90:290-330): Dinv = 1/a_kk in dd, 1/1e-100 for a zero diagonal
+    std::vector<double> dh(n, 0.0), dl(n, 0.0);
+    for (int k = 0; k < n; ++k)
+      for (int j = A.rowptr[k]; j < A.rowptr[k + 1]; ++j)
+        if (A.col[j] == k) {
+          const dd a = mkdd(A.val[j], A.val_lo.empty() ? 0.0 : A.val_lo[j]);
+          const dd v = (a.hi == 0.0 && a.lo == 0.0) ? dd_div(mkdd(1.0, 0.0), mkdd(1.0e-100, 0.0))
+                                                    : dd_div(mkdd(1.0, 0.0), a);
+          dh[k] = v.hi;
+          dl[k] = v.lo;
+        }
+    M.dinv.alloc(n);
+    M.dinv.upload(dh, P.stream);
+    M.dinv_lo.alloc(n);
+    M.dinv_lo.upload(dl, P.stream);
+    PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  } else if (M.kind == PRECON_BJ) {
+    int nblk = P.prm.precon_thblock;
+    if (nblk < 1) {
+      hipDeviceProp_t prop;
+      PARCEL_HIP_CHECK(hipGetDeviceProperties(&prop, P.device));
+      nblk = prop.multiProcessorCount;
+    }
+    nblk = std::max(1, std::min(nblk, n));
+    M.nblocks = nblk;
+    M.mode = (P.prm.ILU_method == PARCEL_ILU0) ? BJ_ILU0 : BJ_DILU;
+    const int nvec = P.prm.independ_nvec < 1 ? 1 : P.prm.independ_nvec;
+    std::vector<BlockSweep> F(nblk), B(nblk);
+    std::vector<int> blo(nblk), bhi(nblk);
+    for (int b = 0; b < nblk; ++b) divide_region0(b, nblk, n, &blo[b], &bhi[b]);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+    for (int b = 0; b < nblk; ++b) {
+      bool ok = (M.mode == BJ_ILU0) ? ilu0_block(A, blo[b], bhi[b], nvec, F[b], B[b], true)
+                                    : dilu_block(A, blo[b], bhi[b], nvec, P.prm.ILU_method, F[b], B[b], true);
+      if (!ok) { bad |= 1; continue; }
+      build_levels(F[b], bhi[b] - blo[b], true);
+      build_levels(B[b], bhi[b] - blo[b], false);
+    }
+    if (bad) throw hip_error("zero diagonal in a block-Jacobi block", PARCEL_HIP_ERR_ZERO_DIAG);
+    int kmax = 1;
+    for (int b = 0; b < nblk; ++b)
+      for (const BlockSweep* S : {&F[b], &B[b]})
+        for (size_t i = 0; i + 1 < S->ptr.size(); ++i) kmax = std::max(kmax, S->ptr[i + 1] - S->ptr[i]);
+    M.nth = 256;
+    // no LDS ring: every dependency is a global row id
+    pack_sweep(F, blo, M.fwd, 0, kmax, M.nth, P.stream);
+    pack_sweep(B, blo, M.bwd, 0, kmax, M.nth, P.stream);
+  } else if (M.kind != PRECON_NONE) {
+    throw hip_error("preconditioner not supported on the double-double path", PARCEL_HIP_ERR_UNSUPPORTED);
+  }
+  M.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void precon_apply_dd(Plan& P, const double* rh, const double* rl, double* sh, double* sl, const int* skip) {
+  Precon& M = P.M;
+  const int n = P.A.n;
+  const int two = M.dd_prec == 2 ? 1 : 0;
+  if (M.kind == PRECON_NONE || M.kind == PRECON_PJ) {
+    const bool pj = M.kind == PRECON_PJ;
+    hipLaunchKernelGGL(k_dd_pj, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, rh, rl, pj ? (const double*)M.dinv.p : nullptr,
+                       (pj && two) ? (const double*)M.dinv_lo.p : nullptr, sh, sl, skip);
+  } else {
+    const DDSweep F = dd_sweep(M.fwd), B = dd_sweep(M.bwd);
+    if (M.mode == BJ_ILU0)
+      hipLaunchKernelGGL(k_dd_bj<BJ_ILU0>, dim3(M.nblocks), dim3(256), 0, P.stream, F, B, rh, rl, sh, sl, two, skip);
+    else
+      hipLaunchKernelGGL(k_dd_bj<BJ_DILU>, dim3(M.nblocks), dim3(256), 0, P.stream, F, B, rh, rl, sh, sl, two, skip);
+  }
+  PARCEL_HIP_CHECK(hipGetLastError());
+}
 
 void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
   precon_apply_t<double>(P, r, s, fuse, skip);

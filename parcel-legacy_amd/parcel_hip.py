@@ -90,6 +90,11 @@ SYMBOLS = {
     "parcel_hip_dcg_mixed": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                   _I, _I, _I, _I, _I, _D, _I, _IP]),
     "parcel_hip_plan_set_inner": (_I, [_P, _D, _I]),
+    "parcel_hip_ddcg": (_I, [_I, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                             _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_plan_create_dd": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(Param),
+                                       _I, _I]),
+    "parcel_hip_plan_solve_dd": (_I, [_P, _P, _P, _P, _P, _IP, _D, _P, _I, _I, _IP]),
     "parcel_hip_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P,
                                     ctypes.POINTER(Param)]),
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
@@ -347,6 +352,24 @@ def dcg_mixed(x, b, rowptr, col, val, inner_rtol=1e-4, inner_itrmax=10000, ipref
                                     ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
                                     iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, inner_rtol,
                                     inner_itrmax, ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
+
+
+def ddcg(xh, xl, bh, bl, rowptr, col, vh, vl, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=1000, rtolmax=1e-20,
+         precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, nBlock=2000, PRECISION_A=2,
+         PRECISION_b=2, PRECISION_x=2, PRECISION_PRECONDITION=2):
+    """parcel_ddcg: double-double PCG, x (hi, lo) in place; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1), np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    opt = lambda a: None if a is None else _ptr(a)
+    _check(lib.parcel_hip_ddcg(0, _ptr(xh), opt(xl), _ptr(bh), opt(bl), n, n, nnz, 1, _ptr(vh), opt(vl),
+                               _ptr(rowptr), _ptr(col), ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it),
+                               rtolmax, _ptr(hist), iovlflag, precon_thblock, independ_nvec, nBlock, PRECISION_A,
+                               PRECISION_b, PRECISION_x, PRECISION_PRECONDITION, ctypes.byref(iret)))
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
