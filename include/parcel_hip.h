@@ -259,6 +259,105 @@ int parcel_hip_memcpy(void* dst, const void* src, unsigned long bytes, int kind)
 int parcel_hip_memset(void* ptr, int value, unsigned long bytes);
 int parcel_hip_device_synchronize(void);
 
+/* --- batched systems: many matrices with ONE sparsity pattern (no
+ * counterpart in the reference, whose solvers take one system per call).
+ * Every system runs the arithmetic of the single-system solver
+ * (KSP_pCG, src/pcg.f90:224-892) on its own vectors; a launch covers all
+ * systems.  Preconditioners:
+ *   NONE, JACOBI      as ipreflag 0 / 1 of parcel_dcg;
+ *   IC0               incomplete Cholesky on the lower five-point pattern;
+ *   FACTOR            M = L L^T from a caller-supplied (or network-predicted)
+ *                     lower factor on the same pattern, BF16 values.
+ * IC0 and FACTOR need the pattern of a 2D five-point grid in natural order
+ * (rows x fastest, grid lines of at most 1024 cells); the factor solves are
+ * line recurrences evaluated as 64-lane scans. */
+#define PARCEL_BATCH_NONE 0
+#define PARCEL_BATCH_JACOBI 1
+#define PARCEL_BATCH_IC0 2
+#define PARCEL_BATCH_FACTOR 3
+
+typedef struct parcel_hip_batch parcel_hip_batch;
+
+typedef struct {
+  double setup_s;          /* create + preconditioner setup (host clock) */
+  double solve_s;          /* last solve (host clock) */
+  double solve_device_ms;  /* last solve, HIP events on the batch stream */
+  int max_niter;           /* largest (outer) iteration count of the last solve */
+  int outer_steps;         /* mixed solves: batched FP64 corrections launched */
+  long inner_total;        /* mixed solves: FP32 iterations summed over systems */
+  int inner_steps;         /* mixed solves: batched FP32 iterations launched */
+  int launches;
+  double apply_ms;         /* factor-apply kernel time (profile on) */
+  int apply_calls;
+  double spmv_ms;          /* fused SpMV kernel time (profile on) */
+  int spmv_calls;
+  int nx, ny;              /* detected five-point grid (0 if none) */
+} parcel_hip_batch_stats;
+
+/* nsys systems of order n; rowptr/col: 1-based CSR (host), shared; vals:
+ * nsys*nnz values in CSR order, system s at vals + s*nnz (host or device). */
+int parcel_hip_batch_create(parcel_hip_batch** batch, int nsys, int n, int nnz, const int* rowptr,
+                            const int* col, const double* vals, int precon);
+/* Factor of every system for PARCEL_BATCH_FACTOR: bf16 bit patterns, each
+ * array nsys*n (host or device): Ls[i] = L(i, i-nx), Lw[i] = L(i, i-1),
+ * Ld[i] = L(i, i) > 0.  Entries outside the pattern must be 0. */
+int parcel_hip_batch_set_factor(parcel_hip_batch* batch, const unsigned short* Ls, const unsigned short* Lw,
+                                const unsigned short* Ld);
+/* Solve all systems to ||b - A x|| / ||b|| < rtol.  precision 64: FP64 PCG
+ * (per system exactly parcel_dcg's iteration); precision 32: FP64
+ * refinement around FP32 PCG with the FP32 preconditioner (as
+ * PARCEL_SOLVER_CG_MIXED; itrmax then counts FP64 corrections).  x, b:
+ * nsys*n (host or device), x holds the initial guess.  hist (nsys*itrmax,
+ * may be NULL), niter and iret (nsys each, may be NULL) are host arrays. */
+int parcel_hip_batch_solve(parcel_hip_batch* batch, double* x, const double* b, int itrmax, double rtol,
+                           int precision, double* hist, int* niter, int* iret);
+int parcel_hip_batch_set_inner(parcel_hip_batch* batch, double inner_rtol, int inner_itrmax);
+int parcel_hip_batch_set_profile(parcel_hip_batch* batch, int on);
+int parcel_hip_batch_get_stats(const parcel_hip_batch* batch, parcel_hip_batch_stats* stats);
+/* y = A x for every system (x, y: nsys*n, host or device) */
+int parcel_hip_batch_spmv(parcel_hip_batch* batch, const double* x, double* y);
+/* z = M^-1 r with the preconditioner of the batch (FP64 arrays) */
+int parcel_hip_batch_precon_apply(parcel_hip_batch* batch, const double* r, double* z);
+/* FP32 apply of the same preconditioner (r, z: nsys*n floats) */
+int parcel_hip_batch_precon_apply32(parcel_hip_batch* batch, const float* r, float* z);
+/* The BF16 factor of every system (FACTOR batches; nsys*n each, host) */
+int parcel_hip_batch_get_factor(const parcel_hip_batch* batch, unsigned short* Ls, unsigned short* Lw,
+                                unsigned short* Ld);
+int parcel_hip_batch_destroy(parcel_hip_batch* batch);
+
+/* --- learned factor: a message-passing network over the graph of a 2D
+ * five-point matrix predicts L on the matrix's lower pattern (M = L L^T,
+ * BF16 factor); its layers run on BF16 MFMA with FP32 accumulation.  No
+ * counterpart in the reference.  Parameters: FP32, parcel_hip_mpnn_nparams()
+ * of them (6 layers of W 32x64 + b 32, head Wo 8x32 + bo 8, row-major). */
+typedef struct parcel_hip_mpnn parcel_hip_mpnn;
+int parcel_hip_mpnn_nparams(void);
+int parcel_hip_mpnn_create(parcel_hip_mpnn** net, const float* params, int nparams);
+/* Predict and install the factor of every system of a PARCEL_BATCH_FACTOR batch */
+int parcel_hip_mpnn_factor(parcel_hip_mpnn* net, parcel_hip_batch* batch);
+/* Network internals for a batch (host arrays, each may be NULL): head
+ * outputs o (nsys*n*8), last activation (nsys*n*32 BF16 bits), scaled
+ * couplings ahat (nsys*n*4: s, w, e, n) and sqrt(diag) sd (nsys*n). */
+int parcel_hip_mpnn_inspect(parcel_hip_mpnn* net, parcel_hip_batch* batch, float* o, unsigned short* hlast,
+                            float* ahat, float* sd);
+int parcel_hip_mpnn_destroy(parcel_hip_mpnn* net);
+
+/* Training on the device: a pool of `capacity` systems of one grid (n cells,
+ * lines of nx), minibatch gradients by the network's backward kernels, Adam. */
+typedef struct parcel_hip_mpnn_trainer parcel_hip_mpnn_trainer;
+int parcel_hip_mpnn_trainer_create(parcel_hip_mpnn_trainer** trainer, int n, int nx, int capacity,
+                                   const float* params);
+/* add a batch's systems with solutions x and right-hand sides b (nsys*n) */
+int parcel_hip_mpnn_trainer_add(parcel_hip_mpnn_trainer* trainer, parcel_hip_batch* batch, const double* x,
+                                const double* b);
+/* loss and gradient (nparams floats, host; may be NULL) of pool systems idx[0..S) */
+int parcel_hip_mpnn_trainer_grad(parcel_hip_mpnn_trainer* trainer, const int* idx, int S, float* grad,
+                                 double* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the minibatch; loss before the step */
+int parcel_hip_mpnn_trainer_step(parcel_hip_mpnn_trainer* trainer, const int* idx, int S, float lr, double* loss);
+int parcel_hip_mpnn_trainer_params(parcel_hip_mpnn_trainer* trainer, float* params);
+int parcel_hip_mpnn_trainer_destroy(parcel_hip_mpnn_trainer* trainer);
+
 /* --- synthetic problem generators (host code): the counterpart of the
  * reference's example generators (src/example/problem.f90) plus the build's
  * own seeded generators.  Output CSR is 1-based with sorted columns. ------ */
@@ -278,6 +377,24 @@ int parcel_gen_poisson3d_spd(int gnx, int gny, int gnz, int* nnz, int* rowptr, i
                              double* val);
 /* x ~ U[-1,1] from the counter-based stream `seed` (SplitMix64, 53-bit) */
 int parcel_gen_uniform(long n, unsigned long long seed, double lo, double hi, double* x);
+/* x ~ N(0,1): normal k of stream `seed` is Box-Muller on uniforms 2*(k/2)
+ * and 2*(k/2)+1 (cosine for even k, sine for odd k) */
+int parcel_gen_normal(long n, unsigned long long seed, double* x);
+/* Gaussian random field on an nx*ny torus (powers of two), squared-
+ * exponential covariance sigma^2 exp(-d^2 / (2 ell^2)), by circulant
+ * embedding in FP64 (white noise: parcel_gen_normal of `seed`) */
+int parcel_gen_grf2d(int nx, int ny, double ell, double sigma, unsigned long long seed, double* g);
+/* 2D five-point diffusion with per-cell conductivity kappa (nx*ny, x
+ * fastest), harmonic-mean faces (make_poisson3D_adv_CRS,
+ * src/example/problem.f90:620-770, in 2D with dx = dy), ghost cells carrying
+ * the boundary cell's kappa; SPD: off-diagonals -face, diagonal the sum. */
+int parcel_gen_diffusion2d(int nx, int ny, const double* kappa, int* nnz, int* rowptr, int* col,
+                           double* val);
+/* nsys systems kappa = exp(grf(seed0 + s)); shared pattern (rowptr/col may be
+ * NULL), values of system s at vals + s*nnz; g_out (nsys*nx*ny) optional.
+ * Pass vals = NULL to query nnz. */
+int parcel_gen_grf_diffusion2d(int nx, int ny, double ell, double sigma, unsigned long long seed0, int nsys,
+                               int* nnz, int* rowptr, int* col, double* vals, double* g_out);
 
 #ifdef __cplusplus
 }

@@ -307,3 +307,198 @@ def last_times():
     t = np.zeros(2)
     load().oracle_last_times(_p(t))
     return float(t[0]), float(t[1])
+
+
+# --- batched solver's factor preconditioners (batched.c) -----------------------
+
+def _sig_batched(lib):
+    sig = {
+        "oracle_pcg_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _RP, _P, _IP, _IP]),
+        "oracle_pcg_mixed_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _RP, _P, _IP, _IP, _IP]),
+        "oracle_ic0_5pt": (_I, [_I, _I, _P, _P, _P]),
+        "oracle_derive": (None, [_I, _I, _I, _P, _P, _P]),
+        "oracle_precon_create_factor": (_P, [_I, _I, _P, _P]),
+        "oracle_lscan_apply": (None, [_I, _I, _P, _P, _P]),
+        "oracle_lscan_apply32": (None, [_I, _I, _P, _P, _P]),
+        "oracle_f32_to_bf16": (ctypes.c_ushort, [ctypes.c_float]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+
+
+def _blib():
+    lib = load()
+    if not getattr(lib, "_batched_sig", False):
+        _sig_batched(lib)
+        lib._batched_sig = True
+    return lib
+
+
+def ic0_5pt(rowptr, val, nx):
+    """IC(0) factor (Ls, Lw, Ld) of a five-point matrix in natural order."""
+    n = rowptr.shape[0] - 1
+    L = np.zeros(3 * n)
+    rc = _blib().oracle_ic0_5pt(n, nx, _p(rowptr), _p(val), _p(L))
+    if rc != 0:
+        raise RuntimeError("IC(0) breakdown")
+    return L[:n].copy(), L[n:2 * n].copy(), L[2 * n:].copy()
+
+
+def derive(Ls, Lw, Ld, nx):
+    """Apply arrays (F64, F32), each (4, n): ls, 1/Ld, a_fwd, a_bwd.  The
+    factor is double (float64 arrays) or bf16 (uint16 bit patterns)."""
+    n = Ls.shape[0]
+    src = 1 if Ls.dtype == np.uint16 else 0
+    L = np.ascontiguousarray(np.concatenate([Ls, Lw, Ld]))
+    F64 = np.zeros(4 * n)
+    F32 = np.zeros(4 * n, np.float32)
+    _blib().oracle_derive(n, nx, src, _p(L), _p(F64), _p(F32))
+    return F64, F32
+
+
+def lscan_apply(F, r, nx):
+    """z = (L L^T)^-1 r with the GPU's lane-scan order (F float64 or float32)."""
+    n = r.shape[0]
+    z = np.zeros_like(r)
+    fn = _blib().oracle_lscan_apply32 if r.dtype == np.float32 else _blib().oracle_lscan_apply
+    fn(nx, n // nx, _p(np.ascontiguousarray(F)), _p(np.ascontiguousarray(r)), _p(z))
+    return z
+
+
+class FactorPrecon:
+    def __init__(self, F64, F32, nx):
+        self.n = F64.shape[0] // 4
+        self.h = _blib().oracle_precon_create_factor(self.n, nx, _p(F64), _p(F32))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_precon_free(self.h)
+            self.h = None
+
+
+def pcg_factor(rowptr, col, val, b, M, x0=None, itrmax=1000, rtol=1e-8, red=TILED):
+    """oracle PCG with a FactorPrecon; returns (x, hist[:niter], niter, iret)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 1))
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    rc = _blib().oracle_pcg_with(M.h, n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), itrmax, rtol,
+                                 ctypes.byref(red), _p(hist), ctypes.byref(niter), ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_pcg_with failed: %d" % rc)
+    return x, hist[: niter.value].copy(), niter.value, iret.value
+
+
+def pcg_mixed_factor(rowptr, col, val, b, M, x0=None, itrmax=50, rtol=1e-10, inner_itrmax=5000, inner_rtol=1e-4,
+                     red=TILED):
+    """Mixed refinement with a FactorPrecon; returns (x, hist[:niter], niter, iret, inner_total)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 1))
+    niter, iret, tot = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    rc = _blib().oracle_pcg_mixed_with(M.h, n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), itrmax, rtol,
+                                       inner_itrmax, inner_rtol, ctypes.byref(red), _p(hist), ctypes.byref(niter),
+                                       ctypes.byref(iret), ctypes.byref(tot))
+    if rc != 0:
+        raise RuntimeError("oracle_pcg_mixed_with failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value, tot.value
+
+
+def f32_to_bf16(x):
+    """Round float32 to bf16 bit patterns, nearest even (NaN -> quiet NaN)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
+
+
+def f32_to_bf16_c(v):
+    """The C oracle's scalar rounding (bit pattern)."""
+    return int(_blib().oracle_f32_to_bf16(float(v)))
+
+
+def bf16_to_f32(h):
+    return (np.asarray(h, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+# --- random-coefficient diffusion (the generator's restatement in numpy) -------
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix_u01(seed, idx):
+    """u_i in [0,1): top 53 bits of SplitMix64 of seed + (i+1)*golden."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.asarray(idx, dtype=np.uint64) + np.uint64(1)) * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(n, seed):
+    k = np.arange(n, dtype=np.uint64)
+    p = k >> np.uint64(1)
+    u1 = splitmix_u01(seed, 2 * p)
+    u2 = splitmix_u01(seed, 2 * p + np.uint64(1))
+    rad = np.sqrt(-2.0 * np.log(1.0 - u1))
+    ang = 6.283185307179586 * u2
+    return np.where((k & np.uint64(1)) == 1, rad * np.sin(ang), rad * np.cos(ang))
+
+
+def grf_spectrum1d(N, ell):
+    """S_N(k) = sum_i exp(-i^2/(2 ell^2)) e^{-2 pi i i k/N} by Poisson summation."""
+    f = np.arange(N) / N
+    m = np.arange(-1, 3)[:, None]
+    return np.sqrt(2 * np.pi) * ell * np.exp(-2 * np.pi ** 2 * ell ** 2 * (f[None, :] - m) ** 2).sum(axis=0)
+
+
+def grf_eigenvalues(nx, ny, ell, sigma):
+    """Eigenvalues of the circulant SE covariance, (ny, nx)."""
+    return sigma * sigma * np.outer(grf_spectrum1d(ny, ell), grf_spectrum1d(nx, ell))
+
+
+def grf2d(nx, ny, ell, sigma, seed):
+    """Circulant-embedding GRF with squared-exponential covariance on the torus."""
+    lam = grf_eigenvalues(nx, ny, ell, sigma)
+    sq = np.sqrt(lam)
+    w = normal(nx * ny, seed).reshape(ny, nx)
+    return np.fft.ifft2(sq * np.fft.fft2(w)).real.reshape(-1)
+
+
+def diffusion2d(nx, ny, kappa):
+    """Five-point harmonic-mean diffusion (1-based CSR), ghost kappa = cell kappa."""
+    k = np.asarray(kappa, dtype=np.float64).reshape(ny, nx)
+
+    def face(a, b):
+        return 1.0 / (0.5 * (1.0 / a + 1.0 / b))
+
+    ks = np.vstack([k[:1], k[:-1]])
+    kw = np.hstack([k[:, :1], k[:, :-1]])
+    ke = np.hstack([k[:, 1:], k[:, -1:]])
+    kn = np.vstack([k[1:], k[-1:]])
+    fs, fw, fe, fn = face(k, ks), face(k, kw), face(k, ke), face(k, kn)
+    diag = ((fs + fw) + fe) + fn
+    n = nx * ny
+    rp = np.zeros(n + 1, np.int32)
+    col = []
+    val = []
+    e = 0
+    for y in range(ny):
+        for x in range(nx):
+            r = x + nx * y
+            rp[r] = e + 1
+            if y > 0:
+                col.append(r - nx + 1); val.append(-fs[y, x]); e += 1
+            if x > 0:
+                col.append(r - 1 + 1); val.append(-fw[y, x]); e += 1
+            col.append(r + 1); val.append(diag[y, x]); e += 1
+            if x < nx - 1:
+                col.append(r + 1 + 1); val.append(-fe[y, x]); e += 1
+            if y < ny - 1:
+                col.append(r + nx + 1); val.append(-fn[y, x]); e += 1
+    rp[n] = e + 1
+    return rp, np.array(col, np.int32), np.array(val, np.float64)

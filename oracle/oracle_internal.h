@@ -19,7 +19,15 @@ struct oracle_precon {
   /* ILU(0): per block split factors, 0-based local CSR */
   int** lptr; int** lcol; double** lval;
   int** uptr; int** ucol; double** uval;
+  /* ORACLE_PRECON_FACTOR: M = L L^T on a five-point grid (batched.c) */
+  int nx, ny;
+  double* F64; /* ls, 1/Ld, a_fwd, a_bwd (4n) */
+  float* F32;  /* the FP32 apply arrays (4n) */
 };
+
+/* factor solves of batched.c */
+void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z);
+void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float* z);
 
 /* D-ILU L/U entry ranges of rows lo..hi (1-based), init_solve_ILU_crs order */
 void oracle_dilu_ranges(oracle_precon* p, int lo, int hi);

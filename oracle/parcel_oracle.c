@@ -426,6 +426,10 @@ oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
 
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   int n = p->n;
+  if (p->ipreflag == ORACLE_PRECON_FACTOR) {
+    oracle_lscan_apply(p->nx, p->ny, p->F64, r, s);
+    return;
+  }
   if (p->ipreflag == 0) {
     memcpy(s, r, sizeof(double) * (size_t)n); /* xcpy, krylov_h.f90:1708 */
   } else if (p->ipreflag == 1) {
@@ -476,6 +480,10 @@ static void ilu0_apply_block32(const oracle_precon* p, int b, int lo, int hi, co
 }
 
 void oracle_precon_apply32(const oracle_precon* p, const float* r, float* s) {
+  if (p->ipreflag == ORACLE_PRECON_FACTOR) {
+    oracle_lscan_apply32(p->nx, p->ny, p->F32, r, s);
+    return;
+  }
   int n = p->n;
   if (p->ipreflag == 0) {
     memcpy(s, r, sizeof(float) * (size_t)n);
@@ -497,6 +505,7 @@ const double* oracle_precon_dinv(const oracle_precon* p) { return p->dinv; }
 void oracle_precon_free(oracle_precon* p) {
   if (!p) return;
   free(p->blo); free(p->bhi); free(p->dinv);
+  free(p->F64); free(p->F32);
   free(p->lstart); free(p->lend); free(p->ustart); free(p->uend);
   if (p->lptr) {
     for (int b = 0; b < p->nblocks; ++b) {
@@ -515,6 +524,10 @@ void oracle_precon_free(oracle_precon* p) {
 static double g_times[2];
 void oracle_last_times(double* t) { t[0] = g_times[0]; t[1] = g_times[1]; }
 
+static int pcg_core(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int itrmax, double rtol, const oracle_red* red,
+                    double* hist, int* niter, int* iret, double t0);
+
 int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
                const double* b, double* x, int ipreflag, int ilu_method,
                int nblocks, int independ_nvec, int itrmax, double rtol,
@@ -524,6 +537,20 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec,
                                           n, rowptr, col, val, &err);
   if (!M) return err;
+  int rc = pcg_core(M, n, rowptr, col, val, b, x, itrmax, rtol, red, hist, niter, iret, t0);
+  oracle_precon_free(M);
+  return rc;
+}
+
+int oracle_pcg_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int itrmax, double rtol, const oracle_red* red,
+                    double* hist, int* niter, int* iret) {
+  return pcg_core(M, n, rowptr, col, val, b, x, itrmax, rtol, red, hist, niter, iret, omp_get_wtime());
+}
+
+static int pcg_core(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int itrmax, double rtol, const oracle_red* red,
+                    double* hist, int* niter, int* iret, double t0) {
   double t1 = omp_get_wtime();
   size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
   double* r = (double*)calloc(1, bytes);
@@ -567,7 +594,6 @@ int oracle_pcg(int n, const int* rowptr, const int* col, const double* val,
   g_times[1] = omp_get_wtime() - t1;
   *iret = (itrmax <= itr) ? 1 : 0;                  /* pcg.f90:851-855 */
   free(r); free(p); free(q); free(s);
-  oracle_precon_free(M);
   return 0;
 }
 
@@ -821,6 +847,15 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
   int err = 0;
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
   if (!M) return err;
+  int rc = oracle_pcg_mixed_with(M, n, rowptr, col, val, b, x, itrmax, rtol, inner_itrmax, inner_rtol, red, hist,
+                                 niter, iret, inner_total);
+  oracle_precon_free(M);
+  return rc;
+}
+
+int oracle_pcg_mixed_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                          const double* b, double* x, int itrmax, double rtol, int inner_itrmax, double inner_rtol,
+                          const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   double* r = (double*)calloc(nn, sizeof(double));
   double* wx = (double*)calloc(nn, sizeof(double));
@@ -850,6 +885,5 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
   *iret = (itrmax <= it) ? 1 : 0;
   *inner_total = total;
   free(r); free(wx); free(wy); free(r32); free(d32); free(ri); free(pi); free(qi); free(si);
-  oracle_precon_free(M);
   return 0;
 }

@@ -135,6 +135,27 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
                      int itrmax, double rtol, int inner_itrmax, double inner_rtol,
                      const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total);
 
+/* --- batched solver's factor preconditioners (batched.c; no reference
+ * counterpart).  M = L L^T on a 2D five-point grid in natural order. ------ */
+#define ORACLE_PRECON_FACTOR 9
+/* PCG / mixed refinement with a prebuilt preconditioner (any kind) */
+int oracle_pcg_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                    const double* b, double* x, int itrmax, double rtol, const oracle_red* red,
+                    double* hist, int* niter, int* iret);
+int oracle_pcg_mixed_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                          const double* b, double* x, int itrmax, double rtol, int inner_itrmax, double inner_rtol,
+                          const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total);
+/* IC(0) factor (Ls, Lw, Ld; n each) of a five-point matrix, 1-based CSR */
+int oracle_ic0_5pt(int n, int nx, const int* rowptr, const double* val, double* L);
+/* apply arrays (ls, 1/Ld, a_fwd, a_bwd) from a double (src 0) or bf16
+ * (src 1) factor; F64 / F32 may be NULL */
+void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F32);
+oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, const float* F32);
+void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z);
+void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float* z);
+float oracle_bf16_to_f32(unsigned short h);
+unsigned short oracle_f32_to_bf16(float f);
+
 /* --- double-double (src/dd_func.f90, src/ddpcg.f90, src/ddmatrix.f90) ---- */
 /* op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (b unused), elementwise over n */
 void oracle_dd_op(int op, int n, const double* ah, const double* al, const double* bh,

@@ -1,0 +1,1011 @@
+// batched.hip -- many systems with one sparsity pattern, solved together.
+//
+// The reference solves one system per call (KSP_pCG, src/pcg.f90:224-892).
+// A batch keeps nsys systems resident at once: the pattern in SELL-64 form
+// once, the values of system s at s*entries, every vector of system s at
+// s*n.  Each CG kernel of cg_kernels.hpp runs with gridDim.y = nsys, so one
+// launch advances every system by one step with exactly the single-system
+// arithmetic; a system that has converged returns at the top of every kernel.
+//
+// Factor preconditioners (IC(0) and a supplied lower factor L, M = L L^T)
+// need a 2D five-point grid in natural order: row i = x + nx*y couples to
+// i-1 and i-nx below the diagonal.  The forward solve L w = r on grid line y
+//     w_x = a_x w_{x-1} + c_x,  a_x = -Lw_x / Ld_x,
+//     c_x = (r_x - Ls_x w_x^{(y-1)}) / Ld_x
+// is a first-order linear recurrence along the line once line y-1 is known,
+// i.e. a scan of affine maps.  One wavefront per system walks the lines; lane
+// l owns E = nx/64 consecutive cells, composes its E maps, then an inclusive
+// 64-lane scan (Hillis-Steele, shuffles 1..32) gives each lane the value
+// entering its cells, and the lane re-applies its maps.  The previous line
+// stays in registers (each lane reads only the cells it owns), so the
+// recurrence never waits on memory.  The backward solve L^T z = w runs the
+// lines top-down with the mirrored (suffix) scan.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "batch.hpp"
+#include "cg_kernels.hpp"
+
+using namespace parcel;
+
+namespace {
+
+// --- bf16 -------------------------------------------------------------------
+__host__ __device__ inline float bf16_to_float(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+template <class S, class T>
+__device__ __forceinline__ T cvt(S v) {
+  return (T)v;
+}
+template <>
+__device__ __forceinline__ float cvt<uint16_t, float>(uint16_t v) {
+  return bf16_to_float(v);
+}
+template <>
+__device__ __forceinline__ double cvt<uint16_t, double>(uint16_t v) {
+  return (double)bf16_to_float(v);
+}
+
+// --- setup kernels ------------------------------------------------------------
+
+// SELL values of every system from CSR values: slot e takes CSR entry idx[e]
+// (-1: padding, 0).
+__global__ void k_b_gather(long entries, long nnz, const int* __restrict__ idx, const double* __restrict__ csr,
+                           double* __restrict__ sell, float* __restrict__ sell32) {
+  const long s = blockIdx.y;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < entries; e += (long)gridDim.x * blockDim.x) {
+    const int j = idx[e];
+    const double v = j >= 0 ? csr[s * nnz + j] : 0.0;
+    sell[s * entries + e] = v;
+    sell32[s * entries + e] = (float)v;
+  }
+}
+
+// Point Jacobi: dinv = 1 / a_ii (the single-system setup), FP32 copy rounded.
+__global__ void k_b_dinv(int n, long entries, const int* __restrict__ dslot, const double* __restrict__ sell,
+                         double* __restrict__ dinv, float* __restrict__ dinv32) {
+  const long s = blockIdx.y;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double d = 1.0 / sell[s * entries + dslot[i]];
+    dinv[s * n + i] = d;
+    dinv32[s * n + i] = (float)d;
+  }
+}
+
+// Apply arrays of the factor solves, in T from factor values of type S:
+//   ls = Ls, inv = 1/Ld, af = -Lw * inv, ab = -Lw(x+1) * inv (0 on the last
+// cell of a line).  L holds Ls, Lw, Ld (each nv apart); F gets ls, inv, af, ab.
+template <class S, class T>
+__global__ void k_b_derive(int n, int nx, long nv, const S* __restrict__ L, T* __restrict__ F) {
+  const long s = blockIdx.y;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long g = s * n + i;
+    const T ld = cvt<S, T>(L[2 * nv + g]);
+    const T inv = (T)1 / ld;
+    const T lw = cvt<S, T>(L[nv + g]);
+    const int x = (int)(i % nx);
+    const T lwn = x + 1 < nx ? cvt<S, T>(L[nv + g + 1]) : (T)0;
+    F[g] = cvt<S, T>(L[g]);
+    F[nv + g] = inv;
+    F[2 * nv + g] = -lw * inv;
+    F[3 * nv + g] = -lwn * inv;
+  }
+}
+
+// --- factor solves: z = (L L^T)^-1 r, one wavefront per system -----------------
+
+__device__ __forceinline__ bool sys_done(const CGState* st, int it) {
+  if (!st) return false;
+  const int d = st[blockIdx.x].done_at;
+  return d != 0 && d <= it;
+}
+
+template <class T, int E, bool FULL>
+__global__ void __launch_bounds__(64) k_lscan(int nx, int ny, long n, long nv, const T* __restrict__ F,
+                                              const T* __restrict__ r, T* __restrict__ w, T* __restrict__ z,
+                                              const CGState* st, int it) {
+  if (sys_done(st, it)) return;
+  const int lane = threadIdx.x;
+  const long so = (long)blockIdx.x * n;
+  const T* __restrict__ ls = F + so;
+  const T* __restrict__ inv = F + nv + so;
+  const T* __restrict__ af = F + 2 * nv + so;
+  const T* __restrict__ ab = F + 3 * nv + so;
+  r += so; w += so; z += so;
+  const int x0 = lane * E;
+  T zp[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) zp[k] = 0;
+  // forward: lines bottom-up, cells left to right
+  for (int y = 0; y < ny; ++y) {
+    const long row = (long)y * nx + x0;
+    T a[E], c[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      if (FULL || x0 + k < nx) {
+        a[k] = af[row + k];
+        c[k] = fma(-ls[row + k], zp[k], r[row + k]) * inv[row + k];
+      } else {
+        a[k] = 0; c[k] = 0;
+      }
+    }
+    T A = 1, C = 0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T Ap = __shfl_up(A, d, 64), Cp = __shfl_up(C, d, 64);
+      if (lane >= d) { C = fma(A, Cp, C); A = A * Ap; }
+    }
+    T v = __shfl_up(C, 1, 64);
+    if (lane == 0) v = 0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      v = fma(a[k], v, c[k]);
+      zp[k] = v;
+      if (FULL || x0 + k < nx) w[row + k] = v;
+    }
+  }
+  // backward: lines top-down, cells right to left
+  T yn[E], lsn[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) { yn[k] = 0; lsn[k] = 0; }
+  for (int y = ny - 1; y >= 0; --y) {
+    const long row = (long)y * nx + x0;
+    T a[E], c[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      if (FULL || x0 + k < nx) {
+        a[k] = ab[row + k];
+        c[k] = fma(-lsn[k], yn[k], w[row + k]) * inv[row + k];
+        lsn[k] = ls[row + k];
+      } else {
+        a[k] = 0; c[k] = 0; lsn[k] = 0;
+      }
+    }
+    T A = 1, C = 0;
+#pragma unroll
+    for (int k = E - 1; k >= 0; --k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T Ap = __shfl_down(A, d, 64), Cp = __shfl_down(C, d, 64);
+      if (lane + d < 64) { C = fma(A, Cp, C); A = A * Ap; }
+    }
+    T v = __shfl_down(C, 1, 64);
+    if (lane == 63) v = 0;
+#pragma unroll
+    for (int k = E - 1; k >= 0; --k) {
+      v = fma(a[k], v, c[k]);
+      yn[k] = v;
+      if (FULL || x0 + k < nx) z[row + k] = v;
+    }
+  }
+}
+
+// s = Dinv r (dinv null: s = r), every system
+template <class T>
+__global__ void __launch_bounds__(kNT) k_b_scale(int n, const T* __restrict__ r, const T* __restrict__ dinv,
+                                                 T* __restrict__ s, Bat B) {
+  const long so = sys_id() * B.vs;
+  r += so; s += so;
+  if (dinv) dinv += so;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) s[i] = dinv ? r[i] * dinv[i] : r[i];
+  }
+}
+
+// K2 of a factor-preconditioned iteration: alpha = rho0 / gamma; r -= alpha q
+template <class T>
+__global__ void __launch_bounds__(kNT) k_b_axpy_r(int n, int it, T* __restrict__ r, const T* __restrict__ q,
+                                                  double* part, CGState* st, int nb_prod, Bat B) {
+  __shared__ double sh[kWaves];
+  st += sys_id();
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
+  const long so = sys_id() * B.vs;
+  r += so; q += so;
+  part = sys_part(part);
+  CGSlot& cur = st->slot[it & 1];
+  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
+  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
+  }
+}
+
+// --- mixed-precision outer loop ---------------------------------------------------
+
+// Before FP64 correction k (k = 1, 2, ...): norms of the current residual
+// (partials of k_residual), exit test, history.  The order follows
+// PARCEL_SOLVER_CG_MIXED: at least one correction; after correction k-1 the
+// solve ends when ||r|| / ||b|| < rtol, or when k-1 = itrmax.
+__global__ void __launch_bounds__(kNT) k_bo_check(int k, int itrmax, double rtol, double* part, int nb,
+                                                  OuterState* os, double* hist, int hist_cap) {
+  __shared__ double sh[kWaves];
+  os += sys_id();
+  if (os->done) return;
+  part = sys_part(part);
+  hist += sys_id() * hist_cap;
+  const double rr = reduce_part(part + PART_RR0 * kNBMax, nb, sh);
+  if (k == 1) {
+    const double bb = reduce_part(part + PART_BB * kNBMax, nb, sh);
+    if (threadIdx.x == 0) {
+      os->bnorm = sqrt(bb);
+      os->rnorm = sqrt(rr);
+      if (hist_cap > 0) hist[0] = os->rnorm / os->bnorm;
+    }
+    return;
+  }
+  if (threadIdx.x != 0) return;
+  const double rnorm = sqrt(rr);
+  os->rnorm = rnorm;
+  const double ratio = rnorm / os->bnorm;
+  if (ratio < rtol) {
+    os->done = 1;
+    os->niter = k - 1 < itrmax ? k - 1 : itrmax;
+    os->iret = itrmax <= k - 1 ? 1 : 0;
+  } else if (k - 1 >= itrmax) {
+    os->done = 1;
+    os->niter = itrmax;
+    os->iret = 1;
+  } else if (k - 1 < hist_cap) {
+    hist[k - 1] = ratio;
+  }
+}
+
+// r32 = (float) r, d32 = 0 for systems still refining
+__global__ void __launch_bounds__(kNT) k_bo_tof32(int n, const double* __restrict__ r, float* __restrict__ r32,
+                                                  float* __restrict__ d32, const OuterState* os, Bat B) {
+  if (os[sys_id()].done) return;
+  const long so = sys_id() * B.vs;
+  r += so; r32 += so; d32 += so;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) { r32[i] = (float)r[i]; d32[i] = 0.0f; }
+  }
+}
+
+// inner solves of finished systems are marked ended before they start
+__global__ void k_bo_mask(const OuterState* os, CGState* st, int nsys) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys && os[s].done) { st[s].done_at = -1; st[s].niter = 0; }
+}
+
+// x += (double) d for systems still refining
+__global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, const float* __restrict__ d,
+                                                 const OuterState* os, Bat B) {
+  if (os[sys_id()].done) return;
+  const long so = sys_id() * B.vs;
+  x += so; d += so;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) x[i] = x[i] + (double)d[i];
+  }
+}
+
+// --- host helpers ------------------------------------------------------------------
+
+template <class T>
+void launch_lscan(const Batch& B, const T* F, const T* r, T* w, T* z, const CGState* st, int it) {
+  const int nx = B.nx;
+  const dim3 g(B.nsys), b(64);
+#define PARCEL_LSCAN(E)                                                                                          \
+  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                           \
+    hipLaunchKernelGGL((k_lscan<T, E, true>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, st, it); \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_lscan<T, E, false>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, st, it);
+  if (nx <= 64) { PARCEL_LSCAN(1) }
+  else if (nx <= 128) { PARCEL_LSCAN(2) }
+  else if (nx <= 256) { PARCEL_LSCAN(4) }
+  else if (nx <= 512) { PARCEL_LSCAN(8) }
+  else { PARCEL_LSCAN(16) }
+#undef PARCEL_LSCAN
+  PARCEL_HIP_CHECK(hipGetLastError());
+}
+
+struct EvPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<EvPair> apply, spmv;
+  hipEvent_t rec(hipStream_t st) {
+    if (!on) return nullptr;
+    if (used == pool.size()) {
+      hipEvent_t e;
+      PARCEL_HIP_CHECK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    hipEvent_t e = pool[used++];
+    PARCEL_HIP_CHECK(hipEventRecord(e, st));
+    return e;
+  }
+  void sum(std::vector<EvPair>& v, double& ms, int& calls) {
+    ms = 0.0; calls = 0;
+    for (auto& p : v) {
+      float t = 0.f;
+      PARCEL_HIP_CHECK(hipEventElapsedTime(&t, p.a, p.b));
+      ms += t; calls++;
+    }
+  }
+  ~Prof() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+template <class T>
+struct BVecs {
+  SellArgsT<T> A;
+  T *r, *q, *s, *p0, *p1, *w;
+  const T* dinv;  // point Jacobi (PARCEL_BATCH_JACOBI) or null
+  const T* F;     // factor apply arrays (IC0 / FACTOR) or null
+  double* hist;
+  int hist_cap;
+};
+
+// One batched PCG: every system from its x to ||r||/||b|| < rtol.  mask (may
+// be null): systems whose OuterState is done take no step.  Returns the
+// final CGState of every system in out.
+template <class T>
+void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, double rtol, const OuterState* mask,
+                  std::vector<CGState>& out, Prof& pf, int& launches) {
+  const int n = B.n, nsys = B.nsys;
+  const hipStream_t st = B.stream;
+  const int nb = tiled_nb(n);
+  const dim3 grid(nb, nsys), block(kNT);
+  const Bat bt{(long)n, B.A.entries};
+  CGState* S = B.st.p;
+  double* part = B.part.p;
+  out.assign(nsys, CGState{});
+  if (itrmax <= 0) {
+    for (auto& c : out) { c.niter = 0; c.iret = 1; c.done_at = 1; }
+    return;
+  }
+  PARCEL_HIP_CHECK(hipMemsetAsync(V.p0, 0, sizeof(T) * (size_t)B.nv(), st));
+  hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, V.A, (const T*)x, b, V.r, part, bt);
+  if (V.F)
+    launch_lscan<T>(B, V.F, V.r, V.w, V.s, nullptr, 0);
+  else
+    hipLaunchKernelGGL(k_b_scale<T>, grid, block, 0, st, n, (const T*)V.r, V.dinv, V.s, bt);
+  hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax,
+                     bt);
+  hipLaunchKernelGGL(k_cg_init, dim3(1, nsys), block, 0, st, (const double*)part, nb, S);
+  launches += 4;
+  if (mask) {
+    hipLaunchKernelGGL(k_bo_mask, dim3((nsys + 63) / 64), dim3(64), 0, st, mask, S, nsys);
+    launches++;
+  }
+  PARCEL_HIP_CHECK(hipGetLastError());
+  CGState* h = nullptr;
+  PARCEL_HIP_CHECK(hipHostMalloc((void**)&h, sizeof(CGState) * nsys, hipHostMallocDefault));
+  int enq = 0, chunk = 8;
+  bool all_done = false;
+  while (enq < itrmax + 1) {
+    const int k = std::min(chunk, itrmax + 1 - enq);
+    for (int j = 1; j <= k; ++j) {
+      const int it = enq + j;
+      T* pold = (it - 1) & 1 ? V.p1 : V.p0;
+      T* pnew = it & 1 ? V.p1 : V.p0;
+      hipEvent_t ea = pf.rec(st);
+      hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
+                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+      launches++;
+      if (pf.on) pf.spmv.push_back({ea, pf.rec(st)});
+      if (it > itrmax) break;
+      if (!V.F) {
+        hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, V.dinv, V.s, part,
+                           S, nb, bt);
+        launches++;
+      } else {
+        hipLaunchKernelGGL(k_b_axpy_r<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
+        hipEvent_t eb = pf.rec(st);
+        launch_lscan<T>(B, V.F, V.r, V.w, V.s, S, it);
+        if (pf.on) pf.apply.push_back({eb, pf.rec(st)});
+        hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                           (const CGState*)S, bt);
+        launches += 3;
+      }
+    }
+    PARCEL_HIP_CHECK(hipGetLastError());
+    enq += k;
+    PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState) * nsys, hipMemcpyDeviceToHost, st));
+    PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+    all_done = true;
+    for (int s = 0; s < nsys; ++s) all_done = all_done && h[s].done_at != 0;
+    if (all_done) break;
+    if (chunk < 64) chunk *= 2;
+  }
+  if (!all_done) {
+    PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState) * nsys, hipMemcpyDeviceToHost, st));
+    PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < nsys; ++s)
+      if (!h[s].done_at) {
+        (void)hipHostFree(h);
+        throw hip_error("batched CG loop did not terminate", PARCEL_HIP_ERR_HIP);
+      }
+  }
+  for (int s = 0; s < nsys; ++s) out[s] = h[s];
+  (void)hipHostFree(h);
+}
+
+void ensure_vectors(Batch& B, bool f32) {
+  const size_t nv = (size_t)B.nv();
+  if (B.r.n < nv) {
+    for (DevBuf<double>* v : {&B.r, &B.q, &B.s, &B.p0, &B.p1, &B.w}) v->alloc(nv);
+  }
+  if (f32 && B.r32.n < nv) {
+    for (DevBuf<float>* v : {&B.b32, &B.d32, &B.r32, &B.q32, &B.s32, &B.p032, &B.p132, &B.w32}) v->alloc(nv);
+  }
+  if (!B.part.p) B.part.alloc((size_t)B.nsys * PART_N * kNBMax);
+  if (!B.st.p) B.st.alloc(B.nsys);
+  if (!B.ost.p) B.ost.alloc(B.nsys);
+}
+
+// Five-point natural-order pattern check; sets nx, ny (0 when it is not one).
+void detect_grid(Batch& B) {
+  B.nx = B.ny = 0;
+  const int n = B.n;
+  if (n < 1) return;
+  const int len0 = B.rowptr[1] - B.rowptr[0];
+  int nx;
+  if (n == 1) nx = 1;
+  else if (len0 == 3) nx = B.col[B.rowptr[0] + 2];
+  else if (len0 == 2) nx = n <= 1024 ? n : 1;
+  else return;
+  if (nx < 1 || n % nx != 0) return;
+  const int ny = n / nx;
+  for (int i = 0; i < n; ++i) {
+    const int x = i % nx, y = i / nx;
+    int expect[5], m = 0;
+    if (y > 0) expect[m++] = i - nx;
+    if (x > 0) expect[m++] = i - 1;
+    expect[m++] = i;
+    if (x < nx - 1) expect[m++] = i + 1;
+    if (y < ny - 1) expect[m++] = i + nx;
+    if (B.rowptr[i + 1] - B.rowptr[i] != m) return;
+    for (int j = 0; j < m; ++j)
+      if (B.col[B.rowptr[i] + j] != expect[j]) return;
+  }
+  if (nx > 1024) return;
+  B.nx = nx;
+  B.ny = ny;
+}
+
+// IC(0) on the five-point pattern, per system (natural order):
+//   Ls_i = a(i,i-nx) / Ld_{i-nx},  Lw_i = a(i,i-1) / Ld_{i-1},
+//   Ld_i = sqrt(fma(-Lw, Lw, fma(-Ls, Ls, a_ii)))
+// (no other fill: rows i and i-1, i and i-nx share no earlier column).
+int ic0_host(const Batch& B, const std::vector<double>& vals, std::vector<double>& L) {
+  const int n = B.n, nx = B.nx, nsys = B.nsys;
+  const long nv = (long)nsys * n;
+  L.assign((size_t)(3 * nv), 0.0);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int s = 0; s < nsys; ++s) {
+    const double* v = vals.data() + (long)s * B.nnz;
+    double* Ls = L.data() + (long)s * n;
+    double* Lw = Ls + nv;
+    double* Ld = Ls + 2 * nv;
+    for (int i = 0; i < n; ++i) {
+      const int e0 = B.rowptr[i];
+      int e = e0;
+      double ls = 0.0, lw = 0.0;
+      if (i >= nx) { ls = v[e] / Ld[i - nx]; ++e; }
+      if (i % nx > 0) { lw = v[e] / Ld[i - 1]; ++e; }
+      const double t = std::fma(-lw, lw, std::fma(-ls, ls, v[e]));
+      if (!(t > 0.0)) { bad = 1; break; }
+      Ls[i] = ls;
+      Lw[i] = lw;
+      Ld[i] = std::sqrt(t);
+    }
+  }
+  return bad ? PARCEL_HIP_ERR_ZERO_DIAG : PARCEL_HIP_OK;
+}
+
+}  // namespace
+
+void parcel::derive_factor(Batch& B) {
+  const long nv = B.nv();
+  B.F64.alloc((size_t)(4 * nv));
+  B.F32.alloc((size_t)(4 * nv));
+  const dim3 g((unsigned)std::min<long>((B.n + 255) / 256, 4096), B.nsys), b(256);
+  if (B.factor_src == 1) {
+    hipLaunchKernelGGL((k_b_derive<double, double>), g, b, 0, B.stream, B.n, B.nx, nv, (const double*)B.L64.p,
+                       B.F64.p);
+    hipLaunchKernelGGL((k_b_derive<double, float>), g, b, 0, B.stream, B.n, B.nx, nv, (const double*)B.L64.p,
+                       B.F32.p);
+  } else {
+    hipLaunchKernelGGL((k_b_derive<uint16_t, double>), g, b, 0, B.stream, B.n, B.nx, nv, (const uint16_t*)B.Lbf.p,
+                       B.F64.p);
+    hipLaunchKernelGGL((k_b_derive<uint16_t, float>), g, b, 0, B.stream, B.n, B.nx, nv, (const uint16_t*)B.Lbf.p,
+                       B.F32.p);
+  }
+  PARCEL_HIP_CHECK(hipGetLastError());
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+}
+
+namespace {
+
+template <class T>
+void fetch_host(const T* src, size_t count, std::vector<T>& dst) {
+  dst.resize(count);
+  if (!count) return;
+  if (api_is_device_ptr(src))
+    PARCEL_HIP_CHECK(hipMemcpy(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost));
+  else
+    std::memcpy(dst.data(), src, count * sizeof(T));
+}
+
+int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const int* col, const double* vals,
+                 int precon) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (nsys < 1 || n < 1 || nnz < 1 || !rowptr || !col || !vals)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: empty batch or null array");
+  if (precon < PARCEL_BATCH_NONE || precon > PARCEL_BATCH_FACTOR)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: unknown preconditioner");
+  B.nsys = nsys; B.n = n; B.nnz = nnz; B.precon = precon;
+  B.rowptr.resize(n + 1);
+  B.col.resize(nnz);
+  if (rowptr[0] != 1 || rowptr[n] != nnz + 1)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: rowptr must be 1-based with rowptr[n] = nnz + 1");
+  for (int i = 0; i <= n; ++i) B.rowptr[i] = rowptr[i] - 1;
+  for (int i = 0; i < n; ++i) {
+    if (B.rowptr[i + 1] < B.rowptr[i]) return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: rowptr decreases");
+    bool diag = false;
+    for (int e = B.rowptr[i]; e < B.rowptr[i + 1]; ++e) {
+      const int c = col[e] - 1;
+      if (c < 0 || c >= n) return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: column out of range");
+      B.col[e] = c;
+      diag = diag || c == i;
+    }
+    if (!diag && precon != PARCEL_BATCH_NONE)
+      return api_fail(PARCEL_HIP_ERR_ZERO_DIAG, "batch_create: row without a diagonal entry");
+  }
+  detect_grid(B);
+  if ((precon == PARCEL_BATCH_IC0 || precon == PARCEL_BATCH_FACTOR) && !B.nx)
+    return api_fail(PARCEL_HIP_ERR_UNSUPPORTED,
+                    "batch_create: IC0/FACTOR need a 2D five-point pattern in natural order (lines <= 1024)");
+  B.stats.nx = B.nx;
+  B.stats.ny = B.ny;
+  // SELL-64 layout of the pattern and the slot -> CSR entry map
+  DevSELL& S = B.A;
+  S.n = n;
+  S.nnz = nnz;
+  const long ns = ((long)n + 63) / 64;
+  S.nslices = ns;
+  std::vector<long> sptr(ns + 1, 0);
+  std::vector<int> sk(ns, 0);
+  for (long s = 0; s < ns; ++s) {
+    int k = 0;
+    for (long r = s * 64; r < std::min((long)n, s * 64 + 64); ++r) k = std::max(k, B.rowptr[r + 1] - B.rowptr[r]);
+    sk[s] = k;
+    sptr[s + 1] = sptr[s] + 64L * k;
+  }
+  const long tot = sptr[ns];
+  S.entries = tot;
+  std::vector<int> scol((size_t)tot), idx((size_t)tot);
+  std::vector<int> dslot(n, 0);
+  std::vector<int4> nslot(B.nx ? n : 0, int4{-1, -1, -1, -1});
+  for (long s = 0; s < ns; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const long r = s * 64 + l;
+      const int len = r < n ? B.rowptr[r + 1] - B.rowptr[r] : 0;
+      for (int j = 0; j < sk[s]; ++j) {
+        const long at = sptr[s] + (long)j * 64 + l;
+        if (j < len) {
+          const int e = B.rowptr[r] + j;
+          scol[at] = B.col[e];
+          idx[at] = e;
+          if (B.col[e] == r) dslot[r] = (int)at;
+          if (B.nx) {
+            const int d = B.col[e] - (int)r;
+            if (d == -B.nx) nslot[r].x = (int)at;
+            else if (d == -1) nslot[r].y = (int)at;
+            else if (d == 1) nslot[r].z = (int)at;
+            else if (d == B.nx) nslot[r].w = (int)at;
+          }
+        } else {
+          scol[at] = -1;
+          idx[at] = -1;
+        }
+      }
+    }
+  S.sptr.upload(sptr, B.stream);
+  S.sk.upload(sk, B.stream);
+  S.col.upload(scol, B.stream);
+  B.diag_slot.upload(dslot, B.stream);
+  if (B.nx) B.nbr_slot.upload(nslot, B.stream);
+  DevBuf<int> didx;
+  didx.upload(idx, B.stream);
+  // values: CSR (host or device) -> SELL order on the device
+  const long nvals = (long)nsys * nnz;
+  DevBuf<double> csr;
+  const double* csr_dev = vals;
+  if (!api_is_device_ptr(vals)) {
+    csr.alloc((size_t)nvals);
+    csr.upload(vals, (size_t)nvals, B.stream);
+    csr_dev = csr.p;
+  }
+  S.val.alloc((size_t)(tot * nsys));
+  S.val32.alloc((size_t)(tot * nsys));
+  {
+    const dim3 g((unsigned)std::min<long>((tot + 255) / 256, 4096), nsys), b(256);
+    hipLaunchKernelGGL(k_b_gather, g, b, 0, B.stream, tot, (long)nnz, (const int*)didx.p, csr_dev, S.val.p,
+                       S.val32.p);
+  }
+  if (precon == PARCEL_BATCH_JACOBI) {
+    B.dinv.alloc((size_t)B.nv());
+    B.dinv32.alloc((size_t)B.nv());
+    const dim3 g((unsigned)std::min<long>((n + 255) / 256, 4096), nsys), b(256);
+    hipLaunchKernelGGL(k_b_dinv, g, b, 0, B.stream, n, tot, (const int*)B.diag_slot.p, (const double*)S.val.p,
+                       B.dinv.p, B.dinv32.p);
+  }
+  PARCEL_HIP_CHECK(hipGetLastError());
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  if (precon == PARCEL_BATCH_IC0) {
+    std::vector<double> hv, L;
+    fetch_host(vals, (size_t)nvals, hv);
+    const int rc = ic0_host(B, hv, L);
+    if (rc != PARCEL_HIP_OK) return api_fail(rc, "batch_create: IC(0) breakdown (non-positive pivot)");
+    B.L64.upload(L, B.stream);
+    B.factor_src = 1;
+    derive_factor(B);
+  }
+  B.stats.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return PARCEL_HIP_OK;
+}
+
+template <class T>
+BVecs<T> vecs_of(Batch& B);
+template <>
+BVecs<double> vecs_of<double>(Batch& B) {
+  BVecs<double> V;
+  V.A = SellArgsT<double>{B.A.sptr.p, B.A.sk.p, B.A.col.p, B.A.val.p};
+  V.r = B.r.p; V.q = B.q.p; V.s = B.s.p; V.p0 = B.p0.p; V.p1 = B.p1.p; V.w = B.w.p;
+  V.dinv = B.precon == PARCEL_BATCH_JACOBI ? B.dinv.p : nullptr;
+  V.F = (B.precon == PARCEL_BATCH_IC0 || B.precon == PARCEL_BATCH_FACTOR) ? B.F64.p : nullptr;
+  V.hist = nullptr; V.hist_cap = 0;
+  return V;
+}
+template <>
+BVecs<float> vecs_of<float>(Batch& B) {
+  BVecs<float> V;
+  V.A = SellArgsT<float>{B.A.sptr.p, B.A.sk.p, B.A.col.p, B.A.val32.p};
+  V.r = B.r32.p; V.q = B.q32.p; V.s = B.s32.p; V.p0 = B.p032.p; V.p1 = B.p132.p; V.w = B.w32.p;
+  V.dinv = B.precon == PARCEL_BATCH_JACOBI ? B.dinv32.p : nullptr;
+  V.F = (B.precon == PARCEL_BATCH_IC0 || B.precon == PARCEL_BATCH_FACTOR) ? B.F32.p : nullptr;
+  V.hist = nullptr; V.hist_cap = 0;
+  return V;
+}
+
+void ensure_hist(Batch& B, int cap) {
+  if (B.hist_cap < cap || !B.hist.p) {
+    B.hist.alloc((size_t)B.nsys * (cap > 0 ? cap : 1));
+    B.hist_cap = cap;
+  }
+}
+
+int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, int precision, double* hist_host,
+                int* niter, int* iret) {
+  if (!x || !b) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: null vector");
+  if (precision != 64 && precision != 32) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: precision must be 64 or 32");
+  if (B.precon == PARCEL_BATCH_FACTOR && !B.factor_src)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: FACTOR preconditioner without a factor (batch_set_factor)");
+  auto t0 = std::chrono::steady_clock::now();
+  const hipStream_t st = B.stream;
+  const size_t nv = (size_t)B.nv();
+  ensure_vectors(B, precision == 32);
+  const int cap = itrmax > 0 ? itrmax : 1;
+  ensure_hist(B, cap);
+  PARCEL_HIP_CHECK(hipMemsetAsync(B.hist.p, 0, sizeof(double) * (size_t)B.nsys * B.hist_cap, st));
+  // inputs resident on the device
+  const bool xdev = api_is_device_ptr(x), bdev = api_is_device_ptr(b);
+  if (!B.xb.p || B.xb.n < nv) { B.xb.alloc(nv); B.bb.alloc(nv); }
+  double* dx = xdev ? x : B.xb.p;
+  const double* db = bdev ? b : B.bb.p;
+  if (!xdev) B.xb.upload(x, nv, st);
+  if (!bdev) B.bb.upload(b, nv, st);
+  Prof pf;
+  pf.on = B.profile;
+  hipEvent_t ev0, ev1;
+  PARCEL_HIP_CHECK(hipEventCreate(&ev0));
+  PARCEL_HIP_CHECK(hipEventCreate(&ev1));
+  PARCEL_HIP_CHECK(hipEventRecord(ev0, st));
+  int launches = 0;
+  std::vector<int> nit(B.nsys, 0), ret(B.nsys, 0);
+  B.stats.outer_steps = 0;
+  B.stats.inner_total = 0;
+  B.stats.inner_steps = 0;
+  if (precision == 64) {
+    BVecs<double> V = vecs_of<double>(B);
+    V.hist = B.hist.p;
+    V.hist_cap = B.hist_cap;
+    std::vector<CGState> out;
+    batch_cg_run<double>(B, V, dx, db, itrmax, rtol, nullptr, out, pf, launches);
+    for (int s = 0; s < B.nsys; ++s) { nit[s] = out[s].niter; ret[s] = out[s].iret; }
+  } else if (itrmax <= 0) {
+    for (int s = 0; s < B.nsys; ++s) { nit[s] = 0; ret[s] = 1; }
+  } else {
+    const int n = B.n, nsys = B.nsys;
+    const int nb = tiled_nb(n);
+    const dim3 grid(nb, nsys), block(kNT);
+    const Bat bt{(long)n, B.A.entries};
+    const SellArgsT<double> A64{B.A.sptr.p, B.A.sk.p, B.A.col.p, B.A.val.p};
+    BVecs<float> V = vecs_of<float>(B);
+    PARCEL_HIP_CHECK(hipMemsetAsync(B.ost.p, 0, sizeof(OuterState) * nsys, st));
+    hipLaunchKernelGGL(k_residual<double>, grid, block, 0, st, n, A64, (const double*)dx, db, B.r.p, B.part.p, bt);
+    std::vector<OuterState> hos(nsys);
+    std::vector<CGState> out;
+    for (int k = 1; k <= itrmax + 1; ++k) {
+      hipLaunchKernelGGL(k_bo_check, dim3(1, nsys), block, 0, st, k, itrmax, rtol, B.part.p, nb, B.ost.p, B.hist.p,
+                         B.hist_cap);
+      launches += 2;
+      PARCEL_HIP_CHECK(hipMemcpyAsync(hos.data(), B.ost.p, sizeof(OuterState) * nsys, hipMemcpyDeviceToHost, st));
+      PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+      bool all = true;
+      for (auto& o : hos) all = all && o.done;
+      if (all) break;
+      hipLaunchKernelGGL(k_bo_tof32, grid, block, 0, st, n, (const double*)B.r.p, B.b32.p,
+                         B.d32.p, (const OuterState*)B.ost.p, bt);
+      batch_cg_run<float>(B, V, B.d32.p, B.b32.p, B.inner_itrmax, B.inner_rtol, B.ost.p, out, pf, launches);
+      int most = 0;
+      for (int s = 0; s < nsys; ++s) {
+        B.stats.inner_total += out[s].niter;
+        most = std::max(most, out[s].niter);
+      }
+      B.stats.inner_steps += most;
+      B.stats.outer_steps++;
+      hipLaunchKernelGGL(k_bo_xadd, grid, block, 0, st, n, dx, (const float*)B.d32.p, (const OuterState*)B.ost.p, bt);
+      hipLaunchKernelGGL(k_residual<double>, grid, block, 0, st, n, A64, (const double*)dx, db, B.r.p, B.part.p, bt);
+      launches += 3;
+    }
+    PARCEL_HIP_CHECK(hipGetLastError());
+    for (int s = 0; s < nsys; ++s) { nit[s] = hos[s].niter; ret[s] = hos[s].iret; }
+  }
+  PARCEL_HIP_CHECK(hipEventRecord(ev1, st));
+  if (!xdev) PARCEL_HIP_CHECK(hipMemcpyAsync(x, dx, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  PARCEL_HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  int mx = 0;
+  for (int s = 0; s < B.nsys; ++s) {
+    if (niter) niter[s] = nit[s];
+    if (iret) iret[s] = ret[s];
+    mx = std::max(mx, nit[s]);
+  }
+  if (hist_host && itrmax > 0) {
+    std::vector<double> h((size_t)B.nsys * B.hist_cap);
+    PARCEL_HIP_CHECK(hipMemcpy(h.data(), B.hist.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < B.nsys; ++s)
+      std::memcpy(hist_host + (long)s * itrmax, h.data() + (long)s * B.hist_cap, sizeof(double) * itrmax);
+  }
+  B.stats.solve_device_ms = ms;
+  B.stats.solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  B.stats.max_niter = mx;
+  B.stats.launches = launches;
+  if (pf.on) {
+    pf.sum(pf.apply, B.stats.apply_ms, B.stats.apply_calls);
+    pf.sum(pf.spmv, B.stats.spmv_ms, B.stats.spmv_calls);
+  } else {
+    B.stats.apply_ms = 0.0; B.stats.apply_calls = 0;
+    B.stats.spmv_ms = 0.0; B.stats.spmv_calls = 0;
+  }
+  return PARCEL_HIP_OK;
+}
+
+#define PARCEL_API_BEGIN try {
+#define PARCEL_API_END                                                    \
+  }                                                                       \
+  catch (const hip_error& e) {                                            \
+    return api_fail(e.code < 0 ? e.code : PARCEL_HIP_ERR_HIP, e.what());  \
+  }                                                                       \
+  catch (const std::exception& e) {                                       \
+    return api_fail(PARCEL_HIP_ERR_HIP, e.what());                        \
+  }
+
+}  // namespace
+
+extern "C" {
+
+int parcel_hip_batch_create(parcel_hip_batch** out, int nsys, int n, int nnz, const int* rowptr, const int* col,
+                            const double* vals, int precon) {
+  PARCEL_API_BEGIN
+  if (!out) return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: null handle");
+  *out = nullptr;
+  int rc = api_ensure_init();
+  if (rc != PARCEL_HIP_OK) return rc;
+  auto* h = new parcel_hip_batch;
+  h->B.device = api_device();
+  PARCEL_HIP_CHECK(hipSetDevice(h->B.device));
+  try {
+    PARCEL_HIP_CHECK(hipStreamCreateWithFlags(&h->B.stream, hipStreamNonBlocking));
+    rc = batch_create(h->B, nsys, n, nnz, rowptr, col, vals, precon);
+  } catch (...) {
+    if (h->B.stream) (void)hipStreamDestroy(h->B.stream);
+    delete h;
+    throw;
+  }
+  if (rc != PARCEL_HIP_OK) {
+    (void)hipStreamDestroy(h->B.stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_set_factor(parcel_hip_batch* h, const unsigned short* Ls, const unsigned short* Lw,
+                                const unsigned short* Ld) {
+  PARCEL_API_BEGIN
+  if (!h || !Ls || !Lw || !Ld) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_factor: null argument");
+  Batch& B = h->B;
+  if (B.precon != PARCEL_BATCH_FACTOR)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_factor: the batch was not created with PARCEL_BATCH_FACTOR");
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  const long nv = B.nv();
+  B.Lbf.alloc((size_t)(3 * nv));
+  const unsigned short* src[3] = {Ls, Lw, Ld};
+  for (int k = 0; k < 3; ++k)
+    PARCEL_HIP_CHECK(hipMemcpyAsync(B.Lbf.p + k * nv, src[k], sizeof(uint16_t) * nv,
+                                    api_is_device_ptr(src[k]) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                    B.stream));
+  B.factor_src = 2;
+  derive_factor(B);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_solve(parcel_hip_batch* h, double* x, const double* b, int itrmax, double rtol, int precision,
+                           double* hist, int* niter, int* iret) {
+  PARCEL_API_BEGIN
+  if (!h) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: null handle");
+  PARCEL_HIP_CHECK(hipSetDevice(h->B.device));
+  return batch_solve(h->B, x, b, itrmax, rtol, precision, hist, niter, iret);
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_set_inner(parcel_hip_batch* h, double inner_rtol, int inner_itrmax) {
+  if (!h || !(inner_rtol > 0.0) || inner_itrmax < 1) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_inner: bad argument");
+  h->B.inner_rtol = inner_rtol;
+  h->B.inner_itrmax = inner_itrmax;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_set_profile(parcel_hip_batch* h, int on) {
+  if (!h) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_profile: null handle");
+  h->B.profile = on != 0;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_get_stats(const parcel_hip_batch* h, parcel_hip_batch_stats* s) {
+  if (!h || !s) return api_fail(PARCEL_HIP_ERR_ARG, "batch_stats: null argument");
+  *s = h->B.stats;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_spmv(parcel_hip_batch* h, const double* x, double* y) {
+  PARCEL_API_BEGIN
+  if (!h || !x || !y) return api_fail(PARCEL_HIP_ERR_ARG, "batch_spmv: null argument");
+  Batch& B = h->B;
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  const size_t nv = (size_t)B.nv();
+  DevBuf<double> tx, ty;
+  const double* dx = x;
+  double* dy = y;
+  if (!api_is_device_ptr(x)) { tx.alloc(nv); tx.upload(x, nv, B.stream); dx = tx.p; }
+  const bool ydev = api_is_device_ptr(y);
+  if (!ydev) { ty.alloc(nv); dy = ty.p; }
+  const SellArgsT<double> A{B.A.sptr.p, B.A.sk.p, B.A.col.p, B.A.val.p};
+  hipLaunchKernelGGL(k_spmv<double>, dim3(tiled_nb(B.n), B.nsys), dim3(kNT), 0, B.stream, B.n, A, dx, dy,
+                     Bat{(long)B.n, B.A.entries});
+  PARCEL_HIP_CHECK(hipGetLastError());
+  if (!ydev) PARCEL_HIP_CHECK(hipMemcpyAsync(y, dy, sizeof(double) * nv, hipMemcpyDeviceToHost, B.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+int precon_apply_any(parcel_hip_batch* h, const T* r, T* z) {
+  if (!h || !r || !z) return api_fail(PARCEL_HIP_ERR_ARG, "batch_precon_apply: null argument");
+  Batch& B = h->B;
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  if (B.precon == PARCEL_BATCH_FACTOR && !B.factor_src)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_precon_apply: no factor set");
+  const size_t nv = (size_t)B.nv();
+  DevBuf<T> tr, tz, tw;
+  const T* dr = r;
+  T* dz = z;
+  if (!api_is_device_ptr(r)) { tr.alloc(nv); tr.upload(r, nv, B.stream); dr = tr.p; }
+  const bool zdev = api_is_device_ptr(z);
+  if (!zdev) { tz.alloc(nv); dz = tz.p; }
+  BVecs<T> V = vecs_of<T>(B);
+  if (V.F) {
+    tw.alloc(nv);
+    launch_lscan<T>(B, V.F, dr, tw.p, dz, nullptr, 0);
+  } else {
+    hipLaunchKernelGGL(k_b_scale<T>, dim3(tiled_nb(B.n), B.nsys), dim3(kNT), 0, B.stream, B.n, dr, V.dinv, dz,
+                       Bat{(long)B.n, B.A.entries});
+  }
+  PARCEL_HIP_CHECK(hipGetLastError());
+  if (!zdev) PARCEL_HIP_CHECK(hipMemcpyAsync(z, dz, sizeof(T) * nv, hipMemcpyDeviceToHost, B.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  return PARCEL_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int parcel_hip_batch_precon_apply(parcel_hip_batch* h, const double* r, double* z) {
+  PARCEL_API_BEGIN
+  return precon_apply_any<double>(h, r, z);
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_precon_apply32(parcel_hip_batch* h, const float* r, float* z) {
+  PARCEL_API_BEGIN
+  return precon_apply_any<float>(h, r, z);
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_get_factor(const parcel_hip_batch* h, unsigned short* Ls, unsigned short* Lw,
+                                unsigned short* Ld) {
+  PARCEL_API_BEGIN
+  if (!h || !Ls || !Lw || !Ld) return api_fail(PARCEL_HIP_ERR_ARG, "batch_get_factor: null argument");
+  const Batch& B = h->B;
+  if (B.factor_src != 2) return api_fail(PARCEL_HIP_ERR_ARG, "batch_get_factor: no BF16 factor set");
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  const long nv = B.nv();
+  unsigned short* dst[3] = {Ls, Lw, Ld};
+  for (int k = 0; k < 3; ++k)
+    PARCEL_HIP_CHECK(hipMemcpyAsync(dst[k], B.Lbf.p + k * nv, sizeof(uint16_t) * nv, hipMemcpyDeviceToHost, B.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_batch_destroy(parcel_hip_batch* h) {
+  if (!h) return PARCEL_HIP_OK;
+  (void)hipSetDevice(h->B.device);
+  (void)hipStreamSynchronize(h->B.stream);
+  hipStream_t s = h->B.stream;
+  delete h;
+  if (s) (void)hipStreamDestroy(s);
+  return PARCEL_HIP_OK;
+}
+
+}  // extern "C"

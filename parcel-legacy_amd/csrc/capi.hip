@@ -111,6 +111,13 @@ void unset_param(parcel_hip_param& prm) {
 
 }  // namespace
 
+namespace parcel {
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+int api_ensure_init() { return ensure_init(); }
+int api_device() { return g_device; }
+bool api_is_device_ptr(const void* p) { return is_device_ptr(p); }
+}  // namespace parcel
+
 extern "C" {
 
 int parcel_hip_init(int myrank) {

@@ -1,0 +1,287 @@
+// cg_kernels.hpp -- the CG kernels, shared by the single-system solver
+// (krylov.hip) and the batched solver (batched.hip).
+//
+// Every kernel takes a Bat: the per-system strides of vectors (vs) and of
+// matrix values (as).  A batched launch has gridDim.y systems; system
+// y = blockIdx.y works on vectors at y*vs, matrix values at y*as, partial
+// sums at y*PART_N*kNBMax, its own CGState and history.  A single-system
+// launch passes {0, 0} and gridDim.y = 1, so every offset is zero.  The
+// arithmetic of a system never depends on the others.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace parcel {
+
+struct Bat {
+  long vs;  // vector stride between systems (0: one system)
+  long as;  // matrix-value stride between systems
+};
+
+namespace {
+
+__device__ __forceinline__ long sys_id() { return (long)blockIdx.y; }
+
+template <class T>
+__device__ __forceinline__ SellArgsT<T> sys_matrix(SellArgsT<T> A, const Bat& B) {
+  A.val += sys_id() * B.as;
+  return A;
+}
+
+__device__ __forceinline__ double* sys_part(double* part) { return part + sys_id() * (PART_N * kNBMax); }
+
+// r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,
+// src/krylov_h.f90:343; pcg.f90:456-470)
+// (T: vector and matrix value type; sums of products accumulate in double)
+template <class T>
+__global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T* __restrict__ x,
+                                                  const T* __restrict__ b, T* __restrict__ r, double* part,
+                                                  Bat B) {
+  __shared__ double sh[kWaves];
+  const long so = sys_id() * B.vs;
+  A = sys_matrix(A, B);
+  x += so; b += so; r += so;
+  part = sys_part(part);
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double a0 = 0.0, a1 = 0.0;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) {
+      const T t = sell_row(A, i, [&](int c) { return x[c]; });
+      const T bi = b[i];
+      const T ri = bi - t;
+      r[i] = ri;
+      a0 = fma((double)bi, (double)bi, a0);
+      a1 = fma((double)ri, (double)ri, a1);
+    }
+  }
+  double s0 = block_sum(a0, sh);
+  double s1 = block_sum(a1, sh);
+  if (threadIdx.x == 0) {
+    part[PART_BB * kNBMax + blockIdx.x] = s0;
+    part[PART_RR0 * kNBMax + blockIdx.x] = s1;
+  }
+}
+
+// partials of <x,y> into part[slot] (slot offset already applied by the
+// caller; systems are PART_N*kNBMax apart)
+template <class T>
+__global__ void __launch_bounds__(kNT) k_dot_part(int n, const T* __restrict__ x, const T* __restrict__ y,
+                                                  double* part, Bat B) {
+  __shared__ double sh[kWaves];
+  const long so = sys_id() * B.vs;
+  x += so; y += so;
+  part = sys_part(part);
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double a0 = 0.0;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) a0 = fma((double)x[i], (double)y[i], a0);
+  }
+  double s0 = block_sum(a0, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+}
+
+__global__ void __launch_bounds__(kNT) k_reduce_one(const double* part, int nb, double* out) {
+  __shared__ double sh[kWaves];
+  double v = reduce_part(part, nb, sh);
+  if (threadIdx.x == 0) *out = v;
+}
+
+// bnorm, rnorm, rho0 of the initial residual (pcg.f90:456-490); slot 0.
+// One workgroup per system.
+__global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGState* st) {
+  __shared__ double sh[kWaves];
+  part = sys_part(const_cast<double*>(part));
+  st += sys_id();
+  const double bb = reduce_part(part + PART_BB * kNBMax, nb, sh);
+  const double rr = reduce_part(part + PART_RR0 * kNBMax, nb, sh);
+  const double rho0 = reduce_part(part + PART_RHO0 * kNBMax, nb, sh);
+  if (threadIdx.x == 0) {
+    st->bnorm = sqrt(bb);
+    st->done_at = 0;
+    st->niter = 0;
+    st->iret = 0;
+    CGSlot& s = st->slot[0];
+    s.rho0 = rho0;
+    s.alpha = 0.0;
+    s.beta = 0.0;
+    s.rnorm = sqrt(rr);
+    s.itr = 0;
+  }
+}
+
+// K1 of iteration `it`
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgsT<T> A,
+                                                 T* __restrict__ x, const T* __restrict__ p_old,
+                                                 const T* __restrict__ s, T* __restrict__ p_new,
+                                                 T* __restrict__ q, double* part, CGState* st,
+                                                 double* hist, int hist_cap, Bat B) {
+  __shared__ double sh[kWaves];
+  st += sys_id();
+  {
+    const int d = st->done_at;
+    if (d != 0 && d < it) return;
+  }
+  const long so = sys_id() * B.vs;
+  A = sys_matrix(A, B);
+  x += so; p_old += so; s += so; p_new += so; q += so;
+  part = sys_part(part);
+  hist += sys_id() * hist_cap;
+  const CGSlot& prev = st->slot[(it - 1) & 1];
+  const double bnorm = st->bnorm;
+  double alpha_prev, beta, rho0, rnorm;
+  bool exit_loop = false;
+  const int nb = gridDim.x;
+  if (it == 1) {
+    alpha_prev = 0.0;
+    beta = 0.0;
+    rho0 = prev.rho0;
+    rnorm = prev.rnorm;
+  } else {
+    // pcg.f90:640-700: rnorm = sqrt(<r,r>); exit tests; beta = rho1/rho0
+    const double rho1 = reduce_part(part + PART_RHO1 * kNBMax, nb, sh);
+    const double rr = reduce_part(part + PART_RR * kNBMax, nb, sh);
+    rnorm = sqrt(rr);
+    const double ratio = rnorm / bnorm;
+    alpha_prev = prev.alpha;
+    exit_loop = ratio < rtol || ratio > 1.0e8 || it - 1 >= itrmax;
+    beta = rho1 / prev.rho0;
+    rho0 = rho1;
+  }
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  if (exit_loop) {
+    // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
+    for (long k = blockIdx.x; k < ntiles; k += nb) {
+      const long i = k * kNT + threadIdx.x;
+      if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done_at = it;
+      st->niter = it - 1;                       // pcg.f90:850
+      st->iret = (it - 1 >= itrmax) ? 1 : 0;    // pcg.f90:851-855
+      st->slot[it & 1].rnorm = rnorm;
+      st->slot[it & 1].itr = it - 1;
+    }
+    return;
+  }
+  double acc = 0.0;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) {
+      const T po = p_old[i];
+      x[i] = fma((T)alpha_prev, po, x[i]);
+      const T pn = fma((T)beta, po, s[i]);
+      p_new[i] = pn;
+      const T t = sell_row(A, i, [&](int c) { return fma((T)beta, p_old[c], s[c]); });
+      q[i] = t;
+      acc = fma((double)pn, (double)t, acc);
+    }
+  }
+  double s0 = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + blockIdx.x] = s0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it - 1 < hist_cap) hist[it - 1] = rnorm / bnorm;  // pcg.f90:512-518
+    CGSlot& cur = st->slot[it & 1];
+    cur.rho0 = rho0;
+    cur.beta = beta;
+    cur.rnorm = rnorm;
+    cur.itr = it;
+  }
+}
+
+// K2 of iteration `it` (point Jacobi or none): alpha; r -= alpha q;
+// s = Dinv r; (rho1, rr) partials.  dinv == nullptr: s = r.
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restrict__ r, const T* __restrict__ q,
+                                                      const T* __restrict__ dinv, T* __restrict__ s, double* part,
+                                                      CGState* st, int nb_prod, Bat B) {
+  __shared__ double sh[kWaves];
+  st += sys_id();
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
+  const long so = sys_id() * B.vs;
+  r += so; q += so; s += so;
+  if (dinv) dinv += so;
+  part = sys_part(part);
+  CGSlot& cur = st->slot[it & 1];
+  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
+  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double a0 = 0.0, a1 = 0.0;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) {
+      const T ri = fma(-(T)alpha, q[i], r[i]);
+      r[i] = ri;
+      const T si = dinv ? ri * dinv[i] : ri;
+      s[i] = si;
+      a0 = fma((double)si, (double)ri, a0);
+      a1 = fma((double)ri, (double)ri, a1);
+    }
+  }
+  double s0 = block_sum(a0, sh);
+  double s1 = block_sum(a1, sh);
+  if (threadIdx.x == 0) {
+    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
+    part[PART_RR * kNBMax + blockIdx.x] = s1;
+  }
+}
+
+// rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30)
+template <class T>
+__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restrict__ s, const T* __restrict__ r,
+                                                 double* part, const CGState* st, Bat B) {
+  __shared__ double sh[kWaves];
+  st += sys_id();
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
+  const long so = sys_id() * B.vs;
+  s += so; r += so;
+  part = sys_part(part);
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double a0 = 0.0, a1 = 0.0;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) {
+      const T ri = r[i];
+      a0 = fma((double)s[i], (double)ri, a0);
+      a1 = fma((double)ri, (double)ri, a1);
+    }
+  }
+  double s0 = block_sum(a0, sh);
+  double s1 = block_sum(a1, sh);
+  if (threadIdx.x == 0) {
+    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
+    part[PART_RR * kNBMax + blockIdx.x] = s1;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgsT<T> A, const T* __restrict__ x, T* __restrict__ y,
+                                              Bat B) {
+  const long so = sys_id() * B.vs;
+  A = sys_matrix(A, B);
+  x += so; y += so;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long k = blockIdx.x; k < ntiles; k += nb) {
+    const long i = k * kNT + threadIdx.x;
+    if (i < n) y[i] = sell_row(A, i, [&](int c) { return x[c]; });
+  }
+}
+
+constexpr Bat kOne{0, 0};
+
+}  // namespace
+}  // namespace parcel

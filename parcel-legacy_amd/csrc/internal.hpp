@@ -2,6 +2,7 @@
 // translation units of libparcel_hip.so (not part of the public C ABI).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -235,5 +236,11 @@ void launch_residual(Plan& P, const double* x, const double* b, double* r, doubl
 void spmv(Plan& P, const double* x, double* y);
 double dot_device(int n, const double* x, const double* y, hipStream_t st);
 double time_kernel(Plan& P, int which, int reps);
+
+// C-API plumbing shared with the batched entry points (capi.hip)
+int api_fail(int code, const std::string& msg);
+int api_ensure_init();
+int api_device();
+bool api_is_device_ptr(const void* p);
 
 }  // namespace parcel

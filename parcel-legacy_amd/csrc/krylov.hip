@@ -29,7 +29,7 @@
 #include <cmath>
 #include <vector>
 
-#include "kernels.hpp"
+#include "cg_kernels.hpp"
 
 namespace parcel {
 
@@ -86,229 +86,6 @@ void sell_build(Plan& P) {
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 
-namespace {
-
-// r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,
-// src/krylov_h.f90:343; pcg.f90:456-470)
-// (T: vector and matrix value type; sums of products accumulate in double)
-template <class T>
-__global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T* __restrict__ x,
-                                                  const T* __restrict__ b, T* __restrict__ r, double* part) {
-  __shared__ double sh[kWaves];
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T t = sell_row(A, i, [&](int c) { return x[c]; });
-      const T bi = b[i];
-      const T ri = bi - t;
-      r[i] = ri;
-      a0 = fma((double)bi, (double)bi, a0);
-      a1 = fma((double)ri, (double)ri, a1);
-    }
-  }
-  double s0 = block_sum(a0, sh);
-  double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) {
-    part[PART_BB * kNBMax + blockIdx.x] = s0;
-    part[PART_RR0 * kNBMax + blockIdx.x] = s1;
-  }
-}
-
-// partials of <x,y> into part[slot]
-template <class T>
-__global__ void __launch_bounds__(kNT) k_dot_part(int n, const T* __restrict__ x, const T* __restrict__ y,
-                                                  double* part) {
-  __shared__ double sh[kWaves];
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) a0 = fma((double)x[i], (double)y[i], a0);
-  }
-  double s0 = block_sum(a0, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s0;
-}
-
-__global__ void __launch_bounds__(kNT) k_reduce_one(const double* part, int nb, double* out) {
-  __shared__ double sh[kWaves];
-  double v = reduce_part(part, nb, sh);
-  if (threadIdx.x == 0) *out = v;
-}
-
-// bnorm, rnorm, rho0 of the initial residual (pcg.f90:456-490); slot 0.
-__global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGState* st) {
-  __shared__ double sh[kWaves];
-  const double bb = reduce_part(part + PART_BB * kNBMax, nb, sh);
-  const double rr = reduce_part(part + PART_RR0 * kNBMax, nb, sh);
-  const double rho0 = reduce_part(part + PART_RHO0 * kNBMax, nb, sh);
-  if (threadIdx.x == 0) {
-    st->bnorm = sqrt(bb);
-    st->done_at = 0;
-    st->niter = 0;
-    st->iret = 0;
-    CGSlot& s = st->slot[0];
-    s.rho0 = rho0;
-    s.alpha = 0.0;
-    s.beta = 0.0;
-    s.rnorm = sqrt(rr);
-    s.itr = 0;
-  }
-}
-
-// K1 of iteration `it`
-template <class T>
-__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgsT<T> A,
-                                                 T* __restrict__ x, const T* __restrict__ p_old,
-                                                 const T* __restrict__ s, T* __restrict__ p_new,
-                                                 T* __restrict__ q, double* part, CGState* st,
-                                                 double* hist, int hist_cap) {
-  __shared__ double sh[kWaves];
-  {
-    const int d = st->done_at;
-    if (d != 0 && d < it) return;
-  }
-  const CGSlot& prev = st->slot[(it - 1) & 1];
-  const double bnorm = st->bnorm;
-  double alpha_prev, beta, rho0, rnorm;
-  bool exit_loop = false;
-  const int nb = gridDim.x;
-  if (it == 1) {
-    alpha_prev = 0.0;
-    beta = 0.0;
-    rho0 = prev.rho0;
-    rnorm = prev.rnorm;
-  } else {
-    // pcg.f90:640-700: rnorm = sqrt(<r,r>); exit tests; beta = rho1/rho0
-    const double rho1 = reduce_part(part + PART_RHO1 * kNBMax, nb, sh);
-    const double rr = reduce_part(part + PART_RR * kNBMax, nb, sh);
-    rnorm = sqrt(rr);
-    const double ratio = rnorm / bnorm;
-    alpha_prev = prev.alpha;
-    exit_loop = ratio < rtol || ratio > 1.0e8 || it - 1 >= itrmax;
-    beta = rho1 / prev.rho0;
-    rho0 = rho1;
-  }
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  if (exit_loop) {
-    // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
-    for (long k = blockIdx.x; k < ntiles; k += nb) {
-      const long i = k * kNT + threadIdx.x;
-      if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->done_at = it;
-      st->niter = it - 1;                       // pcg.f90:850
-      st->iret = (it - 1 >= itrmax) ? 1 : 0;    // pcg.f90:851-855
-      st->slot[it & 1].rnorm = rnorm;
-      st->slot[it & 1].itr = it - 1;
-    }
-    return;
-  }
-  double acc = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T po = p_old[i];
-      x[i] = fma((T)alpha_prev, po, x[i]);
-      const T pn = fma((T)beta, po, s[i]);
-      p_new[i] = pn;
-      const T t = sell_row(A, i, [&](int c) { return fma((T)beta, p_old[c], s[c]); });
-      q[i] = t;
-      acc = fma((double)pn, (double)t, acc);
-    }
-  }
-  double s0 = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + blockIdx.x] = s0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (it - 1 < hist_cap) hist[it - 1] = rnorm / bnorm;  // pcg.f90:512-518
-    CGSlot& cur = st->slot[it & 1];
-    cur.rho0 = rho0;
-    cur.beta = beta;
-    cur.rnorm = rnorm;
-    cur.itr = it;
-  }
-}
-
-// K2 of iteration `it` (point Jacobi or none): alpha; r -= alpha q;
-// s = Dinv r; (rho1, rr) partials
-template <class T>
-__global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restrict__ r, const T* __restrict__ q,
-                                                      const T* __restrict__ dinv, T* __restrict__ s, double* part,
-                                                      CGState* st, int nb_prod) {
-  __shared__ double sh[kWaves];
-  {
-    const int d = st->done_at;
-    if (d != 0 && d <= it) return;
-  }
-  CGSlot& cur = st->slot[it & 1];
-  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
-  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
-  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T ri = fma(-(T)alpha, q[i], r[i]);
-      r[i] = ri;
-      const T si = dinv ? ri * dinv[i] : ri;
-      s[i] = si;
-      a0 = fma((double)si, (double)ri, a0);
-      a1 = fma((double)ri, (double)ri, a1);
-    }
-  }
-  double s0 = block_sum(a0, sh);
-  double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) {
-    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
-    part[PART_RR * kNBMax + blockIdx.x] = s1;
-  }
-}
-
-// rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30)
-template <class T>
-__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restrict__ s, const T* __restrict__ r,
-                                                 double* part, const CGState* st) {
-  __shared__ double sh[kWaves];
-  {
-    const int d = st->done_at;
-    if (d != 0 && d <= it) return;
-  }
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T ri = r[i];
-      a0 = fma((double)s[i], (double)ri, a0);
-      a1 = fma((double)ri, (double)ri, a1);
-    }
-  }
-  double s0 = block_sum(a0, sh);
-  double s1 = block_sum(a1, sh);
-  if (threadIdx.x == 0) {
-    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
-    part[PART_RR * kNBMax + blockIdx.x] = s1;
-  }
-}
-
-__global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgs A, const double* __restrict__ x,
-                                              double* __restrict__ y) {
-  const int nb = gridDim.x;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) y[i] = sell_row(A, i, [&](int c) { return x[c]; });
-  }
-}
-
-}  // namespace
 
 void ensure_vecs(Plan& P, int k) {
   const int n = P.A.n;
@@ -317,7 +94,7 @@ void ensure_vecs(Plan& P, int k) {
 
 void launch_residual(Plan& P, const double* x, const double* b, double* r, double* part) {
   const int n = P.A.n;
-  hipLaunchKernelGGL(k_residual<double>, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, b, r, part);
+  hipLaunchKernelGGL(k_residual<double>, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, b, r, part, kOne);
 }
 
 void plan_workspace(Plan& P, int itrmax) {
@@ -335,14 +112,14 @@ void plan_workspace(Plan& P, int itrmax) {
 
 void spmv(Plan& P, const double* x, double* y) {
   const int n = P.A.n;
-  hipLaunchKernelGGL(k_spmv, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, y);
+  hipLaunchKernelGGL(k_spmv<double>, dim3(tiled_nb(n)), dim3(kNT), 0, P.stream, n, sell_args(P.A), x, y, kOne);
   PARCEL_HIP_CHECK(hipGetLastError());
 }
 
 double dot_device(int n, const double* x, const double* y, hipStream_t st) {
   DevBuf<double> part(kNBMax + 1);
   const int nb = tiled_nb(n);
-  hipLaunchKernelGGL(k_dot_part<double>, dim3(nb), dim3(kNT), 0, st, n, x, y, part.p);
+  hipLaunchKernelGGL(k_dot_part<double>, dim3(nb), dim3(kNT), 0, st, n, x, y, part.p, kOne);
   hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kNT), 0, st, (const double*)part.p, nb, part.p + kNBMax);
   PARCEL_HIP_CHECK(hipGetLastError());
   double h = 0.0;
@@ -413,9 +190,9 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   }
 
   PARCEL_HIP_CHECK(hipMemsetAsync(V.p0, 0, sizeof(T) * (size_t)n, st));
-  hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, A, (const T*)x, b, V.r, part);
+  hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, A, (const T*)x, b, V.r, part, kOne);
   V.precon(P, V.r, V.s, nullptr);
-  hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax);
+  hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax, kOne);
   hipLaunchKernelGGL(k_cg_init, dim3(1), block, 0, st, (const double*)part, nb, S);
   launches += 4;
   PARCEL_HIP_CHECK(hipGetLastError());
@@ -435,14 +212,14 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
       T* pnew = it & 1 ? V.p1 : V.p0;
       hipEvent_t ea = Tm.on ? Tm.rec(st) : nullptr;
       hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, A, x, (const T*)pold,
-                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap);
+                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
       launches++;
       hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
       if (Tm.on) Tm.spmv.push_back({ea, eb, it});
       if (it > itrmax) break;  // the last K1 only closes the loop
       if (!bj) {
         hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, dinv, V.s, part, S,
-                           nb);
+                           nb, kOne);
         launches++;
         if (Tm.on) Tm.vec.push_back({eb, Tm.rec(st), it});
       } else {
@@ -457,7 +234,7 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
         hipEvent_t ec = Tm.on ? Tm.rec(st) : nullptr;
         if (Tm.on) Tm.prec.push_back({eb, ec, it});
         hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
-                           (const CGState*)S);
+                           (const CGState*)S, kOne);
         launches += 2;
         if (Tm.on) Tm.vec.push_back({ec, Tm.rec(st), it});
       }
@@ -529,9 +306,9 @@ double time_kernel(Plan& P, int which, int reps) {
   auto launch = [&](int i) {
     if (which == 1) {
       // iteration 1 semantics every time (no exit test), no history write
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kNT), 0, st, n, 1, 1 << 30, 0.0, sell_args(P.A), P.x.p,
+      hipLaunchKernelGGL(k_cg_spmv<double>, dim3(nb), dim3(kNT), 0, st, n, 1, 1 << 30, 0.0, sell_args(P.A), P.x.p,
                          (i & 1) ? P.p1.p : P.p0.p, P.s.p, (i & 1) ? P.p0.p : P.p1.p, P.q.p, P.part.p, P.cg.p,
-                         P.hist.p, 0);
+                         P.hist.p, 0, kOne);
     } else {
       precon_apply(P, P.r.p, P.s.p, nullptr);
     }

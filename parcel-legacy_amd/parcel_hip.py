@@ -115,7 +115,57 @@ SYMBOLS = {
     "parcel_gen_poisson2d": (_I, [_I, _I, _D, _IP, _P, _P, _P]),
     "parcel_gen_poisson3d_spd": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
     "parcel_gen_uniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
+    "parcel_gen_normal": (_I, [ctypes.c_long, ctypes.c_ulonglong, _P]),
+    "parcel_gen_grf2d": (_I, [_I, _I, _D, _D, ctypes.c_ulonglong, _P]),
+    "parcel_gen_diffusion2d": (_I, [_I, _I, _P, _IP, _P, _P, _P]),
+    "parcel_gen_grf_diffusion2d": (_I, [_I, _I, _D, _D, ctypes.c_ulonglong, _I, _IP, _P, _P, _P, _P]),
+    "parcel_hip_batch_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _P, _P, _P, _I]),
+    "parcel_hip_batch_set_factor": (_I, [_P, _P, _P, _P]),
+    "parcel_hip_batch_solve": (_I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
+    "parcel_hip_batch_set_inner": (_I, [_P, _D, _I]),
+    "parcel_hip_batch_set_profile": (_I, [_P, _I]),
+    "parcel_hip_batch_get_stats": (_I, [_P, _P]),
+    "parcel_hip_batch_spmv": (_I, [_P, _P, _P]),
+    "parcel_hip_batch_precon_apply": (_I, [_P, _P, _P]),
+    "parcel_hip_batch_precon_apply32": (_I, [_P, _P, _P]),
+    "parcel_hip_batch_destroy": (_I, [_P]),
+    "parcel_hip_batch_get_factor": (_I, [_P, _P, _P, _P]),
+    "parcel_hip_mpnn_nparams": (_I, []),
+    "parcel_hip_mpnn_create": (_I, [ctypes.POINTER(_P), _P, _I]),
+    "parcel_hip_mpnn_factor": (_I, [_P, _P]),
+    "parcel_hip_mpnn_inspect": (_I, [_P, _P, _P, _P, _P, _P]),
+    "parcel_hip_mpnn_destroy": (_I, [_P]),
+    "parcel_hip_mpnn_trainer_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _P]),
+    "parcel_hip_mpnn_trainer_add": (_I, [_P, _P, _P, _P]),
+    "parcel_hip_mpnn_trainer_grad": (_I, [_P, _P, _I, _P, _DP]),
+    "parcel_hip_mpnn_trainer_step": (_I, [_P, _P, _I, ctypes.c_float, _DP]),
+    "parcel_hip_mpnn_trainer_params": (_I, [_P, _P]),
+    "parcel_hip_mpnn_trainer_destroy": (_I, [_P]),
 }
+
+BATCH_NONE, BATCH_JACOBI, BATCH_IC0, BATCH_FACTOR = 0, 1, 2, 3
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("setup_s", ctypes.c_double),
+        ("solve_s", ctypes.c_double),
+        ("solve_device_ms", ctypes.c_double),
+        ("max_niter", ctypes.c_int),
+        ("outer_steps", ctypes.c_int),
+        ("inner_total", ctypes.c_long),
+        ("inner_steps", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("apply_ms", ctypes.c_double),
+        ("apply_calls", ctypes.c_int),
+        ("spmv_ms", ctypes.c_double),
+        ("spmv_calls", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 _lib = None
 
@@ -273,6 +323,48 @@ def uniform(n, seed, lo=-1.0, hi=1.0):
     x = np.empty(n, np.float64)
     _check(load().parcel_gen_uniform(n, seed, lo, hi, _ptr(x)))
     return x
+
+
+def normal(n, seed):
+    """x ~ N(0,1) from the counter-based stream `seed` (Box-Muller)."""
+    x = np.empty(n, np.float64)
+    _check(load().parcel_gen_normal(n, seed, _ptr(x)))
+    return x
+
+
+def grf2d(nx, ny, ell, sigma, seed):
+    """Squared-exponential Gaussian random field on an nx*ny torus."""
+    g = np.empty(nx * ny, np.float64)
+    _check(load().parcel_gen_grf2d(nx, ny, ell, sigma, seed, _ptr(g)))
+    return g
+
+
+def diffusion2d(nx, ny, kappa):
+    """Five-point harmonic-mean diffusion with per-cell kappa (1-based CSR)."""
+    lib = load()
+    kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+    nnz = ctypes.c_int(0)
+    _check(lib.parcel_gen_diffusion2d(nx, ny, _ptr(kappa), ctypes.byref(nnz), None, None, None))
+    rp = np.empty(nx * ny + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    _check(lib.parcel_gen_diffusion2d(nx, ny, _ptr(kappa), ctypes.byref(nnz), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def grf_diffusion2d(nx, ny, seed0, nsys, ell=16.0, sigma=2.0, fields=False):
+    """nsys systems kappa = exp(grf(seed0 + s)): (rowptr, col, vals[nsys, nnz][, g[nsys, n]])."""
+    lib = load()
+    nnz = ctypes.c_int(0)
+    _check(lib.parcel_gen_grf_diffusion2d(nx, ny, ell, sigma, seed0, nsys, ctypes.byref(nnz), None, None, None,
+                                          None))
+    rp = np.empty(nx * ny + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    vals = np.empty((nsys, nnz.value), np.float64)
+    g = np.empty((nsys, nx * ny), np.float64) if fields else None
+    _check(lib.parcel_gen_grf_diffusion2d(nx, ny, ell, sigma, seed0, nsys, ctypes.byref(nnz), _ptr(rp), _ptr(col),
+                                          _ptr(vals), _ptr(g)))
+    return (rp, col, vals, g) if fields else (rp, col, vals)
 
 
 # --- solvers -------------------------------------------------------------------
@@ -439,3 +531,161 @@ def dot(x, y):
     r = ctypes.c_double(0.0)
     _check(load().parcel_hip_dot(int(x.shape[0]), _ptr(x), _ptr(y), ctypes.byref(r)))
     return r.value
+
+
+class Batch:
+    """nsys systems with one sparsity pattern (parcel_hip_batch_*).  vals is
+    (nsys, nnz) in CSR order (host array or DeviceArray)."""
+
+    def __init__(self, rowptr, col, vals, precon=BATCH_JACOBI, nsys=None):
+        lib = load()
+        self.lib = lib
+        self.n = int(rowptr.shape[0]) - 1
+        nnz = int(col.shape[0])
+        if nsys is None:
+            nsys = int(vals.shape[0]) if len(vals.shape) == 2 else int(np.prod(vals.shape)) // nnz
+        self.nsys = nsys
+        h = ctypes.c_void_p()
+        _check(lib.parcel_hip_batch_create(ctypes.byref(h), nsys, self.n, nnz, _ptr(rowptr), _ptr(col), _ptr(vals),
+                                           precon))
+        self.h = h
+
+    def set_factor(self, Ls, Lw, Ld):
+        """bf16 bit patterns (uint16 arrays of nsys*n, host or DeviceArray)."""
+        _check(self.lib.parcel_hip_batch_set_factor(self.h, _ptr(Ls), _ptr(Lw), _ptr(Ld)))
+
+    def solve(self, x, b, itrmax, rtol, precision=64, history=True):
+        """x, b: (nsys, n) host arrays or DeviceArrays; returns (hist, niter, iret)."""
+        hist = np.zeros((self.nsys, max(itrmax, 1))) if history else None
+        niter = np.zeros(self.nsys, np.int32)
+        iret = np.zeros(self.nsys, np.int32)
+        _check(self.lib.parcel_hip_batch_solve(self.h, _ptr(x), _ptr(b), itrmax, rtol, precision, _ptr(hist),
+                                               _ptr(niter), _ptr(iret)))
+        return hist, niter, iret
+
+    def set_inner(self, inner_rtol, inner_itrmax):
+        _check(self.lib.parcel_hip_batch_set_inner(self.h, inner_rtol, inner_itrmax))
+
+    def set_profile(self, on):
+        _check(self.lib.parcel_hip_batch_set_profile(self.h, 1 if on else 0))
+
+    def stats(self):
+        st = BatchStats()
+        _check(self.lib.parcel_hip_batch_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def spmv(self, x, y):
+        _check(self.lib.parcel_hip_batch_spmv(self.h, _ptr(x), _ptr(y)))
+
+    def factor(self):
+        """BF16 factor bits (Ls, Lw, Ld), each (nsys*n,) uint16."""
+        out = [np.zeros(self.nsys * self.n, np.uint16) for _ in range(3)]
+        _check(self.lib.parcel_hip_batch_get_factor(self.h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
+        return tuple(out)
+
+    def precon(self, r, z):
+        fn = self.lib.parcel_hip_batch_precon_apply32 if np.dtype(r.dtype) == np.float32 else \
+            self.lib.parcel_hip_batch_precon_apply
+        _check(fn(self.h, _ptr(r), _ptr(z)))
+
+    def close(self):
+        if self.h:
+            self.lib.parcel_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mpnn_nparams():
+    return load().parcel_hip_mpnn_nparams()
+
+
+class Mpnn:
+    """The learned-factor network (inference) with FP32 parameters."""
+
+    def __init__(self, params):
+        self.lib = load()
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        _check(self.lib.parcel_hip_mpnn_create(ctypes.byref(h), _ptr(p), int(p.shape[0])))
+        self.h = h
+
+    def factor(self, batch):
+        """Predict and install the factor of every system of a FACTOR batch."""
+        _check(self.lib.parcel_hip_mpnn_factor(self.h, batch.h))
+
+    def inspect(self, batch):
+        nv = batch.nsys * batch.n
+        o = np.zeros(nv * 8, np.float32)
+        hl = np.zeros(nv * 32, np.uint16)
+        ah = np.zeros(nv * 4, np.float32)
+        sd = np.zeros(nv, np.float32)
+        _check(self.lib.parcel_hip_mpnn_inspect(self.h, batch.h, _ptr(o), _ptr(hl), _ptr(ah), _ptr(sd)))
+        return dict(o=o.reshape(nv, 8), hlast=hl.reshape(nv, 32), ahat=ah.reshape(nv, 4), sd=sd)
+
+    def close(self):
+        if self.h:
+            self.lib.parcel_hip_mpnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MpnnTrainer:
+    """Device-side training pool, minibatch gradients and Adam steps."""
+
+    def __init__(self, n, nx, capacity, params):
+        self.lib = load()
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        _check(self.lib.parcel_hip_mpnn_trainer_create(ctypes.byref(h), n, nx, capacity, _ptr(p)))
+        self.h = h
+        self.count = 0
+
+    def add(self, batch, x, b):
+        _check(self.lib.parcel_hip_mpnn_trainer_add(self.h, batch.h, _ptr(x), _ptr(b)))
+        self.count += batch.nsys
+
+    def grad(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.zeros(mpnn_nparams(), np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib.parcel_hip_mpnn_trainer_grad(self.h, _ptr(idx), int(idx.shape[0]), _ptr(g),
+                                                     ctypes.byref(loss)))
+        return loss.value, g
+
+    def step(self, idx, lr):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib.parcel_hip_mpnn_trainer_step(self.h, _ptr(idx), int(idx.shape[0]), lr, ctypes.byref(loss)))
+        return loss.value
+
+    def params(self):
+        p = np.zeros(mpnn_nparams(), np.float32)
+        _check(self.lib.parcel_hip_mpnn_trainer_params(self.h, _ptr(p)))
+        return p
+
+    def close(self):
+        if self.h:
+            self.lib.parcel_hip_mpnn_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

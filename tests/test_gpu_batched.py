@@ -1,0 +1,179 @@
+"""GPU parity of the batched solver: every system of a batch against the
+oracle's single-system solve on the same inputs, bit for bit (histories,
+iteration counts, solutions), for point Jacobi, IC(0) and a BF16 factor, in
+FP64 and in the mixed FP64/FP32 refinement."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import parcel_hip as ph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    ph.init(0)
+
+
+def batch_problem(nx, ny, nsys, seed0=1000, ell=4.0):
+    nxp = 1 << max(nx - 1, 1).bit_length()
+    nyp = 1 << max(ny - 1, 1).bit_length()
+    vals = []
+    for s in range(nsys):
+        g = ph.grf2d(nxp, nyp, ell, 2.0, seed0 + s).reshape(nyp, nxp)[:ny, :nx].ravel()
+        rp, col, v = ph.diffusion2d(nx, ny, np.exp(g))
+        vals.append(v)
+    vals = np.ascontiguousarray(np.stack(vals))
+    b = np.stack([ph.normal(nx * ny, 77 + s) for s in range(nsys)])
+    return rp, col, vals, np.ascontiguousarray(b)
+
+
+def ic0_factor(rp, val, nx):
+    return orc.ic0_5pt(rp, val, nx)
+
+
+@pytest.mark.parametrize("nx,ny,nsys", [(100, 37, 3), (64, 20, 2)])
+def test_batch_jacobi_fp64_bit_exact(nx, ny, nsys):
+    print("batch jacobi", nx, ny, nsys, flush=True)
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_JACOBI) as B:
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 3000, 1e-10)
+    for s in range(nsys):
+        xo, ho, no, io = orc.pcg(rp, col, vals[s], b[s], ipreflag=1, itrmax=3000, rtol=1e-10)
+        assert niter[s] == no and iret[s] == io, s
+        assert np.array_equal(hist[s, :no], ho), s
+        assert np.array_equal(x[s], xo), s
+
+
+@pytest.mark.parametrize("nx,ny,nsys", [(100, 37, 3), (512, 6, 2), (64, 33, 2), (1000, 3, 1), (1, 40, 1)])
+def test_batch_ic0_fp64_bit_exact(nx, ny, nsys):
+    print("batch ic0", nx, ny, nsys, flush=True)
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0) as B:
+        st = B.stats()
+        # a single column (nx = 1) is a tridiagonal: one grid line of ny cells
+        gx = nx if nx > 1 else ny
+        assert (st["nx"], st["ny"]) == (gx, nx * ny // gx)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 2000, 1e-10)
+    for s in range(nsys):
+        F64, F32 = orc.derive(*ic0_factor(rp, vals[s], gx), gx)
+        M = orc.FactorPrecon(F64, F32, gx)
+        xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=2000, rtol=1e-10)
+        assert niter[s] == no and iret[s] == io, s
+        assert np.array_equal(hist[s, :no], ho), s
+        assert np.array_equal(x[s], xo), s
+
+
+@pytest.mark.parametrize("nx,ny", [(512, 5), (100, 9), (64, 4), (1000, 2)])
+def test_batch_precon_apply_bit_exact(nx, ny):
+    print("batch apply", nx, ny, flush=True)
+    nsys = 2
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0) as B:
+        z = np.zeros_like(b)
+        B.precon(b, z)
+        b32 = b.astype(np.float32)
+        z32 = np.zeros_like(b32)
+        B.precon(b32, z32)
+    for s in range(nsys):
+        F64, F32 = orc.derive(*ic0_factor(rp, vals[s], nx), nx)
+        assert np.array_equal(z[s], orc.lscan_apply(F64, b[s], nx)), s
+        assert np.array_equal(z32[s], orc.lscan_apply(F32, b32[s], nx)), s
+
+
+def test_batch_mixed_ic0_bit_exact():
+    print("batch mixed ic0", flush=True)
+    nx, ny, nsys = 128, 40, 3
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0) as B:
+        B.set_inner(1e-4, 5000)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 30, 1e-10, precision=32)
+        st = B.stats()
+    tot = 0
+    for s in range(nsys):
+        F64, F32 = orc.derive(*ic0_factor(rp, vals[s], nx), nx)
+        M = orc.FactorPrecon(F64, F32, nx)
+        xo, ho, no, io, ti = orc.pcg_mixed_factor(rp, col, vals[s], b[s], M, itrmax=30, rtol=1e-10,
+                                                   inner_itrmax=5000, inner_rtol=1e-4)
+        tot += ti
+        assert niter[s] == no and iret[s] == io == 0, s
+        assert np.array_equal(hist[s, :no], ho), s
+        assert np.array_equal(x[s], xo), s
+    assert st["inner_total"] == tot
+
+
+def test_batch_mixed_jacobi_bit_exact():
+    print("batch mixed jacobi", flush=True)
+    nx, ny, nsys = 64, 30, 2
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_JACOBI) as B:
+        B.set_inner(1e-4, 5000)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 30, 1e-10, precision=32)
+    for s in range(nsys):
+        xo, ho, no, io, ti = orc.pcg_mixed(rp, col, vals[s], b[s], ipreflag=1, itrmax=30, rtol=1e-10,
+                                           inner_itrmax=5000, inner_rtol=1e-4)
+        assert niter[s] == no and iret[s] == io, s
+        assert np.array_equal(hist[s, :no], ho), s
+        assert np.array_equal(x[s], xo), s
+
+
+def test_batch_bf16_factor_bit_exact():
+    print("batch bf16 factor", flush=True)
+    nx, ny, nsys = 128, 24, 2
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    Lb = [[], [], []]
+    for s in range(nsys):
+        for k, a in enumerate(ic0_factor(rp, vals[s], nx)):
+            Lb[k].append(orc.f32_to_bf16(a.astype(np.float32)))
+    Lb = [np.ascontiguousarray(np.concatenate(v)) for v in Lb]
+    with ph.Batch(rp, col, vals, ph.BATCH_FACTOR) as B:
+        B.set_factor(*Lb)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 2000, 1e-10)
+        x32 = np.zeros_like(b)
+        B.set_inner(1e-4, 5000)
+        h32, n32, i32 = B.solve(x32, b, 30, 1e-10, precision=32)
+    n = nx * ny
+    for s in range(nsys):
+        F64, F32 = orc.derive(Lb[0][s * n:(s + 1) * n], Lb[1][s * n:(s + 1) * n], Lb[2][s * n:(s + 1) * n], nx)
+        M = orc.FactorPrecon(F64, F32, nx)
+        xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=2000, rtol=1e-10)
+        assert niter[s] == no and np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s
+        xo, ho, no, io, _ = orc.pcg_mixed_factor(rp, col, vals[s], b[s], M, itrmax=30, rtol=1e-10,
+                                                 inner_itrmax=5000, inner_rtol=1e-4)
+        assert n32[s] == no and np.array_equal(h32[s, :no], ho) and np.array_equal(x32[s], xo), s
+
+
+def test_batch_device_resident_and_spmv():
+    print("batch device resident", flush=True)
+    nx, ny, nsys = 64, 16, 4
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    vd = ph.DeviceArray(vals)
+    with ph.Batch(rp, col, vd, ph.BATCH_JACOBI, nsys=nsys) as B:
+        xd = ph.DeviceArray(np.zeros_like(b))
+        bd = ph.DeviceArray(b)
+        hist, niter, iret = B.solve(xd, bd, 3000, 1e-10)
+        x = xd.numpy()
+        y = np.zeros_like(b)
+        B.spmv(x, y)
+    for s in range(nsys):
+        assert np.array_equal(y[s], orc.spmv(rp, col, vals[s], x[s])), s
+        assert np.linalg.norm(b[s] - y[s]) / np.linalg.norm(b[s]) < 1e-9
+
+
+def test_batch_rejects_non_five_point_for_factor():
+    rp, col, val = ph.poisson3d_spd(4, 4, 4)
+    with pytest.raises(ph.ParcelError) as e:
+        ph.Batch(rp, col, val[None, :].copy(), ph.BATCH_IC0)
+    assert e.value.code == ph.ERR_UNSUPPORTED
+    # Jacobi works on any pattern
+    with ph.Batch(rp, col, val[None, :].copy(), ph.BATCH_JACOBI) as B:
+        b = np.ones((1, 64))
+        x = np.zeros_like(b)
+        _, niter, iret = B.solve(x, b, 500, 1e-10)
+        assert iret[0] == 0

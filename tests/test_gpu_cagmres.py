@@ -71,7 +71,9 @@ def test_cagmres_step_counts(s, qr):
     b = ph.uniform(rp.shape[0] - 1, 42)
     (x, h, ni, ir), (xo, ho, no, io) = _both(rp, col, val, b, s, 1, qr, PRECONS["ilud_8"], 200, 1e-10)
     assert (ni, ir) == (no, io)
-    assert np.array_equal(h, ho) and np.array_equal(x, xo)
+    # s = 15 with CholQR breaks down (the monomial Gram matrix is numerically
+    # singular): NaN in both, at the same place
+    assert np.array_equal(h, ho, equal_nan=True) and np.array_equal(x, xo, equal_nan=True)
 
 
 def test_cagmres_large_capped_tiling():
