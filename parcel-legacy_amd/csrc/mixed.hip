@@ -108,6 +108,8 @@ int mixed_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, do
   for (it = 1; it <= itrmax; ++it) {
     hist[it - 1] = rnorm / bnorm;
     convert(n, P.r.p, r32, st);
+    // every correction solves A d = r from d = 0
+    PARCEL_HIP_CHECK(hipMemsetAsync(d32, 0, sizeof(float) * (size_t)n, st));
     int nit = 0, ret = 0;
     cg_run<float>(P, V, d32, r32, inner_itrmax, inner_rtol, nullptr, &nit, &ret);
     inner_total += nit;

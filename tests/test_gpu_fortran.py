@@ -17,7 +17,7 @@ import parcel_hip as ph
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "parcel-legacy_amd", "build", "example_poisson")
+EXE = os.path.join(ROOT, "parcel-legacy_amd", "bin", "example_poisson")
 
 
 def run_fortran(*args):
