@@ -1,23 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: time-to-solution of preconditioned CG at relative residual 1e-10
-on one MI355X, with HBM GB/s against the roofline.
+"""Benchmark: time-to-solution at FP64 relative residual 1e-10 on one MI355X,
+learned / mixed-precision PCG against plain FP64 PCG, with HBM GB/s against
+the roofline.
 
-Workload (default): 3D 7-point constant-coefficient Poisson 256^3
-(16,777,216 unknowns, 117,047,296 nonzeros, diag 6 / off -1), x_true ~ U[-1,1]
-from SplitMix64 seed 3, b = A x_true, x0 = 0, rtol 1e-10 (FP64 residual).
-Methods:
-  bj        FP64 PCG, block-Jacobi ILU(0), 256 contiguous row blocks
-  pj        FP64 PCG, point Jacobi (the SpMV/CG path)
-  mixed_bj  FP64 iterative refinement around FP32 PCG with FP32 block-Jacobi
-            ILU(0) (256 blocks), inner tolerance 1e-4
-  mixed_pj  the same with FP32 point Jacobi
-A "step" is one full solve from x0 = 0 with the matrix, b and x already in
-HBM; `value` is the mean time-to-solution of the fastest method, and the
-config reports its speed-up over the fastest plain FP64 PCG.  The timed
-solves run without per-kernel events; one more solve with HIP events around
-every kernel gives the per-kernel means behind `roofline`.
+Headline workload (C5): 64 test systems of 2D 5-point diffusion on 512^2
+cells, conductivity exp(g), g a squared-exponential Gaussian random field
+(ell = 16 cells, sigma = 2, field seeds 3048..3111), b ~ N(0,1), x0 = 0.
+  learned_mixed   the trained message-passing network predicts every
+                  system's BF16 factor L (one batched inference), then the
+                  batched FP64 refinement around FP32 PCG with M = L L^T
+                  (inner rtol 1e-4); the time includes the inference
+  untrained_mixed the same with the untrained network (ablation)
+  ic0_mixed/_fp64 the batched solver with the classical IC(0) factor
+  plain_fp64_bj   parcel_dcg's FP64 PCG + block-Jacobi ILU(0) (256 row
+                  blocks), one system at a time, solve time only
+`value` is learned_mixed's mean time for all 64 systems; the config carries
+its speed-up over plain_fp64_bj.
+Second workload (C4): 3D 7-point Poisson 256^3 (n = 16,777,216), x_true ~
+U[-1,1] seed 3, b = A x_true: FP64 PCG with block-Jacobi ILU(0) / point
+Jacobi and the FP64 refinement around FP32 PCG; reported under
+config.c4_mixed_vs_plain (or as the headline with --workload c4).
+A step is one full solve of the workload with matrices, b and x already in
+HBM.  One more solve with HIP events around each kernel gives the per-kernel
+means behind `roofline`.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload c5|c4|both]
 """
 import argparse
 import json
@@ -61,27 +68,10 @@ def ilu0_apply_bytes32(n, nnz_l, nnz_u):
     return 8 * (nnz_l + nnz_u) + 8 * n + 4 * n + 4 * n + 8 * n + 4 * n
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--methods", default="bj,pj,mixed_bj,mixed_pj")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=12)
-    ap.add_argument("--calib", action="store_true",
-                    help="also run one plain 2 GiB device-to-device copy (reference for HBM counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(one GPU holds one rank's whole subdomain); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def run_c4(args):
     import numpy as np
     import parcel_hip as ph
 
-    ph.init(0)
     g = args.grid
     n = g ** 3
     t0 = time.perf_counter()
@@ -224,36 +214,257 @@ def main():
                              (head, no, per_it, tset, H["niter"])}
         log("cpu baseline: %.3f s/it over %d its" % (per_it, no))
 
+    return {
+        "value": H["time_to_solution_s"],
+        "dtype": "f32/f64" if head.startswith("mixed_") else "f64",
+        "workload": "3D 7-point Poisson %d^3 (n=%d, nnz=%d), x_true~U[-1,1] seed 3, b=A x_true, x0=0, "
+                    "rtol 1e-10" % (g, n, nnz),
+        "method": {"bj": "FP64 PCG + block-Jacobi ILU(0), 256 row blocks",
+                   "pj": "FP64 PCG + point Jacobi",
+                   "mixed_bj": "FP64 refinement around FP32 PCG + FP32 block-Jacobi ILU(0), 256 blocks, "
+                               "inner rtol 1e-4",
+                   "mixed_pj": "FP64 refinement around FP32 PCG + FP32 point Jacobi, inner rtol 1e-4"}[head],
+        "headline": head,
+        "speedup_vs_plain_fp64": (round(methods[best_plain]["time_to_solution_s"] / H["time_to_solution_s"], 3)
+                                  if best_plain else None),
+        "plain_fp64_reference": best_plain,
+        "methods": methods,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+def lscan_bytes32(n, nsys):
+    """Factor apply (FP32), per cell: forward reads r, ls, 1/Ld, a_fwd and
+    writes w; backward reads w, 1/Ld, a_bwd, ls and writes z: 40 B."""
+    return 40 * n * nsys
+
+
+def k1_bytes32_batch(n, nnz, nsys):
+    """Batched FP32 fused CG kernel: per system values 4 B/nnz and the
+    vectors of k1_bytes32; columns and row pointers once per launch."""
+    return nsys * (4 * nnz + 24 * n) + 4 * nnz + 4 * (n + 1)
+
+
+def run_c5(args):
+    """Learned-factor mixed PCG on the 64 random-coefficient test systems
+    against plain FP64 block-Jacobi ILU(0) PCG on the same GPU."""
+    import numpy as np
+    import parcel_hip as ph
+    import learned as L
+
+    nsys = args.c5_systems
+    t0 = time.perf_counter()
+    rp, col, vals, b = L.systems(L.TEST_SEED0, nsys)
+    n, nnz = b.shape[1], int(col.shape[0])
+    log("c5: generated %d systems 512^2 (nnz %d) in %.1fs" % (nsys, nnz, time.perf_counter() - t0))
+    D = ph.DeviceArray
+    vals_d, b_d = D(vals), D(b)
+    x_d = D(np.zeros_like(b))
+    y = np.zeros_like(b)
+    weights = args.weights
+    p = L.load_weights(weights)
+    methods = {}
+
+    def true_res(B):
+        B.spmv(x_d, y)
+        return float(max(np.linalg.norm(b[s] - y[s]) / np.linalg.norm(b[s]) for s in range(nsys)))
+
+    def batched(name, kind, precision, params=None, steps=args.steps, warmup=args.warmup):
+        t = time.perf_counter()
+        B = ph.Batch(rp, col, vals_d, kind, nsys=nsys)
+        B.set_inner(args.inner_rtol, 20000)
+        setup = time.perf_counter() - t
+        net = ph.Mpnn(params) if params is not None else None
+        itr = 50 if precision == 32 else 20000
+        times, niter, iret = [], None, None
+        for k in range(warmup + steps):
+            x_d.zero()
+            ph.load().parcel_hip_device_synchronize()
+            t = time.perf_counter()
+            if net is not None:
+                net.factor(B)
+            _, niter, iret = B.solve(x_d, b_d, itr, 1e-10, precision=precision, history=False)
+            if k >= warmup:
+                times.append(time.perf_counter() - t)
+        st = B.stats()
+        res = true_res(B)
+        B.set_profile(True)
+        x_d.zero()
+        B.solve(x_d, b_d, itr, 1e-10, precision=precision, history=False)
+        pst = B.stats()
+        B.close()
+        if net is not None:
+            net.close()
+        m = {"time_to_solution_s": float(np.mean(times)), "steps": len(times), "setup_s": setup,
+             "niter_max": int(niter.max()), "niter_mean": float(niter.mean()), "iret_max": int(iret.max()),
+             "true_rel_res_max": res, "device_ms": st["solve_device_ms"],
+             "apply_ms": pst["apply_ms"] / max(pst["apply_calls"], 1), "apply_calls": pst["apply_calls"],
+             "spmv_ms": pst["spmv_ms"] / max(pst["spmv_calls"], 1), "spmv_calls": pst["spmv_calls"]}
+        if precision == 32:
+            m["fp32_batched_iterations"] = st["inner_steps"]
+            m["fp32_iterations_per_system_mean"] = st["inner_total"] / nsys
+            m["fp64_corrections"] = st["outer_steps"]
+        methods[name] = m
+        log("c5 %s: %.4f s, its max %d, true res %.2e" % (name, m["time_to_solution_s"], m["niter_max"], res))
+        return m
+
+    learned = batched("learned_mixed", ph.BATCH_FACTOR, 32, params=p)
+    batched("learned_fp64", ph.BATCH_FACTOR, 64, params=p, steps=1)
+    batched("untrained_mixed", ph.BATCH_FACTOR, 32, params=L.init_params(0), steps=1, warmup=0)
+    batched("ic0_mixed", ph.BATCH_IC0, 32, steps=1, warmup=0)
+    batched("ic0_fp64", ph.BATCH_IC0, 64, steps=1, warmup=0)
+    batched("jacobi_fp64", ph.BATCH_JACOBI, 64, steps=1, warmup=0)
+
+    # plain FP64 block-Jacobi ILU(0) PCG, parcel_dcg's solver, one system at a time;
+    # the block count is the fastest of a sweep on the first 4 systems
+    sweep = {}
+    for nbc in [int(v) for v in args.c5_blocks.split(",")]:
+        tt = 0.0
+        for s in range(min(4, nsys)):
+            plan = ph.Plan(rp, col, vals[s], ipreflag=2, ILU_method=0, precon_thblock=nbc, profile=0)
+            xs, bs = D(n=n), D(b[s])
+            ph.load().parcel_hip_device_synchronize()
+            t = time.perf_counter()
+            _, niter, _ = plan.solve(xs, bs, 20000, 1e-10)
+            tt += time.perf_counter() - t
+            plan.close()
+        sweep[nbc] = tt / min(4, nsys)
+        log("c5 plain BJ(%d): %.3f s per system (4-system sweep)" % (nbc, sweep[nbc]))
+    nb = min(sweep, key=sweep.get)
+    solve_t, setup_t, its, res, tres = [], [], [], [], []
+    for s in range(nsys):
+        t = time.perf_counter()
+        plan = ph.Plan(rp, col, vals[s], ipreflag=2, ILU_method=0, precon_thblock=nb, profile=0)
+        t1 = time.perf_counter()
+        xs, bs = D(n=n), D(b[s])
+        ph.load().parcel_hip_device_synchronize()
+        t2 = time.perf_counter()
+        hist, niter, iret = plan.solve(xs, bs, 20000, 1e-10)
+        t3 = time.perf_counter()
+        solve_t.append(t3 - t2)
+        setup_t.append(t1 - t)
+        its.append(niter)
+        if s < 4:
+            r_d = D(n=n)
+            plan.spmv(xs, r_d)
+            tres.append(float(np.linalg.norm(b[s] - r_d.numpy()) / np.linalg.norm(b[s])))
+        res.append(float(hist[-1]))
+        plan.close()
+    plain = {"time_to_solution_s": float(np.sum(solve_t)), "setup_s_total": float(np.sum(setup_t)),
+             "block_sweep_s_per_system": sweep,
+             "niter_max": int(max(its)), "niter_mean": float(np.mean(its)), "final_recurrence_res_max": max(res),
+             "true_rel_res_max_first4": max(tres), "blocks": nb}
+    methods["plain_fp64_bj"] = plain
+    log("c5 plain FP64 BJ(%d): %.3f s solve (setup %.2f s), its mean %.0f" % (nb, plain["time_to_solution_s"],
+                                                                           plain["setup_s_total"],
+                                                                           plain["niter_mean"]))
+    # dominant kernel of the learned solve
+    ap_total = learned["apply_ms"] * learned["apply_calls"]
+    sp_total = learned["spmv_ms"] * learned["spmv_calls"]
+    if ap_total >= sp_total:
+        kname, ms, by = "k_lscan<float,8,FULL> (factor apply)", learned["apply_ms"], lscan_bytes32(n, nsys)
+    else:
+        kname, ms, by = "k_cg_spmv<float> (batched)", learned["spmv_ms"], k1_bytes32_batch(n, nnz, nsys)
+    ach = by / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                "mean_ms": round(ms, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        k = args.cpu_iters
+        t = time.perf_counter()
+        orc.pcg(rp, col, vals[0], b[0], ipreflag=2, ilu_method=0, nblocks=nb, itrmax=k, rtol=1e-30)
+        tset, titer = orc.last_times()
+        per_it = titer / k
+        total_its = int(np.sum(its))
+        cpu = {"value": round(per_it * total_its + tset * nsys, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": "oracle FP64 PCG + BJ ILU(0) (%d blocks) on test system 0 for %d iterations "
+                         "(%.4f s/iteration, setup %.2f s), scaled to the plain GPU runs' %d iterations and %d "
+                         "setups over the %d systems" % (nb, k, per_it, tset, total_its, nsys, nsys)}
+    meta = {}
+    mp = os.path.splitext(weights)[0] + ".json"
+    if os.path.exists(mp):
+        meta = {k: v for k, v in json.load(open(mp)).items() if k != "loss_curve_every50"}
+    return {
+        "value": learned["time_to_solution_s"],
+        "dtype": "bf16/f32/f64",
+        "workload": "2D 5-point diffusion 512^2, kappa=exp(GRF, SE ell=16 sigma=2), %d test systems (field seeds "
+                    "3048..%d), b~N(0,1), x0=0, FP64 rtol 1e-10 per system" % (nsys, 3048 + nsys - 1),
+        "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L, M=L L^T) in FP32 batched PCG, "
+                  "FP64 refinement outer, inner rtol %g; time includes the network inference" % args.inner_rtol,
+        "headline": "learned_mixed",
+        "speedup_vs_plain_fp64": round(plain["time_to_solution_s"] / learned["time_to_solution_s"], 3),
+        "plain_fp64_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
+                                "one system at a time, solve time only (factorisation excluded)" % nb,
+        "methods": methods,
+        "training": meta,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c5", "c4", "both"], default="both")
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--methods", default="bj,pj,mixed_bj,mixed_pj")
+    ap.add_argument("--c5-systems", type=int, default=64)
+    ap.add_argument("--c5-blocks", default="8,16,32,64,256",
+                    help="block counts of the plain FP64 BJ ILU(0) baseline sweep (fastest is used)")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "parcel-legacy_amd", "weights", "mpnn_grf512.npy"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one plain 2 GiB device-to-device copy (reference for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one GPU holds one rank's whole subdomain); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import parcel_hip as ph
+
+    ph.init(0)
+    c5 = c4 = None
+    if args.workload in ("c5", "both"):
+        if os.path.exists(args.weights):
+            c5 = run_c5(args)
+        else:
+            log("c5 skipped: no trained weights at %s" % args.weights)
+    if args.workload in ("c4", "both"):
+        c4 = run_c4(args)
+    H = c5 if c5 is not None else c4
+    cfg = {"workload": H["workload"], "method": H["method"], "headline": H["headline"],
+           "speedup_vs_plain_fp64": H["speedup_vs_plain_fp64"], "plain_fp64_reference": H["plain_fp64_reference"],
+           "methods": H["methods"], "parallelism": "1 GPU"}
+    if c5 is not None:
+        cfg["training"] = c5["training"]
+        if c4 is not None:
+            cfg["c4_mixed_vs_plain"] = {k: c4[k] for k in ("workload", "method", "headline", "value",
+                                                            "speedup_vs_plain_fp64", "plain_fp64_reference",
+                                                            "methods", "roofline", "cpu_baseline")}
     line = {
         "metric": METRIC,
-        "value": round(H["time_to_solution_s"], 5),
+        "value": round(H["value"], 5),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(H["time_to_solution_s"] * 1e3, 3),
+        "ms_per_step": round(H["value"] * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64" if head.startswith("mixed_") else "f64",
+        "dtype": H["dtype"],
         "data": "synthetic",
-        "config": {
-            "workload": "3D 7-point Poisson %d^3 (n=%d, nnz=%d), x_true~U[-1,1] seed 3, b=A x_true, x0=0, "
-                        "rtol 1e-10" % (g, n, nnz),
-            "method": {"bj": "FP64 PCG + block-Jacobi ILU(0), 256 row blocks",
-                       "pj": "FP64 PCG + point Jacobi",
-                       "mixed_bj": "FP64 refinement around FP32 PCG + FP32 block-Jacobi ILU(0), 256 blocks, "
-                                   "inner rtol 1e-4",
-                       "mixed_pj": "FP64 refinement around FP32 PCG + FP32 point Jacobi, inner rtol 1e-4"}[head],
-            "headline": head,
-            "speedup_vs_plain_fp64": (round(methods[best_plain]["time_to_solution_s"] / H["time_to_solution_s"], 3)
-                                      if best_plain else None),
-            "plain_fp64_reference": best_plain,
-            "methods": methods,
-            "parallelism": "1 GPU",
-        },
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "config": cfg,
+        "roofline": H["roofline"],
+        "cpu_baseline": H["cpu_baseline"],
     }
     print(json.dumps(line))
 

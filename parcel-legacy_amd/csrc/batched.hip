@@ -110,6 +110,32 @@ __device__ __forceinline__ bool sys_done(const CGState* st, int it) {
   return d != 0 && d <= it;
 }
 
+// Registers of one grid line in flight (the loads of line y + ahead are
+// issued before line y's scan, so memory latency overlaps the recurrence).
+// A line takes 4*E values per lane; up to 128 VGPRs of them stay in flight.
+template <class T, int E>
+constexpr int lines_ahead() {
+  return 4 * E * (int)sizeof(T) / 4 >= 128 ? 1 : (4 * E * (int)sizeof(T) / 4 >= 64 ? 2 : 4);
+}
+template <class T, int E>
+struct LineRegs {
+  T a[E], p[E], q[E], i[E];
+};
+
+template <class T, int E, bool FULL>
+__device__ __forceinline__ void load_line(LineRegs<T, E>& L, const T* __restrict__ a, const T* __restrict__ p,
+                                          const T* __restrict__ q, const T* __restrict__ inv, long row, int x0,
+                                          int nx) {
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    if (FULL || x0 + k < nx) {
+      L.a[k] = a[row + k]; L.p[k] = p[row + k]; L.q[k] = q[row + k]; L.i[k] = inv[row + k];
+    } else {
+      L.a[k] = 0; L.p[k] = 0; L.q[k] = 0; L.i[k] = 0;
+    }
+  }
+}
+
 template <class T, int E, bool FULL>
 __global__ void __launch_bounds__(64) k_lscan(int nx, int ny, long n, long nv, const T* __restrict__ F,
                                               const T* __restrict__ r, T* __restrict__ w, T* __restrict__ z,
@@ -123,71 +149,91 @@ __global__ void __launch_bounds__(64) k_lscan(int nx, int ny, long n, long nv, c
   const T* __restrict__ ab = F + 3 * nv + so;
   r += so; w += so; z += so;
   const int x0 = lane * E;
+  constexpr int kAhead = lines_ahead<T, E>();
   T zp[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) zp[k] = 0;
-  // forward: lines bottom-up, cells left to right
-  for (int y = 0; y < ny; ++y) {
-    const long row = (long)y * nx + x0;
-    T a[E], c[E];
+  LineRegs<T, E> buf[kAhead];
+  // forward: lines bottom-up, cells left to right (p = ls, q = r)
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      if (FULL || x0 + k < nx) {
-        a[k] = af[row + k];
-        c[k] = fma(-ls[row + k], zp[k], r[row + k]) * inv[row + k];
-      } else {
-        a[k] = 0; c[k] = 0;
+  for (int d = 0; d < kAhead; ++d)
+    if (d < ny) load_line<T, E, FULL>(buf[d], af, ls, r, inv, (long)d * nx + x0, x0, nx);
+  for (int y0 = 0; y0 < ny; y0 += kAhead) {
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      const int y = y0 + d;
+      if (y >= ny) break;
+      const long row = (long)y * nx + x0;
+      T a[E], c[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if (FULL || x0 + k < nx) {
+          a[k] = buf[d].a[k];
+          c[k] = fma(-buf[d].p[k], zp[k], buf[d].q[k]) * buf[d].i[k];
+        } else {
+          a[k] = 0; c[k] = 0;
+        }
+      }
+      if (y + kAhead < ny) load_line<T, E, FULL>(buf[d], af, ls, r, inv, row + (long)kAhead * nx, x0, nx);
+      T A = 1, C = 0;
+#pragma unroll
+      for (int k = 0; k < E; ++k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
+#pragma unroll
+      for (int s = 1; s < 64; s <<= 1) {
+        const T Ap = __shfl_up(A, s, 64), Cp = __shfl_up(C, s, 64);
+        if (lane >= s) { C = fma(A, Cp, C); A = A * Ap; }
+      }
+      T v = __shfl_up(C, 1, 64);
+      if (lane == 0) v = 0;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        v = fma(a[k], v, c[k]);
+        zp[k] = v;
+        if (FULL || x0 + k < nx) w[row + k] = v;
       }
     }
-    T A = 1, C = 0;
-#pragma unroll
-    for (int k = 0; k < E; ++k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const T Ap = __shfl_up(A, d, 64), Cp = __shfl_up(C, d, 64);
-      if (lane >= d) { C = fma(A, Cp, C); A = A * Ap; }
-    }
-    T v = __shfl_up(C, 1, 64);
-    if (lane == 0) v = 0;
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      v = fma(a[k], v, c[k]);
-      zp[k] = v;
-      if (FULL || x0 + k < nx) w[row + k] = v;
-    }
   }
-  // backward: lines top-down, cells right to left
+  // backward: lines top-down, cells right to left (p = ls, q = w)
   T yn[E], lsn[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) { yn[k] = 0; lsn[k] = 0; }
-  for (int y = ny - 1; y >= 0; --y) {
-    const long row = (long)y * nx + x0;
-    T a[E], c[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      if (FULL || x0 + k < nx) {
-        a[k] = ab[row + k];
-        c[k] = fma(-lsn[k], yn[k], w[row + k]) * inv[row + k];
-        lsn[k] = ls[row + k];
-      } else {
-        a[k] = 0; c[k] = 0; lsn[k] = 0;
+  for (int d = 0; d < kAhead; ++d)
+    if (ny - 1 - d >= 0) load_line<T, E, FULL>(buf[d], ab, ls, w, inv, (long)(ny - 1 - d) * nx + x0, x0, nx);
+  for (int y0 = ny - 1; y0 >= 0; y0 -= kAhead) {
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      const int y = y0 - d;
+      if (y < 0) break;
+      const long row = (long)y * nx + x0;
+      T a[E], c[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        if (FULL || x0 + k < nx) {
+          a[k] = buf[d].a[k];
+          c[k] = fma(-lsn[k], yn[k], buf[d].q[k]) * buf[d].i[k];
+          lsn[k] = buf[d].p[k];
+        } else {
+          a[k] = 0; c[k] = 0; lsn[k] = 0;
+        }
       }
-    }
-    T A = 1, C = 0;
+      if (y - kAhead >= 0) load_line<T, E, FULL>(buf[d], ab, ls, w, inv, row - (long)kAhead * nx, x0, nx);
+      T A = 1, C = 0;
 #pragma unroll
-    for (int k = E - 1; k >= 0; --k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
+      for (int k = E - 1; k >= 0; --k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const T Ap = __shfl_down(A, d, 64), Cp = __shfl_down(C, d, 64);
-      if (lane + d < 64) { C = fma(A, Cp, C); A = A * Ap; }
-    }
-    T v = __shfl_down(C, 1, 64);
-    if (lane == 63) v = 0;
+      for (int s = 1; s < 64; s <<= 1) {
+        const T Ap = __shfl_down(A, s, 64), Cp = __shfl_down(C, s, 64);
+        if (lane + s < 64) { C = fma(A, Cp, C); A = A * Ap; }
+      }
+      T v = __shfl_down(C, 1, 64);
+      if (lane == 63) v = 0;
 #pragma unroll
-    for (int k = E - 1; k >= 0; --k) {
-      v = fma(a[k], v, c[k]);
-      yn[k] = v;
-      if (FULL || x0 + k < nx) z[row + k] = v;
+      for (int k = E - 1; k >= 0; --k) {
+        v = fma(a[k], v, c[k]);
+        yn[k] = v;
+        if (FULL || x0 + k < nx) z[row + k] = v;
+      }
     }
   }
 }
