@@ -6,15 +6,18 @@ the roofline.
 Headline workload (C5): 64 test systems of 2D 5-point diffusion on 512^2
 cells, conductivity exp(g), g a squared-exponential Gaussian random field
 (ell = 16 cells, sigma = 2, field seeds 3048..3111), b ~ N(0,1), x0 = 0.
-  learned_mixed   the trained message-passing network predicts every
+  learned_apply32 the trained message-passing network predicts every
                   system's BF16 factor L (one batched inference), then the
-                  batched FP64 refinement around FP32 PCG with M = L L^T
-                  (inner rtol 1e-4); the time includes the inference
-  untrained_mixed the same with the untrained network (ablation)
-  ic0_mixed/_fp64 the batched solver with the classical IC(0) factor
+                  batched FP64 PCG with M = L L^T applied in FP32; the time
+                  includes the inference
+  learned_refine32 / learned_fp64  FP64 refinement around FP32 PCG / all
+                  FP64, same factor
+  untrained_apply32, ic0_apply32, ic0_fp64, jacobi_fp64  the same solver
+                  with the untrained network / the classical IC(0) factor /
+                  point Jacobi (ablation)
   plain_fp64_bj   parcel_dcg's FP64 PCG + block-Jacobi ILU(0) (256 row
                   blocks), one system at a time, solve time only
-`value` is learned_mixed's mean time for all 64 systems; the config carries
+`value` is learned_apply32's mean time for all 64 systems; the config carries
 its speed-up over plain_fp64_bj.
 Second workload (C4): 3D 7-point Poisson 256^3 (n = 16,777,216), x_true ~
 U[-1,1] seed 3, b = A x_true: FP64 PCG with block-Jacobi ILU(0) / point
@@ -234,16 +237,18 @@ def run_c4(args):
     }
 
 
-def lscan_bytes32(n, nsys):
-    """Factor apply (FP32), per cell: forward reads r, ls, 1/Ld, a_fwd and
-    writes w; backward reads w, 1/Ld, a_bwd, ls and writes z: 40 B."""
-    return 40 * n * nsys
+def lscan_bytes(n, nsys, t=4, tv=4):
+    """Factor apply per cell: forward reads r (tv), ls, 1/Ld, a_fwd and writes
+    w; backward reads w, 1/Ld, a_bwd, ls and writes z (tv); factor and w in t
+    bytes: 8t + 2tv (40 B in FP32, 48 B for the FP32 apply on FP64 vectors)."""
+    return (8 * t + 2 * tv) * n * nsys
 
 
-def k1_bytes32_batch(n, nnz, nsys):
-    """Batched FP32 fused CG kernel: per system values 4 B/nnz and the
-    vectors of k1_bytes32; columns and row pointers once per launch."""
-    return nsys * (4 * nnz + 24 * n) + 4 * nnz + 4 * (n + 1)
+def k1_bytes_batch(n, nnz, nsys, t=8):
+    """Batched fused CG kernel: per system values t B/nnz and the vectors of
+    k1_bytes (6 vector passes of t bytes); columns and row pointers once per
+    launch."""
+    return nsys * (t * nnz + 6 * t * n) + 4 * nnz + 4 * (n + 1)
 
 
 def run_c5(args):
@@ -276,7 +281,7 @@ def run_c5(args):
         B.set_inner(args.inner_rtol, 20000)
         setup = time.perf_counter() - t
         net = ph.Mpnn(params) if params is not None else None
-        itr = 50 if precision == 32 else 20000
+        itr = 50 if precision == ph.BATCH_REFINE32 else 20000
         times, niter, iret = [], None, None
         for k in range(warmup + steps):
             x_d.zero()
@@ -301,7 +306,7 @@ def run_c5(args):
              "true_rel_res_max": res, "device_ms": st["solve_device_ms"],
              "apply_ms": pst["apply_ms"] / max(pst["apply_calls"], 1), "apply_calls": pst["apply_calls"],
              "spmv_ms": pst["spmv_ms"] / max(pst["spmv_calls"], 1), "spmv_calls": pst["spmv_calls"]}
-        if precision == 32:
+        if precision == ph.BATCH_REFINE32:
             m["fp32_batched_iterations"] = st["inner_steps"]
             m["fp32_iterations_per_system_mean"] = st["inner_total"] / nsys
             m["fp64_corrections"] = st["outer_steps"]
@@ -309,12 +314,16 @@ def run_c5(args):
         log("c5 %s: %.4f s, its max %d, true res %.2e" % (name, m["time_to_solution_s"], m["niter_max"], res))
         return m
 
-    learned = batched("learned_mixed", ph.BATCH_FACTOR, 32, params=p)
-    batched("learned_fp64", ph.BATCH_FACTOR, 64, params=p, steps=1)
-    batched("untrained_mixed", ph.BATCH_FACTOR, 32, params=L.init_params(0), steps=1, warmup=0)
-    batched("ic0_mixed", ph.BATCH_IC0, 32, steps=1, warmup=0)
-    batched("ic0_fp64", ph.BATCH_IC0, 64, steps=1, warmup=0)
-    batched("jacobi_fp64", ph.BATCH_JACOBI, 64, steps=1, warmup=0)
+    # headline: FP64 PCG whose preconditioner (the network's BF16 factor) is
+    # applied in FP32; then the FP64-refinement form, the all-FP64 form, and
+    # the same solver with the untrained network / the classical IC(0) factor
+    learned = batched("learned_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p)
+    batched("learned_refine32", ph.BATCH_FACTOR, ph.BATCH_REFINE32, params=p, steps=1)
+    batched("learned_fp64", ph.BATCH_FACTOR, ph.BATCH_FP64, params=p, steps=1)
+    batched("untrained_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=L.init_params(0), steps=1, warmup=0)
+    batched("ic0_apply32", ph.BATCH_IC0, ph.BATCH_APPLY32, steps=1, warmup=0)
+    batched("ic0_fp64", ph.BATCH_IC0, ph.BATCH_FP64, steps=1, warmup=0)
+    batched("jacobi_fp64", ph.BATCH_JACOBI, ph.BATCH_FP64, steps=1, warmup=0)
 
     # plain FP64 block-Jacobi ILU(0) PCG, parcel_dcg's solver, one system at a time;
     # the block count is the fastest of a sweep on the first 4 systems
@@ -363,9 +372,10 @@ def run_c5(args):
     ap_total = learned["apply_ms"] * learned["apply_calls"]
     sp_total = learned["spmv_ms"] * learned["spmv_calls"]
     if ap_total >= sp_total:
-        kname, ms, by = "k_lscan<float,8,FULL> (factor apply)", learned["apply_ms"], lscan_bytes32(n, nsys)
+        kname, ms, by = ("k_lscan<float,double,8,FULL> (FP32 factor apply on FP64 vectors)", learned["apply_ms"],
+                         lscan_bytes(n, nsys, 4, 8))
     else:
-        kname, ms, by = "k_cg_spmv<float> (batched)", learned["spmv_ms"], k1_bytes32_batch(n, nnz, nsys)
+        kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
     ach = by / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "bytes_per_launch": by,
@@ -393,9 +403,9 @@ def run_c5(args):
         "dtype": "bf16/f32/f64",
         "workload": "2D 5-point diffusion 512^2, kappa=exp(GRF, SE ell=16 sigma=2), %d test systems (field seeds "
                     "3048..%d), b~N(0,1), x0=0, FP64 rtol 1e-10 per system" % (nsys, 3048 + nsys - 1),
-        "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L, M=L L^T) in FP32 batched PCG, "
-                  "FP64 refinement outer, inner rtol %g; time includes the network inference" % args.inner_rtol,
-        "headline": "learned_mixed",
+        "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L, M = L L^T) applied in FP32 inside "
+                  "batched FP64 PCG (64 systems per launch); time includes the network inference",
+        "headline": "learned_apply32",
         "speedup_vs_plain_fp64": round(plain["time_to_solution_s"] / learned["time_to_solution_s"], 3),
         "plain_fp64_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
                                 "one system at a time, solve time only (factorisation excluded)" % nb,

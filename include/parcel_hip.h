@@ -275,6 +275,10 @@ int parcel_hip_device_synchronize(void);
 #define PARCEL_BATCH_JACOBI 1
 #define PARCEL_BATCH_IC0 2
 #define PARCEL_BATCH_FACTOR 3
+/* precision of parcel_hip_batch_solve */
+#define PARCEL_BATCH_FP64 64       /* FP64 PCG (parcel_dcg's iteration) */
+#define PARCEL_BATCH_REFINE32 32   /* FP64 refinement around FP32 PCG */
+#define PARCEL_BATCH_APPLY32 48    /* FP64 PCG with the factor apply in FP32 */
 
 typedef struct parcel_hip_batch parcel_hip_batch;
 
@@ -306,7 +310,9 @@ int parcel_hip_batch_set_factor(parcel_hip_batch* batch, const unsigned short* L
 /* Solve all systems to ||b - A x|| / ||b|| < rtol.  precision 64: FP64 PCG
  * (per system exactly parcel_dcg's iteration); precision 32: FP64
  * refinement around FP32 PCG with the FP32 preconditioner (as
- * PARCEL_SOLVER_CG_MIXED; itrmax then counts FP64 corrections).  x, b:
+ * PARCEL_SOLVER_CG_MIXED; itrmax then counts FP64 corrections); precision
+ * 48: FP64 PCG whose factor apply (IC0 / FACTOR) runs in FP32 (r rounded to
+ * FP32, z widened back).  x, b:
  * nsys*n (host or device), x holds the initial guess.  hist (nsys*itrmax,
  * may be NULL), niter and iret (nsys each, may be NULL) are host arrays. */
 int parcel_hip_batch_solve(parcel_hip_batch* batch, double* x, const double* b, int itrmax, double rtol,

@@ -100,93 +100,90 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
   }
 }
 
-/* The GPU's wavefront scan, lane by lane.  Lane l owns cells l*E .. l*E+E-1
- * of a line; cells past nx are padding with a = c = 0.
- * Forward line: c = fma(-ls, zprev, r) * inv; lane map A, C composed in cell
- * order (C = fma(a, C, c); A = a*A); inclusive scan over lanes with offsets
- * 1, 2, .., 32 (C = fma(A, C[l-d], C); A = A * A[l-d], from the values before
- * the step); the lane's entry value is C[l-1] (0 for lane 0); cells re-run
- * v = fma(a, v, c).  Backward: mirrored, cells and lanes from the right. */
-#define ORACLE_LSCAN(NAME, T, FMA)                                                           \
-  static void NAME##_fwd(int nx, int ny, int E, const T* F, long n, const T* r, T* w) {      \
-    const int W = 64 * E;                                                                    \
-    T* zp = (T*)calloc((size_t)W, sizeof(T));                                               \
-    T* a = (T*)calloc((size_t)W, sizeof(T));                                                \
-    T* c = (T*)calloc((size_t)W, sizeof(T));                                                \
-    T A[64], C[64], A2[64], C2[64];                                                          \
-    const T *ls = F, *inv = F + n, *af = F + 2 * n;                                         \
-    for (int y = 0; y < ny; ++y) {                                                           \
-      const long row = (long)y * nx;                                                         \
-      for (int x = 0; x < W; ++x) {                                                          \
-        if (x < nx) {                                                                        \
-          a[x] = af[row + x];                                                                \
-          c[x] = FMA(-ls[row + x], zp[x], r[row + x]) * inv[row + x];                       \
-        } else { a[x] = 0; c[x] = 0; }                                                       \
-      }                                                                                      \
-      for (int l = 0; l < 64; ++l) {                                                         \
-        T Al = 1, Cl = 0;                                                                    \
-        for (int k = 0; k < E; ++k) { const int x = l * E + k; Cl = FMA(a[x], Cl, c[x]); Al = a[x] * Al; } \
-        A[l] = Al; C[l] = Cl;                                                                \
-      }                                                                                      \
-      for (int d = 1; d < 64; d <<= 1) {                                                     \
-        for (int l = 0; l < 64; ++l) {                                                       \
-          if (l >= d) { C2[l] = FMA(A[l], C[l - d], C[l]); A2[l] = A[l] * A[l - d]; }       \
-          else { C2[l] = C[l]; A2[l] = A[l]; }                                               \
-        }                                                                                    \
-        memcpy(A, A2, sizeof(A)); memcpy(C, C2, sizeof(C));                                  \
-      }                                                                                      \
-      for (int l = 0; l < 64; ++l) {                                                         \
-        T v = l > 0 ? C[l - 1] : (T)0;                                                       \
-        for (int k = 0; k < E; ++k) {                                                        \
-          const int x = l * E + k;                                                           \
-          v = FMA(a[x], v, c[x]);                                                            \
-          zp[x] = v;                                                                         \
-          if (x < nx) w[row + x] = v;                                                        \
-        }                                                                                    \
-      }                                                                                      \
-    }                                                                                        \
-    free(zp); free(a); free(c);                                                              \
-  }                                                                                          \
-  static void NAME##_bwd(int nx, int ny, int E, const T* F, long n, const T* w, T* z) {      \
-    const int W = 64 * E;                                                                    \
-    T* yn = (T*)calloc((size_t)W, sizeof(T));                                               \
-    T* lsn = (T*)calloc((size_t)W, sizeof(T));                                              \
-    T* a = (T*)calloc((size_t)W, sizeof(T));                                                \
-    T* c = (T*)calloc((size_t)W, sizeof(T));                                                \
-    T A[64], C[64], A2[64], C2[64];                                                          \
-    const T *ls = F, *inv = F + n, *ab = F + 3 * n;                                         \
-    for (int y = ny - 1; y >= 0; --y) {                                                      \
-      const long row = (long)y * nx;                                                         \
-      for (int x = 0; x < W; ++x) {                                                          \
-        if (x < nx) {                                                                        \
-          a[x] = ab[row + x];                                                                \
-          c[x] = FMA(-lsn[x], yn[x], w[row + x]) * inv[row + x];                            \
-          lsn[x] = ls[row + x];                                                              \
-        } else { a[x] = 0; c[x] = 0; lsn[x] = 0; }                                           \
-      }                                                                                      \
-      for (int l = 0; l < 64; ++l) {                                                         \
-        T Al = 1, Cl = 0;                                                                    \
-        for (int k = E - 1; k >= 0; --k) { const int x = l * E + k; Cl = FMA(a[x], Cl, c[x]); Al = a[x] * Al; } \
-        A[l] = Al; C[l] = Cl;                                                                \
-      }                                                                                      \
-      for (int d = 1; d < 64; d <<= 1) {                                                     \
-        for (int l = 0; l < 64; ++l) {                                                       \
-          if (l + d < 64) { C2[l] = FMA(A[l], C[l + d], C[l]); A2[l] = A[l] * A[l + d]; }   \
-          else { C2[l] = C[l]; A2[l] = A[l]; }                                               \
-        }                                                                                    \
-        memcpy(A, A2, sizeof(A)); memcpy(C, C2, sizeof(C));                                  \
-      }                                                                                      \
-      for (int l = 0; l < 64; ++l) {                                                         \
-        T v = l < 63 ? C[l + 1] : (T)0;                                                      \
-        for (int k = E - 1; k >= 0; --k) {                                                   \
-          const int x = l * E + k;                                                           \
-          v = FMA(a[x], v, c[x]);                                                            \
-          yn[x] = v;                                                                         \
-          if (x < nx) z[row + x] = v;                                                        \
-        }                                                                                    \
-      }                                                                                      \
-    }                                                                                        \
-    free(yn); free(lsn); free(a); free(c);                                                   \
+/* The GPU's wavefront scan, lane by lane.  Forward sweep: lane l owns cells
+ * x = l*E + k (k = 0 .. E-1); backward sweep: lane l owns cells
+ * x = 64E - 1 - (l*E + k), so the right-to-left recurrence runs in lane
+ * order too.  Cells past nx are padding with a = c = 0.
+ * A cell's map: v = a v_prev + c, c = fma(-p, carry, q) * inv (forward:
+ * p = ls, carry = w of the same cell on line y-1, q = r; backward: p = ls of
+ * line y+1, carry = z of the same cell on line y+1, q = w).  The lane's map
+ * (A, C) is composed in k order (C = fma(a, C, c); A = a*A), then a scan in
+ * lane order: shifts 1, 2, 4, 8 inside rows of 16 lanes, then lane 16r-1
+ * into rows 1 and 3, then lane 31 into rows 2 and 3; a lane that receives
+ * nothing in a step combines with the identity (1, 0); each step combines
+ * C = fma(A, Cp, C), A = A*Ap from the values before the step.  The value
+ * entering lane l is C[l-1] (0 for lane 0), and the lane re-applies its maps
+ * v = fma(a, v, c) in k order. */
+#define ORACLE_LSCAN(NAME, T, FMA)                                                              \
+  static void NAME##_scan(T* A, T* C) {                                                         \
+    T A2[64], C2[64];                                                                           \
+    for (int step = 0; step < 6; ++step) {                                                      \
+      for (int l = 0; l < 64; ++l) {                                                            \
+        int src = -1;                                                                           \
+        if (step < 4) {                                                                         \
+          const int s = 1 << step;                                                              \
+          if (l % 16 >= s) src = l - s;                                                         \
+        } else if (step == 4) {                                                                 \
+          if ((l / 16) % 2 == 1) src = 16 * (l / 16) - 1;                                       \
+        } else {                                                                                \
+          if (l / 16 >= 2) src = 31;                                                            \
+        }                                                                                       \
+        const T Ap = src >= 0 ? A[src] : (T)1, Cp = src >= 0 ? C[src] : (T)0;                   \
+        C2[l] = FMA(A[l], Cp, C[l]);                                                            \
+        A2[l] = A[l] * Ap;                                                                      \
+      }                                                                                         \
+      memcpy(A, A2, sizeof(A2));                                                                \
+      memcpy(C, C2, sizeof(C2));                                                                \
+    }                                                                                           \
+  }                                                                                             \
+  static void NAME##_sweep(int nx, int ny, int E, int bwd, const T* a_, const T* ls, const T* q, \
+                           const T* inv, T* out) {                                              \
+    const int W = 64 * E;                                                                       \
+    T* carry = (T*)calloc((size_t)W, sizeof(T));                                                \
+    T* pn = (T*)calloc((size_t)W, sizeof(T));                                                   \
+    T* a = (T*)calloc((size_t)W, sizeof(T));                                                    \
+    T* c = (T*)calloc((size_t)W, sizeof(T));                                                    \
+    T A[64], C[64];                                                                             \
+    for (int t = 0; t < ny; ++t) {                                                              \
+      const int y = bwd ? ny - 1 - t : t;                                                       \
+      const long row = (long)y * nx;                                                            \
+      /* slot u = l*E + k in lane order; cell x(u) */                                           \
+      for (int u = 0; u < W; ++u) {                                                             \
+        const int x = bwd ? W - 1 - u : u;                                                      \
+        if (x < nx) {                                                                           \
+          a[u] = a_[row + x];                                                                   \
+          const T p = bwd ? pn[u] : ls[row + x];                                                \
+          c[u] = FMA(-p, carry[u], q[row + x]) * inv[row + x];                                  \
+          if (bwd) pn[u] = ls[row + x];                                                         \
+        } else {                                                                                \
+          a[u] = 0; c[u] = 0;                                                                   \
+          if (bwd) pn[u] = 0;                                                                   \
+        }                                                                                       \
+      }                                                                                         \
+      for (int l = 0; l < 64; ++l) {                                                            \
+        T Al = 1, Cl = 0;                                                                       \
+        for (int k = 0; k < E; ++k) { const int u = l * E + k; Cl = FMA(a[u], Cl, c[u]); Al = a[u] * Al; } \
+        A[l] = Al; C[l] = Cl;                                                                   \
+      }                                                                                         \
+      NAME##_scan(A, C);                                                                        \
+      for (int l = 0; l < 64; ++l) {                                                            \
+        T v = l > 0 ? C[l - 1] : (T)0;                                                          \
+        for (int k = 0; k < E; ++k) {                                                           \
+          const int u = l * E + k;                                                              \
+          const int x = bwd ? W - 1 - u : u;                                                    \
+          v = FMA(a[u], v, c[u]);                                                               \
+          carry[u] = v;                                                                         \
+          if (x < nx) out[row + x] = v;                                                         \
+        }                                                                                       \
+      }                                                                                         \
+    }                                                                                           \
+    free(carry); free(pn); free(a); free(c);                                                    \
+  }                                                                                             \
+  static void NAME##_fwd(int nx, int ny, int E, const T* F, long n, const T* r, T* w) {         \
+    NAME##_sweep(nx, ny, E, 0, F + 2 * n, F, r, F + n, w);                                      \
+  }                                                                                             \
+  static void NAME##_bwd(int nx, int ny, int E, const T* F, long n, const T* w, T* z) {         \
+    NAME##_sweep(nx, ny, E, 1, F + 3 * n, F, w, F + n, z);                                      \
   }
 
 ORACLE_LSCAN(lscan64, double, fma)
@@ -210,6 +207,18 @@ void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float*
   free(w);
 }
 
+/* FP32 apply for double vectors: r rounded to float, z widened back */
+void oracle_lscan_apply32in64(int nx, int ny, const float* F, const double* r, double* z) {
+  const long n = (long)nx * ny;
+  float* r32 = (float*)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
+  float* z32 = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+  for (long i = 0; i < n; ++i) r32[i] = (float)r[i];
+  oracle_lscan_apply32(nx, ny, F, r32, z32);
+  for (long i = 0; i < n; ++i) z[i] = (double)z32[i];
+  free(r32);
+  free(z32);
+}
+
 oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, const float* F32) {
   oracle_precon* p = (oracle_precon*)calloc(1, sizeof(oracle_precon));
   p->ipreflag = ORACLE_PRECON_FACTOR;
@@ -223,3 +232,5 @@ oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, con
   memcpy(p->F32, F32, sizeof(float) * 4 * (size_t)n);
   return p;
 }
+
+void oracle_precon_factor_apply32(oracle_precon* p, int on) { p->apply32in64 = on; }

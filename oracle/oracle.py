@@ -321,6 +321,7 @@ def _sig_batched(lib):
         "oracle_lscan_apply": (None, [_I, _I, _P, _P, _P]),
         "oracle_lscan_apply32": (None, [_I, _I, _P, _P, _P]),
         "oracle_f32_to_bf16": (ctypes.c_ushort, [ctypes.c_float]),
+        "oracle_precon_factor_apply32": (None, [_P, _I]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -368,9 +369,12 @@ def lscan_apply(F, r, nx):
 
 
 class FactorPrecon:
-    def __init__(self, F64, F32, nx):
+    def __init__(self, F64, F32, nx, apply32=False):
+        """apply32: FP64 vectors through the FP32 apply (r rounded, z widened)."""
         self.n = F64.shape[0] // 4
         self.h = _blib().oracle_precon_create_factor(self.n, nx, _p(F64), _p(F32))
+        if apply32:
+            _blib().oracle_precon_factor_apply32(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

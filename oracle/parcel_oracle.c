@@ -427,7 +427,10 @@ oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   int n = p->n;
   if (p->ipreflag == ORACLE_PRECON_FACTOR) {
-    oracle_lscan_apply(p->nx, p->ny, p->F64, r, s);
+    if (p->apply32in64)
+      oracle_lscan_apply32in64(p->nx, p->ny, p->F32, r, s);
+    else
+      oracle_lscan_apply(p->nx, p->ny, p->F64, r, s);
     return;
   }
   if (p->ipreflag == 0) {

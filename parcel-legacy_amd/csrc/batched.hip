@@ -122,120 +122,140 @@ struct LineRegs {
   T a[E], p[E], q[E], i[E];
 };
 
-template <class T, int E, bool FULL>
+// The lane's E cells x0 .. x0+E-1 of one line; MIRROR stores them in reverse
+// (slot k holds cell x0+E-1-k), which the backward sweep uses.
+template <class T, class TQ, int E, bool FULL, bool MIRROR>
 __device__ __forceinline__ void load_line(LineRegs<T, E>& L, const T* __restrict__ a, const T* __restrict__ p,
-                                          const T* __restrict__ q, const T* __restrict__ inv, long row, int x0,
+                                          const TQ* __restrict__ q, const T* __restrict__ inv, long row, int x0,
                                           int nx) {
 #pragma unroll
-  for (int k = 0; k < E; ++k) {
-    if (FULL || x0 + k < nx) {
-      L.a[k] = a[row + k]; L.p[k] = p[row + k]; L.q[k] = q[row + k]; L.i[k] = inv[row + k];
+  for (int j = 0; j < E; ++j) {
+    const int k = MIRROR ? E - 1 - j : j;
+    if (FULL || x0 + j < nx) {
+      L.a[k] = a[row + x0 + j]; L.p[k] = p[row + x0 + j]; L.q[k] = (T)q[row + x0 + j]; L.i[k] = inv[row + x0 + j];
     } else {
       L.a[k] = 0; L.p[k] = 0; L.q[k] = 0; L.i[k] = 0;
     }
   }
 }
 
-template <class T, int E, bool FULL>
+// DPP lane moves (GFX9 controls): row_shr:n = 0x110+n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143, wave_shr:1 = 0x138.  Lanes without a source, or in
+// rows outside RM, receive `old`.
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, RM, 0xf, false));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, RM, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, RM, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One scan step: the lane's map (A, C) after the map (Ap, Cp) moved in by
+// CTRL; lanes that receive nothing get the identity (1, 0).
+template <int CTRL, int RM, class T>
+__device__ __forceinline__ void scan_step(T& A, T& C) {
+  const T Ap = dpp<CTRL, RM>((T)1, A), Cp = dpp<CTRL, RM>((T)0, C);
+  C = fma(A, Cp, C);
+  A = A * Ap;
+}
+
+// Inclusive scan of affine maps over the 64 lanes in lane order: shifts
+// 1, 2, 4, 8 inside each row of 16, then row 0's last map into rows 1 and 3
+// (row_bcast:15) and lane 31's into rows 2 and 3 (row_bcast:31).  Returns
+// the value entering the lane's cells (lane 0: 0).
+template <class T>
+__device__ __forceinline__ T scan_maps(T A, T C) {
+  scan_step<0x111, 0xf>(A, C);
+  scan_step<0x112, 0xf>(A, C);
+  scan_step<0x114, 0xf>(A, C);
+  scan_step<0x118, 0xf>(A, C);
+  scan_step<0x142, 0xa>(A, C);
+  scan_step<0x143, 0xc>(A, C);
+  return dpp<0x138, 0xf>((T)0, C);
+}
+
+// One sweep over the lines (forward: y = 0 .. ny-1, lane cells in increasing
+// x; backward: y = ny-1 .. 0, lane cells mirrored so the recurrence still
+// runs in lane order).  Map of a cell: v = a v_prev + c with
+// c = fma(-p, carry, q) * inv, carry = the value of the same cell on the
+// previous line, p = ls of that previous line for the backward sweep.
+template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
+__device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, const T* __restrict__ ls,
+                                      const TQ* __restrict__ q, const T* __restrict__ inv, TO* __restrict__ out) {
+  constexpr int kAhead = lines_ahead<T, E>();
+  const int lane = threadIdx.x;
+  const int x0 = BWD ? 64 * E - (lane + 1) * E : lane * E;
+  T carry[E], pn[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) { carry[k] = 0; pn[k] = 0; }
+  LineRegs<T, E> buf[kAhead];
+#pragma unroll
+  for (int d = 0; d < kAhead; ++d) {
+    const int y = BWD ? ny - 1 - d : d;
+    if (d < ny) load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, (long)y * nx, x0, nx);
+  }
+  for (int t0 = 0; t0 < ny; t0 += kAhead) {
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      const int t = t0 + d;
+      if (t >= ny) break;
+      const int y = BWD ? ny - 1 - t : t;
+      const long row = (long)y * nx;
+      T am[E], c[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int x = BWD ? x0 + E - 1 - k : x0 + k;
+        if (FULL || x < nx) {
+          am[k] = buf[d].a[k];
+          // forward: p is this line's ls; backward: the previous (upper) line's
+          const T pk = BWD ? pn[k] : buf[d].p[k];
+          c[k] = fma(-pk, carry[k], buf[d].q[k]) * buf[d].i[k];
+          if (BWD) pn[k] = buf[d].p[k];
+        } else {
+          am[k] = 0; c[k] = 0;
+          if (BWD) pn[k] = 0;
+        }
+      }
+      if (t + kAhead < ny) {
+        const int yn = BWD ? y - kAhead : y + kAhead;
+        load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, (long)yn * nx, x0, nx);
+      }
+      T A = 1, C = 0;
+#pragma unroll
+      for (int k = 0; k < E; ++k) { C = fma(am[k], C, c[k]); A = am[k] * A; }
+      T v = scan_maps(A, C);
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int x = BWD ? x0 + E - 1 - k : x0 + k;
+        v = fma(am[k], v, c[k]);
+        carry[k] = v;
+        if (FULL || x < nx) out[row + x] = (TO)v;
+      }
+    }
+  }
+}
+
+// z = (L L^T)^-1 r for every system, one wavefront per system: forward sweep
+// into w, then the backward sweep into z.  T: arithmetic and factor type; TV:
+// the type of r and z (double with T = float: the FP32 apply of an FP64 PCG).
+template <class T, class TV, int E, bool FULL>
 __global__ void __launch_bounds__(64) k_lscan(int nx, int ny, long n, long nv, const T* __restrict__ F,
-                                              const T* __restrict__ r, T* __restrict__ w, T* __restrict__ z,
+                                              const TV* __restrict__ r, T* __restrict__ w, TV* __restrict__ z,
                                               const CGState* st, int it) {
   if (sys_done(st, it)) return;
-  const int lane = threadIdx.x;
   const long so = (long)blockIdx.x * n;
-  const T* __restrict__ ls = F + so;
-  const T* __restrict__ inv = F + nv + so;
-  const T* __restrict__ af = F + 2 * nv + so;
-  const T* __restrict__ ab = F + 3 * nv + so;
-  r += so; w += so; z += so;
-  const int x0 = lane * E;
-  constexpr int kAhead = lines_ahead<T, E>();
-  T zp[E];
-#pragma unroll
-  for (int k = 0; k < E; ++k) zp[k] = 0;
-  LineRegs<T, E> buf[kAhead];
-  // forward: lines bottom-up, cells left to right (p = ls, q = r)
-#pragma unroll
-  for (int d = 0; d < kAhead; ++d)
-    if (d < ny) load_line<T, E, FULL>(buf[d], af, ls, r, inv, (long)d * nx + x0, x0, nx);
-  for (int y0 = 0; y0 < ny; y0 += kAhead) {
-#pragma unroll
-    for (int d = 0; d < kAhead; ++d) {
-      const int y = y0 + d;
-      if (y >= ny) break;
-      const long row = (long)y * nx + x0;
-      T a[E], c[E];
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        if (FULL || x0 + k < nx) {
-          a[k] = buf[d].a[k];
-          c[k] = fma(-buf[d].p[k], zp[k], buf[d].q[k]) * buf[d].i[k];
-        } else {
-          a[k] = 0; c[k] = 0;
-        }
-      }
-      if (y + kAhead < ny) load_line<T, E, FULL>(buf[d], af, ls, r, inv, row + (long)kAhead * nx, x0, nx);
-      T A = 1, C = 0;
-#pragma unroll
-      for (int k = 0; k < E; ++k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
-#pragma unroll
-      for (int s = 1; s < 64; s <<= 1) {
-        const T Ap = __shfl_up(A, s, 64), Cp = __shfl_up(C, s, 64);
-        if (lane >= s) { C = fma(A, Cp, C); A = A * Ap; }
-      }
-      T v = __shfl_up(C, 1, 64);
-      if (lane == 0) v = 0;
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        v = fma(a[k], v, c[k]);
-        zp[k] = v;
-        if (FULL || x0 + k < nx) w[row + k] = v;
-      }
-    }
-  }
-  // backward: lines top-down, cells right to left (p = ls, q = w)
-  T yn[E], lsn[E];
-#pragma unroll
-  for (int k = 0; k < E; ++k) { yn[k] = 0; lsn[k] = 0; }
-#pragma unroll
-  for (int d = 0; d < kAhead; ++d)
-    if (ny - 1 - d >= 0) load_line<T, E, FULL>(buf[d], ab, ls, w, inv, (long)(ny - 1 - d) * nx + x0, x0, nx);
-  for (int y0 = ny - 1; y0 >= 0; y0 -= kAhead) {
-#pragma unroll
-    for (int d = 0; d < kAhead; ++d) {
-      const int y = y0 - d;
-      if (y < 0) break;
-      const long row = (long)y * nx + x0;
-      T a[E], c[E];
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        if (FULL || x0 + k < nx) {
-          a[k] = buf[d].a[k];
-          c[k] = fma(-lsn[k], yn[k], buf[d].q[k]) * buf[d].i[k];
-          lsn[k] = buf[d].p[k];
-        } else {
-          a[k] = 0; c[k] = 0; lsn[k] = 0;
-        }
-      }
-      if (y - kAhead >= 0) load_line<T, E, FULL>(buf[d], ab, ls, w, inv, row - (long)kAhead * nx, x0, nx);
-      T A = 1, C = 0;
-#pragma unroll
-      for (int k = E - 1; k >= 0; --k) { C = fma(a[k], C, c[k]); A = a[k] * A; }
-#pragma unroll
-      for (int s = 1; s < 64; s <<= 1) {
-        const T Ap = __shfl_down(A, s, 64), Cp = __shfl_down(C, s, 64);
-        if (lane + s < 64) { C = fma(A, Cp, C); A = A * Ap; }
-      }
-      T v = __shfl_down(C, 1, 64);
-      if (lane == 63) v = 0;
-#pragma unroll
-      for (int k = E - 1; k >= 0; --k) {
-        v = fma(a[k], v, c[k]);
-        yn[k] = v;
-        if (FULL || x0 + k < nx) z[row + k] = v;
-      }
-    }
-  }
+  const T* ls = F + so;
+  const T* inv = F + nv + so;
+  const T* af = F + 2 * nv + so;
+  const T* ab = F + 3 * nv + so;
+  sweep<T, TV, T, E, FULL, false>(nx, ny, af, ls, r + so, inv, w + so);
+  // the backward sweep reads w cells written by other lanes
+  __syncthreads();
+  sweep<T, T, TV, E, FULL, true>(nx, ny, ab, ls, w + so, inv, z + so);
 }
 
 // s = Dinv r (dinv null: s = r), every system
@@ -354,15 +374,17 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
 
 // --- host helpers ------------------------------------------------------------------
 
-template <class T>
-void launch_lscan(const Batch& B, const T* F, const T* r, T* w, T* z, const CGState* st, int it) {
+template <class T, class TV>
+void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TV* z, const CGState* st, int it) {
   const int nx = B.nx;
   const dim3 g(B.nsys), b(64);
-#define PARCEL_LSCAN(E)                                                                                          \
-  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                           \
-    hipLaunchKernelGGL((k_lscan<T, E, true>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, st, it); \
-  else                                                                                                           \
-    hipLaunchKernelGGL((k_lscan<T, E, false>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, st, it);
+#define PARCEL_LSCAN(E)                                                                                     \
+  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                      \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, true>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, \
+                       st, it);                                                                             \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, false>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w,   \
+                       z, st, it);
   if (nx <= 64) { PARCEL_LSCAN(1) }
   else if (nx <= 128) { PARCEL_LSCAN(2) }
   else if (nx <= 256) { PARCEL_LSCAN(4) }
@@ -411,6 +433,8 @@ struct BVecs {
   T *r, *q, *s, *p0, *p1, *w;
   const T* dinv;  // point Jacobi (PARCEL_BATCH_JACOBI) or null
   const T* F;     // factor apply arrays (IC0 / FACTOR) or null
+  const float* Fmix = nullptr;  // FP32 factor apply arrays used on T vectors (FP32 apply in FP64 PCG)
+  float* wmix = nullptr;        // its FP32 intermediate vector
   double* hist;
   int hist_cap;
 };
@@ -435,8 +459,11 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
   }
   PARCEL_HIP_CHECK(hipMemsetAsync(V.p0, 0, sizeof(T) * (size_t)B.nv(), st));
   hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, V.A, (const T*)x, b, V.r, part, bt);
+  const bool fac = V.F || V.Fmix;
   if (V.F)
-    launch_lscan<T>(B, V.F, V.r, V.w, V.s, nullptr, 0);
+    launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, nullptr, 0);
+  else if (V.Fmix)
+    launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, V.s, nullptr, 0);
   else
     hipLaunchKernelGGL(k_b_scale<T>, grid, block, 0, st, n, (const T*)V.r, V.dinv, V.s, bt);
   hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax,
@@ -464,14 +491,17 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
       launches++;
       if (pf.on) pf.spmv.push_back({ea, pf.rec(st)});
       if (it > itrmax) break;
-      if (!V.F) {
+      if (!fac) {
         hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, V.dinv, V.s, part,
                            S, nb, bt);
         launches++;
       } else {
         hipLaunchKernelGGL(k_b_axpy_r<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
         hipEvent_t eb = pf.rec(st);
-        launch_lscan<T>(B, V.F, V.r, V.w, V.s, S, it);
+        if (V.F)
+          launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, S, it);
+        else
+          launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, V.s, S, it);
         if (pf.on) pf.apply.push_back({eb, pf.rec(st)});
         hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
                            (const CGState*)S, bt);
@@ -760,13 +790,16 @@ void ensure_hist(Batch& B, int cap) {
 int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, int precision, double* hist_host,
                 int* niter, int* iret) {
   if (!x || !b) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: null vector");
-  if (precision != 64 && precision != 32) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: precision must be 64 or 32");
+  if (precision != PARCEL_BATCH_FP64 && precision != PARCEL_BATCH_REFINE32 && precision != PARCEL_BATCH_APPLY32)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: precision must be 64, 32 or 48");
+  if (precision == PARCEL_BATCH_APPLY32 && B.precon != PARCEL_BATCH_IC0 && B.precon != PARCEL_BATCH_FACTOR)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: precision 48 needs a factor preconditioner");
   if (B.precon == PARCEL_BATCH_FACTOR && !B.factor_src)
     return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: FACTOR preconditioner without a factor (batch_set_factor)");
   auto t0 = std::chrono::steady_clock::now();
   const hipStream_t st = B.stream;
   const size_t nv = (size_t)B.nv();
-  ensure_vectors(B, precision == 32);
+  ensure_vectors(B, precision != PARCEL_BATCH_FP64);
   const int cap = itrmax > 0 ? itrmax : 1;
   ensure_hist(B, cap);
   PARCEL_HIP_CHECK(hipMemsetAsync(B.hist.p, 0, sizeof(double) * (size_t)B.nsys * B.hist_cap, st));
@@ -788,8 +821,13 @@ int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, i
   B.stats.outer_steps = 0;
   B.stats.inner_total = 0;
   B.stats.inner_steps = 0;
-  if (precision == 64) {
+  if (precision == PARCEL_BATCH_FP64 || precision == PARCEL_BATCH_APPLY32) {
     BVecs<double> V = vecs_of<double>(B);
+    if (precision == PARCEL_BATCH_APPLY32) {
+      V.F = nullptr;
+      V.Fmix = B.F32.p;
+      V.wmix = B.w32.p;
+    }
     V.hist = B.hist.p;
     V.hist_cap = B.hist_cap;
     std::vector<CGState> out;
@@ -1001,7 +1039,7 @@ int precon_apply_any(parcel_hip_batch* h, const T* r, T* z) {
   BVecs<T> V = vecs_of<T>(B);
   if (V.F) {
     tw.alloc(nv);
-    launch_lscan<T>(B, V.F, dr, tw.p, dz, nullptr, 0);
+    launch_lscan<T, T>(B, V.F, dr, tw.p, dz, nullptr, 0);
   } else {
     hipLaunchKernelGGL(k_b_scale<T>, dim3(tiled_nb(B.n), B.nsys), dim3(kNT), 0, B.stream, B.n, dr, V.dinv, dz,
                        Bat{(long)B.n, B.A.entries});

@@ -144,6 +144,7 @@ SYMBOLS = {
 }
 
 BATCH_NONE, BATCH_JACOBI, BATCH_IC0, BATCH_FACTOR = 0, 1, 2, 3
+BATCH_FP64, BATCH_REFINE32, BATCH_APPLY32 = 64, 32, 48
 
 
 class BatchStats(ctypes.Structure):

@@ -177,3 +177,32 @@ def test_batch_rejects_non_five_point_for_factor():
         x = np.zeros_like(b)
         _, niter, iret = B.solve(x, b, 500, 1e-10)
         assert iret[0] == 0
+
+
+@pytest.mark.parametrize("kind", ["ic0", "bf16"])
+def test_batch_fp32_apply_in_fp64_pcg_bit_exact(kind):
+    print("batch fp32 apply", kind, flush=True)
+    nx, ny, nsys = 512, 12, 2
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    n = nx * ny
+    if kind == "ic0":
+        B = ph.Batch(rp, col, vals, ph.BATCH_IC0)
+        facs = [orc.derive(*ic0_factor(rp, vals[s], nx), nx) for s in range(nsys)]
+    else:
+        Lb = [[], [], []]
+        for s in range(nsys):
+            for k, a in enumerate(ic0_factor(rp, vals[s], nx)):
+                Lb[k].append(orc.f32_to_bf16(a.astype(np.float32)))
+        Lb = [np.ascontiguousarray(np.concatenate(v)) for v in Lb]
+        B = ph.Batch(rp, col, vals, ph.BATCH_FACTOR)
+        B.set_factor(*Lb)
+        facs = [orc.derive(Lb[0][s * n:(s + 1) * n], Lb[1][s * n:(s + 1) * n], Lb[2][s * n:(s + 1) * n], nx)
+                for s in range(nsys)]
+    with B:
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 3000, 1e-10, precision=ph.BATCH_APPLY32)
+    for s in range(nsys):
+        M = orc.FactorPrecon(facs[s][0], facs[s][1], nx, apply32=True)
+        xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=3000, rtol=1e-10)
+        assert iret[s] == 0 and niter[s] == no, s
+        assert np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s
