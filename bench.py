@@ -275,10 +275,12 @@ def run_c5(args):
         B.spmv(x_d, y)
         return float(max(np.linalg.norm(b[s] - y[s]) / np.linalg.norm(b[s]) for s in range(nsys)))
 
-    def batched(name, kind, precision, params=None, steps=args.steps, warmup=args.warmup):
+    def batched(name, kind, precision, params=None, steps=args.steps, warmup=args.warmup, strips=1):
         t = time.perf_counter()
         B = ph.Batch(rp, col, vals_d, kind, nsys=nsys)
         B.set_inner(args.inner_rtol, 20000)
+        if kind != ph.BATCH_JACOBI:
+            B.set_strips(strips)
         setup = time.perf_counter() - t
         net = ph.Mpnn(params) if params is not None else None
         itr = 50 if precision == ph.BATCH_REFINE32 else 20000
@@ -301,7 +303,7 @@ def run_c5(args):
         B.close()
         if net is not None:
             net.close()
-        m = {"time_to_solution_s": float(np.mean(times)), "steps": len(times), "setup_s": setup,
+        m = {"time_to_solution_s": float(np.mean(times)), "steps": len(times), "setup_s": setup, "strips": strips,
              "niter_max": int(niter.max()), "niter_mean": float(niter.mean()), "iret_max": int(iret.max()),
              "true_rel_res_max": res, "device_ms": st["solve_device_ms"],
              "apply_ms": pst["apply_ms"] / max(pst["apply_calls"], 1), "apply_calls": pst["apply_calls"],
@@ -317,12 +319,22 @@ def run_c5(args):
     # headline: FP64 PCG whose preconditioner (the network's BF16 factor) is
     # applied in FP32; then the FP64-refinement form, the all-FP64 form, and
     # the same solver with the untrained network / the classical IC(0) factor
-    learned = batched("learned_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p)
-    batched("learned_refine32", ph.BATCH_FACTOR, ph.BATCH_REFINE32, params=p, steps=1)
-    batched("learned_fp64", ph.BATCH_FACTOR, ph.BATCH_FP64, params=p, steps=1)
-    batched("untrained_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=L.init_params(0), steps=1, warmup=0)
-    batched("ic0_apply32", ph.BATCH_IC0, ph.BATCH_APPLY32, steps=1, warmup=0)
-    batched("ic0_fp64", ph.BATCH_IC0, ph.BATCH_FP64, steps=1, warmup=0)
+    # strips per system for the factor solves: the fastest of a one-solve sweep
+    ssweep = {}
+    for ns in [int(v) for v in args.c5_strips.split(",")]:
+        ssweep[ns] = batched("strips_%d" % ns, ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, steps=1,
+                             warmup=0, strips=ns)["time_to_solution_s"]
+        del methods["strips_%d" % ns]
+    ns = min(ssweep, key=ssweep.get)
+    learned = batched("learned_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, strips=ns)
+    learned["strip_sweep_s"] = ssweep
+    batched("learned_apply32_exact_factor", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, steps=1, strips=1)
+    batched("learned_refine32", ph.BATCH_FACTOR, ph.BATCH_REFINE32, params=p, steps=1, strips=ns)
+    batched("learned_fp64", ph.BATCH_FACTOR, ph.BATCH_FP64, params=p, steps=1, strips=ns)
+    batched("untrained_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=L.init_params(0), steps=1, warmup=0,
+            strips=ns)
+    batched("ic0_apply32", ph.BATCH_IC0, ph.BATCH_APPLY32, steps=1, warmup=0, strips=ns)
+    batched("ic0_fp64", ph.BATCH_IC0, ph.BATCH_FP64, steps=1, warmup=0, strips=ns)
     batched("jacobi_fp64", ph.BATCH_JACOBI, ph.BATCH_FP64, steps=1, warmup=0)
 
     # plain FP64 block-Jacobi ILU(0) PCG, parcel_dcg's solver, one system at a time;
@@ -403,8 +415,9 @@ def run_c5(args):
         "dtype": "bf16/f32/f64",
         "workload": "2D 5-point diffusion 512^2, kappa=exp(GRF, SE ell=16 sigma=2), %d test systems (field seeds "
                     "3048..%d), b~N(0,1), x0=0, FP64 rtol 1e-10 per system" % (nsys, 3048 + nsys - 1),
-        "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L, M = L L^T) applied in FP32 inside "
-                  "batched FP64 PCG (64 systems per launch); time includes the network inference",
+        "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L) applied in FP32 inside batched FP64 "
+                  "PCG (64 systems per launch), factor solves on %d strip(s) of grid lines per system "
+                  "(M = diag(L_k L_k^T)); time includes the network inference" % ns,
         "headline": "learned_apply32",
         "speedup_vs_plain_fp64": round(plain["time_to_solution_s"] / learned["time_to_solution_s"], 3),
         "plain_fp64_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
@@ -428,6 +441,7 @@ def main():
     ap.add_argument("--c5-blocks", default="8,16,32,64,256",
                     help="block counts of the plain FP64 BJ ILU(0) baseline sweep (fastest is used)")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--c5-strips", default="1,2,4,8", help="strips per system of the factor solves (sweep)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "parcel-legacy_amd", "weights", "mpnn_grf512.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)

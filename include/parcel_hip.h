@@ -318,6 +318,11 @@ int parcel_hip_batch_set_factor(parcel_hip_batch* batch, const unsigned short* L
 int parcel_hip_batch_solve(parcel_hip_batch* batch, double* x, const double* b, int itrmax, double rtol,
                            int precision, double* hist, int* niter, int* iret);
 int parcel_hip_batch_set_inner(parcel_hip_batch* batch, double inner_rtol, int inner_itrmax);
+/* Factor solves on nstrip independent strips of grid lines per system (lines
+ * [ny k / nstrip, ny (k+1) / nstrip) for strip k): M = diag(L_k L_k^T), the
+ * factor's couplings between strips dropped; nstrip wavefronts per system.
+ * Default 1 (M = L L^T exactly). */
+int parcel_hip_batch_set_strips(parcel_hip_batch* batch, int nstrip);
 int parcel_hip_batch_set_profile(parcel_hip_batch* batch, int on);
 int parcel_hip_batch_get_stats(const parcel_hip_batch* batch, parcel_hip_batch_stats* stats);
 /* y = A x for every system (x, y: nsys*n, host or device) */

@@ -178,42 +178,55 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
       }                                                                                         \
     }                                                                                           \
     free(carry); free(pn); free(a); free(c);                                                    \
-  }                                                                                             \
-  static void NAME##_fwd(int nx, int ny, int E, const T* F, long n, const T* r, T* w) {         \
-    NAME##_sweep(nx, ny, E, 0, F + 2 * n, F, r, F + n, w);                                      \
-  }                                                                                             \
-  static void NAME##_bwd(int nx, int ny, int E, const T* F, long n, const T* w, T* z) {         \
-    NAME##_sweep(nx, ny, E, 1, F + 3 * n, F, w, F + n, z);                                      \
   }
 
 ORACLE_LSCAN(lscan64, double, fma)
 ORACLE_LSCAN(lscan32, float, fmaf)
 
-void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z) {
+/* Strips: lines [ny k / nstrip, ny (k+1) / nstrip) solved independently. */
+void oracle_lscan_apply_strips(int nx, int ny, int nstrip, const double* F, const double* r, double* z) {
   const long n = (long)nx * ny;
-  double* w = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double* w = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
   const int E = lanes_e(nx);
-  lscan64_fwd(nx, ny, E, F, n, r, w);
-  lscan64_bwd(nx, ny, E, F, n, w, z);
+  for (int k = 0; k < nstrip; ++k) {
+    const int y0 = (int)((long)ny * k / nstrip), y1 = (int)((long)ny * (k + 1) / nstrip);
+    if (y1 <= y0) continue;
+    const long o = (long)y0 * nx;
+    lscan64_sweep(nx, y1 - y0, E, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
+    lscan64_sweep(nx, y1 - y0, E, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
+  }
   free(w);
+}
+
+void oracle_lscan_apply32_strips(int nx, int ny, int nstrip, const float* F, const float* r, float* z) {
+  const long n = (long)nx * ny;
+  float* w = (float*)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
+  const int E = lanes_e(nx);
+  for (int k = 0; k < nstrip; ++k) {
+    const int y0 = (int)((long)ny * k / nstrip), y1 = (int)((long)ny * (k + 1) / nstrip);
+    if (y1 <= y0) continue;
+    const long o = (long)y0 * nx;
+    lscan32_sweep(nx, y1 - y0, E, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
+    lscan32_sweep(nx, y1 - y0, E, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
+  }
+  free(w);
+}
+
+void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z) {
+  oracle_lscan_apply_strips(nx, ny, 1, F, r, z);
 }
 
 void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float* z) {
-  const long n = (long)nx * ny;
-  float* w = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
-  const int E = lanes_e(nx);
-  lscan32_fwd(nx, ny, E, F, n, r, w);
-  lscan32_bwd(nx, ny, E, F, n, w, z);
-  free(w);
+  oracle_lscan_apply32_strips(nx, ny, 1, F, r, z);
 }
 
 /* FP32 apply for double vectors: r rounded to float, z widened back */
-void oracle_lscan_apply32in64(int nx, int ny, const float* F, const double* r, double* z) {
+void oracle_lscan_apply32in64(int nx, int ny, int nstrip, const float* F, const double* r, double* z) {
   const long n = (long)nx * ny;
   float* r32 = (float*)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
   float* z32 = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
   for (long i = 0; i < n; ++i) r32[i] = (float)r[i];
-  oracle_lscan_apply32(nx, ny, F, r32, z32);
+  oracle_lscan_apply32_strips(nx, ny, nstrip, F, r32, z32);
   for (long i = 0; i < n; ++i) z[i] = (double)z32[i];
   free(r32);
   free(z32);
@@ -226,6 +239,7 @@ oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, con
   p->nx = nx;
   p->ny = n / nx;
   p->nblocks = 1;
+  p->nstrip = 1;
   p->F64 = (double*)malloc(sizeof(double) * 4 * (size_t)n);
   p->F32 = (float*)malloc(sizeof(float) * 4 * (size_t)n);
   memcpy(p->F64, F64, sizeof(double) * 4 * (size_t)n);
@@ -234,3 +248,4 @@ oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, con
 }
 
 void oracle_precon_factor_apply32(oracle_precon* p, int on) { p->apply32in64 = on; }
+void oracle_precon_factor_strips(oracle_precon* p, int nstrip) { p->nstrip = nstrip < 1 ? 1 : nstrip; }

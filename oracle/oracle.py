@@ -320,8 +320,10 @@ def _sig_batched(lib):
         "oracle_precon_create_factor": (_P, [_I, _I, _P, _P]),
         "oracle_lscan_apply": (None, [_I, _I, _P, _P, _P]),
         "oracle_lscan_apply32": (None, [_I, _I, _P, _P, _P]),
+        "oracle_lscan_apply_strips": (None, [_I, _I, _I, _P, _P, _P]),
         "oracle_f32_to_bf16": (ctypes.c_ushort, [ctypes.c_float]),
         "oracle_precon_factor_apply32": (None, [_P, _I]),
+        "oracle_precon_factor_strips": (None, [_P, _I]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -359,22 +361,30 @@ def derive(Ls, Lw, Ld, nx):
     return F64, F32
 
 
-def lscan_apply(F, r, nx):
+def lscan_apply(F, r, nx, nstrip=1):
     """z = (L L^T)^-1 r with the GPU's lane-scan order (F float64 or float32)."""
     n = r.shape[0]
     z = np.zeros_like(r)
+    if nstrip != 1:
+        assert r.dtype == np.float64
+        _blib().oracle_lscan_apply_strips(nx, n // nx, nstrip, _p(np.ascontiguousarray(F)),
+                                          _p(np.ascontiguousarray(r)), _p(z))
+        return z
     fn = _blib().oracle_lscan_apply32 if r.dtype == np.float32 else _blib().oracle_lscan_apply
     fn(nx, n // nx, _p(np.ascontiguousarray(F)), _p(np.ascontiguousarray(r)), _p(z))
     return z
 
 
 class FactorPrecon:
-    def __init__(self, F64, F32, nx, apply32=False):
-        """apply32: FP64 vectors through the FP32 apply (r rounded, z widened)."""
+    def __init__(self, F64, F32, nx, apply32=False, nstrip=1):
+        """apply32: FP64 vectors through the FP32 apply (r rounded, z widened);
+        nstrip: independent strips of grid lines."""
         self.n = F64.shape[0] // 4
         self.h = _blib().oracle_precon_create_factor(self.n, nx, _p(F64), _p(F32))
         if apply32:
             _blib().oracle_precon_factor_apply32(self.h, 1)
+        if nstrip != 1:
+            _blib().oracle_precon_factor_strips(self.h, nstrip)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -24,12 +24,15 @@ struct oracle_precon {
   double* F64; /* ls, 1/Ld, a_fwd, a_bwd (4n) */
   float* F32;  /* the FP32 apply arrays (4n) */
   int apply32in64; /* FP64 vectors through the FP32 apply */
+  int nstrip;      /* independent strips of grid lines */
 };
 
 /* factor solves of batched.c */
 void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z);
 void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float* z);
-void oracle_lscan_apply32in64(int nx, int ny, const float* F, const double* r, double* z);
+void oracle_lscan_apply32in64(int nx, int ny, int nstrip, const float* F, const double* r, double* z);
+void oracle_lscan_apply_strips(int nx, int ny, int nstrip, const double* F, const double* r, double* z);
+void oracle_lscan_apply32_strips(int nx, int ny, int nstrip, const float* F, const float* r, float* z);
 
 /* D-ILU L/U entry ranges of rows lo..hi (1-based), init_solve_ILU_crs order */
 void oracle_dilu_ranges(oracle_precon* p, int lo, int hi);

@@ -428,9 +428,9 @@ void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   int n = p->n;
   if (p->ipreflag == ORACLE_PRECON_FACTOR) {
     if (p->apply32in64)
-      oracle_lscan_apply32in64(p->nx, p->ny, p->F32, r, s);
+      oracle_lscan_apply32in64(p->nx, p->ny, p->nstrip, p->F32, r, s);
     else
-      oracle_lscan_apply(p->nx, p->ny, p->F64, r, s);
+      oracle_lscan_apply_strips(p->nx, p->ny, p->nstrip, p->F64, r, s);
     return;
   }
   if (p->ipreflag == 0) {
@@ -484,7 +484,7 @@ static void ilu0_apply_block32(const oracle_precon* p, int b, int lo, int hi, co
 
 void oracle_precon_apply32(const oracle_precon* p, const float* r, float* s) {
   if (p->ipreflag == ORACLE_PRECON_FACTOR) {
-    oracle_lscan_apply32(p->nx, p->ny, p->F32, r, s);
+    oracle_lscan_apply32_strips(p->nx, p->ny, p->nstrip, p->F32, r, s);
     return;
   }
   int n = p->n;

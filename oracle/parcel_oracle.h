@@ -153,6 +153,8 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
 oracle_precon* oracle_precon_create_factor(int n, int nx, const double* F64, const float* F32);
 /* on != 0: FP64 vectors go through the FP32 apply (r rounded, z widened) */
 void oracle_precon_factor_apply32(oracle_precon* p, int on);
+/* factor solves on nstrip independent strips of grid lines (default 1) */
+void oracle_precon_factor_strips(oracle_precon* p, int nstrip);
 void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z);
 void oracle_lscan_apply32(int nx, int ny, const float* F, const float* r, float* z);
 float oracle_bf16_to_f32(unsigned short h);

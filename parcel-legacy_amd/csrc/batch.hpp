@@ -21,6 +21,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   int nsys = 0, n = 0, nnz = 0;
   int nx = 0, ny = 0;  // five-point grid (0: the pattern is not one)
+  int nstrip = 1;      // factor solves: independent strips of grid lines per system
   int precon = PARCEL_BATCH_NONE;
   int factor_src = 0;  // 0 none, 1 FP64 factor (IC(0)), 2 BF16 factor
   DevSELL A;           // shared pattern; A.val / A.val32 hold nsys*entries values

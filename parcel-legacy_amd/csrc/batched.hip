@@ -104,9 +104,9 @@ __global__ void k_b_derive(int n, int nx, long nv, const S* __restrict__ L, T* _
 
 // --- factor solves: z = (L L^T)^-1 r, one wavefront per system -----------------
 
-__device__ __forceinline__ bool sys_done(const CGState* st, int it) {
+__device__ __forceinline__ bool sys_done(const CGState* st, int sys, int it) {
   if (!st) return false;
-  const int d = st[blockIdx.x].done_at;
+  const int d = st[sys].done_at;
   return d != 0 && d <= it;
 }
 
@@ -239,23 +239,30 @@ __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, c
   }
 }
 
-// z = (L L^T)^-1 r for every system, one wavefront per system: forward sweep
-// into w, then the backward sweep into z.  T: arithmetic and factor type; TV:
-// the type of r and z (double with T = float: the FP32 apply of an FP64 PCG).
+// z = (L L^T)^-1 r for every system: forward sweep into w, then the
+// backward sweep into z, one wavefront per strip of grid lines.  With nstrip
+// strips per system, strip k holds lines [ny k / nstrip, ny (k+1) / nstrip)
+// and drops the factor's couplings to the neighbouring strips (block-diagonal
+// M = diag(L_k L_k^T)); nstrip = 1 is the exact M = L L^T.  T: arithmetic and
+// factor type; TV: the type of r and z (double with T = float: the FP32
+// apply of an FP64 PCG).
 template <class T, class TV, int E, bool FULL>
-__global__ void __launch_bounds__(64) k_lscan(int nx, int ny, long n, long nv, const T* __restrict__ F,
+__global__ void __launch_bounds__(64) k_lscan(int nx, int ny, int nstrip, long n, long nv, const T* __restrict__ F,
                                               const TV* __restrict__ r, T* __restrict__ w, TV* __restrict__ z,
                                               const CGState* st, int it) {
-  if (sys_done(st, it)) return;
-  const long so = (long)blockIdx.x * n;
+  const int sys = blockIdx.x / nstrip, strip = blockIdx.x - sys * nstrip;
+  if (sys_done(st, sys, it)) return;
+  const int y0 = (int)((long)ny * strip / nstrip), y1 = (int)((long)ny * (strip + 1) / nstrip);
+  if (y1 <= y0) return;
+  const long so = (long)sys * n + (long)y0 * nx;
   const T* ls = F + so;
   const T* inv = F + nv + so;
   const T* af = F + 2 * nv + so;
   const T* ab = F + 3 * nv + so;
-  sweep<T, TV, T, E, FULL, false>(nx, ny, af, ls, r + so, inv, w + so);
+  sweep<T, TV, T, E, FULL, false>(nx, y1 - y0, af, ls, r + so, inv, w + so);
   // the backward sweep reads w cells written by other lanes
   __syncthreads();
-  sweep<T, T, TV, E, FULL, true>(nx, ny, ab, ls, w + so, inv, z + so);
+  sweep<T, T, TV, E, FULL, true>(nx, y1 - y0, ab, ls, w + so, inv, z + so);
 }
 
 // s = Dinv r (dinv null: s = r), every system
@@ -377,14 +384,14 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
 template <class T, class TV>
 void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TV* z, const CGState* st, int it) {
   const int nx = B.nx;
-  const dim3 g(B.nsys), b(64);
-#define PARCEL_LSCAN(E)                                                                                     \
-  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                      \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, true>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w, z, \
-                       st, it);                                                                             \
-  else                                                                                                      \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, false>), g, b, 0, B.stream, nx, B.ny, (long)B.n, B.nv(), F, r, w,   \
-                       z, st, it);
+  const dim3 g(B.nsys * B.nstrip), b(64);
+#define PARCEL_LSCAN(E)                                                                                          \
+  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                           \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, true>), g, b, 0, B.stream, nx, B.ny, B.nstrip, (long)B.n, B.nv(), F, r, \
+                       w, z, st, it);                                                                            \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, false>), g, b, 0, B.stream, nx, B.ny, B.nstrip, (long)B.n, B.nv(), F,   \
+                       r, w, z, st, it);
   if (nx <= 64) { PARCEL_LSCAN(1) }
   else if (nx <= 128) { PARCEL_LSCAN(2) }
   else if (nx <= 256) { PARCEL_LSCAN(4) }
@@ -981,6 +988,13 @@ int parcel_hip_batch_set_inner(parcel_hip_batch* h, double inner_rtol, int inner
   if (!h || !(inner_rtol > 0.0) || inner_itrmax < 1) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_inner: bad argument");
   h->B.inner_rtol = inner_rtol;
   h->B.inner_itrmax = inner_itrmax;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_set_strips(parcel_hip_batch* h, int nstrip) {
+  if (!h || nstrip < 1) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_strips: bad argument");
+  if (h->B.ny && nstrip > h->B.ny) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_strips: more strips than lines");
+  h->B.nstrip = nstrip;
   return PARCEL_HIP_OK;
 }
 

@@ -123,6 +123,7 @@ SYMBOLS = {
     "parcel_hip_batch_set_factor": (_I, [_P, _P, _P, _P]),
     "parcel_hip_batch_solve": (_I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
     "parcel_hip_batch_set_inner": (_I, [_P, _D, _I]),
+    "parcel_hip_batch_set_strips": (_I, [_P, _I]),
     "parcel_hip_batch_set_profile": (_I, [_P, _I]),
     "parcel_hip_batch_get_stats": (_I, [_P, _P]),
     "parcel_hip_batch_spmv": (_I, [_P, _P, _P]),
@@ -566,6 +567,9 @@ class Batch:
 
     def set_inner(self, inner_rtol, inner_itrmax):
         _check(self.lib.parcel_hip_batch_set_inner(self.h, inner_rtol, inner_itrmax))
+
+    def set_strips(self, nstrip):
+        _check(self.lib.parcel_hip_batch_set_strips(self.h, nstrip))
 
     def set_profile(self, on):
         _check(self.lib.parcel_hip_batch_set_profile(self.h, 1 if on else 0))

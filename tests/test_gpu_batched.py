@@ -206,3 +206,25 @@ def test_batch_fp32_apply_in_fp64_pcg_bit_exact(kind):
         xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=3000, rtol=1e-10)
         assert iret[s] == 0 and niter[s] == no, s
         assert np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s
+
+
+@pytest.mark.parametrize("nstrip,precision", [(4, 64), (8, 48), (3, 32)])
+def test_batch_strips_bit_exact(nstrip, precision):
+    print("batch strips", nstrip, precision, flush=True)
+    nx, ny, nsys = 512, 16, 2
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0) as B:
+        B.set_strips(nstrip)
+        B.set_inner(1e-4, 5000)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 30 if precision == 32 else 3000, 1e-10, precision=precision)
+    for s in range(nsys):
+        F64, F32 = orc.derive(*ic0_factor(rp, vals[s], nx), nx)
+        M = orc.FactorPrecon(F64, F32, nx, apply32=precision == 48, nstrip=nstrip)
+        if precision == 32:
+            xo, ho, no, io, _ = orc.pcg_mixed_factor(rp, col, vals[s], b[s], M, itrmax=30, rtol=1e-10,
+                                                     inner_itrmax=5000, inner_rtol=1e-4)
+        else:
+            xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=3000, rtol=1e-10)
+        assert iret[s] == 0 and niter[s] == no, s
+        assert np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s

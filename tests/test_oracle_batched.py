@@ -169,3 +169,23 @@ def test_factor_pcg_oracle_converges_faster_than_jacobi():
     _, _, itj, _ = orc.pcg(rp, col, val, b, ipreflag=1, itrmax=2000, rtol=1e-10)
     assert iret == 0 and it < 0.6 * itj
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+
+
+@pytest.mark.parametrize("nx,ny,nstrip", [(100, 37, 4), (64, 16, 16), (512, 9, 2)])
+def test_lscan_strips_solve_block_diagonal(nx, ny, nstrip):
+    # strips drop the factor's couplings between line blocks: the apply equals
+    # solving diag(L_k L_k^T) block by block
+    rp, col, val = five_point(nx, ny)
+    n = nx * ny
+    Ls, Lw, Ld = orc.ic0_5pt(rp, val, nx)
+    F64, _ = orc.derive(Ls, Lw, Ld, nx)
+    r = np.random.default_rng(4).standard_normal(n)
+    z = orc.lscan_apply(F64, r, nx, nstrip=nstrip)
+    ref = np.zeros(n)
+    for k in range(nstrip):
+        y0, y1 = ny * k // nstrip, ny * (k + 1) // nstrip
+        sl = slice(y0 * nx, y1 * nx)
+        Lk = lower_factor(Ls[sl].copy(), Lw[sl].copy(), Ld[sl].copy(), nx)
+        Lk = sp.csr_matrix(Lk)
+        ref[sl] = spl.spsolve_triangular(Lk.T.tocsr(), spl.spsolve_triangular(Lk, r[sl], lower=True), lower=False)
+    assert np.max(np.abs(z - ref)) < 1e-12 * np.max(np.abs(ref))
