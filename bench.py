@@ -319,6 +319,16 @@ def run_c5(args):
     # headline: FP64 PCG whose preconditioner (the network's BF16 factor) is
     # applied in FP32; then the FP64-refinement form, the all-FP64 form, and
     # the same solver with the untrained network / the classical IC(0) factor
+    def ablations(ns):
+        batched("learned_apply32_exact_factor", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, steps=1, strips=1)
+        batched("learned_refine32", ph.BATCH_FACTOR, ph.BATCH_REFINE32, params=p, steps=1, strips=ns)
+        batched("learned_fp64", ph.BATCH_FACTOR, ph.BATCH_FP64, params=p, steps=1, strips=ns)
+        batched("untrained_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=L.init_params(0), steps=1,
+                warmup=0, strips=ns)
+        batched("ic0_apply32", ph.BATCH_IC0, ph.BATCH_APPLY32, steps=1, warmup=0, strips=ns)
+        batched("ic0_fp64", ph.BATCH_IC0, ph.BATCH_FP64, steps=1, warmup=0, strips=ns)
+        batched("jacobi_fp64", ph.BATCH_JACOBI, ph.BATCH_FP64, steps=1, warmup=0)
+
     # strips per system for the factor solves: the fastest of a one-solve sweep
     ssweep = {}
     for ns in [int(v) for v in args.c5_strips.split(",")]:
@@ -328,14 +338,15 @@ def run_c5(args):
     ns = min(ssweep, key=ssweep.get)
     learned = batched("learned_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, strips=ns)
     learned["strip_sweep_s"] = ssweep
-    batched("learned_apply32_exact_factor", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, steps=1, strips=1)
-    batched("learned_refine32", ph.BATCH_FACTOR, ph.BATCH_REFINE32, params=p, steps=1, strips=ns)
-    batched("learned_fp64", ph.BATCH_FACTOR, ph.BATCH_FP64, params=p, steps=1, strips=ns)
-    batched("untrained_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=L.init_params(0), steps=1, warmup=0,
-            strips=ns)
-    batched("ic0_apply32", ph.BATCH_IC0, ph.BATCH_APPLY32, steps=1, warmup=0, strips=ns)
-    batched("ic0_fp64", ph.BATCH_IC0, ph.BATCH_FP64, steps=1, warmup=0, strips=ns)
-    batched("jacobi_fp64", ph.BATCH_JACOBI, ph.BATCH_FP64, steps=1, warmup=0)
+    if args.c5_ablations:
+        ablations(ns)
+    return finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights)
+
+
+def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights):
+    import numpy as np
+    import parcel_hip as ph
+    D = ph.DeviceArray
 
     # plain FP64 block-Jacobi ILU(0) PCG, parcel_dcg's solver, one system at a time;
     # the block count is the fastest of a sweep on the first 4 systems
@@ -354,7 +365,8 @@ def run_c5(args):
         log("c5 plain BJ(%d): %.3f s per system (4-system sweep)" % (nbc, sweep[nbc]))
     nb = min(sweep, key=sweep.get)
     solve_t, setup_t, its, res, tres = [], [], [], [], []
-    for s in range(nsys):
+    nplain = min(nsys, args.c5_plain_systems) if args.c5_plain_systems > 0 else nsys
+    for s in range(nplain):
         t = time.perf_counter()
         plan = ph.Plan(rp, col, vals[s], ipreflag=2, ILU_method=0, precon_thblock=nb, profile=0)
         t1 = time.perf_counter()
@@ -372,7 +384,9 @@ def run_c5(args):
             tres.append(float(np.linalg.norm(b[s] - r_d.numpy()) / np.linalg.norm(b[s])))
         res.append(float(hist[-1]))
         plan.close()
-    plain = {"time_to_solution_s": float(np.sum(solve_t)), "setup_s_total": float(np.sum(setup_t)),
+    # (with --c5-plain-systems below nsys, a profiling run: the total is scaled)
+    plain = {"time_to_solution_s": float(np.sum(solve_t)) * nsys / nplain, "systems_timed": nplain,
+             "setup_s_total": float(np.sum(setup_t)) * nsys / nplain,
              "block_sweep_s_per_system": sweep,
              "niter_max": int(max(its)), "niter_mean": float(np.mean(its)), "final_recurrence_res_max": max(res),
              "true_rel_res_max_first4": max(tres), "blocks": nb}
@@ -417,7 +431,7 @@ def run_c5(args):
                     "3048..%d), b~N(0,1), x0=0, FP64 rtol 1e-10 per system" % (nsys, 3048 + nsys - 1),
         "method": "learned factor (6x32 MPNN, BF16 MFMA inference, BF16 L) applied in FP32 inside batched FP64 "
                   "PCG (64 systems per launch), factor solves on %d strip(s) of grid lines per system "
-                  "(M = diag(L_k L_k^T)); time includes the network inference" % ns,
+                  "(M = diag(L_k L_k^T)); time includes the network inference" % learned["strips"],
         "headline": "learned_apply32",
         "speedup_vs_plain_fp64": round(plain["time_to_solution_s"] / learned["time_to_solution_s"], 3),
         "plain_fp64_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
@@ -442,6 +456,9 @@ def main():
                     help="block counts of the plain FP64 BJ ILU(0) baseline sweep (fastest is used)")
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--c5-strips", default="1,2,4,8", help="strips per system of the factor solves (sweep)")
+    ap.add_argument("--c5-ablations", type=int, default=1, help="0: only the headline and the plain baseline")
+    ap.add_argument("--c5-plain-systems", type=int, default=0,
+                    help="time the plain baseline on this many systems and scale (profiling runs; 0: all)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "parcel-legacy_amd", "weights", "mpnn_grf512.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)

@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kNT) k_b_scale(int n, const T* __restrict__ r,
   if (dinv) dinv += so;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) s[i] = dinv ? r[i] * dinv[i] : r[i];
   }
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(kNT) k_b_axpy_r(int n, int it, T* __restrict__
   if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
   }
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(kNT) k_bo_tof32(int n, const double* __restric
   r += so; r32 += so; d32 += so;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) { r32[i] = (float)r[i]; d32[i] = 0.0f; }
   }
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
   x += so; d += so;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) x[i] = x[i] + (double)d[i];
   }

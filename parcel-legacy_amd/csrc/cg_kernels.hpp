@@ -22,6 +22,17 @@ namespace {
 
 __device__ __forceinline__ long sys_id() { return (long)blockIdx.y; }
 
+// Owner index of this workgroup in the tiled layout.  Workgroups are dealt to
+// the 8 XCDs round-robin by linear id, so with nb a multiple of 8 the owners
+// of XCD k are made the contiguous range [k nb/8, (k+1) nb/8): a stencil's
+// neighbour tiles then sit in the same XCD's L2.  A bijection on [0, nb):
+// owner o still holds tiles o, o+nb, ... and writes partial slot o, so every
+// sum is unchanged.
+__device__ __forceinline__ int tile_owner() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 template <class T>
 __device__ __forceinline__ SellArgsT<T> sys_matrix(SellArgsT<T> A, const Bat& B) {
   A.val += sys_id() * B.as;
@@ -45,7 +56,7 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
       const T t = sell_row(A, i, [&](int c) { return x[c]; });
@@ -59,8 +70,8 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
   if (threadIdx.x == 0) {
-    part[PART_BB * kNBMax + blockIdx.x] = s0;
-    part[PART_RR0 * kNBMax + blockIdx.x] = s1;
+    part[PART_BB * kNBMax + tile_owner()] = s0;
+    part[PART_RR0 * kNBMax + tile_owner()] = s1;
   }
 }
 
@@ -76,12 +87,12 @@ __global__ void __launch_bounds__(kNT) k_dot_part(int n, const T* __restrict__ x
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) a0 = fma((double)x[i], (double)y[i], a0);
   }
   double s0 = block_sum(a0, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+  if (threadIdx.x == 0) part[tile_owner()] = s0;
 }
 
 __global__ void __launch_bounds__(kNT) k_reduce_one(const double* part, int nb, double* out) {
@@ -155,7 +166,7 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
   const long ntiles = ((long)n + kNT - 1) / kNT;
   if (exit_loop) {
     // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
-    for (long k = blockIdx.x; k < ntiles; k += nb) {
+    for (long k = tile_owner(); k < ntiles; k += nb) {
       const long i = k * kNT + threadIdx.x;
       if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
     }
@@ -169,7 +180,7 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
     return;
   }
   double acc = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
       const T po = p_old[i];
@@ -182,7 +193,7 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
     }
   }
   double s0 = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + blockIdx.x] = s0;
+  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + tile_owner()] = s0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (it - 1 < hist_cap) hist[it - 1] = rnorm / bnorm;  // pcg.f90:512-518
     CGSlot& cur = st->slot[it & 1];
@@ -216,7 +227,7 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restri
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
       const T ri = fma(-(T)alpha, q[i], r[i]);
@@ -230,8 +241,8 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restri
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
   if (threadIdx.x == 0) {
-    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
-    part[PART_RR * kNBMax + blockIdx.x] = s1;
+    part[PART_RHO1 * kNBMax + tile_owner()] = s0;
+    part[PART_RR * kNBMax + tile_owner()] = s1;
   }
 }
 
@@ -251,7 +262,7 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restr
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
   double a0 = 0.0, a1 = 0.0;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) {
       const T ri = r[i];
@@ -262,8 +273,8 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restr
   double s0 = block_sum(a0, sh);
   double s1 = block_sum(a1, sh);
   if (threadIdx.x == 0) {
-    part[PART_RHO1 * kNBMax + blockIdx.x] = s0;
-    part[PART_RR * kNBMax + blockIdx.x] = s1;
+    part[PART_RHO1 * kNBMax + tile_owner()] = s0;
+    part[PART_RR * kNBMax + tile_owner()] = s1;
   }
 }
 
@@ -275,7 +286,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgsT<T> A, const T* __
   x += so; y += so;
   const int nb = gridDim.x;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = blockIdx.x; k < ntiles; k += nb) {
+  for (long k = tile_owner(); k < ntiles; k += nb) {
     const long i = k * kNT + threadIdx.x;
     if (i < n) y[i] = sell_row(A, i, [&](int c) { return x[c]; });
   }
