@@ -204,6 +204,22 @@ int parcel_hip_ddcg(int icomm, double* vecx_hi, double* vecx_lo, const double* v
                     int precon_thblock, int independ_nvec, int nBlock, int PRECISION_A,
                     int PRECISION_b, int PRECISION_x, int PRECISION_PRECONDITION, int* iret);
 
+/* parcel_dcg_ddm (src/ddm_kernel.f90:2893-2986 GPU build, :2266-2360 CPU
+ * build): PCG on a stencil matrix given as nnd diagonals over the rank's
+ * ghost-padded grid (num_grids interior cells per direction, margin ghost
+ * cells on each side, nrow = the padded size; ioff_dia the linear offsets,
+ * ioff_grids(nnd, ncomp_grids) the per-direction offsets); x, b padded too.
+ * ipreflag 0 none, 1 point Jacobi, 2 block Jacobi = D-ILU on boxes of the
+ * GPU build (nx/4 x ny/4 x nz/4 boxes, init_DILU_DDM).  One GPU holds one
+ * rank's whole grid: num_neighbor_ranks must be 0.  Additive Schwarz
+ * (ipreflag 3) is not available. */
+int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd, const int* ioff_dia,
+                       const double* val_dia, int num_neighbor_ranks, const int* neighbor_ranks, int margin,
+                       int ncomp_grids, const int* ndiv_grids, const int* num_grids, const int* ioff_grids,
+                       int div_direc_th, int ipreflag, int addL, int iflagAS, int* itrmax, double rtolmax,
+                       double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                       int nBlock, int* iret);
+
 /* --- plan API: setup once (upload, factorise), solve many times.  This is the
  * split that KSP_CRS performs internally (src/krylov.f90:447-735: set_KSP_CRS,
  * then START_KSP_SOLVER); exposing it lets a caller keep the matrix resident
@@ -386,6 +402,12 @@ int parcel_gen_poisson2d(int gnx, int gny, double scale, int* nnz, int* rowptr, 
 /* 3D 7-point constant Poisson, diag 6 off -1 (SPD) */
 int parcel_gen_poisson3d_spd(int gnx, int gny, int gnz, int* nnz, int* rowptr, int* col,
                              double* val);
+/* make_poisson_3D_DDM (src/example/ddm_setup.f90:133-349) for one rank:
+ * 7 diagonals over the (gnx+2m)(gny+2m)(gnz+2m) padded grid (val_dia of
+ * 7*padded size, order -z -y -x 0 +x +y +z), ioff_dia (7), ioff_grids (7x3,
+ * column-major), b and the exact solution in the padded layout (ghosts 0). */
+int parcel_gen_poisson3d_ddm(int gnx, int gny, int gnz, int margin, double* val_dia, int* ioff_dia,
+                             int* ioff_grids, double* vecb, double* vecx_exact);
 /* x ~ U[-1,1] from the counter-based stream `seed` (SplitMix64, 53-bit) */
 int parcel_gen_uniform(long n, unsigned long long seed, double lo, double hi, double* x);
 /* x ~ N(0,1): normal k of stream `seed` is Box-Muller on uniforms 2*(k/2)

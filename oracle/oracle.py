@@ -44,6 +44,8 @@ def load():
         "oracle_divide_region": (None, [_I, _I, _I, _IP, _IP]),
         "oracle_spmv": (None, [_I, _P, _P, _P, _P, _P]),
         "oracle_precon_create": (_P, [_I, _I, _I, _I, _I, _P, _P, _P, _IP]),
+        "oracle_precon_create_blocks": (_P, [_I, _I, _I, _P, _I, _I, _P, _P, _P, _IP]),
+        "oracle_pcg_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_precon_apply": (None, [_P, _P, _P]),
         "oracle_precon_free": (None, [_P]),
         "oracle_precon_nblocks": (_I, [_P]),
@@ -103,12 +105,20 @@ def spmv(rowptr, col, val, x):
 
 
 class Precon:
-    def __init__(self, rowptr, col, val, ipreflag, ilu_method=1, nblocks=1, independ_nvec=1):
-        self._keep = (rowptr, col, val)
+    def __init__(self, rowptr, col, val, ipreflag, ilu_method=1, nblocks=1, independ_nvec=1, starts=None):
+        """starts: explicit block-Jacobi blocks (0-based starts, nblocks+1 entries)."""
+        self._keep = (rowptr, col, val, starts)
         self.n = rowptr.shape[0] - 1
         err = ctypes.c_int(0)
-        self.h = load().oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, self.n, _p(rowptr),
-                                             _p(col), _p(val), ctypes.byref(err))
+        if starts is None:
+            self.h = load().oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, self.n, _p(rowptr),
+                                                 _p(col), _p(val), ctypes.byref(err))
+        else:
+            starts = np.ascontiguousarray(starts, dtype=np.int32)
+            self._keep = (rowptr, col, val, starts)
+            self.h = load().oracle_precon_create_blocks(ipreflag, ilu_method, int(starts.shape[0]) - 1, _p(starts),
+                                                        independ_nvec, self.n, _p(rowptr), _p(col), _p(val),
+                                                        ctypes.byref(err))
         if not self.h:
             raise RuntimeError("oracle precon setup failed: %d" % err.value)
 
@@ -393,12 +403,13 @@ class FactorPrecon:
 
 
 def pcg_factor(rowptr, col, val, b, M, x0=None, itrmax=1000, rtol=1e-8, red=TILED):
-    """oracle PCG with a FactorPrecon; returns (x, hist[:niter], niter, iret)."""
+    """oracle PCG with a prebuilt preconditioner (FactorPrecon or Precon);
+    returns (x, hist[:niter], niter, iret)."""
     n = rowptr.shape[0] - 1
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     hist = np.zeros(max(itrmax, 1))
     niter, iret = ctypes.c_int(0), ctypes.c_int(0)
-    rc = _blib().oracle_pcg_with(M.h, n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), itrmax, rtol,
+    rc = load().oracle_pcg_with(M.h, n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), itrmax, rtol,
                                  ctypes.byref(red), _p(hist), ctypes.byref(niter), ctypes.byref(iret))
     if rc != 0:
         raise RuntimeError("oracle_pcg_with failed: %d" % rc)

@@ -370,7 +370,25 @@ static void ilu0_apply_block(const oracle_precon* p, int b, int lo, int hi,
   }
 }
 
+static oracle_precon* precon_create(int ipreflag, int ilu_method, int nblocks, const int* starts,
+                                    int independ_nvec, int n, const int* rowptr,
+                                    const int* col, const double* val, int* err);
+
 oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
+                                    int independ_nvec, int n, const int* rowptr,
+                                    const int* col, const double* val, int* err) {
+  return precon_create(ipreflag, ilu_method, nblocks, NULL, independ_nvec, n, rowptr, col, val, err);
+}
+
+/* Block Jacobi with explicit blocks: block b is 0-based rows
+ * [starts[b], starts[b+1]) (the box blocks of the stencil entry point). */
+oracle_precon* oracle_precon_create_blocks(int ipreflag, int ilu_method, int nblocks, const int* starts,
+                                           int independ_nvec, int n, const int* rowptr,
+                                           const int* col, const double* val, int* err) {
+  return precon_create(ipreflag, ilu_method, nblocks, starts, independ_nvec, n, rowptr, col, val, err);
+}
+
+static oracle_precon* precon_create(int ipreflag, int ilu_method, int nblocks, const int* starts,
                                     int independ_nvec, int n, const int* rowptr,
                                     const int* col, const double* val, int* err) {
   oracle_precon* p = (oracle_precon*)calloc(1, sizeof(oracle_precon));
@@ -379,12 +397,17 @@ oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
   p->independ_nvec = independ_nvec < 1 ? 1 : independ_nvec; /* type.f90:1377 */
   p->n = n;
   p->rowptr = rowptr; p->col = col; p->val = val;
-  if (nblocks < 1) nblocks = 1;
-  if (nblocks > n) nblocks = n;
+  if (!starts) {
+    if (nblocks < 1) nblocks = 1;
+    if (nblocks > n) nblocks = n;
+  }
   p->nblocks = nblocks;
   p->blo = (int*)malloc(sizeof(int) * (size_t)nblocks);
   p->bhi = (int*)malloc(sizeof(int) * (size_t)nblocks);
-  for (int b = 0; b < nblocks; ++b) oracle_divide_region(b, nblocks, n, &p->blo[b], &p->bhi[b]);
+  for (int b = 0; b < nblocks; ++b) {
+    if (starts) { p->blo[b] = starts[b] + 1; p->bhi[b] = starts[b + 1]; }
+    else oracle_divide_region(b, nblocks, n, &p->blo[b], &p->bhi[b]);
+  }
   *err = 0;
   if (ipreflag == 1) {
     p->dinv = (double*)calloc((size_t)n, sizeof(double));

@@ -66,6 +66,10 @@ typedef struct oracle_precon oracle_precon;
 oracle_precon* oracle_precon_create(int ipreflag, int ilu_method, int nblocks,
                                     int independ_nvec, int n, const int* rowptr,
                                     const int* col, const double* val, int* err);
+/* the same with explicit blocks: block b is 0-based rows [starts[b], starts[b+1]) */
+oracle_precon* oracle_precon_create_blocks(int ipreflag, int ilu_method, int nblocks, const int* starts,
+                                           int independ_nvec, int n, const int* rowptr,
+                                           const int* col, const double* val, int* err);
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s);
 void oracle_precon_free(oracle_precon* p);
 /* The same preconditioner in FP32: sweeps in float with the factors rounded

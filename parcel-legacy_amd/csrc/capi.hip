@@ -3,6 +3,7 @@
 // The one-shot entry points keep the argument lists of the reference Fortran
 // wrappers (src/wrapper.f90); internally each one is plan_create +
 // plan_solve + plan_destroy, the split KSP_CRS performs (src/krylov.f90:447-735).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -152,7 +153,7 @@ namespace {
 // (dd only) may be null (PRECISION_A = 1).
 int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const double* crsA,
                 const double* crsA_lo, const int* crsRow_ptr, const int* crsCol, const parcel_hip_param* prm,
-                int dd_prec) {
+                int dd_prec, const std::vector<int>* block_starts = nullptr) {
   PARCEL_API_BEGIN
   if (!out) return fail(PARCEL_HIP_ERR_ARG, "plan pointer is NULL");
   *out = nullptr;
@@ -206,6 +207,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
       if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
         throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
     }
+    if (block_starts) P.block_starts = *block_starts;
     if (dd_prec) precon_setup_dd(P, dd_prec);
     else precon_setup(P);
     P.stats.precon_setup_s = P.M.setup_seconds;
@@ -635,6 +637,159 @@ int parcel_hip_memset(void* ptr, int value, unsigned long bytes) {
 int parcel_hip_device_synchronize(void) {
   PARCEL_API_BEGIN
   PARCEL_HIP_CHECK(hipDeviceSynchronize());
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+}  // extern "C"
+
+// --- stencil (DDM) entry point -------------------------------------------------
+//
+// parcel_dcg_ddm (src/ddm_kernel.f90:2893-2986): the matrix comes as nnd
+// diagonals over the rank's ghost-padded grid (margin cells on every side),
+// vectors in the same padded layout.  With the block-Jacobi preconditioner the
+// GPU build of the reference solves D-ILU on boxes (init_DILU_DDM with
+// nx/4, ny/4, nz/4 boxes per direction, src/src_DDM/ddm_kernel.f90:1009-1016
+// and src/src_DDM/precon_ddm_GPU.f90:370-470; get_DILU_DDM :2646-2922).  Here
+// the interior cells are renumbered box by box (boxes x fastest, cells x
+// fastest inside a box), the diagonals become a CRS matrix in that order, and
+// the boxes become contiguous block-Jacobi blocks of the CRS path, whose
+// D-ILU (ILU_method 1) is the box D-ILU above.
+
+namespace {
+
+struct DdmGrid {
+  int dim = 0;
+  int nx = 1, ny = 1, nz = 1;        // interior cells per direction
+  int mx = 1, my = 1, mz = 1;        // padded
+  int margin = 0;
+};
+
+// Box split of one direction exactly as init_DILU_DDM: div = max(n/4, 1)
+// boxes of n_local = ceil(n/div) cells, the last ones cut at n.
+std::vector<int> box_edges(int n) {
+  int div = std::max(n / 4, 1);
+  const int loc = (n % div) ? n / div + 1 : n / div;
+  std::vector<int> e;
+  for (int i = 0; i < div; ++i) {
+    const int s0 = loc * i, s1 = std::min(n, loc * (i + 1));
+    if (s1 > s0) e.push_back(s0);
+  }
+  e.push_back(n);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd, const int* ioff_dia,
+                       const double* val_dia, int num_neighbor_ranks, const int* neighbor_ranks, int margin,
+                       int ncomp_grids, const int* ndiv_grids, const int* num_grids, const int* ioff_grids,
+                       int div_direc_th, int ipreflag, int addL, int iflagAS, int* itrmax, double rtolmax,
+                       double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                       int nBlock, int* iret) {
+  PARCEL_API_BEGIN
+  (void)icomm; (void)neighbor_ranks; (void)ndiv_grids; (void)div_direc_th; (void)addL; (void)iflagAS;
+  (void)iovlflag; (void)precon_thblock; (void)Neuman_itr; (void)nBlock;
+  if (!x || !b || !ioff_dia || !val_dia || !num_grids || !ioff_grids || !itrmax || !iret)
+    return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: null argument");
+  if (num_neighbor_ranks != 0)
+    return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: one GPU holds one rank's whole grid (no neighbour ranks)");
+  if (ncomp_grids < 1 || ncomp_grids > 3 || margin < 0 || nnd < 1)
+    return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: bad grid description");
+  if (ipreflag != PARCEL_PRECON_NONE && ipreflag != PARCEL_PRECON_POINTJACOBI &&
+      ipreflag != PARCEL_PRECON_BLOCKJACOBI)
+    return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: preconditioner not available on the stencil path");
+  DdmGrid G;
+  G.dim = ncomp_grids;
+  G.margin = margin;
+  int* nn[3] = {&G.nx, &G.ny, &G.nz};
+  for (int d = 0; d < ncomp_grids; ++d) *nn[d] = num_grids[d];
+  if (G.nx < 1 || G.ny < 1 || G.nz < 1) return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: empty grid");
+  G.mx = G.nx + 2 * margin;
+  G.my = ncomp_grids >= 2 ? G.ny + 2 * margin : 1;
+  G.mz = ncomp_grids >= 3 ? G.nz + 2 * margin : 1;
+  const long m = (long)G.mx * G.my * G.mz;
+  if (m != nrow) return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: nrow differs from the padded grid size");
+  const int sx = margin, sy = ncomp_grids >= 2 ? margin : 0, sz = ncomp_grids >= 3 ? margin : 0;
+  const long n = (long)G.nx * G.ny * G.nz;
+  // box-major numbering of the interior cells
+  const std::vector<int> ex = box_edges(G.nx), ey = box_edges(G.ny), ez = box_edges(G.nz);
+  std::vector<int> perm(n), pos(m, -1), starts;
+  long p = 0;
+  for (size_t bz = 0; bz + 1 < ez.size(); ++bz)
+    for (size_t by = 0; by + 1 < ey.size(); ++by)
+      for (size_t bx = 0; bx + 1 < ex.size(); ++bx) {
+        starts.push_back((int)p);
+        for (int jz = ez[bz]; jz < ez[bz + 1]; ++jz)
+          for (int jy = ey[by]; jy < ey[by + 1]; ++jy)
+            for (int jx = ex[bx]; jx < ex[bx + 1]; ++jx) {
+              const long j = (jx + sx) + (long)G.mx * (jy + sy) + (long)G.mx * G.my * (jz + sz);
+              perm[p] = (int)j;
+              pos[j] = (int)p;
+              ++p;
+            }
+      }
+  starts.push_back((int)p);
+  std::vector<int> ioff(nnd);
+  std::vector<double> vd;
+  fetch(ioff_dia, (size_t)nnd, ioff);
+  fetch(val_dia, (size_t)m * nnd, vd);
+  // CRS in the new numbering, columns ascending; entries coupling to ghost
+  // cells must be zero (a single rank has no halo values)
+  std::vector<int> rp(n + 1), col;
+  std::vector<double> val;
+  col.reserve((size_t)n * nnd);
+  val.reserve((size_t)n * nnd);
+  std::vector<std::pair<int, double>> row;
+  for (long r = 0; r < n; ++r) {
+    const long j = perm[r];
+    row.clear();
+    for (int ip = 0; ip < nnd; ++ip) {
+      const long jn = j + ioff[ip];
+      const double v = vd[(size_t)ip * m + j];
+      const bool inside = jn >= 0 && jn < m && pos[jn] >= 0;
+      if (!inside) {
+        if (v != 0.0) throw hip_error("dcg_ddm: coupling to a ghost cell (halo values need >1 rank)",
+                                      PARCEL_HIP_ERR_UNSUPPORTED);
+        continue;
+      }
+      if (v == 0.0 && jn != j) continue;
+      row.emplace_back(pos[jn], v);
+    }
+    std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& c) {
+      return a.first < c.first;
+    });
+    rp[r] = (int)col.size() + 1;
+    for (auto& e : row) { col.push_back(e.first + 1); val.push_back(e.second); }
+  }
+  rp[n] = (int)col.size() + 1;
+  // vectors in the new numbering
+  std::vector<double> xb, bb, xp(n), bp(n);
+  fetch(x, (size_t)m, xb);
+  fetch(b, (size_t)m, bb);
+  for (long r = 0; r < n; ++r) { xp[r] = xb[perm[r]]; bp[r] = bb[perm[r]]; }
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = PARCEL_ILUD;
+  prm.independ_nvec = independ_nvec;
+  prm.precon_thblock = (int)starts.size() - 1;
+  prm.profile = 0;
+  parcel_hip_plan* plan = nullptr;
+  int rc = create_impl(&plan, (int)n, (int)n, (int)col.size(), 1, val.data(), nullptr, rp.data(), col.data(), &prm,
+                       0, ipreflag == PARCEL_PRECON_BLOCKJACOBI ? &starts : nullptr);
+  if (rc) return rc;
+  rc = parcel_hip_plan_solve(plan, xp.data(), bp.data(), itrmax, rtolmax, reshistory, iret);
+  parcel_hip_plan_destroy(plan);
+  if (rc) return rc;
+  for (long r = 0; r < n; ++r) xb[perm[r]] = xp[r];
+  if (is_device_ptr(x))
+    PARCEL_HIP_CHECK(hipMemcpy(x, xb.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+  else
+    std::memcpy(x, xb.data(), sizeof(double) * m);
   return PARCEL_HIP_OK;
   PARCEL_API_END
 }

@@ -11,6 +11,7 @@
 // stream can be generated in parallel.
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "parcel_hip.h"
 
@@ -152,6 +153,43 @@ int parcel_gen_poisson2d(int gnx, int gny, double scale, int* nnz, int* rowptr, 
       if (iy < gny) { col[e] = (int)(row + gnx); val[e] = off; ++e; }
     }
   rowptr[n] = (int)(e + 1);
+  return PARCEL_HIP_OK;
+}
+
+// make_poisson_3D_DDM (src/example/ddm_setup.f90:133-349), one rank: the
+// right-hand side is make_poisson3D_right_vector's (same formula), placed in
+// the padded layout.
+int parcel_gen_poisson3d_ddm(int gnx, int gny, int gnz, int margin, double* val_dia, int* ioff_dia,
+                             int* ioff_grids, double* vecb, double* vecx_exact) {
+  if (gnx < 1 || gny < 1 || gnz < 1 || margin < 1 || !val_dia || !ioff_dia || !ioff_grids || !vecb)
+    return PARCEL_HIP_ERR_ARG;
+  const long mx = gnx + 2L * margin, my = gny + 2L * margin, mz = gnz + 2L * margin, m = mx * my * mz;
+  const long n = (long)gnx * gny * gnz;
+  for (long k = 0; k < 7 * m; ++k) val_dia[k] = 0.0;
+  for (long k = 0; k < m; ++k) { vecb[k] = 0.0; if (vecx_exact) vecx_exact[k] = 0.0; }
+  static const int od[7][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int ip = 0; ip < 7; ++ip) {
+    ioff_dia[ip] = (int)(mx * my * od[ip][2] + mx * od[ip][1] + od[ip][0]);
+    for (int d = 0; d < 3; ++d) ioff_grids[ip + 7 * d] = od[ip][d];
+  }
+  std::vector<double> b(n), xe(n);
+  int rc = parcel_gen_poisson3d_rhs_parcel(gnx, gny, gnz, b.data(), xe.data());
+  if (rc) return rc;
+  for (int jz = 0; jz < gnz; ++jz)
+    for (int jy = 0; jy < gny; ++jy)
+      for (int jx = 0; jx < gnx; ++jx) {
+        const long j = (jx + margin) + mx * (jy + margin) + mx * my * (jz + margin);
+        const long i = jx + (long)gnx * jy + (long)gnx * gny * jz;
+        if (jz > 0) val_dia[j + 0 * m] = 1.0;
+        if (jy > 0) val_dia[j + 1 * m] = 1.0;
+        if (jx > 0) val_dia[j + 2 * m] = 1.0;
+        val_dia[j + 3 * m] = -6.0;
+        if (jx < gnx - 1) val_dia[j + 4 * m] = 1.0;
+        if (jy < gny - 1) val_dia[j + 5 * m] = 1.0;
+        if (jz < gnz - 1) val_dia[j + 6 * m] = 1.0;
+        vecb[j] = b[i];
+        if (vecx_exact) vecx_exact[j] = xe[i];
+      }
   return PARCEL_HIP_OK;
 }
 

@@ -155,6 +155,7 @@ struct Plan {
   DevBuf<double> basis2;            // second basis (orthonormal Q of CA-GMRES with MGS/CGS)
   DevBuf<double> wpart;             // partial-sum slots of the other solvers
   std::vector<DevBuf<float>> vec32; // FP32 work vectors (mixed-precision solve)
+  std::vector<int> block_starts;    // explicit block-Jacobi blocks (nblocks+1 starts), empty: divide_region
   double inner_rtol = 1e-4;         // mixed-precision inner PCG tolerance
   int inner_itrmax = 10000;
   DevBuf<char> solver_state;        // device-resident scalars of the other solvers

@@ -39,6 +39,11 @@ int apply_k(int k) {
   return k;
 }
 
+// Block-Jacobi block b is rows [blo[b], bhi[b]): the plan's explicit block
+// starts when it has them (the box blocks of the stencil entry point), else
+// divide_region's split.
+void block_ranges(const Plan& P, int nblk, int n, std::vector<int>& blo, std::vector<int>& bhi);
+
 // divide_region (src/common.f90:125-182), 0-based half-open result.
 void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   if (n == 1) { *lo = 0; *hi = 1; return; }
@@ -47,6 +52,14 @@ void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   if (rank < over) { l = 1 + iblock * rank + rank; h = l + iblock; }
   else { l = 1 + iblock * rank + over; h = l + iblock - 1; }
   *lo = l - 1; *hi = h;
+}
+
+void block_ranges(const Plan& P, int nblk, int n, std::vector<int>& blo, std::vector<int>& bhi) {
+  if (!P.block_starts.empty()) {
+    for (int b = 0; b < nblk; ++b) { blo[b] = P.block_starts[b]; bhi[b] = P.block_starts[b + 1]; }
+    return;
+  }
+  for (int b = 0; b < nblk; ++b) divide_region0(b, nblk, n, &blo[b], &bhi[b]);
 }
 
 struct BlockSweep {
@@ -778,12 +791,13 @@ void precon_setup(Plan& P) {
     }
     if (nblk > n) nblk = n;
     if (nblk < 1) nblk = 1;
+    if (!P.block_starts.empty()) nblk = (int)P.block_starts.size() - 1;
     M.nblocks = nblk;
     M.mode = (P.prm.ILU_method == PARCEL_ILU0) ? BJ_ILU0 : BJ_DILU;
     const int nvec = P.prm.independ_nvec < 1 ? 1 : P.prm.independ_nvec;
     std::vector<BlockSweep> F(nblk), B(nblk);
     std::vector<int> blo(nblk), bhi(nblk);
-    for (int b = 0; b < nblk; ++b) divide_region0(b, nblk, n, &blo[b], &bhi[b]);
+    block_ranges(P, nblk, n, blo, bhi);
     int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
     for (int b = 0; b < nblk; ++b) {
@@ -995,12 +1009,13 @@ void precon_setup_dd(Plan& P, int prec) {
       nblk = prop.multiProcessorCount;
     }
     nblk = std::max(1, std::min(nblk, n));
+    if (!P.block_starts.empty()) nblk = (int)P.block_starts.size() - 1;
     M.nblocks = nblk;
     M.mode = (P.prm.ILU_method == PARCEL_ILU0) ? BJ_ILU0 : BJ_DILU;
     const int nvec = P.prm.independ_nvec < 1 ? 1 : P.prm.independ_nvec;
     std::vector<BlockSweep> F(nblk), B(nblk);
     std::vector<int> blo(nblk), bhi(nblk);
-    for (int b = 0; b < nblk; ++b) divide_region0(b, nblk, n, &blo[b], &bhi[b]);
+    block_ranges(P, nblk, n, blo, bhi);
     int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
     for (int b = 0; b < nblk; ++b) {

@@ -116,6 +116,9 @@ SYMBOLS = {
     "parcel_gen_poisson3d_spd": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
     "parcel_gen_uniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
     "parcel_gen_normal": (_I, [ctypes.c_long, ctypes.c_ulonglong, _P]),
+    "parcel_gen_poisson3d_ddm": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
+    "parcel_hip_dcg_ddm": (_I, [_I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D,
+                                _P, _I, _I, _I, _I, _I, _IP]),
     "parcel_gen_grf2d": (_I, [_I, _I, _D, _D, ctypes.c_ulonglong, _P]),
     "parcel_gen_diffusion2d": (_I, [_I, _I, _P, _IP, _P, _P, _P]),
     "parcel_gen_grf_diffusion2d": (_I, [_I, _I, _D, _D, ctypes.c_ulonglong, _I, _IP, _P, _P, _P, _P]),
@@ -325,6 +328,40 @@ def uniform(n, seed, lo=-1.0, hi=1.0):
     x = np.empty(n, np.float64)
     _check(load().parcel_gen_uniform(n, seed, lo, hi, _ptr(x)))
     return x
+
+
+def poisson3d_ddm(gnx, gny, gnz, margin=1):
+    """make_poisson_3D_DDM (src/example/ddm_setup.f90:133-349), one rank:
+    dict with val_dia (7*m), ioff_dia (7), ioff_grids (7*3, column-major), b,
+    x_exact (padded layout), m and the grid."""
+    m = (gnx + 2 * margin) * (gny + 2 * margin) * (gnz + 2 * margin)
+    val = np.zeros(7 * m)
+    ioff = np.zeros(7, np.int32)
+    iog = np.zeros(21, np.int32)
+    b = np.zeros(m)
+    xe = np.zeros(m)
+    _check(load().parcel_gen_poisson3d_ddm(gnx, gny, gnz, margin, _ptr(val), _ptr(ioff), _ptr(iog), _ptr(b),
+                                           _ptr(xe)))
+    return dict(val_dia=val, ioff_dia=ioff, ioff_grids=iog, b=b, x_exact=xe, m=m, grid=(gnx, gny, gnz),
+                margin=margin)
+
+
+def dcg_ddm(x, b, grid, margin, val_dia, ioff_dia, ioff_grids, ipreflag=PRECON_BJ, itrmax=1000, rtolmax=1e-8,
+            independ_nvec=1):
+    """parcel_dcg_ddm (src/ddm_kernel.f90:2893-2986): x, b in the padded
+    layout; returns (hist[:niter], niter, iret)."""
+    nnd = int(ioff_dia.shape[0])
+    m = int(b.shape[0])
+    ng = np.ascontiguousarray(grid, dtype=np.int32)
+    nd = np.ones(3, np.int32)
+    hist = np.zeros(max(itrmax, 1))
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(load().parcel_hip_dcg_ddm(0, _ptr(x), _ptr(b), m, nnd, _ptr(ioff_dia), _ptr(val_dia), 0, None, margin,
+                                     int(ng.shape[0]), _ptr(nd), _ptr(ng), _ptr(ioff_grids), 1, ipreflag, 0, 0,
+                                     ctypes.byref(it), rtolmax, _ptr(hist), 0, -1, 1, independ_nvec, 2000,
+                                     ctypes.byref(iret)))
+    return hist[: it.value].copy(), it.value, iret.value
 
 
 def normal(n, seed):

@@ -33,3 +33,55 @@ def convdiff3d(g, peclet=0.5, seed=None):
     A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
     A.sort_indices()
     return A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.astype(np.float64), A
+
+
+def ddm_box_split(n):
+    """One direction of the GPU build's box partition (init_DILU_DDM with n/4
+    boxes, src/src_DDM/precon_ddm_GPU.f90:399-470): box starts and the end."""
+    div = max(n // 4, 1)
+    loc = n // div + (1 if n % div else 0)
+    edges = [loc * i for i in range(div) if min(n, loc * (i + 1)) > loc * i]
+    return edges + [n]
+
+
+def ddm_box_crs(d):
+    """The stencil problem d (parcel_hip.poisson3d_ddm) as 1-based CSR in
+    box-major numbering: (rowptr, col, val, perm, starts), perm[r] = padded
+    index of row r, starts = 0-based first row of every box (and n)."""
+    nx, ny, nz = d["grid"]
+    mg = d["margin"]
+    mx, my = nx + 2 * mg, ny + 2 * mg
+    m = d["m"]
+    val = d["val_dia"].reshape(7, m)
+    ioff = d["ioff_dia"]
+    ex, ey, ez = ddm_box_split(nx), ddm_box_split(ny), ddm_box_split(nz)
+    perm, starts = [], []
+    for bz in range(len(ez) - 1):
+        for by in range(len(ey) - 1):
+            for bx in range(len(ex) - 1):
+                starts.append(len(perm))
+                for jz in range(ez[bz], ez[bz + 1]):
+                    for jy in range(ey[by], ey[by + 1]):
+                        for jx in range(ex[bx], ex[bx + 1]):
+                            perm.append((jx + mg) + mx * (jy + mg) + mx * my * (jz + mg))
+    starts.append(len(perm))
+    perm = np.array(perm)
+    pos = -np.ones(m, np.int64)
+    pos[perm] = np.arange(perm.size)
+    rows, cols, vals = [], [], []
+    for r, j in enumerate(perm):
+        ent = []
+        for ip in range(7):
+            v = val[ip, j]
+            jn = j + ioff[ip]
+            if pos[jn] < 0 or (v == 0.0 and jn != j):
+                continue
+            ent.append((pos[jn], v))
+        for c, v in sorted(ent):
+            rows.append(r)
+            cols.append(c)
+            vals.append(v)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(perm.size, perm.size))
+    A.sort_indices()
+    return (A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.copy(), perm,
+            np.array(starts, np.int32))
