@@ -403,8 +403,16 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
     else:
         kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
     ach = by / (ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "batched_pcg_c5_grf512_traffic.json")
+    if nsys == 64 and n == 512 * 512 and os.path.exists(tf):
+        # HBM bytes per launch of the same kernel on the same workload, from the
+        # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy kernel)
+        tk = json.load(open(tf))["kernels"].get(kname)
+        if tk:
+            traffic = tk["traffic_bytes"]
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "bytes_per_launch": by,
                 "mean_ms": round(ms, 4)}
     cpu = None
     if not args.no_cpu_baseline:
