@@ -176,6 +176,22 @@ int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int 
                         int independ_nvec, int nBlock, int CAGMRES_sstep, int CAGMRES_tstep,
                         int CAGMRES_basis, int CAGMRES_QRflag, int* iret);
 
+/* parcel_dcbcg (src/wrapper.f90:1139-1233 CPU build; the GPU build's
+ * counterpart has the same arguments): parcel_dcg's arguments plus
+ * CBCG_kstep (k, 1..8), CBCG_Eigenflag (1 power method, 2 CA-Arnoldi; any
+ * other value uses Emax = 1, Emin = 0), power_method_itrmax and the
+ * CA-Arnoldi parameters CAARNOLDI_sstep/_tstep/_basis/_QRflag.  Chebyshev-
+ * basis CG of src/pcbcg.f90:1182-2025: reshistory(j) = ||r|| / ||b|| at the
+ * start of outer step j (k iterations each); niter = min(itrmax, j) of the
+ * last step (itrmax / k + 1 when no step converged, its history entry 0). */
+int parcel_hip_dcbcg(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz,
+                     int istart, const double* crsA, const int* crsRow_ptr, const int* crsCol,
+                     int ipreflag, int ILU_method, int addL, int iflagAS, int* itrmax,
+                     double rtolmax, double* reshistory, int iovlflag, int precon_thblock,
+                     int Neuman_itr, int independ_nvec, int nBlock, int CBCG_kstep,
+                     int CBCG_Eigenflag, int power_method_itrmax, int CAARNOLDI_sstep,
+                     int CAARNOLDI_tstep, int CAARNOLDI_basis, int CAARNOLDI_QRflag, int* iret);
+
 /* Mixed-precision PCG (no counterpart in the reference; its FP64 PCG is
  * KSP_pCG, src/pcg.f90:224-892): parcel_dcg's arguments plus the inner
  * tolerance and iteration limit.  FP64 refinement x += d, r = b - A x, each

@@ -69,6 +69,11 @@ def load():
         "oracle_chol_upper": (_I, [_I, _P, _I]),
         "oracle_hqr_eigs": (_I, [_I, _P, _I, _P, _P]),
         "oracle_leja": (None, [_I, _P, _P]),
+        "oracle_sym_pinv": (None, [_I, _P, _I, _P, _I]),
+        "oracle_power_emax": (_D, [_P, _I, _P, _P, _P, _P, _I, _RP]),
+        "oracle_cbcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _RP, _P, _IP, _IP,
+                             ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+        "oracle_cbcg_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _D, _RP, _P, _IP, _IP]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -179,6 +184,54 @@ def cagmres(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, i
     if rc != 0:
         raise RuntimeError("oracle_cagmres failed: %d" % rc)
     return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
+
+
+def cbcg(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=1000,
+         rtol=1e-8, k=4, eigflag=1, pm_itrmax=100, red=TILED):
+    """Chebyshev-basis CG; returns (x, hist[:niter], niter, iret, (Emax, Emin))."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 0) + 2)
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    emax, emin = ctypes.c_double(0), ctypes.c_double(0)
+    rc = load().oracle_cbcg(n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), ipreflag, ilu_method, nblocks,
+                            independ_nvec, itrmax, rtol, k, eigflag, pm_itrmax, ctypes.byref(red), _p(hist),
+                            ctypes.byref(niter), ctypes.byref(iret), ctypes.byref(emax), ctypes.byref(emin))
+    if rc != 0:
+        raise RuntimeError("oracle_cbcg failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value, (emax.value, emin.value)
+
+
+def cbcg_bounds(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1,
+                itrmax=1000, rtol=1e-8, k=4, emax=1.0, emin=0.0, red=TILED):
+    """CBCG with given eigenvalue bounds; returns (x, hist[:niter], niter, iret)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    hist = np.zeros(max(itrmax, 0) + 2)
+    niter, iret = ctypes.c_int(0), ctypes.c_int(0)
+    pre = Precon(rowptr, col, val, ipreflag, ilu_method, nblocks, independ_nvec)
+    rc = load().oracle_cbcg_with(pre.h, n, _p(rowptr), _p(col), _p(val), _p(b), _p(x), itrmax, rtol, k, emax,
+                                 emin, ctypes.byref(red), _p(hist), ctypes.byref(niter), ctypes.byref(iret))
+    if rc != 0:
+        raise RuntimeError("oracle_cbcg_with failed: %d" % rc)
+    return x, hist[: max(niter.value, 0)].copy(), niter.value, iret.value
+
+
+def power_emax(rowptr, col, val, x0, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=100, red=TILED):
+    """Power-method estimate of the largest eigenvalue of A M^-1."""
+    n = rowptr.shape[0] - 1
+    pre = Precon(rowptr, col, val, ipreflag, ilu_method, nblocks, independ_nvec)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    return load().oracle_power_emax(pre.h, n, _p(rowptr), _p(col), _p(val), _p(x0), itrmax, ctypes.byref(red))
+
+
+def sym_pinv(a):
+    """Pseudo-inverse of a symmetric matrix (upper triangle read)."""
+    at = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)  # column-major view
+    k = at.shape[0]
+    pt = np.zeros((k, k))
+    load().oracle_sym_pinv(k, _p(at), k, _p(pt), k)
+    return pt.T.copy()
 
 
 def pcg_mixed(rowptr, col, val, b, x0=None, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=50,

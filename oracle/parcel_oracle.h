@@ -118,6 +118,24 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val,
                    int t, int basis, int qrflag, const oracle_red* red, double* hist,
                    int* niter, int* iret);
 
+/* KSP_pCBCG (src/pcbcg.f90:1182-2025), k-step Chebyshev-basis CG, k in 1..8.
+ * eigflag 1: Emax from the power method (pm_itrmax steps from x0), Emin = 0;
+ * 2: CA-Arnoldi (returns -4); other: Emax = 1, Emin = 0.  hist must hold
+ * itrmax / k + 1 entries; hist[j-1] = ||r|| / ||b|| at the start of outer
+ * step j; niter = min(itrmax, j_last) (entries never written are 0). */
+int oracle_cbcg(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
+                int ipreflag, int ilu_method, int nblocks, int independ_nvec, int itrmax, double rtol, int k,
+                int eigflag, int pm_itrmax, const oracle_red* red, double* hist, int* niter, int* iret,
+                double* emax_out, double* emin_out);
+/* the same with a prebuilt preconditioner and given bounds */
+int oracle_cbcg_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                     const double* b, double* x, int itrmax, double rtol, int k, double Emax, double Emin,
+                     const oracle_red* red, double* hist, int* niter, int* iret);
+double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                         const double* x0, int itrmax, const oracle_red* red);
+/* pseudo-inverse of a symmetric n x n matrix (upper triangle of A read) */
+void oracle_sym_pinv(int n, const double* A, int lda, double* P, int ldp);
+
 /* Dense kernels used by CA-GMRES (column-major, 0-based); exposed for tests. */
 void oracle_gemm_nn(int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc);

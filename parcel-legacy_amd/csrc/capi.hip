@@ -194,7 +194,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
       throw hip_error("the double-double path has CG only", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
         P.prm.solver != PARCEL_SOLVER_GMRES && P.prm.solver != PARCEL_SOLVER_CAGMRES &&
-        P.prm.solver != PARCEL_SOLVER_CG_MIXED)
+        P.prm.solver != PARCEL_SOLVER_CBCG && P.prm.solver != PARCEL_SOLVER_CG_MIXED)
       throw hip_error("solver not available on this path yet", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver == PARCEL_SOLVER_GMRES && (P.prm.GMRES_m < 1 || P.prm.GMRES_m > 256))
       throw hip_error("GMRES_m must be in 1..256", PARCEL_HIP_ERR_ARG);
@@ -207,6 +207,8 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
       if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
         throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
     }
+    if (P.prm.solver == PARCEL_SOLVER_CBCG && (P.prm.CBCG_kstep < 1 || P.prm.CBCG_kstep > 8))
+      throw hip_error("CBCG_kstep must be in 1..8", PARCEL_HIP_ERR_ARG);
     if (block_starts) P.block_starts = *block_starts;
     if (dd_prec) precon_setup_dd(P, dd_prec);
     else precon_setup(P);
@@ -379,6 +381,7 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
     case PARCEL_SOLVER_BICG: rc = bicg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_GMRES: rc = gmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_CAGMRES: rc = cagmres_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
+    case PARCEL_SOLVER_CBCG: rc = cbcg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     case PARCEL_SOLVER_CG_MIXED: rc = mixed_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
     default: rc = cg_solve(P, x, b, maxit, rtolmax, reshistory, &niter, &ret); break;
   }
@@ -501,6 +504,36 @@ int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int 
   prm.CAGMRES_tstep = CAGMRES_tstep;
   prm.CAGMRES_basis = CAGMRES_basis;
   prm.CAGMRES_QRflag = CAGMRES_QRflag;
+  return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
+                  iret);
+}
+
+int parcel_hip_dcbcg(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                     const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag, int ILU_method,
+                     int addL, int iflagAS, int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                     int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock, int CBCG_kstep,
+                     int CBCG_Eigenflag, int power_method_itrmax, int CAARNOLDI_sstep, int CAARNOLDI_tstep,
+                     int CAARNOLDI_basis, int CAARNOLDI_QRflag, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CBCG;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+  prm.CBCG_kstep = CBCG_kstep;
+  prm.CBCG_Eigenflag = CBCG_Eigenflag;
+  prm.power_method_itrmax = power_method_itrmax;
+  prm.CAARNOLDI_sstep = CAARNOLDI_sstep;
+  prm.CAARNOLDI_tstep = CAARNOLDI_tstep;
+  prm.CAARNOLDI_basis = CAARNOLDI_basis;
+  prm.CAARNOLDI_QRflag = CAARNOLDI_QRflag;
   return one_shot(prm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, itrmax, rtolmax, reshistory,
                   iret);
 }

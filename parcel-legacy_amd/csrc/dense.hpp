@@ -464,6 +464,71 @@ inline void leja_order(int n, double* re, double* im) {
   }
 }
 
+/* Pseudo-inverse of a symmetric n x n matrix (moore_penrose,
+ * src/Moore_Penrose.f90:3-67: A+ = V diag(1/sigma_i, sigma_i > 0) U^T by
+ * DGESVD).  For a symmetric A the singular triplets are (|lambda_i|, v_i,
+ * sign(lambda_i) v_i), so A+ = sum over lambda_i != 0 of v_i v_i^T / lambda_i;
+ * the eigenpairs come from cyclic Jacobi rotations (rows p < q in order; an
+ * off-diagonal entry at or below 2^-53 sqrt(|a_pp a_qq|) is set to zero; stop
+ * after a sweep without rotations, at most 64 sweeps).  Plain expressions
+ * without fma.  A is read from the upper triangle; P may not alias A. */
+inline void sym_pinv(int n, const double* A, int lda, double* P, int ldp) {
+  std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) a[IDX(i, j, n)] = (i <= j) ? A[IDX(i, j, lda)] : A[IDX(j, i, lda)];
+  for (int i = 0; i < n; ++i) v[IDX(i, i, n)] = 1.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[IDX(p, q, n)];
+        if (apq == 0.0) continue;
+        const double app = a[IDX(p, p, n)], aqq = a[IDX(q, q, n)];
+        if (fabs(apq) <= 0x1p-53 * sqrt(fabs(app * aqq))) {
+          a[IDX(p, q, n)] = 0.0;
+          a[IDX(q, p, n)] = 0.0;
+          continue;
+        }
+        rotated = true;
+        const double theta = (aqq - app) / (2.0 * apq);
+        double t;
+        if (fabs(theta) > 1e150) {
+          t = 0.5 / theta;
+        } else {
+          t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+          if (theta < 0.0) t = -t;
+        }
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int r = 0; r < n; ++r) {
+          if (r == p || r == q) continue;
+          const double arp = a[IDX(r, p, n)], arq = a[IDX(r, q, n)];
+          const double np_ = c * arp - s * arq, nq_ = s * arp + c * arq;
+          a[IDX(r, p, n)] = np_; a[IDX(p, r, n)] = np_;
+          a[IDX(r, q, n)] = nq_; a[IDX(q, r, n)] = nq_;
+        }
+        a[IDX(p, p, n)] = app - t * apq;
+        a[IDX(q, q, n)] = aqq + t * apq;
+        a[IDX(p, q, n)] = 0.0;
+        a[IDX(q, p, n)] = 0.0;
+        for (int r = 0; r < n; ++r) {
+          const double vrp = v[IDX(r, p, n)], vrq = v[IDX(r, q, n)];
+          v[IDX(r, p, n)] = c * vrp - s * vrq;
+          v[IDX(r, q, n)] = s * vrp + c * vrq;
+        }
+      }
+    if (!rotated) break;
+  }
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < n; ++r) {
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double lam = a[IDX(i, i, n)];
+        if (lam != 0.0) acc = acc + v[IDX(r, i, n)] * (v[IDX(c, i, n)] / lam);
+      }
+      P[IDX(r, c, ldp)] = acc;
+    }
+}
+
 #undef IDX
 
 }  // namespace dense

@@ -225,6 +225,8 @@ int gmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double 
                 double* hist_host, int* niter, int* iret);
 int cagmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                   double* hist_host, int* niter, int* iret);
+int cbcg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
+               double* hist_host, int* niter, int* iret);
 int mixed_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                 double* hist_host, int* niter, int* iret);
 int ddcg_solve(Plan& P, double* xh, double* xl, const double* bh, const double* bl, int itrmax, double rtol,

@@ -87,6 +87,8 @@ SYMBOLS = {
                                _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcagmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                  _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dcbcg": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
+                              _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcg_mixed": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                   _I, _I, _I, _I, _I, _D, _I, _IP]),
     "parcel_hip_plan_set_inner": (_I, [_P, _D, _I]),
@@ -466,6 +468,25 @@ def dcagmres(x, b, rowptr, col, val, CAGMRES_sstep=4, CAGMRES_tstep=1, CAGMRES_b
                                    ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
                                    iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep,
                                    CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
+
+
+def dcbcg(x, b, rowptr, col, val, CBCG_kstep=4, CBCG_Eigenflag=2, power_method_itrmax=100, CAARNOLDI_sstep=4,
+          CAARNOLDI_tstep=10, CAARNOLDI_basis=1, CAARNOLDI_QRflag=3, ipreflag=PRECON_NONE, ILU_method=ILUD,
+          itrmax=1000, rtolmax=1e-8, precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0,
+          Neuman_itr=1, nBlock=2000):
+    """parcel_dcbcg: k-step Chebyshev-basis CG; solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    n = int(rowptr.shape[0]) - 1
+    nnz = int(col.shape[0])
+    hist = np.zeros(max(itrmax, 1) + 1, np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(lib.parcel_hip_dcbcg(0, _ptr(x), _ptr(b), n, n, nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col),
+                                ipreflag, ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                                iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, CBCG_kstep,
+                                CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep,
+                                CAARNOLDI_basis, CAARNOLDI_QRflag, ctypes.byref(iret)))
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
