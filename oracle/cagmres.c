@@ -551,19 +551,134 @@ static void q_vrinv(int n, int nc, const double* V, size_t ldv, const double* Ri
     }
 }
 
+/* QRDecomposition_ (QRDecomposition.f90:1331-1551) of nc columns of V into
+ * R (nc x nc, ld nc) and, with qmat (or GS), Q = V R^-1.  nc == 1 always
+ * uses MGS.  W: scratch of nc columns (CholeskyQR2's Q1). */
+static void qr_cols(const oracle_red* red, int n, int nc, double* V, size_t ldv, double* Q, size_t ldq, double* R,
+                    int qrflag, int qmat, double* W, double* R2, double* Rinv) {
+  memset(R, 0, sizeof(double) * (size_t)nc * nc);
+  if (nc == 1 || qrflag == 1 || qrflag == 2) {
+    const int mgs = (nc == 1 || qrflag == 1);
+    /* GS_QR_: MGS or CGS, R(k,i) and Q formed; V modified in place */
+    for (int i = 0; i < nc; ++i) {
+      double* vi = V + (size_t)i * ldv;
+      if (mgs) {
+        for (int k = 0; k < i; ++k) {
+          const double* qk = Q + (size_t)k * ldq;
+          const double rk = oracle_dot(red, n, vi, qk);
+          R[IDX(k, i, nc)] = rk;
+          for (int q = 0; q < n; ++q) vi[q] = fma(-rk, qk[q], vi[q]);
+        }
+      } else {
+        for (int k = 0; k < i; ++k) R[IDX(k, i, nc)] = oracle_dot(red, n, Q + (size_t)k * ldq, vi);
+        for (int q = 0; q < n; ++q) {
+          double w = vi[q];
+          for (int k = 0; k < i; ++k) w = fma(-R[IDX(k, i, nc)], Q[(size_t)k * ldq + q], w);
+          vi[q] = w;
+        }
+      }
+      const double rii = sqrt(oracle_dot(red, n, vi, vi));
+      R[IDX(i, i, nc)] = rii;
+      const double sc = 1.0 / rii;
+      double* qi = Q + (size_t)i * ldq;
+      for (int q = 0; q < n; ++q) qi[q] = sc * vi[q];
+    }
+    return;
+  }
+  cholqr_r(red, n, nc, V, ldv, R);
+  if (qrflag == 5) {                               /* CholeskyQR2_Ronly_ */
+    memcpy(Rinv, R, sizeof(double) * (size_t)nc * nc);
+    oracle_trinv_upper(nc, Rinv, nc);
+    q_vrinv(n, nc, V, ldv, Rinv, W, (size_t)n);   /* Q1 = V R^-1 */
+    cholqr_r(red, n, nc, W, (size_t)n, R2);
+    oracle_trmm_lunn(nc, R2, nc, R, nc);           /* R = R2 R */
+  }
+  if (qmat) {                                      /* calcQ_VRinv_ */
+    memcpy(Rinv, R, sizeof(double) * (size_t)nc * nc);
+    oracle_trinv_upper(nc, Rinv, nc);
+    q_vrinv(n, nc, V, ldv, Rinv, Q, ldq);
+  }
+}
+
+/* calc_HessenbergMatrix for block k >= 2 (pcagmres.f90:879-1051, with
+ * calc_Rk_Bk_Rkinv :850 and calc_hke1eskRRinv :818).  Rp = R' (c1 x (s+1),
+ * ld c1 = s(k-1)+1) from the block CGS, Rk the block's s x s R factor, H the
+ * (st+1) x st Hessenberg; *hk_pre is read and updated.  The code's effect on
+ * its local arrays: BlockR_ubar = R'(1:s(k-1), 1:s+1), BlockR = R'(1:s(k-1),
+ * 1:s), Rtri0 = [R'(c1, 1:s+1); 0 Rk]. */
+void oracle_hessenberg_block(int s, int t, int k, const double* B1, double* H, const double* Rk, const double* Rp,
+                             double* hk_pre) {
+  const int m = s * t, c0 = s * (k - 1), c1 = c0 + 1, s1 = s + 1, ldh = m + 1;
+  double* Rt0 = (double*)calloc((size_t)s1 * s1, sizeof(double));
+  double* Rinv = (double*)calloc((size_t)s * s, sizeof(double));
+  double* Rub = (double*)calloc((size_t)c0 * s1, sizeof(double));
+  double* BR = (double*)calloc((size_t)c0 * s, sizeof(double));
+  double* BH = (double*)calloc((size_t)c0 * c0, sizeof(double));
+  double* BHR = (double*)calloc((size_t)c0 * s, sizeof(double));
+  double* RubB = (double*)calloc((size_t)c0 * s, sizeof(double));
+  double* Hk = (double*)calloc((size_t)s * s, sizeof(double));
+  double* RB = (double*)calloc((size_t)s * s, sizeof(double));
+  double* B0 = (double*)calloc((size_t)s * s, sizeof(double));
+  double* Rs = (double*)calloc((size_t)s * s, sizeof(double));
+  double* E = (double*)calloc((size_t)s * s, sizeof(double));
+  double* H0 = (double*)calloc((size_t)s * s, sizeof(double));
+  for (int i = 0; i < s1; ++i) Rt0[IDX(0, i, s1)] = Rp[IDX(c0, i, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < s; ++i) Rt0[IDX(1 + i, j + 1, s1)] = Rk[IDX(i, j, s)];
+  for (int j = 0; j < s1; ++j)
+    for (int i = 0; i < c0; ++i) Rub[IDX(i, j, c0)] = Rp[IDX(i, j, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < c0; ++i) BR[IDX(i, j, c0)] = Rp[IDX(i, j, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < s; ++i) Rinv[IDX(i, j, s)] = Rt0[IDX(i, j, s1)];
+  oracle_trinv_upper(s, Rinv, s);
+  for (int j = 0; j < c0; ++j)
+    for (int i = 0; i < c0; ++i) BH[IDX(i, j, c0)] = H[IDX(i, j, ldh)];
+  oracle_gemm_nn(c0, s, c0, BH, c0, BR, c0, BHR, c0);
+  oracle_gemm_nn(c0, s, s1, Rub, c0, B1, s1, RubB, c0);
+  for (int i = 0; i < c0 * s; ++i) BHR[i] = -BHR[i] + RubB[i];
+  oracle_gemm_nn(c0, s, s, BHR, c0, Rinv, s, RubB, c0);
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < c0; ++j) H[IDX(j, c0 + i, ldh)] = RubB[IDX(j, i, c0)];
+  const double bk = B1[IDX(s, s - 1, s1)];
+  const double rho_tilde = Rt0[IDX(s - 1, s - 1, s1)], rho = Rt0[IDX(s, s, s1)];
+  const double hk = rho / rho_tilde * bk;
+  H[IDX(s * k, s * k - 1, ldh)] = hk;
+  /* calc_Rk_Bk_Rkinv */
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < s; ++j) {
+      B0[IDX(j, i, s)] = B1[IDX(j, i, s1)];
+      Rs[IDX(j, i, s)] = Rt0[IDX(j, i, s1)];
+    }
+  oracle_gemm_nn(s, s, s, Rs, s, B0, s, RB, s);
+  oracle_gemm_nn(s, s, s, RB, s, Rinv, s, Hk, s);
+  for (int i = 0; i < s; ++i) Hk[IDX(i, s - 1, s)] = fma(bk / rho_tilde, Rt0[IDX(i, s, s1)], Hk[IDX(i, s - 1, s)]);
+  /* calc_hke1eskRRinv */
+  for (int i = 0; i < s; ++i) E[IDX(0, i, s)] = BR[IDX(c0 - 1, i, c0)] * *hk_pre;
+  oracle_gemm_nn(s, s, s, E, s, Rinv, s, H0, s);
+  for (int i = 0; i < s * s; ++i) Hk[i] = Hk[i] - H0[i];
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < s; ++j) H[IDX(c0 + j, c0 + i, ldh)] = Hk[IDX(j, i, s)];
+  *hk_pre = hk;
+  free(Rt0); free(Rinv); free(Rub); free(BR); free(BH); free(BHR); free(RubB); free(Hk); free(RB); free(B0);
+  free(Rs); free(E); free(H0);
+}
+
 int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
                    int ipreflag, int ilu_method, int nblocks, int independ_nvec, int itrmax, double rtol, int s,
                    int t, int basis, int qrflag, const oracle_red* red, double* hist, int* niter, int* iret) {
-  if (s < 1 || t != 1) return -4;
+  if (s < 1 || t < 1) return -4;
   if (qrflag != 1 && qrflag != 2 && qrflag != 4 && qrflag != 5) return -4;
   int err = 0;
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
   if (!M) return err;
   const int m = s * t, s1 = s + 1;
+  const int c1max = s * (t - 1) + 1;
   const size_t nn = (size_t)(n > 0 ? n : 1);
   double* r = (double*)calloc(nn, sizeof(double));
   double* V = (double*)calloc(nn * (size_t)s1, sizeof(double));
-  double* Q = (double*)calloc(nn * (size_t)s1, sizeof(double));
+  double* Q = (double*)calloc(nn * (size_t)(m + 1), sizeof(double));
+  double* W = (double*)calloc(nn * (size_t)s1, sizeof(double));
   double* pv = (double*)calloc(nn, sizeof(double));
   double* v0 = (double*)calloc(nn, sizeof(double));
   double* x0 = (double*)calloc(nn, sizeof(double));
@@ -571,6 +686,7 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
   double* R = (double*)calloc((size_t)s1 * s1, sizeof(double));
   double* R2 = (double*)calloc((size_t)s1 * s1, sizeof(double));
   double* Rinv = (double*)calloc((size_t)s1 * s1, sizeof(double));
+  double* Rp = (double*)calloc((size_t)c1max * s1, sizeof(double));
   double* RB = (double*)calloc((size_t)s1 * s, sizeof(double));
   double* Hes = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
   double* Hg = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
@@ -583,8 +699,8 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
   double* er = (double*)calloc((size_t)m, sizeof(double));
   double* ei = (double*)calloc((size_t)m, sizeof(double));
   for (int i = 0; i < s; ++i) B1[IDX(i + 1, i, s1)] = 1.0; /* sub-diagonal 1 (monomial) */
-  int iflag_exit = 0, iflag_basis = 0, itr, itr0 = 1, mm = 0;
-  double rnorm;
+  int iflag_exit = 0, iflag_basis = 0, itr, itr0 = 1, mm = 0, qmat = 0;
+  double rnorm, hk_pre = 0.0;
   oracle_spmv(n, rowptr, col, val, x, r);
   for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
   const double bnorm = sqrt(oracle_dot(red, n, b, b));
@@ -596,6 +712,8 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
     memset(Hes, 0, sizeof(double) * (m + 1) * m);
     memset(Hg, 0, sizeof(double) * (m + 1) * m);
     memset(R, 0, sizeof(double) * s1 * s1);
+    memset(Rp, 0, sizeof(double) * (size_t)c1max * s1);
+    hk_pre = 0.0;
     oracle_spmv(n, rowptr, col, val, x, r);           /* r = b - A x */
     for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
     rnorm = sqrt(oracle_dot(red, n, r, r));
@@ -604,89 +722,79 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
     zeta[0] = -rnorm;
     const double a = -1.0 / rnorm;
     for (int i = 0; i < n; ++i) V[i] = a * r[i];       /* v1 = -r/||r|| */
-    /* k = 1 (t = 1) */
-    if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
-    for (int i = 1; i <= s; ++i) {
-      const double* vi = V + (size_t)(i - 1) * nn;
-      double* vn = V + (size_t)i * nn;
-      oracle_precon_apply(M, vi, pv);
-      oracle_spmv(n, rowptr, col, val, pv, vn);        /* V_{i+1} = A M^-1 V_i */
-      if (iflag_basis && basis == 1) {                 /* calc_NewtonBasisVec */
-        const double alpha = -B1[IDX(i - 1, i - 1, s1)];
-        if (i != 1) {
-          const double beta = B1[IDX(i - 2, i - 1, s1)];
-          const double* vp = V + (size_t)(i - 2) * nn;
-          for (int q = 0; q < n; ++q) vn[q] = fma(beta, vp[q], fma(alpha, vi[q], vn[q]));
-        } else {
-          for (int q = 0; q < n; ++q) vn[q] = fma(alpha, vi[q], vn[q]);
+    for (int k = 1; k <= t; ++k) {
+      if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
+      qmat = (k != t);
+      for (int i = 1; i <= s; ++i) {
+        const double* vi = V + (size_t)(i - 1) * nn;
+        double* vn = V + (size_t)i * nn;
+        oracle_precon_apply(M, vi, pv);
+        oracle_spmv(n, rowptr, col, val, pv, vn);      /* V_{i+1} = A M^-1 V_i */
+        if (iflag_basis && basis == 1) {               /* calc_NewtonBasisVec */
+          const double alpha = -B1[IDX(i - 1, i - 1, s1)];
+          if (i != 1) {
+            const double beta = B1[IDX(i - 2, i - 1, s1)];
+            const double* vp = V + (size_t)(i - 2) * nn;
+            for (int q = 0; q < n; ++q) vn[q] = fma(beta, vp[q], fma(alpha, vi[q], vn[q]));
+          } else {
+            for (int q = 0; q < n; ++q) vn[q] = fma(alpha, vi[q], vn[q]);
+          }
         }
       }
-    }
-    /* QR of V (n x (s+1)) */
-    if (qrflag == 1 || qrflag == 2) {
-      /* GS_QR_: MGS or CGS, R(k,i) and Q formed; V modified in place */
-      for (int i = 0; i < s1; ++i) {
-        double* vi = V + (size_t)i * nn;
-        if (qrflag == 1) {
-          for (int k = 0; k < i; ++k) {
-            const double* qk = Q + (size_t)k * nn;
-            const double rk = oracle_dot(red, n, vi, qk);
-            R[IDX(k, i, s1)] = rk;
-            for (int q = 0; q < n; ++q) vi[q] = fma(-rk, qk[q], vi[q]);
-          }
-        } else {
-          for (int k = 0; k < i; ++k) R[IDX(k, i, s1)] = oracle_dot(red, n, Q + (size_t)k * nn, vi);
+      if (k == 1) {
+        qr_cols(red, n, s1, V, nn, Q, nn, R, qrflag, qmat, W, R2, Rinv);
+        /* calc_HessenbergMatrix, k = 1: H = R B1 R(1:s,1:s)^-1 */
+        for (int j = 0; j < s; ++j)
+          for (int i = 0; i < s; ++i) Rinv[IDX(i, j, s)] = R[IDX(i, j, s1)];
+        oracle_trinv_upper(s, Rinv, s);
+        oracle_gemm_nn(s1, s, s1, R, s1, B1, s1, RB, s1);
+        oracle_gemm_nn(s1, s, s, RB, s1, Rinv, s, Hes, m + 1);
+        hk_pre = Hes[IDX(s, s - 1, m + 1)];
+      } else {
+        /* block CGS (:2049-2112): R' = Q(:,1:c1)^T V, V(:,2:s+1) -= Q(:,1:c1) R'(:,2:s+1) */
+        const int c1 = s * (k - 1) + 1;
+        for (int j = 0; j < s1; ++j)
+          for (int i = 0; i < c1; ++i)
+            Rp[IDX(i, j, c1)] = oracle_dot(red, n, Q + (size_t)i * nn, V + (size_t)j * nn);
+        for (int j = 1; j < s1; ++j) {
+          double* vj = V + (size_t)j * nn;
           for (int q = 0; q < n; ++q) {
-            double w = vi[q];
-            for (int k = 0; k < i; ++k) w = fma(-R[IDX(k, i, s1)], Q[(size_t)k * nn + q], w);
-            vi[q] = w;
+            double w = vj[q];
+            for (int i = 0; i < c1; ++i) w = fma(-Rp[IDX(i, j, c1)], Q[(size_t)i * nn + q], w);
+            vj[q] = w;
           }
         }
-        const double rii = sqrt(oracle_dot(red, n, vi, vi));
-        R[IDX(i, i, s1)] = rii;
-        const double sc = 1.0 / rii;
-        double* qi = Q + (size_t)i * nn;
-        for (int q = 0; q < n; ++q) qi[q] = sc * vi[q];
+        qr_cols(red, n, s, V + nn, nn, Q + (size_t)c1 * nn, nn, R, qrflag, qmat, W, R2, Rinv);
+        oracle_hessenberg_block(s, t, k, B1, Hes, R, Rp, &hk_pre);
       }
-    } else {
-      cholqr_r(red, n, s1, V, nn, R);
-      if (qrflag == 5) {                               /* CholeskyQR2_Ronly_ */
-        memcpy(Rinv, R, sizeof(double) * s1 * s1);
-        oracle_trinv_upper(s1, Rinv, s1);
-        q_vrinv(n, s1, V, nn, Rinv, Q, nn);            /* Q1 = V R^-1 */
-        cholqr_r(red, n, s1, Q, nn, R2);
-        oracle_trmm_lunn(s1, R2, s1, R, s1);           /* R = R2 R */
+      if (k != t) memcpy(V, Q + (size_t)(s * k) * nn, sizeof(double) * nn);
+      for (int i = s * (k - 1); i < s * k; ++i)
+        for (int j = 0; j <= s * k; ++j) Hg[IDX(j, i, m + 1)] = Hes[IDX(j, i, m + 1)];
+      for (int i = s * (k - 1) + 1; i <= s * k; ++i) {
+        mm = i;
+        givens_m(m, Hg, i, co, si, zeta);
+        rnorm = fabs(zeta[i]);
+        itr0 = i + (itr - 1) + 1;
+        hist[itr0 - 1] = rnorm / bnorm;
+        if (rnorm / bnorm < rtol) { iflag_exit = 1; break; }
+        if (itr0 == itrmax) { iflag_exit = 1; break; }
+      }
+      if (iflag_exit) break;
+      if (basis == 1) {
+        iflag_basis = 1;
+        const int sk = s * k;
+        for (int j = 0; j < sk; ++j)
+          for (int i = 0; i < sk; ++i) Hs[IDX(i, j, sk)] = Hes[IDX(i, j, m + 1)];
+        memset(er, 0, sizeof(double) * m);
+        memset(ei, 0, sizeof(double) * m);
+        (void)oracle_hqr_eigs(sk, Hs, sk, er, ei);
+        oracle_leja(sk, er, ei);
       }
     }
-    /* calc_HessenbergMatrix, k = 1: H = R B1 R(1:s,1:s)^-1 */
-    for (int j = 0; j < s; ++j)
-      for (int i = 0; i < s; ++i) Rinv[IDX(i, j, s)] = R[IDX(i, j, s1)];
-    oracle_trinv_upper(s, Rinv, s);
-    oracle_gemm_nn(s1, s, s1, R, s1, B1, s1, RB, s1);
-    oracle_gemm_nn(s1, s, s, RB, s1, Rinv, s, Hes, m + 1);
-    for (int i = 0; i < s; ++i)
-      for (int j = 0; j <= s; ++j) Hg[IDX(j, i, m + 1)] = Hes[IDX(j, i, m + 1)];
-    for (int i = 1; i <= s; ++i) {
-      mm = i;
-      givens_m(m, Hg, i, co, si, zeta);
-      rnorm = fabs(zeta[i]);
-      itr0 = i + (itr - 1) + 1;
-      hist[itr0 - 1] = rnorm / bnorm;
-      if (rnorm / bnorm < rtol) { iflag_exit = 1; break; }
-      if (itr0 == itrmax) { iflag_exit = 1; break; }
-    }
-    if (!iflag_exit && basis == 1) {
-      iflag_basis = 1;
-      for (int j = 0; j < s; ++j)
-        for (int i = 0; i < s; ++i) Hs[IDX(i, j, s)] = Hes[IDX(i, j, m + 1)];
-      memset(er, 0, sizeof(double) * m);
-      memset(ei, 0, sizeof(double) * m);
-      (void)oracle_hqr_eigs(s, Hs, s, er, ei);
-      oracle_leja(s, er, ei);
-    }
-    /* solution update */
-    if (qrflag == 1 || qrflag == 2) {
-      /* update_xVec with Q (:1136-1236) */
+    /* solution update (:2287-2430) */
+    const int gs = (qrflag == 1 || qrflag == 2);
+    if (gs || qmat || (t > 1 && s == 1)) {
+      /* update_xVec with the orthonormal Q (:1136-1236) */
       for (int i = mm; i >= 1; --i) {
         double yi = zeta[i - 1];
         for (int j = mm; j >= i + 1; --j) yi = yi - Hg[IDX(i - 1, j - 1, m + 1)] * y[j - 1];
@@ -699,19 +807,25 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
         v0[q] = acc;
       }
     } else {
-      /* update_xVec_cagmres (:1338-1478): y = H^-1 zeta, then y = R(1:s,1:s)^-1 y */
+      /* update_xVec_cagmres (:1338-1478): y = H^-1 zeta; the last Vstep
+       * entries times R(1:sv-1,1:sv-1)^-1 (sv = s+1 for t = 1, else s);
+       * Qy = sum of the first mm - Vstep Q columns and Vstep V columns */
       for (int i = mm; i >= 1; --i) {
         double yi = zeta[i - 1];
         for (int j = mm; j >= i + 1; --j) yi = yi - Hg[IDX(i - 1, j - 1, m + 1)] * y[j - 1];
         y[i - 1] = yi / Hg[IDX(i - 1, i - 1, m + 1)];
       }
-      for (int i = 0; i < s; ++i) yt[i] = (i < mm) ? y[i] : 0.0;
-      memcpy(Rinv, R, sizeof(double) * s1 * s1);
-      oracle_trinv_upper(s1, Rinv, s1);
-      oracle_gemm_nn(s, 1, s, Rinv, s1, yt, s, y, s);
+      const int sv = (t == 1) ? s + 1 : s;
+      const int vstep = (sv - 1) - (m - mm);
+      for (int i = 0; i < sv - 1; ++i) yt[i] = (i < vstep) ? y[mm - vstep + i] : 0.0;
+      memcpy(Rinv, R, sizeof(double) * sv * sv);
+      oracle_trinv_upper(sv, Rinv, sv);
+      oracle_gemm_nn(sv - 1, 1, sv - 1, Rinv, sv, yt, sv - 1, y + (mm - vstep), sv - 1);
+      const double* Vv = (t == 1) ? V : V + nn;
       for (int q = 0; q < n; ++q) {
         double acc = 0.0;
-        for (int k = 0; k < mm; ++k) acc = fma(y[k], V[(size_t)k * nn + q], acc);
+        for (int k = 0; k < mm - vstep; ++k) acc = fma(y[k], Q[(size_t)k * nn + q], acc);
+        for (int k = 0; k < vstep; ++k) acc = fma(y[mm - vstep + k], Vv[(size_t)k * nn + q], acc);
         v0[q] = acc;
       }
     }
@@ -721,8 +835,8 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
   }
   *niter = itr0 < itrmax ? itr0 : itrmax;            /* :2466 */
   *iret = (itrmax <= itr) ? 1 : 0;
-  free(r); free(V); free(Q); free(pv); free(v0); free(x0); free(B1); free(R); free(R2); free(Rinv);
-  free(RB); free(Hes); free(Hg); free(Hs); free(co); free(si); free(zeta); free(y); free(yt); free(er);
+  free(r); free(V); free(Q); free(W); free(pv); free(v0); free(x0); free(B1); free(R); free(R2); free(Rinv);
+  free(Rp); free(RB); free(Hes); free(Hg); free(Hs); free(co); free(si); free(zeta); free(y); free(yt); free(er);
   free(ei);
   oracle_precon_free(M);
   return 0;

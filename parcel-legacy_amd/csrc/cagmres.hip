@@ -1,18 +1,21 @@
 // cagmres.hip -- CA-GMRES(s,t), KSP_pCAGMRES (src/pcagmres.f90:1557-2517),
-// for t = 1, with right preconditioning, monomial or Newton basis
-// (calc_NewtonBasis / calc_NewtonBasisVec, :586-611,1480-1552) and the QR
-// variants of QRDecomposition_ (src/QRDecomposition.f90:1331-1551):
-// MGS / CGS (GS_QR_), CholeskyQR (CholeskyQR_Ronly_) and CholeskyQR2.
+// with right preconditioning, monomial or Newton basis (calc_NewtonBasis /
+// calc_NewtonBasisVec, :586-611,1480-1552) and the QR variants of
+// QRDecomposition_ (src/QRDecomposition.f90:1331-1551): MGS / CGS (GS_QR_),
+// CholeskyQR (CholeskyQR_Ronly_) and CholeskyQR2, Q = V R^-1 where a later
+// block needs it (calcQ_VRinv_, :603-671).
 //
-// Device work per restart cycle: the residual, s preconditioner applies and
+// Device work per block of a restart cycle: s preconditioner applies and
 // SpMVs that build the basis V (n x (s+1), the Newton recurrence fused into
-// the SpMV's store), one pass computing every entry of the Gram matrix V^T V
-// at once (CholeskyQR; CholeskyQR2 adds one pass over V R^-1 formed on the
-// fly, never stored), and the solution update.  The (s+1) x (s+1) dense
-// work -- Cholesky, Hessenberg matrix R B R^-1 (calc_HessenbergMatrix, k=1,
-// :879-1051), Givens rotations, Ritz values and Leja ordering -- runs on the
-// host as in the reference, after one device-to-host copy of the Gram matrix
-// and the residual norm per cycle.
+// the SpMV's store); for blocks k >= 2 the block classical Gram-Schmidt
+// against the s(k-1)+1 earlier Q columns (one pass computing every entry of
+// Q^T V, one pass subtracting Q R'); one pass computing every entry of the
+// Gram matrix V^T V at once (CholeskyQR; CholeskyQR2 adds one pass over
+// V R^-1 formed on the fly, never stored); and after the cycle the solution
+// update.  The small dense work -- Cholesky, the Hessenberg matrix
+// (calc_HessenbergMatrix, :879-1051), Givens rotations, Ritz values and
+// Leja ordering -- runs on the host as in the reference, after one
+// device-to-host copy of the Gram matrix (and of R') per block.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -26,6 +29,7 @@ namespace parcel {
 namespace {
 
 constexpr int kMaxCols = 16;  // s + 1 <= 16
+constexpr int kMaxM = 128;    // s * t
 
 // st[0] = bnorm, st[1] = rnorm, st[2] = -1/rnorm
 __global__ void __launch_bounds__(kNT) k_ca_restart(const double* part_bb, const double* part_rr, int nb,
@@ -238,17 +242,112 @@ __global__ void __launch_bounds__(kNT) k_ca_qr_norm(int n, const double* __restr
     qi[j] = sc * vi[j];
 }
 
-// ---- solution update ---------------------------------------------------
-struct Coef {
-  double c[kMaxCols];
-};
+// ---- block CGS between blocks (t > 1) ------------------------------------
+// Partials of <Q_i, V_j> for Q columns i in [4*blockIdx.y, 4*blockIdx.y + 4)
+// (below c1) and the NV columns of V; slot i + c1*j.
+template <int NV>
+__global__ void __launch_bounds__(kNT) k_ca_bdots(int n, const double* Q, long ldq, int c1, const double* V,
+                                                  long ldv, double* part) {
+  constexpr int NP = 4 * NV;
+  __shared__ double sh[kWaves * NP];
+  const int i0 = blockIdx.y * 4;
+  const int ni = (c1 - i0) < 4 ? (c1 - i0) : 4;
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) acc[p] = 0.0;
+  for (long t = blockIdx.x; t < ntiles; t += nb) {
+    const long q = t * kNT + threadIdx.x;
+    if (q < n) {
+      double v[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) v[j] = V[(long)j * ldv + q];
+#pragma unroll
+      for (int ii = 0; ii < 4; ++ii) {
+        if (ii < ni) {
+          const double qv = Q[(long)(i0 + ii) * ldq + q];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) acc[ii * NV + j] = fma(qv, v[j], acc[ii * NV + j]);
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const double v = wave_sum(acc[p]);
+    if (lane == 0) sh[wave * NP + p] = v;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+    const int ii = p / NV, j = p % NV;
+    if (ii >= ni) continue;
+    double v = sh[p];
+    for (int w = 1; w < kWaves; ++w) v += sh[w * NP + p];
+    part[(long)(i0 + ii + c1 * j) * kNBMax + blockIdx.x] = v;
+  }
+}
 
-// v0 = sum_{k < mm} fma(c_k, W_k, .) from 0 (sum_vec_cagmres / sum_vec_FP64)
-__global__ void __launch_bounds__(kNT) k_ca_sumvec(int n, const double* W, long ldw, Coef y, int mm,
-                                                   double* __restrict__ v0) {
+// out[slot] = reduction of slot's partials (one workgroup per slot)
+__global__ void __launch_bounds__(kNT) k_ca_reduce_slots(const double* part, int nb, double* out) {
+  __shared__ double sh[kWaves];
+  const double v = reduce_part(part + (long)blockIdx.x * kNBMax, nb, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// V_j -= sum_i R'(i,j) Q_i for j = 1..nv (columns of V), i = 0..c1-1 in
+// order, as fma(-R'(i,j), Q_i, .) (calc_gemm with the negated R',
+// vec.f90:403-440); R' is c1 x (nv+1), ld c1, column 0 unused
+__global__ void __launch_bounds__(kNT) k_ca_bupd(int n, const double* Q, long ldq, int c1, double* V, long ldv,
+                                                 int nv, const double* Rp) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double w[kMaxCols];
+#pragma unroll
+    for (int j = 0; j < kMaxCols; ++j)
+      if (j < nv) w[j] = V[(long)(j + 1) * ldv + q];
+    for (int i = 0; i < c1; ++i) {
+      const double qv = Q[(long)i * ldq + q];
+#pragma unroll
+      for (int j = 0; j < kMaxCols; ++j)
+        if (j < nv) w[j] = fma(-Rp[i + (long)c1 * (j + 1)], qv, w[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxCols; ++j)
+      if (j < nv) V[(long)(j + 1) * ldv + q] = w[j];
+  }
+}
+
+// Q = V R^-1 (calcQ_VRinv_, QRDecomposition.f90:603-671): column c sums
+// fma(V_k, Rinv(k,c), .) over k = 0..c from 0; Rinv is cnt x cnt, ld cnt
+__global__ void __launch_bounds__(kNT) k_ca_vrinv(int n, const double* V, long ldv, int cnt, const double* Rinv,
+                                                  double* Q, long ldq) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    double w[kMaxCols];
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c)
+      if (c < cnt) w[c] = V[(long)c * ldv + q];
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) {
+      if (c < cnt) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k <= c; ++k) a = fma(w[k], Rinv[k + c * cnt], a);
+        Q[(long)c * ldq + q] = a;
+      }
+    }
+  }
+}
+
+// ---- solution update ---------------------------------------------------
+// v0 = sum_{k < nq} fma(y_k, Q_k, .) then sum_{k < nv} fma(y_{nq+k}, W_k, .),
+// from 0 (sum_vec_cagmres, cu_pcagmres.cu:9-60; nv = 0: sum_vec)
+__global__ void __launch_bounds__(kNT) k_ca_sumvec(int n, const double* Q, long ldq, int nq, const double* W,
+                                                   long ldw, int nv, const double* y, double* __restrict__ v0) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double a = 0.0;
-    for (int k = 0; k < mm; ++k) a = fma(y.c[k], W[(long)k * ldw + i], a);
+    for (int k = 0; k < nq; ++k) a = fma(y[k], Q[(long)k * ldq + i], a);
+    for (int k = 0; k < nv; ++k) a = fma(y[nq + k], W[(long)k * ldw + i], a);
     v0[i] = a;
   }
 }
@@ -285,6 +384,18 @@ void cgs_dots(int cnt, dim3 grid, hipStream_t st, int n, const double* Q, long l
     PARCEL_CGSD(14) PARCEL_CGSD(15)
 #undef PARCEL_CGSD
     default: throw hip_error("CA-GMRES: CGS column count out of range", PARCEL_HIP_ERR_ARG);
+  }
+}
+
+void bdots(int nv, dim3 grid, hipStream_t st, int n, const double* Q, long ldq, int c1, const double* V, long ldv,
+           double* part) {
+  switch (nv) {
+#define PARCEL_BD(C) \
+  case C: hipLaunchKernelGGL(k_ca_bdots<C>, grid, dim3(kNT), 0, st, n, Q, ldq, c1, V, ldv, part); break;
+    PARCEL_BD(2) PARCEL_BD(3) PARCEL_BD(4) PARCEL_BD(5) PARCEL_BD(6) PARCEL_BD(7) PARCEL_BD(8) PARCEL_BD(9)
+    PARCEL_BD(10) PARCEL_BD(11) PARCEL_BD(12) PARCEL_BD(13) PARCEL_BD(14) PARCEL_BD(15) PARCEL_BD(16)
+#undef PARCEL_BD
+    default: throw hip_error("CA-GMRES: s + 1 must be in 2..16", PARCEL_HIP_ERR_ARG);
   }
 }
 
@@ -342,37 +453,43 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
   const int s = P.prm.CAGMRES_sstep, t = P.prm.CAGMRES_tstep;
   const int basis = P.prm.CAGMRES_basis, qrflag = P.prm.CAGMRES_QRflag;
   if (s < 1 || s + 1 > kMaxCols) throw hip_error("CAGMRES_sstep must be in 1..15", PARCEL_HIP_ERR_ARG);
-  if (t != 1) throw hip_error("CAGMRES_tstep > 1 (block CGS between blocks) is not supported",
-                              PARCEL_HIP_ERR_UNSUPPORTED);
+  if (t < 1 || s * t > kMaxM) throw hip_error("CAGMRES_tstep must be >= 1 with s*t <= 128", PARCEL_HIP_ERR_ARG);
   if (qrflag != PARCEL_QR_MGS && qrflag != PARCEL_QR_CGS && qrflag != PARCEL_QR_CHOLESKYQR &&
       qrflag != PARCEL_QR_CHOLESKYQR2)
     throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
   const bool gs = qrflag == PARCEL_QR_MGS || qrflag == PARCEL_QR_CGS;
   const int m = s * t, nc = s + 1;
+  const int c1max = s * (t - 1) + 1;
+  const bool needQ = gs || t > 1;
   plan_workspace(P, 1);
   const long ldv = n;
   if (P.basis.n < (size_t)nc * (size_t)n) P.basis.alloc((size_t)nc * (size_t)n);
-  if (gs && P.basis2.n < (size_t)nc * (size_t)n) P.basis2.alloc((size_t)nc * (size_t)n);
+  if (needQ && P.basis2.n < (size_t)(m + 1) * (size_t)n) P.basis2.alloc((size_t)(m + 1) * (size_t)n);
   ensure_vecs(P, 3);
   double* V = P.basis.p;
-  double* Q = gs ? P.basis2.p : nullptr;
+  double* Q = needQ ? P.basis2.p : nullptr;
   double* pv = P.vec[0].p;
   double* v0 = P.vec[1].p;
   double* x0 = P.vec[2].p;
   const int np = nc * (nc + 1) / 2;
-  const size_t nslot = (size_t)std::max(np, 2) * kNBMax;
+  const size_t nslot = (size_t)std::max(std::max(np, 2), t > 1 ? c1max * nc : 0) * kNBMax;
   if (P.wpart.n < nslot) P.wpart.alloc(nslot);
   double* gp = P.wpart.p;
-  // device scalars: st[0..2], then G / R (nc x nc), then Rinv (nc x nc)
-  const size_t dbytes = sizeof(double) * (4 + 2 * (size_t)nc * nc);
-  if (P.solver_state.n < dbytes) P.solver_state.alloc(dbytes);
+  // device scalars: st[0..3], G / R (nc x nc), Rinv (nc x nc), R' (c1max x nc), y (m + 1)
+  const size_t nrp = t > 1 ? (size_t)c1max * nc : 0;
+  const size_t dcount = 4 + 2 * (size_t)nc * nc + nrp + (size_t)m + 1;
+  if (P.solver_state.n < sizeof(double) * dcount) P.solver_state.alloc(sizeof(double) * dcount);
   double* dst = reinterpret_cast<double*>(P.solver_state.p);
   double* dG = dst + 4;
   double* dRinv = dG + (size_t)nc * nc;
-  double* hbuf = nullptr;  // pinned: st (4) + G (nc^2) + Rinv (nc^2)
-  PARCEL_HIP_CHECK(hipHostMalloc((void**)&hbuf, dbytes, hipHostMallocDefault));
+  double* dRp = dRinv + (size_t)nc * nc;
+  double* dY = dRp + nrp;
+  double* hbuf = nullptr;  // pinned mirror of the device scalars
+  PARCEL_HIP_CHECK(hipHostMalloc((void**)&hbuf, sizeof(double) * dcount, hipHostMallocDefault));
   double* hG = hbuf + 4;
   double* hRinv = hG + (size_t)nc * nc;
+  double* hRp = hRinv + (size_t)nc * nc;
+  double* hY = hRp + nrp;
   const int nb = tiled_nb(n);
   const dim3 grid(nb), block(kNT);
   const SellArgs A = sell_args(P.A);
@@ -387,27 +504,100 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
   std::vector<double> hist((size_t)std::max(itrmax, 0) + m + 1, 0.0);
   std::vector<double> B1((size_t)nc * s, 0.0), R((size_t)nc * nc), R2((size_t)nc * nc), Rinv((size_t)nc * nc);
   std::vector<double> RB((size_t)nc * s), Hes((size_t)(m + 1) * m), Hg((size_t)(m + 1) * m), Hs((size_t)m * m);
+  std::vector<double> Rp(std::max(nrp, (size_t)1));
   std::vector<double> co(m + 1), si(m + 1), zeta(m + 1), y(m + 1), yt(m + 1), er(m, 0.0), ei(m, 0.0);
   for (int i = 0; i < s; ++i) B1[(i + 1) + (size_t)i * nc] = 1.0;  // monomial: sub-diagonal 1
-  bool iflag_exit = false, iflag_basis = false;
+  bool iflag_exit = false, iflag_basis = false, qmat = false;
   int itr = 1, itr0 = 1, mm = 0;
-  double bnorm = 0.0, rnorm = 0.0;
+  double bnorm = 0.0, rnorm = 0.0, hk_pre = 0.0;
   auto fetch = [&](size_t count) {
     PARCEL_HIP_CHECK(hipMemcpyAsync(hbuf, dst, sizeof(double) * count, hipMemcpyDeviceToHost, st));
     PARCEL_HIP_CHECK(hipStreamSynchronize(st));
   };
-  auto cholqr = [&](const double* rinv_dev, std::vector<double>& Rout) {
-    gram(nc, grid, st, n, V, ldv, rinv_dev, gp);
-    hipLaunchKernelGGL(k_ca_gram_reduce, dim3(np), block, 0, st, (const double*)gp, nb, nc, dG);
+  // Gram matrix of cnt columns (of V R^-1 with rinv_dev) -> Cholesky factor
+  auto cholqr = [&](const double* Vb, int cnt, const double* rinv_dev, std::vector<double>& Rout) {
+    const int npc = cnt * (cnt + 1) / 2;
+    PARCEL_HIP_CHECK(hipMemsetAsync(dG, 0, sizeof(double) * (size_t)cnt * cnt, st));
+    gram(cnt, grid, st, n, Vb, ldv, rinv_dev, gp);
+    hipLaunchKernelGGL(k_ca_gram_reduce, dim3(npc), block, 0, st, (const double*)gp, nb, cnt, dG);
     launches += 2;
-    fetch(4 + (size_t)nc * nc);
-    for (int j = 0; j < nc; ++j)
-      for (int i = 0; i < nc; ++i) Rout[i + (size_t)j * nc] = (i <= j) ? hG[i + (size_t)j * nc] : 0.0;
-    (void)dense::chol_upper(nc, Rout.data(), nc);
-    for (int j = 0; j < nc; ++j)
-      for (int i = j + 1; i < nc; ++i) Rout[i + (size_t)j * nc] = 0.0;
+    fetch(4 + (size_t)cnt * cnt);
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < cnt; ++i) Rout[i + (size_t)j * cnt] = (i <= j) ? hG[i + (size_t)j * cnt] : 0.0;
+    (void)dense::chol_upper(cnt, Rout.data(), cnt);
+    for (int j = 0; j < cnt; ++j)
+      for (int i = j + 1; i < cnt; ++i) Rout[i + (size_t)j * cnt] = 0.0;
   };
-  PARCEL_HIP_CHECK(hipMemsetAsync(dG, 0, sizeof(double) * (size_t)nc * nc, st));
+  auto upload_rinv = [&](const std::vector<double>& Rsrc, int cnt) {
+    std::memcpy(Rinv.data(), Rsrc.data(), sizeof(double) * (size_t)cnt * cnt);
+    dense::trinv_upper(cnt, Rinv.data(), cnt);
+    std::memcpy(hRinv, Rinv.data(), sizeof(double) * (size_t)cnt * cnt);
+    PARCEL_HIP_CHECK(hipMemcpyAsync(dRinv, hRinv, sizeof(double) * (size_t)cnt * cnt, hipMemcpyHostToDevice, st));
+  };
+  // QRDecomposition_ (QRDecomposition.f90:1331-1551) of cnt columns Vb into
+  // R (cnt x cnt) and, for GS or with qmat, Qb = the orthonormal factor
+  auto qr_block = [&](double* Vb, int cnt, double* Qb, bool qm, std::vector<double>& Rout) {
+    if (cnt == 1 || gs) {
+      const bool mgs = (cnt == 1 || qrflag == PARCEL_QR_MGS);
+      double* dR = dG;
+      PARCEL_HIP_CHECK(hipMemsetAsync(dR, 0, sizeof(double) * (size_t)cnt * cnt, st));
+      for (int i = 0; i < cnt; ++i) {
+        double* vi = Vb + (long)i * ldv;
+        double* sl = gp;
+        int fin;  // slot holding the partials of <v_i,v_i>
+        if (mgs) {
+          if (i == 0) {
+            hipLaunchKernelGGL(k_ca_dot, grid, block, 0, st, n, (const double*)vi, (const double*)vi, sl);
+            fin = 0;
+            ++launches;
+          } else {
+            hipLaunchKernelGGL(k_ca_dot, grid, block, 0, st, n, (const double*)vi, (const double*)Qb, sl);
+            ++launches;
+            for (int k = 0; k < i; ++k) {
+              const double* qnext = (k + 1 < i) ? Qb + (long)(k + 1) * ldv : nullptr;
+              hipLaunchKernelGGL(k_ca_mgs, grid, block, 0, st, n, vi, (const double*)(Qb + (long)k * ldv), qnext,
+                                 (const double*)(gp + (size_t)(k & 1) * kNBMax),
+                                 gp + (size_t)((k + 1) & 1) * kNBMax, nb, dR, cnt, k, i);
+              ++launches;
+            }
+            fin = i & 1;
+          }
+        } else {
+          if (i > 0) {
+            cgs_dots(i, grid, st, n, Qb, ldv, vi, gp);
+            hipLaunchKernelGGL(k_ca_cgs_reduce, dim3(i), block, 0, st, (const double*)gp, nb, dR, cnt, i);
+            launches += 2;
+          }
+          // the reduce kernel has consumed slots 0..i-1 before this one writes slot 0
+          hipLaunchKernelGGL(k_ca_cgs_update, grid, block, 0, st, n, (const double*)Qb, ldv, vi, (const double*)dR,
+                             cnt, i, gp);
+          ++launches;
+          fin = 0;
+        }
+        const double* pfin = gp + (size_t)fin * kNBMax;
+        hipLaunchKernelGGL(k_ca_qr_norm, grid, block, 0, st, n, (const double*)vi, Qb + (long)i * ldv, pfin, nb, dR,
+                           cnt, i);
+        ++launches;
+      }
+      fetch(4 + (size_t)cnt * cnt);
+      for (size_t q = 0; q < (size_t)cnt * cnt; ++q) Rout[q] = hG[q];
+      return;
+    }
+    cholqr(Vb, cnt, nullptr, Rout);
+    if (qrflag == PARCEL_QR_CHOLESKYQR2) {
+      // Q1 = V R^-1 (never stored), R2 = chol(Q1^T Q1), R = R2 R
+      upload_rinv(Rout, cnt);
+      cholqr(Vb, cnt, dRinv, R2);
+      dense::trmm_lunn(cnt, R2.data(), cnt, Rout.data(), cnt);
+    }
+    if (qm) {  // calcQ_VRinv_
+      upload_rinv(Rout, cnt);
+      hipLaunchKernelGGL(k_ca_vrinv, grid, block, 0, st, n, (const double*)Vb, ldv, cnt, (const double*)dRinv, Qb,
+                         ldv);
+      ++launches;
+    }
+  };
+
   for (itr = 1; itr <= itrmax; itr += s * t) {
     itr0 = itr;
     std::fill(co.begin(), co.end(), 0.0);
@@ -416,125 +606,102 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
     std::fill(Hes.begin(), Hes.end(), 0.0);
     std::fill(Hg.begin(), Hg.end(), 0.0);
     std::fill(R.begin(), R.end(), 0.0);
+    std::fill(Rp.begin(), Rp.end(), 0.0);
+    hk_pre = 0.0;
     // r = b - A x, ||r|| (and ||b|| on the first cycle); v1 = -r/||r||
     launch_residual(P, x, b, P.r.p, P.part.p);
     hipLaunchKernelGGL(k_ca_restart, dim3(1), block, 0, st, (const double*)(P.part.p + PART_BB * kNBMax),
                        (const double*)(P.part.p + PART_RR0 * kNBMax), nb, dst, itr == 1 ? 1 : 0);
     hipLaunchKernelGGL(k_ca_scale, grid, block, 0, st, n, (const double*)P.r.p, V, (const double*)dst);
     launches += 3;
-    // basis (the restart test below uses the norm fetched with the Gram
-    // matrix; a converged restart leaves x untouched either way)
-    if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
-    for (int i = 1; i <= s; ++i) {
-      const double* vi = V + (long)(i - 1) * ldv;
-      precon_apply(P, vi, pv, nullptr);
-      int mode = 0;
-      double alpha = 0.0, beta = 0.0;
-      if (iflag_basis && basis == 1) {
-        alpha = -B1[(i - 1) + (size_t)(i - 1) * nc];
-        mode = 1;
-        if (i != 1) {
-          beta = B1[(i - 2) + (size_t)(i - 1) * nc];
-          mode = 2;
+    bool converged = false;
+    for (int k = 1; k <= t; ++k) {
+      if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
+      qmat = (k != t);
+      for (int i = 1; i <= s; ++i) {
+        const double* vi = V + (long)(i - 1) * ldv;
+        precon_apply(P, vi, pv, nullptr);
+        int mode = 0;
+        double alpha = 0.0, beta = 0.0;
+        if (iflag_basis && basis == 1) {
+          alpha = -B1[(i - 1) + (size_t)(i - 1) * nc];
+          mode = 1;
+          if (i != 1) {
+            beta = B1[(i - 2) + (size_t)(i - 1) * nc];
+            mode = 2;
+          }
+        }
+        hipLaunchKernelGGL(k_ca_basis, grid, block, 0, st, n, A, (const double*)pv, V + (long)i * ldv, vi,
+                           i >= 2 ? (const double*)(V + (long)(i - 2) * ldv) : nullptr, alpha, beta, mode);
+        launches += 2;
+      }
+      if (k == 1) {
+        qr_block(V, nc, Q, gs || qmat, R);
+        // the restart test (pcagmres.f90:1977, abstol = 0) uses the norm
+        // fetched with R; a converged restart leaves x untouched either way
+        if (itr == 1) bnorm = hbuf[0];
+        rnorm = hbuf[1];
+        if (rnorm / bnorm < rtol) {
+          converged = true;
+          break;
+        }
+        if (itr == 1) hist[0] = rnorm / bnorm;
+        zeta[0] = -rnorm;
+        // calc_HessenbergMatrix, k = 1: H = R B1 R(1:s,1:s)^-1
+        for (int j = 0; j < s; ++j)
+          for (int i = 0; i < s; ++i) Rinv[i + (size_t)j * s] = R[i + (size_t)j * nc];
+        dense::trinv_upper(s, Rinv.data(), s);
+        dense::gemm_nn(nc, s, nc, R.data(), nc, B1.data(), nc, RB.data(), nc);
+        dense::gemm_nn(nc, s, s, RB.data(), nc, Rinv.data(), s, Hes.data(), m + 1);
+        hk_pre = Hes[s + (size_t)(s - 1) * (m + 1)];
+      } else {
+        // block CGS: R' = Q(:,1:c1)^T V, V(:,2:s+1) -= Q(:,1:c1) R'(:,2:s+1)
+        const int c1 = s * (k - 1) + 1;
+        bdots(nc, dim3(nb, (c1 + 3) / 4), st, n, Q, ldv, c1, V, ldv, gp);
+        hipLaunchKernelGGL(k_ca_reduce_slots, dim3(c1 * nc), block, 0, st, (const double*)gp, nb, dRp);
+        hipLaunchKernelGGL(k_ca_bupd, grid, block, 0, st, n, (const double*)Q, ldv, c1, V, ldv, s,
+                           (const double*)dRp);
+        launches += 3;
+        PARCEL_HIP_CHECK(hipMemcpyAsync(hRp, dRp, sizeof(double) * (size_t)c1 * nc, hipMemcpyDeviceToHost, st));
+        std::vector<double> Rk((size_t)s * s, 0.0);
+        qr_block(V + ldv, s, Q + (long)c1 * ldv, gs || qmat, Rk);  // synchronises the stream
+        for (size_t q = 0; q < (size_t)c1 * nc; ++q) Rp[q] = hRp[q];
+        dense::hessenberg_block(s, t, k, B1.data(), Hes.data(), Rk.data(), Rp.data(), &hk_pre);
+        std::memcpy(R.data(), Rk.data(), sizeof(double) * (size_t)s * s);
+      }
+      if (k != t)
+        PARCEL_HIP_CHECK(hipMemcpyAsync(V, Q + (long)(s * k) * ldv, sizeof(double) * (size_t)n,
+                                        hipMemcpyDeviceToDevice, st));
+      for (int i = s * (k - 1); i < s * k; ++i)
+        for (int j = 0; j <= s * k; ++j) Hg[j + (size_t)i * (m + 1)] = Hes[j + (size_t)i * (m + 1)];
+      for (int i = s * (k - 1) + 1; i <= s * k; ++i) {
+        mm = i;
+        givens(m, Hg.data(), i, co.data(), si.data(), zeta.data());
+        rnorm = std::fabs(zeta[i]);
+        itr0 = i + (itr - 1) + 1;
+        hist[itr0 - 1] = rnorm / bnorm;
+        if (rnorm / bnorm < rtol || itr0 == itrmax) {
+          iflag_exit = true;
+          break;
         }
       }
-      hipLaunchKernelGGL(k_ca_basis, grid, block, 0, st, n, A, (const double*)pv, V + (long)i * ldv, vi,
-                         i >= 2 ? (const double*)(V + (long)(i - 2) * ldv) : nullptr, alpha, beta, mode);
-      launches += 2;
-    }
-    // QR of V
-    if (gs) {
-      double* dR = dG;
-      PARCEL_HIP_CHECK(hipMemsetAsync(dR, 0, sizeof(double) * (size_t)nc * nc, st));
-      for (int i = 0; i < nc; ++i) {
-        double* vi = V + (long)i * ldv;
-        double* sl = gp;
-        int fin;  // slot holding the partials of <v_i,v_i>
-        if (qrflag == PARCEL_QR_MGS) {
-          if (i == 0) {
-            hipLaunchKernelGGL(k_ca_dot, grid, block, 0, st, n, (const double*)vi, (const double*)vi, sl);
-            fin = 0;
-            ++launches;
-          } else {
-            hipLaunchKernelGGL(k_ca_dot, grid, block, 0, st, n, (const double*)vi, (const double*)Q, sl);
-            ++launches;
-            for (int k = 0; k < i; ++k) {
-              const double* qnext = (k + 1 < i) ? Q + (long)(k + 1) * ldv : nullptr;
-              hipLaunchKernelGGL(k_ca_mgs, grid, block, 0, st, n, vi, (const double*)(Q + (long)k * ldv), qnext,
-                                 (const double*)(gp + (size_t)(k & 1) * kNBMax),
-                                 gp + (size_t)((k + 1) & 1) * kNBMax, nb, dR, nc, k, i);
-              ++launches;
-            }
-            fin = i & 1;
-          }
-        } else {
-          if (i > 0) {
-            cgs_dots(i, grid, st, n, Q, ldv, vi, gp);
-            hipLaunchKernelGGL(k_ca_cgs_reduce, dim3(i), block, 0, st, (const double*)gp, nb, dR, nc, i);
-            launches += 2;
-          }
-          // the reduce kernel has consumed slots 0..i-1 before this one writes slot 0
-          hipLaunchKernelGGL(k_ca_cgs_update, grid, block, 0, st, n, (const double*)Q, ldv, vi, (const double*)dR,
-                             nc, i, gp);
-          ++launches;
-          fin = 0;
-        }
-        const double* pfin = gp + (size_t)fin * kNBMax;
-        hipLaunchKernelGGL(k_ca_qr_norm, grid, block, 0, st, n, (const double*)vi, Q + (long)i * ldv, pfin, nb, dR,
-                           nc, i);
-        ++launches;
-      }
-      fetch(4 + (size_t)nc * nc);
-      for (size_t q = 0; q < (size_t)nc * nc; ++q) R[q] = hG[q];
-    } else {
-      cholqr(nullptr, R);
-      if (qrflag == PARCEL_QR_CHOLESKYQR2) {
-        // Q1 = V R^-1 (never stored), R2 = chol(Q1^T Q1), R = R2 R
-        std::memcpy(Rinv.data(), R.data(), sizeof(double) * (size_t)nc * nc);
-        dense::trinv_upper(nc, Rinv.data(), nc);
-        std::memcpy(hRinv, Rinv.data(), sizeof(double) * (size_t)nc * nc);
-        PARCEL_HIP_CHECK(hipMemcpyAsync(dRinv, hRinv, sizeof(double) * (size_t)nc * nc, hipMemcpyHostToDevice, st));
-        cholqr(dRinv, R2);
-        dense::trmm_lunn(nc, R2.data(), nc, R.data(), nc);
+      if (iflag_exit) break;
+      if (basis == 1) {
+        iflag_basis = true;
+        const int sk = s * k;
+        for (int j = 0; j < sk; ++j)
+          for (int i = 0; i < sk; ++i) Hs[i + (size_t)j * sk] = Hes[i + (size_t)j * (m + 1)];
+        std::fill(er.begin(), er.end(), 0.0);
+        std::fill(ei.begin(), ei.end(), 0.0);
+        (void)dense::hqr_eigs(sk, Hs.data(), sk, er.data(), ei.data());
+        dense::leja_order(sk, er.data(), ei.data());
       }
     }
-    if (itr == 1) bnorm = hbuf[0];
-    rnorm = hbuf[1];
-    if (rnorm / bnorm < rtol) break;  // pcagmres.f90:1977 (abstol = 0)
-    if (itr == 1) hist[0] = rnorm / bnorm;
-    zeta[0] = -rnorm;
-    // calc_HessenbergMatrix, k = 1: H = R B1 R(1:s,1:s)^-1
-    for (int j = 0; j < s; ++j)
-      for (int i = 0; i < s; ++i) Rinv[i + (size_t)j * s] = R[i + (size_t)j * nc];
-    dense::trinv_upper(s, Rinv.data(), s);
-    dense::gemm_nn(nc, s, nc, R.data(), nc, B1.data(), nc, RB.data(), nc);
-    dense::gemm_nn(nc, s, s, RB.data(), nc, Rinv.data(), s, Hes.data(), m + 1);
-    for (int i = 0; i < s; ++i)
-      for (int j = 0; j <= s; ++j) Hg[j + (size_t)i * (m + 1)] = Hes[j + (size_t)i * (m + 1)];
-    for (int i = 1; i <= s; ++i) {
-      mm = i;
-      givens(m, Hg.data(), i, co.data(), si.data(), zeta.data());
-      rnorm = std::fabs(zeta[i]);
-      itr0 = i + (itr - 1) + 1;
-      hist[itr0 - 1] = rnorm / bnorm;
-      if (rnorm / bnorm < rtol || itr0 == itrmax) {
-        iflag_exit = true;
-        break;
-      }
-    }
-    if (!iflag_exit && basis == 1) {
-      iflag_basis = true;
-      for (int j = 0; j < s; ++j)
-        for (int i = 0; i < s; ++i) Hs[i + (size_t)j * s] = Hes[i + (size_t)j * (m + 1)];
-      std::fill(er.begin(), er.end(), 0.0);
-      std::fill(ei.begin(), ei.end(), 0.0);
-      (void)dense::hqr_eigs(s, Hs.data(), s, er.data(), ei.data());
-      dense::leja_order(s, er.data(), ei.data());
-    }
-    // solution update
-    Coef cy{};
-    const double* W;
-    if (gs) {
+    if (converged) break;
+    // solution update (pcagmres.f90:2287-2430)
+    int nq, nv;
+    const double* W = nullptr;
+    if (gs || qmat || (t > 1 && s == 1)) {
       // update_xVec (pcagmres.f90:1136-1236) with the orthonormal Q
       for (int i = mm; i >= 1; --i) {
         double yi = zeta[i - 1];
@@ -542,26 +709,36 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
         const double d = Hg[(i - 1) + (size_t)(i - 1) * (m + 1)];
         y[i - 1] = (d == 0.0) ? yi / 1.0e-300 : yi / d;
       }
-      for (int k = 0; k < mm; ++k) cy.c[k] = y[k];
-      W = Q;
+      nq = mm;
+      nv = 0;
     } else {
-      // update_xVec_cagmres (pcagmres.f90:1338-1478): y = H^-1 zeta, then R(1:s,1:s)^-1 y
+      // update_xVec_cagmres (pcagmres.f90:1338-1478): y = H^-1 zeta; its last
+      // vstep entries times R(1:sv-1,1:sv-1)^-1 (sv = s+1 for t = 1, else s);
+      // Qy = the first mm - vstep Q columns and vstep V columns
       for (int i = mm; i >= 1; --i) {
         double yi = zeta[i - 1];
         for (int j = mm; j >= i + 1; --j) yi = yi - Hg[(i - 1) + (size_t)(j - 1) * (m + 1)] * y[j - 1];
         y[i - 1] = yi / Hg[(i - 1) + (size_t)(i - 1) * (m + 1)];
       }
-      for (int i = 0; i < s; ++i) yt[i] = (i < mm) ? y[i] : 0.0;
-      std::memcpy(Rinv.data(), R.data(), sizeof(double) * (size_t)nc * nc);
-      dense::trinv_upper(nc, Rinv.data(), nc);
-      dense::gemm_nn(s, 1, s, Rinv.data(), nc, yt.data(), s, y.data(), s);
-      for (int k = 0; k < mm; ++k) cy.c[k] = y[k];
-      W = V;
+      const int sv = (t == 1) ? s + 1 : s;
+      const int vstep = (sv - 1) - (m - mm);
+      for (int i = 0; i < sv - 1; ++i) yt[i] = (i < vstep) ? y[mm - vstep + i] : 0.0;
+      std::memcpy(Rinv.data(), R.data(), sizeof(double) * (size_t)sv * sv);
+      dense::trinv_upper(sv, Rinv.data(), sv);
+      dense::gemm_nn(sv - 1, 1, sv - 1, Rinv.data(), sv, yt.data(), sv - 1, y.data() + (mm - vstep), sv - 1);
+      nq = mm - vstep;
+      nv = vstep;
+      W = (t == 1) ? V : V + ldv;
     }
-    hipLaunchKernelGGL(k_ca_sumvec, grid, block, 0, st, n, W, ldv, cy, mm, v0);
+    for (int k = 0; k < nq + nv; ++k) hY[k] = y[k];
+    PARCEL_HIP_CHECK(hipMemcpyAsync(dY, hY, sizeof(double) * (size_t)std::max(nq + nv, 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ca_sumvec, grid, block, 0, st, n, (const double*)Q, ldv, nq, W, ldv, nv,
+                       (const double*)dY, v0);
     precon_apply(P, v0, x0, nullptr);
     hipLaunchKernelGGL(k_ca_xadd, grid, block, 0, st, n, x, (const double*)x0);
     launches += 3;
+    // hY is rewritten by the next cycle only after the stream has drained
+    PARCEL_HIP_CHECK(hipStreamSynchronize(st));
     PARCEL_HIP_CHECK(hipGetLastError());
     if (iflag_exit) break;
   }

@@ -201,8 +201,8 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
     if (P.prm.solver == PARCEL_SOLVER_CAGMRES) {
       if (P.prm.CAGMRES_sstep < 1 || P.prm.CAGMRES_sstep > 15)
         throw hip_error("CAGMRES_sstep must be in 1..15", PARCEL_HIP_ERR_ARG);
-      if (P.prm.CAGMRES_tstep != 1)
-        throw hip_error("CAGMRES_tstep > 1 is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
+      if (P.prm.CAGMRES_tstep < 1 || P.prm.CAGMRES_sstep * P.prm.CAGMRES_tstep > 128)
+        throw hip_error("CAGMRES_tstep must be >= 1 with s*t <= 128", PARCEL_HIP_ERR_ARG);
       const int q = P.prm.CAGMRES_QRflag;
       if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
         throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);

@@ -464,6 +464,61 @@ inline void leja_order(int n, double* re, double* im) {
   }
 }
 
+/* calc_HessenbergMatrix for block k >= 2 (src/pcagmres.f90:879-1051, with
+ * calc_Rk_Bk_Rkinv :850 and calc_hke1eskRRinv :818).  Rp = R' = Q(:,1:c1)^T V
+ * (c1 x (s+1), ld c1 = s(k-1)+1) from the block CGS, Rk the block's s x s R
+ * factor (ld s), H the (st+1) x st Hessenberg (ld st+1); *hk_pre is read and
+ * updated.  What the reference's local arrays hold: BlockR_ubar =
+ * R'(1:s(k-1), 1:s+1), BlockR = R'(1:s(k-1), 1:s), Rtri0 = [R'(c1, 1:s+1);
+ * 0 Rk].  New columns of H:
+ *   H(1:s(k-1), new) = (-H_prev BlockR + BlockR_ubar B1) Rtri0(1:s,1:s)^-1
+ *   H(new, new)      = Rtri0 B1 Rinv + (bk/rho~) z e_s^T - hk_pre e1 BlockR(end,:) Rinv
+ *   H(sk+1, sk)      = rho / rho~ * bk                                      */
+inline void hessenberg_block(int s, int t, int k, const double* B1, double* H, const double* Rk, const double* Rp,
+                             double* hk_pre) {
+  const int m = s * t, c0 = s * (k - 1), c1 = c0 + 1, s1 = s + 1, ldh = m + 1;
+  std::vector<double> Rt0((size_t)s1 * s1, 0.0), Rinv((size_t)s * s, 0.0), Rub((size_t)c0 * s1, 0.0),
+      BR((size_t)c0 * s, 0.0), BH((size_t)c0 * c0, 0.0), BHR((size_t)c0 * s, 0.0), RubB((size_t)c0 * s, 0.0),
+      Hk((size_t)s * s, 0.0), RB((size_t)s * s, 0.0), B0((size_t)s * s, 0.0), Rs((size_t)s * s, 0.0),
+      E((size_t)s * s, 0.0), H0((size_t)s * s, 0.0);
+  for (int i = 0; i < s1; ++i) Rt0[IDX(0, i, s1)] = Rp[IDX(c0, i, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < s; ++i) Rt0[IDX(1 + i, j + 1, s1)] = Rk[IDX(i, j, s)];
+  for (int j = 0; j < s1; ++j)
+    for (int i = 0; i < c0; ++i) Rub[IDX(i, j, c0)] = Rp[IDX(i, j, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < c0; ++i) BR[IDX(i, j, c0)] = Rp[IDX(i, j, c1)];
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < s; ++i) Rinv[IDX(i, j, s)] = Rt0[IDX(i, j, s1)];
+  trinv_upper(s, Rinv.data(), s);
+  for (int j = 0; j < c0; ++j)
+    for (int i = 0; i < c0; ++i) BH[IDX(i, j, c0)] = H[IDX(i, j, ldh)];
+  gemm_nn(c0, s, c0, BH.data(), c0, BR.data(), c0, BHR.data(), c0);
+  gemm_nn(c0, s, s1, Rub.data(), c0, B1, s1, RubB.data(), c0);
+  for (int i = 0; i < c0 * s; ++i) BHR[i] = -BHR[i] + RubB[i];
+  gemm_nn(c0, s, s, BHR.data(), c0, Rinv.data(), s, RubB.data(), c0);
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < c0; ++j) H[IDX(j, c0 + i, ldh)] = RubB[IDX(j, i, c0)];
+  const double bk = B1[IDX(s, s - 1, s1)];
+  const double rho_tilde = Rt0[IDX(s - 1, s - 1, s1)], rho = Rt0[IDX(s, s, s1)];
+  const double hk = rho / rho_tilde * bk;
+  H[IDX(s * k, s * k - 1, ldh)] = hk;
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < s; ++j) {
+      B0[IDX(j, i, s)] = B1[IDX(j, i, s1)];
+      Rs[IDX(j, i, s)] = Rt0[IDX(j, i, s1)];
+    }
+  gemm_nn(s, s, s, Rs.data(), s, B0.data(), s, RB.data(), s);
+  gemm_nn(s, s, s, RB.data(), s, Rinv.data(), s, Hk.data(), s);
+  for (int i = 0; i < s; ++i) Hk[IDX(i, s - 1, s)] = fma(bk / rho_tilde, Rt0[IDX(i, s, s1)], Hk[IDX(i, s - 1, s)]);
+  for (int i = 0; i < s; ++i) E[IDX(0, i, s)] = BR[IDX(c0 - 1, i, c0)] * *hk_pre;
+  gemm_nn(s, s, s, E.data(), s, Rinv.data(), s, H0.data(), s);
+  for (int i = 0; i < s * s; ++i) Hk[i] = Hk[i] - H0[i];
+  for (int i = 0; i < s; ++i)
+    for (int j = 0; j < s; ++j) H[IDX(c0 + j, c0 + i, ldh)] = Hk[IDX(j, i, s)];
+  *hk_pre = hk;
+}
+
 /* Pseudo-inverse of a symmetric n x n matrix (moore_penrose,
  * src/Moore_Penrose.f90:3-67: A+ = V diag(1/sigma_i, sigma_i > 0) U^T by
  * DGESVD).  For a symmetric A the singular triplets are (|lambda_i|, v_i,

@@ -38,15 +38,15 @@ PRECONS = {
 }
 
 
-def _both(rp, col, val, b, s, basis, qr, kw, itrmax, rtol, x0=None):
+def _both(rp, col, val, b, s, basis, qr, kw, itrmax, rtol, x0=None, t=1):
     n = rp.shape[0] - 1
     x = np.zeros(n) if x0 is None else x0.copy()
-    hist, niter, iret = ph.dcagmres(x, b, rp, col, val, CAGMRES_sstep=s, CAGMRES_basis=basis, CAGMRES_QRflag=qr,
-                                    itrmax=itrmax, rtolmax=rtol, ipreflag=kw["ipreflag"],
+    hist, niter, iret = ph.dcagmres(x, b, rp, col, val, CAGMRES_sstep=s, CAGMRES_tstep=t, CAGMRES_basis=basis,
+                                    CAGMRES_QRflag=qr, itrmax=itrmax, rtolmax=rtol, ipreflag=kw["ipreflag"],
                                     ILU_method=kw.get("ILU_method", 1), precon_thblock=kw.get("precon_thblock", 1))
     xo, ho, no, io = orc.cagmres(rp, col, val, b, x0=x0, ipreflag=kw["ipreflag"],
                                  ilu_method=kw.get("ILU_method", 1), nblocks=kw.get("precon_thblock", 1),
-                                 itrmax=itrmax, rtol=rtol, s=s, basis=basis, qrflag=qr)
+                                 itrmax=itrmax, rtol=rtol, s=s, t=t, basis=basis, qrflag=qr)
     return (x, hist, niter, iret), (xo, ho, no, io)
 
 
@@ -103,8 +103,44 @@ def test_cagmres_converged_at_restart_and_errors():
     (x, h, ni, ir), (xo, ho, no, io) = _both(rp, col, val, b, 4, 1, 4, PRECONS["pj"], 50, 1e-8, x0=xs)
     assert (ni, ir) == (no, io) == (1, 0)
     assert np.array_equal(x, xs)
-    for kw, code in ((dict(CAGMRES_sstep=0), -1), (dict(CAGMRES_sstep=16), -1), (dict(CAGMRES_tstep=2), -4),
-                     (dict(CAGMRES_QRflag=3), -4)):
+    for kw, code in ((dict(CAGMRES_sstep=0), -1), (dict(CAGMRES_sstep=16), -1), (dict(CAGMRES_tstep=0), -1),
+                     (dict(CAGMRES_sstep=8, CAGMRES_tstep=17), -1), (dict(CAGMRES_QRflag=3), -4)):
         with pytest.raises(ph.ParcelError) as e:
             ph.dcagmres(np.zeros(n), b, rp, col, val, itrmax=10, **kw)
         assert e.value.code == code
+
+
+# --- t > 1: block CGS between blocks -------------------------------------------
+
+@pytest.mark.parametrize("qr", [1, 2, 4, 5])
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("s,t", [(3, 2), (4, 5), (1, 6), (8, 3)])
+def test_cagmres_blocks_bit_exact(s, t, basis, qr):
+    say("cagmres blocks", s, t, basis, qr)
+    rp, col, val = PROBLEMS["convdiff_12"]()
+    b = ph.uniform(rp.shape[0] - 1, 46)
+    (x, h, ni, ir), (xo, ho, no, io) = _both(rp, col, val, b, s, basis, qr, PRECONS["ilu0_4"], 250, 1e-9, t=t)
+    assert (ni, ir) == (no, io)
+    assert np.array_equal(h, ho, equal_nan=True) and np.array_equal(x, xo, equal_nan=True)
+
+
+@pytest.mark.parametrize("itrmax", [2, 4, 7, 9, 10, 23])
+@pytest.mark.parametrize("qr", [2, 5])
+def test_cagmres_blocks_exit_points(itrmax, qr):
+    # exits inside a block before the last (orthonormal Q update) and inside
+    # the last block (V columns times R^-1), and at cycle ends
+    say("cagmres blocks itrmax", itrmax, qr)
+    rp, col, val = PROBLEMS["parcel3d_20"]()
+    b = ph.uniform(rp.shape[0] - 1, 47)
+    (x, h, ni, ir), (xo, ho, no, io) = _both(rp, col, val, b, 3, 1, qr, PRECONS["pj"], itrmax, 1e-30, t=3)
+    assert (ni, ir) == (no, io)
+    assert np.array_equal(h, ho) and np.array_equal(x, xo)
+
+
+def test_cagmres_blocks_large():
+    say("cagmres blocks poisson2d_760")
+    rp, col, val = PROBLEMS["poisson2d_760"]()
+    b = ph.uniform(rp.shape[0] - 1, 48)
+    (x, h, ni, ir), (xo, ho, no, io) = _both(rp, col, val, b, 4, 1, 5, PRECONS["ilu0_4"], 50, 1e-12, t=4)
+    assert (ni, ir) == (no, io)
+    assert np.array_equal(h, ho) and np.array_equal(x, xo)
