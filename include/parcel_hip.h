@@ -163,11 +163,10 @@ int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn
                       int* iret);
 
 /* parcel_dcagmres (src/wrapper.f90:1355-1454 GPU build, :2807-2906 CPU build):
- * parcel_dcg's arguments plus CAGMRES_sstep (s, 1..15), CAGMRES_tstep (t;
- * t = 1 only), CAGMRES_basis (0 monomial, 1 Newton with Leja-ordered Ritz
- * shifts) and CAGMRES_QRflag (PARCEL_QR_MGS, _CGS, _CHOLESKYQR,
- * _CHOLESKYQR2; TSQR is not available).  CA-GMRES of
- * src/pcagmres.f90:1557-2517. */
+ * parcel_dcg's arguments plus CAGMRES_sstep (s, 1..15), CAGMRES_tstep (t >= 1,
+ * s*t <= 128), CAGMRES_basis (0 monomial, 1 Newton with Leja-ordered Ritz
+ * shifts) and CAGMRES_QRflag (PARCEL_QR_MGS, _CGS, _TSQR, _CHOLESKYQR,
+ * _CHOLESKYQR2).  CA-GMRES of src/pcagmres.f90:1557-2517. */
 int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int gn,
                         int nnz, int istart, const double* crsA, const int* crsRow_ptr,
                         const int* crsCol, int ipreflag, int ILU_method, int addL,
@@ -191,6 +190,12 @@ int parcel_hip_dcbcg(int icomm, double* vecx, const double* vecb, int n, int gn,
                      int Neuman_itr, int independ_nvec, int nBlock, int CBCG_kstep,
                      int CBCG_Eigenflag, int power_method_itrmax, int CAARNOLDI_sstep,
                      int CAARNOLDI_tstep, int CAARNOLDI_basis, int CAARNOLDI_QRflag, int* iret);
+
+/* The R factor of TSQR_Ronly_ (src/QRDecomposition.f90:887-1002, the
+ * CAGMRES_QRflag = 3 path of QRDecomposition_) for the n x nc column-major
+ * matrix V (ld ldv >= n; host or device memory; nc <= 16): nc x nc upper
+ * triangular with a non-negative diagonal, written to host memory R. */
+int parcel_hip_tsqr_r(int n, int nc, const double* V, int ldv, double* R);
 
 /* Mixed-precision PCG (no counterpart in the reference; its FP64 PCG is
  * KSP_pCG, src/pcg.f90:224-892): parcel_dcg's arguments plus the inner

@@ -585,8 +585,12 @@ static void qr_cols(const oracle_red* red, int n, int nc, double* V, size_t ldv,
     }
     return;
   }
-  cholqr_r(red, n, nc, V, ldv, R);
-  if (qrflag == 5) {                               /* CholeskyQR2_Ronly_ */
+  if (qrflag == 3 && n > nc && n > 64) {           /* TSQR_Ronly_ (tsqr.c) */
+    (void)oracle_tsqr_r(n, nc, V, (int)ldv, R);
+  } else {
+    cholqr_r(red, n, nc, V, ldv, R);
+  }
+  if (qrflag == 5 || (qrflag == 3 && !(n > nc && n > 64))) {   /* CholeskyQR2_Ronly_ */
     memcpy(Rinv, R, sizeof(double) * (size_t)nc * nc);
     oracle_trinv_upper(nc, Rinv, nc);
     q_vrinv(n, nc, V, ldv, Rinv, W, (size_t)n);   /* Q1 = V R^-1 */
@@ -668,7 +672,7 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
                    int ipreflag, int ilu_method, int nblocks, int independ_nvec, int itrmax, double rtol, int s,
                    int t, int basis, int qrflag, const oracle_red* red, double* hist, int* niter, int* iret) {
   if (s < 1 || t < 1) return -4;
-  if (qrflag != 1 && qrflag != 2 && qrflag != 4 && qrflag != 5) return -4;
+  if (qrflag < 1 || qrflag > 5) return -4;
   int err = 0;
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
   if (!M) return err;

@@ -70,6 +70,9 @@ def load():
         "oracle_hqr_eigs": (_I, [_I, _P, _I, _P, _P]),
         "oracle_leja": (None, [_I, _P, _P]),
         "oracle_sym_pinv": (None, [_I, _P, _I, _P, _I]),
+        "oracle_tsqr_r": (_I, [_I, _I, _P, _I, _P]),
+        "oracle_house_r": (None, [_I, _I, _P, _I, _P]),
+        "oracle_tsqr_waves": (_I, [_I]),
         "oracle_power_emax": (_D, [_P, _I, _P, _P, _P, _P, _I, _RP]),
         "oracle_cbcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _RP, _P, _IP, _IP,
                              ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -223,6 +226,28 @@ def power_emax(rowptr, col, val, x0, ipreflag=0, ilu_method=1, nblocks=1, indepe
     pre = Precon(rowptr, col, val, ipreflag, ilu_method, nblocks, independ_nvec)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
     return load().oracle_power_emax(pre.h, n, _p(rowptr), _p(col), _p(val), _p(x0), itrmax, ctypes.byref(red))
+
+
+def tsqr_r(V):
+    """TSQR R factor (non-negative diagonal) of the n x nc matrix V."""
+    V = np.asarray(V, dtype=np.float64)
+    n, nc = V.shape
+    vt = np.ascontiguousarray(V.T)  # column-major n x nc
+    rt = np.zeros((nc, nc))
+    rc = load().oracle_tsqr_r(n, nc, _p(vt), n, _p(rt))
+    if rc != 0:
+        raise RuntimeError("oracle_tsqr_r failed")
+    return rt.T.copy()
+
+
+def house_r(A):
+    """House_local_Ronly_omp R factor of the m x s matrix A (diagonal sign as produced)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, s = A.shape
+    at = np.ascontiguousarray(A.T)
+    rt = np.zeros((s, s))
+    load().oracle_house_r(m, s, _p(at), m, _p(rt))
+    return rt.T.copy()
 
 
 def sym_pinv(a):

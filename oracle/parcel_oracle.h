@@ -108,7 +108,8 @@ int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
 
 /* KSP_pCAGMRES (src/pcagmres.f90:1557-2517), CA-GMRES(s,t) with right
  * preconditioning; t = 1.  basis: 0 monomial, 1 Newton with Leja-ordered
- * Ritz shifts.  qrflag: 1 MGS, 2 CGS, 4 CholeskyQR, 5 CholeskyQR2.  hist must
+ * Ritz shifts.  qrflag: 1 MGS, 2 CGS, 3 TSQR (CholeskyQR2 when n <= max(s+1, 64)),
+ * 4 CholeskyQR, 5 CholeskyQR2.  hist must
  * hold itrmax + s entries (the reference writes reshistory(itr+i) up to
  * itrmax + s when a cycle starts at itr = itrmax).  Returns -4 for t != 1
  * or another qrflag. */
@@ -133,6 +134,13 @@ int oracle_cbcg_with(const oracle_precon* M, int n, const int* rowptr, const int
                      const oracle_red* red, double* hist, int* niter, int* iret);
 double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
                          const double* x0, int itrmax, const oracle_red* red);
+/* TSQR R factor (tsqr.c): R (nc x nc, ld nc, nc <= 16) of V (n x nc, ld ldv)
+ * with a non-negative diagonal; oracle_house_r is the second level
+ * (House_local_Ronly_omp) on its own; oracle_tsqr_waves the number of
+ * first-level factors. */
+int oracle_tsqr_r(int n, int nc, const double* V, int ldv, double* R);
+void oracle_house_r(int m, int s, double* A, int lda, double* R);
+int oracle_tsqr_waves(int n);
 /* pseudo-inverse of a symmetric n x n matrix (upper triangle of A read) */
 void oracle_sym_pinv(int n, const double* A, int lda, double* P, int ldp);
 

@@ -2,7 +2,7 @@
 // with right preconditioning, monomial or Newton basis (calc_NewtonBasis /
 // calc_NewtonBasisVec, :586-611,1480-1552) and the QR variants of
 // QRDecomposition_ (src/QRDecomposition.f90:1331-1551): MGS / CGS (GS_QR_),
-// CholeskyQR (CholeskyQR_Ronly_) and CholeskyQR2, Q = V R^-1 where a later
+// TSQR (TSQR_Ronly_), CholeskyQR (CholeskyQR_Ronly_) and CholeskyQR2, Q = V R^-1 where a later
 // block needs it (calcQ_VRinv_, :603-671).
 //
 // Device work per block of a restart cycle: s preconditioner applies and
@@ -339,6 +339,81 @@ __global__ void __launch_bounds__(kNT) k_ca_vrinv(int n, const double* V, long l
   }
 }
 
+// ---- TSQR, first level (TSQR_local_Ronly_, TSQR.f90:114-261) -------------
+// 64-row panels; panel p belongs to wavefront p mod W.  A wavefront keeps an
+// NC x NC upper-triangular R (lane l holds row l) and folds each panel P in
+// by Householder QR of [R; P], House_local_Ronly_omp's reflector
+// v = [y1; P(:,i)] on the stacked matrix: the 64-lane sums are xor
+// butterflies, the R entries are broadcast from their row's lane.  The W
+// factors go to Rout (NC x NC each, column-major) for the host's second level.
+constexpr int kTsqrWaves = 1024;
+inline int tsqr_waves(int n) {
+  const int np = (n + 63) / 64;
+  return np < kTsqrWaves ? (np > 0 ? np : 1) : kTsqrWaves;
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kNT) k_tsqr(int n, const double* V, long ldv, int W, double* Rout) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (w >= W) return;  // whole wavefronts only; no workgroup barrier below
+  const int np = (n + 63) / 64;
+  double Rrow[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) Rrow[c] = 0.0;
+  for (int p = w; p < np; p += W) {
+    const long row = (long)p * 64 + lane;
+    double P[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) P[c] = row < n ? V[(long)c * ldv + row] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const double x = P[i];
+      double d[NC];
+#pragma unroll
+      for (int k = i; k < NC; ++k) d[k] = x * P[k];
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+        for (int k = i; k < NC; ++k) d[k] += __shfl_xor(d[k], o, 64);
+      const double alpha = __shfl(Rrow[i], i, 64);
+      const double s2 = d[i];
+      const double nrm = sqrt(fma(alpha, alpha, s2));
+      const double y1 = alpha > 0.0 ? alpha + nrm : alpha - nrm;
+      double den = fma(y1, y1, s2);
+      if (den == 0.0) den = 1.0 - 300;
+      const double t1 = 2.0 / den;
+      const double na = fma(-(t1 * y1), fma(y1, alpha, s2), alpha);
+#pragma unroll
+      for (int k = i + 1; k < NC; ++k) {
+        const double rik = __shfl(Rrow[k], i, 64);
+        const double tk = t1 * fma(y1, rik, d[k]);
+        P[k] = fma(-x, tk, P[k]);
+        const double nr = fma(-y1, tk, rik);
+        if (lane == i) Rrow[k] = nr;
+      }
+      if (lane == i) Rrow[i] = na;
+    }
+  }
+  if (lane < NC) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) Rout[(long)w * NC * NC + lane + c * NC] = (c >= lane) ? Rrow[c] : 0.0;
+  }
+}
+
+void tsqr_local(int nc, int n, const double* V, long ldv, int W, double* Rout, hipStream_t st) {
+  const dim3 g((W + kWaves - 1) / kWaves), b(kNT);
+  switch (nc) {
+#define PARCEL_TS(C) \
+  case C: hipLaunchKernelGGL(k_tsqr<C>, g, b, 0, st, n, V, ldv, W, Rout); break;
+    PARCEL_TS(1) PARCEL_TS(2) PARCEL_TS(3) PARCEL_TS(4) PARCEL_TS(5) PARCEL_TS(6) PARCEL_TS(7) PARCEL_TS(8)
+    PARCEL_TS(9) PARCEL_TS(10) PARCEL_TS(11) PARCEL_TS(12) PARCEL_TS(13) PARCEL_TS(14) PARCEL_TS(15)
+    PARCEL_TS(16)
+#undef PARCEL_TS
+    default: throw hip_error("TSQR: column count must be in 1..16", PARCEL_HIP_ERR_ARG);
+  }
+}
+
 // ---- solution update ---------------------------------------------------
 // v0 = sum_{k < nq} fma(y_k, Q_k, .) then sum_{k < nv} fma(y_{nq+k}, W_k, .),
 // from 0 (sum_vec_cagmres, cu_pcagmres.cu:9-60; nv = 0: sum_vec)
@@ -446,6 +521,26 @@ void givens(int m, double* H, int i, double* c, double* s, double* e) {
 
 }  // namespace
 
+// TSQR_Ronly_ (QRDecomposition.f90:887-1002): the first level on the device
+// into dT (tsqr_waves(n) factors), House_local_Ronly_omp over the stacked
+// factors on the host (hT: pinned, same size), TSQR_GLOBAL (the identity on
+// one rank), then rows with a negative diagonal negated.  R: nc x nc, ld nc.
+void tsqr_r(int n, int nc, const double* V, long ldv, double* dT, double* hT, double* R, hipStream_t st) {
+  const int W = tsqr_waves(n);
+  tsqr_local(nc, n, V, ldv, W, dT, st);
+  PARCEL_HIP_CHECK(hipMemcpyAsync(hT, dT, sizeof(double) * (size_t)W * nc * nc, hipMemcpyDeviceToHost, st));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+  const int mrows = W * nc;
+  std::vector<double> S((size_t)mrows * nc);
+  for (int w = 0; w < W; ++w)
+    for (int j = 0; j < nc; ++j)
+      for (int i = 0; i < nc; ++i) S[(size_t)w * nc + i + (size_t)j * mrows] = hT[(size_t)w * nc * nc + i + (size_t)j * nc];
+  dense::house_r(mrows, nc, S.data(), mrows, R);
+  for (int j = 0; j < nc; ++j)
+    if (R[j + (size_t)j * nc] < 0.0)
+      for (int i = j; i < nc; ++i) R[j + (size_t)i * nc] = -R[j + (size_t)i * nc];
+}
+
 int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, double* hist_host, int* niter,
                   int* iret) {
   const int n = P.A.n;
@@ -454,9 +549,8 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
   const int basis = P.prm.CAGMRES_basis, qrflag = P.prm.CAGMRES_QRflag;
   if (s < 1 || s + 1 > kMaxCols) throw hip_error("CAGMRES_sstep must be in 1..15", PARCEL_HIP_ERR_ARG);
   if (t < 1 || s * t > kMaxM) throw hip_error("CAGMRES_tstep must be >= 1 with s*t <= 128", PARCEL_HIP_ERR_ARG);
-  if (qrflag != PARCEL_QR_MGS && qrflag != PARCEL_QR_CGS && qrflag != PARCEL_QR_CHOLESKYQR &&
-      qrflag != PARCEL_QR_CHOLESKYQR2)
-    throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
+  if (qrflag < PARCEL_QR_MGS || qrflag > PARCEL_QR_CHOLESKYQR2)
+    throw hip_error("CAGMRES_QRflag must be in 1..5", PARCEL_HIP_ERR_ARG);
   const bool gs = qrflag == PARCEL_QR_MGS || qrflag == PARCEL_QR_CGS;
   const int m = s * t, nc = s + 1;
   const int c1max = s * (t - 1) + 1;
@@ -477,19 +571,23 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
   double* gp = P.wpart.p;
   // device scalars: st[0..3], G / R (nc x nc), Rinv (nc x nc), R' (c1max x nc), y (m + 1)
   const size_t nrp = t > 1 ? (size_t)c1max * nc : 0;
-  const size_t dcount = 4 + 2 * (size_t)nc * nc + nrp + (size_t)m + 1;
+  const int tsw = tsqr_waves(n);
+  const size_t ntsqr = qrflag == PARCEL_QR_TSQR ? (size_t)tsw * nc * nc : 0;
+  const size_t dcount = 4 + 2 * (size_t)nc * nc + nrp + (size_t)m + 1 + ntsqr;
   if (P.solver_state.n < sizeof(double) * dcount) P.solver_state.alloc(sizeof(double) * dcount);
   double* dst = reinterpret_cast<double*>(P.solver_state.p);
   double* dG = dst + 4;
   double* dRinv = dG + (size_t)nc * nc;
   double* dRp = dRinv + (size_t)nc * nc;
   double* dY = dRp + nrp;
+  double* dT = dY + m + 1;  // first-level TSQR factors
   double* hbuf = nullptr;  // pinned mirror of the device scalars
   PARCEL_HIP_CHECK(hipHostMalloc((void**)&hbuf, sizeof(double) * dcount, hipHostMallocDefault));
   double* hG = hbuf + 4;
   double* hRinv = hG + (size_t)nc * nc;
   double* hRp = hRinv + (size_t)nc * nc;
   double* hY = hRp + nrp;
+  double* hT = hY + m + 1;
   const int nb = tiled_nb(n);
   const dim3 grid(nb), block(kNT);
   const SellArgs A = sell_args(P.A);
@@ -583,12 +681,21 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
       for (size_t q = 0; q < (size_t)cnt * cnt; ++q) Rout[q] = hG[q];
       return;
     }
-    cholqr(Vb, cnt, nullptr, Rout);
-    if (qrflag == PARCEL_QR_CHOLESKYQR2) {
-      // Q1 = V R^-1 (never stored), R2 = chol(Q1^T Q1), R = R2 R
-      upload_rinv(Rout, cnt);
-      cholqr(Vb, cnt, dRinv, R2);
-      dense::trmm_lunn(cnt, R2.data(), cnt, Rout.data(), cnt);
+    if (qrflag == PARCEL_QR_TSQR && n > cnt && n > 64) {
+      // TSQR_Ronly_ (QRDecomposition.f90:887-1002): first level on the
+      // device, House_local_Ronly_omp over the stacked factors on the host,
+      // TSQR_GLOBAL (the identity on one rank), then rows made non-negative
+      tsqr_r(n, cnt, Vb, ldv, dT, hT, Rout.data(), st);
+      ++launches;
+      fetch(4);
+    } else {
+      cholqr(Vb, cnt, nullptr, Rout);
+      if (qrflag == PARCEL_QR_CHOLESKYQR2 || qrflag == PARCEL_QR_TSQR) {
+        // Q1 = V R^-1 (never stored), R2 = chol(Q1^T Q1), R = R2 R
+        upload_rinv(Rout, cnt);
+        cholqr(Vb, cnt, dRinv, R2);
+        dense::trmm_lunn(cnt, R2.data(), cnt, Rout.data(), cnt);
+      }
     }
     if (qm) {  // calcQ_VRinv_
       upload_rinv(Rout, cnt);
@@ -762,3 +869,38 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
 }
 
 }  // namespace parcel
+
+extern "C" int parcel_hip_tsqr_r(int n, int nc, const double* V, int ldv, double* R) {
+  using namespace parcel;
+  if (n < 1 || nc < 1 || nc > 16 || ldv < n || !V || !R) return api_fail(PARCEL_HIP_ERR_ARG, "tsqr_r: bad arguments");
+  try {
+    if (int rc = api_ensure_init()) return rc;
+    hipStream_t st = nullptr;
+    const bool dev = api_is_device_ptr(V);
+    DevBuf<double> dv;
+    const double* Vd = V;
+    if (!dev) {
+      dv.alloc((size_t)ldv * nc);
+      PARCEL_HIP_CHECK(hipMemcpy(dv.p, V, sizeof(double) * (size_t)ldv * nc, hipMemcpyHostToDevice));
+      Vd = dv.p;
+    }
+    const int W = tsqr_waves(n);
+    DevBuf<double> dT((size_t)W * nc * nc);
+    double* hT = nullptr;
+    PARCEL_HIP_CHECK(hipHostMalloc((void**)&hT, sizeof(double) * (size_t)W * nc * nc, hipHostMallocDefault));
+    std::vector<double> Rh((size_t)nc * nc);
+    try {
+      tsqr_r(n, nc, Vd, ldv, dT.p, hT, Rh.data(), st);
+    } catch (...) {
+      (void)hipHostFree(hT);
+      throw;
+    }
+    (void)hipHostFree(hT);
+    std::memcpy(R, Rh.data(), sizeof(double) * (size_t)nc * nc);
+    return PARCEL_HIP_OK;
+  } catch (const hip_error& e) {
+    return api_fail(e.code < 0 ? e.code : PARCEL_HIP_ERR_HIP, e.what());
+  } catch (const std::exception& e) {
+    return api_fail(PARCEL_HIP_ERR_HIP, e.what());
+  }
+}

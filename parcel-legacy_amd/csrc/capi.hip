@@ -204,8 +204,8 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
       if (P.prm.CAGMRES_tstep < 1 || P.prm.CAGMRES_sstep * P.prm.CAGMRES_tstep > 128)
         throw hip_error("CAGMRES_tstep must be >= 1 with s*t <= 128", PARCEL_HIP_ERR_ARG);
       const int q = P.prm.CAGMRES_QRflag;
-      if (q != PARCEL_QR_MGS && q != PARCEL_QR_CGS && q != PARCEL_QR_CHOLESKYQR && q != PARCEL_QR_CHOLESKYQR2)
-        throw hip_error("CAGMRES_QRflag: TSQR is not supported", PARCEL_HIP_ERR_UNSUPPORTED);
+      if (q < PARCEL_QR_MGS || q > PARCEL_QR_CHOLESKYQR2)
+        throw hip_error("CAGMRES_QRflag must be in 1..5", PARCEL_HIP_ERR_ARG);
     }
     if (P.prm.solver == PARCEL_SOLVER_CBCG && (P.prm.CBCG_kstep < 1 || P.prm.CBCG_kstep > 8))
       throw hip_error("CBCG_kstep must be in 1..8", PARCEL_HIP_ERR_ARG);

@@ -464,6 +464,35 @@ inline void leja_order(int n, double* re, double* im) {
   }
 }
 
+/* House_local_Ronly_omp (src/TSQR.f90:836-927) on the m x s matrix A (ld
+ * lda, overwritten): reflector v = [y1; A(i+1:m, i)], y1 = a_ii + sign(a_ii)
+ * ||A(i:m, i)||, t1 = 2 / v^T v (1.0 - 300 for a zero v^T v, the reference's
+ * guard), sequential fma sums in row order; R (s x s, ld s) gets the upper
+ * triangle (diagonal -sign(a_ii) ||.||). */
+inline void house_r(int m, int s, double* A, int lda, double* R) {
+  for (int i = 0; i < s; ++i) {
+    double tmp2 = 0.0;
+    for (int j = i + 1; j < m; ++j) tmp2 = fma(A[IDX(j, i, lda)], A[IDX(j, i, lda)], tmp2);
+    const double aii = A[IDX(i, i, lda)];
+    const double tmp3 = sqrt(fma(aii, aii, tmp2));
+    const double y1 = aii > 0.0 ? aii + tmp3 : aii - tmp3;
+    const double tmp = fma(y1, aii, tmp2);
+    double den = fma(y1, y1, tmp2);
+    if (den == 0.0) den = 1.0 - 300;
+    const double t1 = 2.0 / den;
+    A[IDX(i, i, lda)] = fma(-(t1 * y1), tmp, aii);
+    for (int k = i + 1; k < s; ++k) {
+      double d = y1 * A[IDX(i, k, lda)];
+      for (int j = i + 1; j < m; ++j) d = fma(A[IDX(j, i, lda)], A[IDX(j, k, lda)], d);
+      d = t1 * d;
+      for (int j = i + 1; j < m; ++j) A[IDX(j, k, lda)] = fma(-A[IDX(j, i, lda)], d, A[IDX(j, k, lda)]);
+      A[IDX(i, k, lda)] = fma(-y1, d, A[IDX(i, k, lda)]);
+    }
+  }
+  for (int j = 0; j < s; ++j)
+    for (int i = 0; i < s; ++i) R[IDX(i, j, s)] = (i <= j) ? A[IDX(i, j, lda)] : 0.0;
+}
+
 /* calc_HessenbergMatrix for block k >= 2 (src/pcagmres.f90:879-1051, with
  * calc_Rk_Bk_Rkinv :850 and calc_hke1eskRRinv :818).  Rp = R' = Q(:,1:c1)^T V
  * (c1 x (s+1), ld c1 = s(k-1)+1) from the block CGS, Rk the block's s x s R

@@ -87,6 +87,7 @@ SYMBOLS = {
                                _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcagmres": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                                  _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_tsqr_r": (_I, [_I, _I, _P, _I, _P]),
     "parcel_hip_dcbcg": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
                               _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcg_mixed": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D, _P,
@@ -469,6 +470,17 @@ def dcagmres(x, b, rowptr, col, val, CAGMRES_sstep=4, CAGMRES_tstep=1, CAGMRES_b
                                    iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep,
                                    CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, ctypes.byref(iret)))
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
+
+
+def tsqr_r(V):
+    """R factor (non-negative diagonal) of the n x nc matrix V by the library's TSQR."""
+    lib = load()
+    V = np.asarray(V, dtype=np.float64)
+    n, nc = V.shape
+    vt = np.ascontiguousarray(V.T)  # column-major n x nc
+    rt = np.zeros((nc, nc))
+    _check(lib.parcel_hip_tsqr_r(n, nc, _ptr(vt), n, _ptr(rt)))
+    return rt.T.copy()
 
 
 def dcbcg(x, b, rowptr, col, val, CBCG_kstep=4, CBCG_Eigenflag=2, power_method_itrmax=100, CAARNOLDI_sstep=4,

@@ -104,7 +104,7 @@ def test_cagmres_converged_at_restart_and_errors():
     assert (ni, ir) == (no, io) == (1, 0)
     assert np.array_equal(x, xs)
     for kw, code in ((dict(CAGMRES_sstep=0), -1), (dict(CAGMRES_sstep=16), -1), (dict(CAGMRES_tstep=0), -1),
-                     (dict(CAGMRES_sstep=8, CAGMRES_tstep=17), -1), (dict(CAGMRES_QRflag=3), -4)):
+                     (dict(CAGMRES_sstep=8, CAGMRES_tstep=17), -1), (dict(CAGMRES_QRflag=6), -1)):
         with pytest.raises(ph.ParcelError) as e:
             ph.dcagmres(np.zeros(n), b, rp, col, val, itrmax=10, **kw)
         assert e.value.code == code

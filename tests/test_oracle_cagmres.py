@@ -356,6 +356,6 @@ def test_cagmres_blocks_converge():
 def test_cagmres_unsupported_options():
     rp, col, val, A = convdiff3d(3)
     b = np.ones(A.shape[0])
-    for kw in (dict(qrflag=3),):
+    for kw in (dict(qrflag=6), dict(s=0)):
         with pytest.raises(RuntimeError):
             orc.cagmres(rp, col, val, b, itrmax=5, **kw)
