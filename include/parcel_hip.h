@@ -177,8 +177,10 @@ int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int 
 
 /* parcel_dcbcg (src/wrapper.f90:1139-1233 CPU build; the GPU build's
  * counterpart has the same arguments): parcel_dcg's arguments plus
- * CBCG_kstep (k, 1..8), CBCG_Eigenflag (1 power method, 2 CA-Arnoldi; any
- * other value uses Emax = 1, Emin = 0), power_method_itrmax and the
+ * CBCG_kstep (k, 1..8), CBCG_Eigenflag (1 power method; 2 CA-Arnoldi, which
+ * the reference's GPU build runs as the power method, src/pcagmres.f90:
+ * 2647-2651, and so does this library; any other value uses Emax = 1,
+ * Emin = 0), power_method_itrmax and the
  * CA-Arnoldi parameters CAARNOLDI_sstep/_tstep/_basis/_QRflag.  Chebyshev-
  * basis CG of src/pcbcg.f90:1182-2025: reshistory(j) = ||r|| / ||b|| at the
  * start of outer step j (k iterations each); niter = min(itrmax, j) of the

@@ -237,12 +237,9 @@ int oracle_cbcg(int n, const int* rowptr, const int* col, const double* val, con
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
   if (!M) return err ? err : -1;
   double Emax = 0.0, Emin = 0.0;
-  if (eigflag == 1) {
-    Emax = oracle_power_emax(M, n, rowptr, col, val, x, pm_itrmax, red);
-  } else if (eigflag == 2) {
-    oracle_precon_free(M);
-    return -4;
-  }
+  /* flag 2 (CA-Arnoldi): the reference's GPU build runs the power method
+   * there (KSP_pCAARNOLDI, src/pcagmres.f90:2647-2651) */
+  if (eigflag == 1 || eigflag == 2) Emax = oracle_power_emax(M, n, rowptr, col, val, x, pm_itrmax, red);
   if (emax_out) *emax_out = Emax;
   if (emin_out) *emin_out = Emin;
   int rc = oracle_cbcg_with(M, n, rowptr, col, val, b, x, itrmax, rtol, k, Emax, Emin, red, hist, niter, iret);

@@ -120,8 +120,9 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val,
                    int* niter, int* iret);
 
 /* KSP_pCBCG (src/pcbcg.f90:1182-2025), k-step Chebyshev-basis CG, k in 1..8.
- * eigflag 1: Emax from the power method (pm_itrmax steps from x0), Emin = 0;
- * 2: CA-Arnoldi (returns -4); other: Emax = 1, Emin = 0.  hist must hold
+ * eigflag 1 and 2: Emax from the power method (pm_itrmax steps from x0),
+ * Emin = 0 (2 is CA-Arnoldi, whose GPU-build body is the power method);
+ * other: Emax = 1, Emin = 0.  hist must hold
  * itrmax / k + 1 entries; hist[j-1] = ||r|| / ||b|| at the start of outer
  * step j; niter = min(itrmax, j_last) (entries never written are 0). */
 int oracle_cbcg(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,

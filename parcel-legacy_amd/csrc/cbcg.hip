@@ -1,7 +1,8 @@
 // cbcg.hip -- Chebyshev-basis CG, KSP_pCBCG (src/pcbcg.f90:1182-2025), with
 // the eigenvalue bounds of getEigenValue (src/pcbcg.f90:179-196): the power
-// method (power_method_symMatrix, src/power_method.f90:188-437) or CA-Arnoldi
-// (KSP_pCAARNOLDI, src/pcagmres.f90:3199-3556, in caarnoldi.hip).
+// method (power_method_symMatrix, src/power_method.f90:188-437) for
+// CBCG_Eigenflag 1 and, as in the reference's GPU build, for 2 (CA-Arnoldi,
+// whose GPU-build body is the power method, src/pcagmres.f90:2647-2651).
 //
 // One outer step of k-step CBCG:
 //   r   -= AQ alpha                                       (calc_gemv)
@@ -333,11 +334,10 @@ int cbcg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, dou
   // eigenvalue bounds (getEigenValue, src/pcbcg.f90:179-196, and the guards
   // of KSP_pCBCG, :1500-1512)
   double Emax = 0.0, Emin = 0.0;
-  if (P.prm.CBCG_Eigenflag == 1) {
+  // flag 2 (CA-Arnoldi) is the power method in the reference's GPU build:
+  // KSP_pCAARNOLDI compiles to power_method_symMatrix (pcagmres.f90:2647-2651)
+  if (P.prm.CBCG_Eigenflag == 1 || P.prm.CBCG_Eigenflag == 2)
     Emax = power_method_emax(P, x, P.prm.power_method_itrmax, &launches);
-  } else if (P.prm.CBCG_Eigenflag == 2) {
-    throw hip_error("CBCG_Eigenflag 2 (CA-Arnoldi bounds) is not supported yet", PARCEL_HIP_ERR_UNSUPPORTED);
-  }
   if (Emax == Emin) {
     if (Emax < 0.0) Emax = 0.0;
     else Emin = 0.0;

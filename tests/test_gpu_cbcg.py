@@ -80,6 +80,9 @@ def test_cbcg_large_and_fixed_bound():
     say("cbcg large", n)
     g, o = _both(rp, col, val, b, 4, 1, PRECONS["ilu0_4"], itrmax=80, rtol=1e-10, pm=10)
     _same(g, o)
+    # flag 2 (CA-Arnoldi, the power method in the GPU build), the default
+    g, o = _both(rp, col, val, b, 4, 2, PRECONS["ilud_8"], itrmax=80, rtol=1e-10, pm=10)
+    _same(g, o)
     # an unknown eigenvalue flag uses Emax = 1, Emin = 0 (a valid Krylov basis)
     g, o = _both(rp, col, val, b, 3, 0, PRECONS["pj"], itrmax=60, rtol=1e-10)
     _same(g, o)

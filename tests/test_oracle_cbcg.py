@@ -145,3 +145,14 @@ def test_cbcg_exit_semantics():
     x, h, nit, ret, _ = orc.cbcg(rp, col, val, b, x0=xe, ipreflag=1, itrmax=40, rtol=1e-6, k=4, eigflag=1,
                                  pm_itrmax=3)
     assert nit == 1 and ret == 0 and h[0] < 1e-6
+
+
+def test_cbcg_caarnoldi_flag_is_the_power_method():
+    # KSP_pCAARNOLDI's GPU-build body is power_method_symMatrix
+    # (src/pcagmres.f90:2647-2651): flag 2 gives flag 1's bound and history
+    rp, col, val = ph.poisson3d_spd(8, 7, 6)
+    b = np.random.default_rng(9).standard_normal(rp.shape[0] - 1)
+    r1 = orc.cbcg(rp, col, val, b, ipreflag=1, itrmax=200, rtol=1e-9, k=4, eigflag=1, pm_itrmax=20)
+    r2 = orc.cbcg(rp, col, val, b, ipreflag=1, itrmax=200, rtol=1e-9, k=4, eigflag=2, pm_itrmax=20)
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1]) and r1[2:] == r2[2:]
+
