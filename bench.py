@@ -251,6 +251,13 @@ def k1_bytes_batch(n, nnz, nsys, t=8):
     return nsys * (t * nnz + 6 * t * n) + 4 * nnz + 4 * (n + 1)
 
 
+def k1_bytes_stencil(n, nsys, t=8):
+    """Batched fused CG kernel on the five-point stencil form: per system the
+    three coefficient arrays (c, e, n; the w and s couplings are the e and n
+    of the neighbouring rows) and the six vector passes of k1_bytes."""
+    return nsys * (3 * t * n + 6 * t * n)
+
+
 def run_c5(args):
     """Learned-factor mixed PCG on the 64 random-coefficient test systems
     against plain FP64 block-Jacobi ILU(0) PCG on the same GPU."""
@@ -307,7 +314,8 @@ def run_c5(args):
              "niter_max": int(niter.max()), "niter_mean": float(niter.mean()), "iret_max": int(iret.max()),
              "true_rel_res_max": res, "device_ms": st["solve_device_ms"],
              "apply_ms": pst["apply_ms"] / max(pst["apply_calls"], 1), "apply_calls": pst["apply_calls"],
-             "spmv_ms": pst["spmv_ms"] / max(pst["spmv_calls"], 1), "spmv_calls": pst["spmv_calls"]}
+             "spmv_ms": pst["spmv_ms"] / max(pst["spmv_calls"], 1), "spmv_calls": pst["spmv_calls"],
+             "stencil5": int(pst["stencil5"])}
         if precision == ph.BATCH_REFINE32:
             m["fp32_batched_iterations"] = st["inner_steps"]
             m["fp32_iterations_per_system_mean"] = st["inner_total"] / nsys
@@ -401,7 +409,11 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
         kname, ms, by = ("k_lscan<float,double,8,FULL> (FP32 factor apply on FP64 vectors)", learned["apply_ms"],
                          lscan_bytes(n, nsys, 4, 8))
     else:
-        kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
+        if learned.get("stencil5"):
+            kname, ms, by = ("k_cg_spmv<double,4,St5> (batched, five-point stencil)", learned["spmv_ms"],
+                             k1_bytes_stencil(n, nsys, 8))
+        else:
+            kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
     ach = by / (ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "batched_pcg_c5_grf512_traffic.json")

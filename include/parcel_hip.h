@@ -335,6 +335,7 @@ typedef struct {
   double spmv_ms;          /* fused SpMV kernel time (profile on) */
   int spmv_calls;
   int nx, ny;              /* detected five-point grid (0 if none) */
+  int stencil5;            /* 1: every system exactly symmetric, the CG SpMV reads c/e/n arrays */
 } parcel_hip_batch_stats;
 
 /* nsys systems of order n; rowptr/col: 1-based CSR (host), shared; vals:

@@ -28,6 +28,8 @@ struct Batch {
   std::vector<int> rowptr, col;  // 0-based host pattern
   DevBuf<int> diag_slot;         // SELL slot of each row's diagonal
   DevBuf<int4> nbr_slot;         // five-point grids: SELL slots of the s, w, e, n entries (-1: none)
+  DevBuf<double> st5;     // five-point stencil c, e, n (3 * nsys * n) when the pattern is one and
+  bool st5_ok = false;    // every system is exactly symmetric (the fused CG SpMV then reads these)
   DevBuf<double> dinv;           // nsys*n
   DevBuf<float> dinv32;
   DevBuf<double> L64;            // 3*nsys*n: Ls, Lw, Ld

@@ -71,6 +71,27 @@ __global__ void k_b_gather(long entries, long nnz, const int* __restrict__ idx, 
   }
 }
 
+// Five-point stencil arrays c = a(i,i), e = a(i,i+1), n = a(i,i+nx) (0 where
+// the entry is absent) of every system, and bad = 1 when a system is not
+// exactly symmetric (a(i,i-1) != a(i-1,i) or a(i,i-nx) != a(i-nx,i), bitwise).
+__global__ void k_b_st5(int n, int nx, int ny, long entries, long nv, const int* __restrict__ dslot,
+                        const int4* __restrict__ nslot, const double* __restrict__ sell, double* __restrict__ out,
+                        int* bad) {
+  const long s = blockIdx.y;
+  const double* v = sell + s * entries;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx), y = (int)(i / nx);
+    const int4 sl = nslot[i];
+    out[s * n + i] = v[dslot[i]];
+    out[nv + s * n + i] = x < nx - 1 ? v[sl.z] : 0.0;
+    out[2 * nv + s * n + i] = y < ny - 1 ? v[sl.w] : 0.0;
+    bool ok = true;
+    if (x > 0) ok = ok && __double_as_longlong(v[sl.y]) == __double_as_longlong(v[nslot[i - 1].z]);
+    if (y > 0) ok = ok && __double_as_longlong(v[sl.x]) == __double_as_longlong(v[nslot[i - nx].w]);
+    if (!ok) *bad = 1;
+  }
+}
+
 // Point Jacobi: dinv = 1 / a_ii (the single-system setup), FP32 copy rounded.
 __global__ void k_b_dinv(int n, long entries, const int* __restrict__ dslot, const double* __restrict__ sell,
                          double* __restrict__ dinv, float* __restrict__ dinv32) {
@@ -281,7 +302,7 @@ __global__ void __launch_bounds__(kNT) k_b_scale(int n, const T* __restrict__ r,
 }
 
 // K2 of a factor-preconditioned iteration: alpha = rho0 / gamma; r -= alpha q
-template <class T>
+template <class T, int LB>
 __global__ void __launch_bounds__(kNT) k_b_axpy_r(int n, int it, T* __restrict__ r, const T* __restrict__ q,
                                                   double* part, CGState* st, int nb_prod, Bat B) {
   __shared__ double sh[kWaves];
@@ -297,11 +318,15 @@ __global__ void __launch_bounds__(kNT) k_b_axpy_r(int n, int it, T* __restrict__
   const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
   const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
   if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
-  const int nb = gridDim.x;
+  const int nb = gridDim.x * LB;
+  const int o0 = tile_owner() * LB;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long k = tile_owner(); k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
+  for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const long i = (base + o0 + j) * kNT + threadIdx.x;
+      if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
+    }
   }
 }
 
@@ -434,6 +459,8 @@ struct Prof {
   }
 };
 
+constexpr int kLB = 4;  // logical tile owners per workgroup in the batched iteration kernels
+
 template <class T>
 struct BVecs {
   SellArgsT<T> A;
@@ -442,6 +469,7 @@ struct BVecs {
   const T* F;     // factor apply arrays (IC0 / FACTOR) or null
   const float* Fmix = nullptr;  // FP32 factor apply arrays used on T vectors (FP32 apply in FP64 PCG)
   float* wmix = nullptr;        // its FP32 intermediate vector
+  const T* st5 = nullptr;       // five-point stencil arrays (c, e, n; nv apart) or null
   double* hist;
   int hist_cap;
 };
@@ -456,6 +484,9 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
   const hipStream_t st = B.stream;
   const int nb = tiled_nb(n);
   const dim3 grid(nb, nsys), block(kNT);
+  // the per-iteration kernels run kLB owners per workgroup (same sums)
+  const bool lb = nb % (8 * kLB) == 0;
+  const dim3 gridl(lb ? nb / kLB : nb, nsys);
   const Bat bt{(long)n, B.A.entries};
   CGState* S = B.st.p;
   double* part = B.part.p;
@@ -493,8 +524,20 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
       T* pold = (it - 1) & 1 ? V.p1 : V.p0;
       T* pnew = it & 1 ? V.p1 : V.p0;
       hipEvent_t ea = pf.rec(st);
-      hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
-                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+      if (V.st5) {
+        const St5<T> A5{V.st5, V.st5 + B.nv(), V.st5 + 2 * B.nv(), B.nx, B.ny};
+        if (lb)
+          hipLaunchKernelGGL((k_cg_spmv<T, kLB, St5<T>>), gridl, block, 0, st, n, it, itrmax, rtol, A5, x,
+                             (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+        else
+          hipLaunchKernelGGL((k_cg_spmv<T, 1, St5<T>>), grid, block, 0, st, n, it, itrmax, rtol, A5, x,
+                             (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+      } else if (lb)
+        hipLaunchKernelGGL((k_cg_spmv<T, kLB>), gridl, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
+                           (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+      else
+        hipLaunchKernelGGL((k_cg_spmv<T, 1>), grid, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
+                           (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
       launches++;
       if (pf.on) pf.spmv.push_back({ea, pf.rec(st)});
       if (it > itrmax) break;
@@ -503,15 +546,22 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
                            S, nb, bt);
         launches++;
       } else {
-        hipLaunchKernelGGL(k_b_axpy_r<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
+        if (lb)
+          hipLaunchKernelGGL((k_b_axpy_r<T, kLB>), gridl, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
+        else
+          hipLaunchKernelGGL((k_b_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
         hipEvent_t eb = pf.rec(st);
         if (V.F)
           launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, S, it);
         else
           launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, V.s, S, it);
         if (pf.on) pf.apply.push_back({eb, pf.rec(st)});
-        hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
-                           (const CGState*)S, bt);
+        if (lb)
+          hipLaunchKernelGGL((k_cg_dot2<T, kLB>), gridl, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                             (const CGState*)S, bt);
+        else
+          hipLaunchKernelGGL((k_cg_dot2<T, 1>), grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                             (const CGState*)S, bt);
         launches += 3;
       }
     }
@@ -742,6 +792,20 @@ int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const in
     hipLaunchKernelGGL(k_b_gather, g, b, 0, B.stream, tot, (long)nnz, (const int*)didx.p, csr_dev, S.val.p,
                        S.val32.p);
   }
+  if (B.nx) {
+    B.st5.alloc((size_t)(3 * B.nv()));
+    DevBuf<int> bad(1);
+    PARCEL_HIP_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), B.stream));
+    const dim3 g((unsigned)std::min<long>((n + 255) / 256, 4096), nsys), b(256);
+    hipLaunchKernelGGL(k_b_st5, g, b, 0, B.stream, n, B.nx, B.ny, tot, B.nv(), (const int*)B.diag_slot.p,
+                       (const int4*)B.nbr_slot.p, (const double*)S.val.p, B.st5.p, bad.p);
+    int hb = 1;
+    PARCEL_HIP_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, B.stream));
+    PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+    B.st5_ok = hb == 0;
+    if (!B.st5_ok) B.st5.release();
+    B.stats.stencil5 = B.st5_ok ? 1 : 0;
+  }
   if (precon == PARCEL_BATCH_JACOBI) {
     B.dinv.alloc((size_t)B.nv());
     B.dinv32.alloc((size_t)B.nv());
@@ -773,6 +837,7 @@ BVecs<double> vecs_of<double>(Batch& B) {
   V.r = B.r.p; V.q = B.q.p; V.s = B.s.p; V.p0 = B.p0.p; V.p1 = B.p1.p; V.w = B.w.p;
   V.dinv = B.precon == PARCEL_BATCH_JACOBI ? B.dinv.p : nullptr;
   V.F = (B.precon == PARCEL_BATCH_IC0 || B.precon == PARCEL_BATCH_FACTOR) ? B.F64.p : nullptr;
+  V.st5 = B.st5_ok ? B.st5.p : nullptr;
   V.hist = nullptr; V.hist_cap = 0;
   return V;
 }

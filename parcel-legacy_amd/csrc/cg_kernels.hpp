@@ -33,9 +33,71 @@ __device__ __forceinline__ int tile_owner() {
   return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// N workgroup sums at once, each equal to block_sum's (butterfly per wave,
+// then the wave totals left to right); sh must hold kWaves*N doubles.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < N; ++d) v[d] = wave_sum(v[d]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < N; ++d) sh[w * N + d] = v[d];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    double t = sh[d];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) t = t + sh[k * N + d];
+    v[d] = t;
+  }
+}
+
 template <class T>
 __device__ __forceinline__ SellArgsT<T> sys_matrix(SellArgsT<T> A, const Bat& B) {
   A.val += sys_id() * B.as;
+  return A;
+}
+
+// Matrix access of the fused CG kernel: the SELL-64 matrix, or a five-point
+// stencil in natural order with symmetric couplings stored once per row.
+template <class T, class G>
+__device__ __forceinline__ T mat_row(const SellArgsT<T>& A, long i, G g) {
+  return sell_row(A, i, g);
+}
+
+// Five-point stencil: c = a(i,i), e = a(i,i+1) = a(i+1,i), nn = a(i,i+nx) =
+// a(i+nx,i); one system's arrays are vs apart.  A row sums its entries in
+// the CSR order s, w, c, e, n (sell_row's order for that pattern), so the
+// product is bitwise the SELL product when the matrix is exactly symmetric.
+template <class T>
+struct St5 {
+  const T* c;
+  const T* e;
+  const T* nn;
+  int nx, ny;
+};
+
+template <class T, class G>
+__device__ __forceinline__ T mat_row(const St5<T>& A, long i, G g) {
+  const int x = (int)(i % A.nx), y = (int)(i / A.nx);
+  T t = 0;
+  if (y > 0) t = fma(A.nn[i - A.nx], g(i - A.nx), t);
+  if (x > 0) t = fma(A.e[i - 1], g(i - 1), t);
+  t = fma(A.c[i], g(i), t);
+  if (x < A.nx - 1) t = fma(A.e[i], g(i + 1), t);
+  if (y < A.ny - 1) t = fma(A.nn[i], g(i + A.nx), t);
+  return t;
+}
+
+template <class T>
+__device__ __forceinline__ St5<T> sys_matrix(St5<T> A, const Bat& B) {
+  const long o = sys_id() * B.vs;
+  A.c += o;
+  A.e += o;
+  A.nn += o;
   return A;
 }
 
@@ -124,14 +186,19 @@ __global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGS
   }
 }
 
-// K1 of iteration `it`
-template <class T>
-__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, SellArgsT<T> A,
+// K1 of iteration `it`.  LB logical owners per workgroup: the tiled layout
+// has nb = LB * gridDim.x owners; workgroup w runs owners tile_owner(w)*LB
+// + j, j < LB, round by round (tile o + r*nb of every owner o in round r),
+// and writes each owner's partial, so every sum equals the LB = 1 launch's
+// while the partial reduction at the start is paid once per workgroup and
+// LB rows per thread are in flight.
+template <class T, int LB = 1, class MT = SellArgsT<T>>
+__global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, MT A,
                                                  T* __restrict__ x, const T* __restrict__ p_old,
                                                  const T* __restrict__ s, T* __restrict__ p_new,
                                                  T* __restrict__ q, double* part, CGState* st,
                                                  double* hist, int hist_cap, Bat B) {
-  __shared__ double sh[kWaves];
+  __shared__ double sh[kWaves * LB];
   st += sys_id();
   {
     const int d = st->done_at;
@@ -146,7 +213,8 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
   const double bnorm = st->bnorm;
   double alpha_prev, beta, rho0, rnorm;
   bool exit_loop = false;
-  const int nb = gridDim.x;
+  const int nb = gridDim.x * LB;
+  const int o0 = tile_owner() * LB;
   if (it == 1) {
     alpha_prev = 0.0;
     beta = 0.0;
@@ -166,9 +234,12 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
   const long ntiles = ((long)n + kNT - 1) / kNT;
   if (exit_loop) {
     // loop ended after iteration it-1: final x += alpha p (pcg.f90:733)
-    for (long k = tile_owner(); k < ntiles; k += nb) {
-      const long i = k * kNT + threadIdx.x;
-      if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
+    for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        const long i = (base + o0 + j) * kNT + threadIdx.x;
+        if (i < n) x[i] = fma((T)alpha_prev, p_old[i], x[i]);
+      }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done_at = it;
@@ -179,21 +250,29 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
     }
     return;
   }
-  double acc = 0.0;
-  for (long k = tile_owner(); k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T po = p_old[i];
-      x[i] = fma((T)alpha_prev, po, x[i]);
-      const T pn = fma((T)beta, po, s[i]);
-      p_new[i] = pn;
-      const T t = sell_row(A, i, [&](int c) { return fma((T)beta, p_old[c], s[c]); });
-      q[i] = t;
-      acc = fma((double)pn, (double)t, acc);
+  double acc[LB];
+#pragma unroll
+  for (int j = 0; j < LB; ++j) acc[j] = 0.0;
+  for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const long i = (base + o0 + j) * kNT + threadIdx.x;
+      if (i < n) {
+        const T po = p_old[i];
+        x[i] = fma((T)alpha_prev, po, x[i]);
+        const T pn = fma((T)beta, po, s[i]);
+        p_new[i] = pn;
+        const T t = mat_row(A, i, [&](long c) { return fma((T)beta, p_old[c], s[c]); });
+        q[i] = t;
+        acc[j] = fma((double)pn, (double)t, acc[j]);
+      }
     }
   }
-  double s0 = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[PART_GAMMA * kNBMax + tile_owner()] = s0;
+  block_sum_n<LB>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) part[PART_GAMMA * kNBMax + o0 + j] = acc[j];
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (it - 1 < hist_cap) hist[it - 1] = rnorm / bnorm;  // pcg.f90:512-518
     CGSlot& cur = st->slot[it & 1];
@@ -246,11 +325,12 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restri
   }
 }
 
-// rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30)
-template <class T>
+// rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30); LB logical
+// owners per workgroup as in k_cg_spmv
+template <class T, int LB = 1>
 __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restrict__ s, const T* __restrict__ r,
                                                  double* part, const CGState* st, Bat B) {
-  __shared__ double sh[kWaves];
+  __shared__ double sh[kWaves * 2 * LB];
   st += sys_id();
   {
     const int d = st->done_at;
@@ -259,22 +339,30 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restr
   const long so = sys_id() * B.vs;
   s += so; r += so;
   part = sys_part(part);
-  const int nb = gridDim.x;
+  const int nb = gridDim.x * LB;
+  const int o0 = tile_owner() * LB;
   const long ntiles = ((long)n + kNT - 1) / kNT;
-  double a0 = 0.0, a1 = 0.0;
-  for (long k = tile_owner(); k < ntiles; k += nb) {
-    const long i = k * kNT + threadIdx.x;
-    if (i < n) {
-      const T ri = r[i];
-      a0 = fma((double)s[i], (double)ri, a0);
-      a1 = fma((double)ri, (double)ri, a1);
+  double acc[2 * LB];
+#pragma unroll
+  for (int j = 0; j < 2 * LB; ++j) acc[j] = 0.0;
+  for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const long i = (base + o0 + j) * kNT + threadIdx.x;
+      if (i < n) {
+        const T ri = r[i];
+        acc[j] = fma((double)s[i], (double)ri, acc[j]);
+        acc[LB + j] = fma((double)ri, (double)ri, acc[LB + j]);
+      }
     }
   }
-  double s0 = block_sum(a0, sh);
-  double s1 = block_sum(a1, sh);
+  block_sum_n<2 * LB>(acc, sh);
   if (threadIdx.x == 0) {
-    part[PART_RHO1 * kNBMax + tile_owner()] = s0;
-    part[PART_RR * kNBMax + tile_owner()] = s1;
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      part[PART_RHO1 * kNBMax + o0 + j] = acc[j];
+      part[PART_RR * kNBMax + o0 + j] = acc[LB + j];
+    }
   }
 }
 

@@ -170,6 +170,7 @@ class BatchStats(ctypes.Structure):
         ("spmv_calls", ctypes.c_int),
         ("nx", ctypes.c_int),
         ("ny", ctypes.c_int),
+        ("stencil5", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -228,3 +228,34 @@ def test_batch_strips_bit_exact(nstrip, precision):
             xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=3000, rtol=1e-10)
         assert iret[s] == 0 and niter[s] == no, s
         assert np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s
+
+
+@pytest.mark.parametrize("nx,ny,nsys,sym", [(128, 128, 3, True), (256, 96, 2, True), (128, 128, 2, False),
+                                            (100, 37, 2, True)])
+def test_batch_stencil_spmv_bit_exact(nx, ny, nsys, sym):
+    # exactly symmetric five-point systems take the stencil SpMV (three
+    # coefficient arrays, several tile owners per workgroup when the tile
+    # count allows); a nonsymmetric one keeps the SELL product; both must
+    # give the oracle's histories bit for bit
+    print("batch stencil", nx, ny, nsys, sym, flush=True)
+    rp, col, vals, b = batch_problem(nx, ny, nsys)
+    if not sym:
+        rng = np.random.default_rng(5)
+        vals = vals * (1.0 + 0.01 * (rng.random(vals.shape) - 0.5) * (col - 1 != np.repeat(
+            np.arange(nx * ny), np.diff(rp)))[None, :])
+        vals = np.ascontiguousarray(vals)
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0 if sym else ph.BATCH_JACOBI) as B:
+        assert B.stats()["stencil5"] == (1 if sym else 0)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 4000, 1e-10)
+    for s in range(nsys):
+        if sym:
+            F64, F32 = orc.derive(*ic0_factor(rp, vals[s], nx), nx)
+            M = orc.FactorPrecon(F64, F32, nx)
+            xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=4000, rtol=1e-10)
+        else:
+            xo, ho, no, io = orc.pcg(rp, col, vals[s], b[s], ipreflag=1, itrmax=4000, rtol=1e-10)
+        assert niter[s] == no and iret[s] == io, s
+        assert np.array_equal(hist[s, :no], ho), s
+        assert np.array_equal(x[s], xo), s
+
