@@ -233,9 +233,13 @@ int parcel_hip_ddcg(int icomm, double* vecx_hi, double* vecx_lo, const double* v
  * cells on each side, nrow = the padded size; ioff_dia the linear offsets,
  * ioff_grids(nnd, ncomp_grids) the per-direction offsets); x, b padded too.
  * ipreflag 0 none, 1 point Jacobi, 2 block Jacobi = D-ILU on boxes of the
- * GPU build (nx/4 x ny/4 x nz/4 boxes, init_DILU_DDM).  One GPU holds one
- * rank's whole grid: num_neighbor_ranks must be 0.  Additive Schwarz
- * (ipreflag 3) is not available. */
+ * GPU build (nx/4 x ny/4 x nz/4 boxes, init_DILU_DDM), 3 additive Schwarz
+ * (solve_DILU_AS_DDM_GPU, src/src_DDM/precon_ddm_GPU.f90:1573-1868): the same
+ * boxes extended by addL >= 0 cells on every side facing another box, D-ILU
+ * per extended box, each box's own cells kept (restricted additive Schwarz,
+ * what the reference computes at one rank; iflagAS only selects how rank
+ * sleeves are averaged).  One GPU holds one rank's whole grid:
+ * num_neighbor_ranks must be 0. */
 int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd, const int* ioff_dia,
                        const double* val_dia, int num_neighbor_ranks, const int* neighbor_ranks, int margin,
                        int ncomp_grids, const int* ndiv_grids, const int* num_grids, const int* ioff_grids,

@@ -70,6 +70,7 @@ def load():
         "oracle_hqr_eigs": (_I, [_I, _P, _I, _P, _P]),
         "oracle_leja": (None, [_I, _P, _P]),
         "oracle_sym_pinv": (None, [_I, _P, _I, _P, _I]),
+        "oracle_precon_create_ras": (_P, [_I, _I, _P, _P, _P, _P, _P, _I, _P, _IP]),
         "oracle_tsqr_r": (_I, [_I, _I, _P, _I, _P]),
         "oracle_house_r": (None, [_I, _I, _P, _I, _P]),
         "oracle_tsqr_waves": (_I, [_I]),
@@ -110,6 +111,32 @@ def spmv(rowptr, col, val, x):
     y = np.empty(n, np.float64)
     load().oracle_spmv(n, _p(rowptr), _p(col), _p(val), _p(x), _p(y))
     return y
+
+
+class RasPrecon:
+    """Restricted additive Schwarz: gather src, block D-ILU of the expanded
+    matrix (rp_e, col_e, val_e; 1-based CSR, blocks at starts), scatter to dst."""
+
+    def __init__(self, n, src, dst, rp_e, col_e, val_e, starts):
+        self._keep = [np.ascontiguousarray(a, dtype=t) for a, t in
+                      ((src, np.int32), (dst, np.int32), (rp_e, np.int32), (col_e, np.int32), (val_e, np.float64),
+                       (starts, np.int32))]
+        s, d, r, c, v, st = self._keep
+        self.n = n
+        err = ctypes.c_int(0)
+        self.h = load().oracle_precon_create_ras(n, int(s.shape[0]), _p(s), _p(d), _p(r), _p(c), _p(v),
+                                                 int(st.shape[0]) - 1, _p(st), ctypes.byref(err))
+        if not self.h:
+            raise RuntimeError("oracle RAS setup failed: %d" % err.value)
+
+    def apply(self, r):
+        s = np.empty(self.n, np.float64)
+        load().oracle_precon_apply(self.h, _p(np.ascontiguousarray(r, dtype=np.float64)), _p(s))
+        return s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_precon_free(self.h)
 
 
 class Precon:

@@ -25,6 +25,13 @@ struct oracle_precon {
   float* F32;  /* the FP32 apply arrays (4n) */
   int apply32in64; /* FP64 vectors through the FP32 apply */
   int nstrip;      /* independent strips of grid lines */
+  /* ORACLE_PRECON_RAS: block D-ILU of the expanded matrix, gather / scatter */
+  oracle_precon* inner;
+  int ras_n;
+  int* ras_src;
+  int* ras_dst;
+  double* ras_re;
+  double* ras_ze;
 };
 
 /* factor solves of batched.c */

@@ -449,6 +449,13 @@ static oracle_precon* precon_create(int ipreflag, int ilu_method, int nblocks, c
 
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   int n = p->n;
+  if (p->ipreflag == ORACLE_PRECON_RAS) {
+    for (int q = 0; q < p->ras_n; ++q) p->ras_re[q] = r[p->ras_src[q]];
+    oracle_precon_apply(p->inner, p->ras_re, p->ras_ze);
+    for (int q = 0; q < p->ras_n; ++q)
+      if (p->ras_dst[q] >= 0) s[p->ras_dst[q]] = p->ras_ze[q];
+    return;
+  }
   if (p->ipreflag == ORACLE_PRECON_FACTOR) {
     if (p->apply32in64)
       oracle_lscan_apply32in64(p->nx, p->ny, p->nstrip, p->F32, r, s);
@@ -528,8 +535,38 @@ void oracle_precon_apply32(const oracle_precon* p, const float* r, float* s) {
 int oracle_precon_nblocks(const oracle_precon* p) { return p->nblocks; }
 const double* oracle_precon_dinv(const oracle_precon* p) { return p->dinv; }
 
+/* Restricted additive Schwarz (solve_DILU_AS_DDM_GPU at one rank,
+ * src/src_DDM/precon_ddm_GPU.f90:1573-1868): expanded row q takes r[src[q]];
+ * the expanded block-diagonal matrix (one block per extended box, 1-based
+ * CSR, block b = 0-based rows [starts[b], starts[b+1])) is D-ILU (ILUD)
+ * factorised per block; s[dst[q]] = z[q] where dst[q] >= 0. */
+oracle_precon* oracle_precon_create_ras(int n, int ne, const int* src, const int* dst, const int* rp_e,
+                                        const int* col_e, const double* val_e, int nblocks, const int* starts,
+                                        int* err) {
+  oracle_precon* in = oracle_precon_create_blocks(2, 1, nblocks, starts, 1, ne, rp_e, col_e, val_e, err);
+  if (!in) return NULL;
+  oracle_precon* p = (oracle_precon*)calloc(1, sizeof(oracle_precon));
+  p->ipreflag = ORACLE_PRECON_RAS;
+  p->n = n;
+  p->inner = in;
+  p->ras_n = ne;
+  p->ras_src = (int*)malloc(sizeof(int) * (size_t)ne);
+  p->ras_dst = (int*)malloc(sizeof(int) * (size_t)ne);
+  memcpy(p->ras_src, src, sizeof(int) * (size_t)ne);
+  memcpy(p->ras_dst, dst, sizeof(int) * (size_t)ne);
+  p->ras_re = (double*)malloc(sizeof(double) * (size_t)ne);
+  p->ras_ze = (double*)malloc(sizeof(double) * (size_t)ne);
+  return p;
+}
+
 void oracle_precon_free(oracle_precon* p) {
   if (!p) return;
+  if (p->ipreflag == ORACLE_PRECON_RAS) {
+    oracle_precon_free(p->inner);
+    free(p->ras_src); free(p->ras_dst); free(p->ras_re); free(p->ras_ze);
+    free(p);
+    return;
+  }
   free(p->blo); free(p->bhi); free(p->dinv);
   free(p->F64); free(p->F32);
   free(p->lstart); free(p->lend); free(p->ustart); free(p->uend);

@@ -169,6 +169,11 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
 /* --- batched solver's factor preconditioners (batched.c; no reference
  * counterpart).  M = L L^T on a 2D five-point grid in natural order. ------ */
 #define ORACLE_PRECON_FACTOR 9
+/* restricted additive Schwarz over overlapping blocks (see oracle_precon_create_ras) */
+#define ORACLE_PRECON_RAS 10
+oracle_precon* oracle_precon_create_ras(int n, int ne, const int* src, const int* dst, const int* rp_e,
+                                        const int* col_e, const double* val_e, int nblocks, const int* starts,
+                                        int* err);
 /* PCG / mixed refinement with a prebuilt preconditioner (any kind) */
 int oracle_pcg_with(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
                     const double* b, double* x, int itrmax, double rtol, const oracle_red* red,

@@ -302,34 +302,6 @@ __global__ void __launch_bounds__(kNT) k_b_scale(int n, const T* __restrict__ r,
 }
 
 // K2 of a factor-preconditioned iteration: alpha = rho0 / gamma; r -= alpha q
-template <class T, int LB>
-__global__ void __launch_bounds__(kNT) k_b_axpy_r(int n, int it, T* __restrict__ r, const T* __restrict__ q,
-                                                  double* part, CGState* st, int nb_prod, Bat B) {
-  __shared__ double sh[kWaves];
-  st += sys_id();
-  {
-    const int d = st->done_at;
-    if (d != 0 && d <= it) return;
-  }
-  const long so = sys_id() * B.vs;
-  r += so; q += so;
-  part = sys_part(part);
-  CGSlot& cur = st->slot[it & 1];
-  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
-  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
-  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
-  const int nb = gridDim.x * LB;
-  const int o0 = tile_owner() * LB;
-  const long ntiles = ((long)n + kNT - 1) / kNT;
-  for (long base = 0; base < ntiles; base += nb) {
-#pragma unroll
-    for (int j = 0; j < LB; ++j) {
-      const long i = (base + o0 + j) * kNT + threadIdx.x;
-      if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
-    }
-  }
-}
-
 // --- mixed-precision outer loop ---------------------------------------------------
 
 // Before FP64 correction k (k = 1, 2, ...): norms of the current residual
@@ -547,9 +519,9 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
         launches++;
       } else {
         if (lb)
-          hipLaunchKernelGGL((k_b_axpy_r<T, kLB>), gridl, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
+          hipLaunchKernelGGL((k_cg_axpy_r<T, kLB>), gridl, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
         else
-          hipLaunchKernelGGL((k_b_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
+          hipLaunchKernelGGL((k_cg_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
         hipEvent_t eb = pf.rec(st);
         if (V.F)
           launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, S, it);

@@ -328,6 +328,12 @@ int parcel_hip_plan_destroy(parcel_hip_plan* plan) {
   if (!plan) return PARCEL_HIP_OK;
   hipStream_t s = plan->P.stream;
   if (s) (void)hipStreamSynchronize(s);
+  if (plan->P.ras_owner) {
+    plan->P.ras_owner->P.stream = nullptr;  // shares this plan's stream
+    (void)parcel_hip_plan_destroy(plan->P.ras_owner);
+    plan->P.ras_owner = nullptr;
+    plan->P.ras_plan = nullptr;
+  }
   delete plan;
   if (s) (void)hipStreamDestroy(s);
   return PARCEL_HIP_OK;
@@ -716,6 +722,97 @@ std::vector<int> box_edges(int n) {
 
 extern "C" {
 
+// Restricted additive Schwarz on the box-major system: expanded row p of box
+// b is a cell of b's extended box (lexicographic z, y, x order inside the
+// box); the expanded matrix keeps the couplings inside the extended box; its
+// block D-ILU (ILUD, one block per box) runs in a second plan sharing this
+// plan's stream.
+static int ras_build(parcel_hip_plan* plan, const DdmGrid& G, int sx, int sy, int sz, const std::vector<int>& ex,
+                     const std::vector<int>& ey, const std::vector<int>& ez, const std::vector<int>& pos,
+                     const std::vector<int>& ioff, const std::vector<double>& vd, long m, int nnd, int addL) {
+  Plan& P = plan->P;
+  const int nbx = (int)ex.size() - 1, nby = (int)ey.size() - 1, nbz = (int)ez.size() - 1;
+  auto lo = [&](const std::vector<int>& e, int k) { return k > 0 ? e[k] - addL : e[k]; };
+  auto hi = [&](const std::vector<int>& e, int k, int nb) { return k < nb - 1 ? e[k + 1] + addL : e[k + 1]; };
+  for (int k = 1; k < nbx; ++k)
+    if (ex[k] - addL < 0 || ex[k] + addL > G.nx) return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: addL wider than the grid");
+  for (int k = 1; k < nby; ++k)
+    if (ey[k] - addL < 0 || ey[k] + addL > G.ny) return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: addL wider than the grid");
+  for (int k = 1; k < nbz; ++k)
+    if (ez[k] - addL < 0 || ez[k] + addL > G.nz) return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: addL wider than the grid");
+  std::vector<int> src, dst, starts, rp(1, 1), col;
+  std::vector<double> val;
+  std::vector<long> cell;  // padded index of each expanded row of the current box
+  std::vector<int> local;  // padded index -> expanded position (current box), -1 outside
+  std::vector<std::pair<int, double>> row;
+  for (int bz = 0; bz < nbz; ++bz)
+    for (int by = 0; by < nby; ++by)
+      for (int bx = 0; bx < nbx; ++bx) {
+        const int x0 = lo(ex, bx), x1 = hi(ex, bx, nbx), y0 = lo(ey, by), y1 = hi(ey, by, nby);
+        const int z0 = lo(ez, bz), z1 = hi(ez, bz, nbz);
+        const int base = (int)src.size();
+        starts.push_back(base);
+        cell.clear();
+        for (int jz = z0; jz < z1; ++jz)
+          for (int jy = y0; jy < y1; ++jy)
+            for (int jx = x0; jx < x1; ++jx) {
+              const long j = (jx + sx) + (long)G.mx * (jy + sy) + (long)G.mx * G.my * (jz + sz);
+              const bool own = jx >= ex[bx] && jx < ex[bx + 1] && jy >= ey[by] && jy < ey[by + 1] &&
+                               jz >= ez[bz] && jz < ez[bz + 1];
+              src.push_back(pos[j]);
+              dst.push_back(own ? pos[j] : -1);
+              cell.push_back(j);
+            }
+        if (local.empty()) local.assign((size_t)m, -1);
+        for (size_t q = 0; q < cell.size(); ++q) local[cell[q]] = base + (int)q;
+        for (size_t q = 0; q < cell.size(); ++q) {
+          const long j = cell[q];
+          row.clear();
+          for (int ip = 0; ip < nnd; ++ip) {
+            const long jn = j + ioff[ip];
+            if (jn < 0 || jn >= m || local[jn] < 0) continue;  // outside the extended box
+            const double v = vd[(size_t)ip * m + j];
+            if (v == 0.0 && jn != j) continue;
+            row.emplace_back(local[jn], v);
+          }
+          std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& c) {
+            return a.first < c.first;
+          });
+          for (auto& e : row) { col.push_back(e.first + 1); val.push_back(e.second); }
+          rp.push_back((int)col.size() + 1);
+        }
+        for (size_t q = 0; q < cell.size(); ++q) local[cell[q]] = -1;
+      }
+  const int ne = (int)src.size();
+  starts.push_back(ne);
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = PARCEL_PRECON_BLOCKJACOBI;
+  prm.ILU_method = PARCEL_ILUD;
+  prm.independ_nvec = 1;
+  prm.precon_thblock = (int)starts.size() - 1;
+  prm.profile = 0;
+  parcel_hip_plan* aux = nullptr;
+  int rc = create_impl(&aux, ne, ne, (int)col.size(), 1, val.data(), nullptr, rp.data(), col.data(), &prm, 0,
+                       &starts);
+  if (rc) return rc;
+  // the block solves run on the outer plan's stream
+  (void)hipStreamSynchronize(aux->P.stream);
+  (void)hipStreamDestroy(aux->P.stream);
+  aux->P.stream = P.stream;
+  P.ras_owner = aux;
+  P.ras_plan = &aux->P;
+  P.ras_n = ne;
+  P.ras_src.upload(src, P.stream);
+  P.ras_dst.upload(dst, P.stream);
+  P.ras_r.alloc((size_t)ne);
+  P.ras_s.alloc((size_t)ne);
+  P.M.kind = PRECON_RAS;
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  return PARCEL_HIP_OK;
+}
+
 int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd, const int* ioff_dia,
                        const double* val_dia, int num_neighbor_ranks, const int* neighbor_ranks, int margin,
                        int ncomp_grids, const int* ndiv_grids, const int* num_grids, const int* ioff_grids,
@@ -723,7 +820,7 @@ int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd,
                        double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
                        int nBlock, int* iret) {
   PARCEL_API_BEGIN
-  (void)icomm; (void)neighbor_ranks; (void)ndiv_grids; (void)div_direc_th; (void)addL; (void)iflagAS;
+  (void)icomm; (void)neighbor_ranks; (void)ndiv_grids; (void)div_direc_th; (void)iflagAS;
   (void)iovlflag; (void)precon_thblock; (void)Neuman_itr; (void)nBlock;
   if (!x || !b || !ioff_dia || !val_dia || !num_grids || !ioff_grids || !itrmax || !iret)
     return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: null argument");
@@ -732,8 +829,10 @@ int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd,
   if (ncomp_grids < 1 || ncomp_grids > 3 || margin < 0 || nnd < 1)
     return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: bad grid description");
   if (ipreflag != PARCEL_PRECON_NONE && ipreflag != PARCEL_PRECON_POINTJACOBI &&
-      ipreflag != PARCEL_PRECON_BLOCKJACOBI)
+      ipreflag != PARCEL_PRECON_BLOCKJACOBI && ipreflag != PARCEL_PRECON_ADDITIVESCHWARZ)
     return fail(PARCEL_HIP_ERR_UNSUPPORTED, "dcg_ddm: preconditioner not available on the stencil path");
+  const bool as = ipreflag == PARCEL_PRECON_ADDITIVESCHWARZ;
+  if (as && addL < 0) return fail(PARCEL_HIP_ERR_ARG, "dcg_ddm: addL must be >= 0");
   DdmGrid G;
   G.dim = ncomp_grids;
   G.margin = margin;
@@ -806,7 +905,7 @@ int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd,
   parcel_hip_param prm;
   unset_param(prm);
   prm.solver = PARCEL_SOLVER_CG;
-  prm.ipreflag = ipreflag;
+  prm.ipreflag = as ? PARCEL_PRECON_NONE : ipreflag;
   prm.ILU_method = PARCEL_ILUD;
   prm.independ_nvec = independ_nvec;
   prm.precon_thblock = (int)starts.size() - 1;
@@ -815,6 +914,18 @@ int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd,
   int rc = create_impl(&plan, (int)n, (int)n, (int)col.size(), 1, val.data(), nullptr, rp.data(), col.data(), &prm,
                        0, ipreflag == PARCEL_PRECON_BLOCKJACOBI ? &starts : nullptr);
   if (rc) return rc;
+  if (as) {
+    // solve_DILU_AS_DDM_GPU (precon_ddm_GPU.f90:1573-1868) at one rank: the
+    // boxes of init_AS_DDM (:370-643) extended by addL cells on every side
+    // facing another box, D-ILU on each extended box, the box's own cells
+    // kept (the overlap sums of the update kernel only reach extension
+    // cells, which are never copied out)
+    rc = ras_build(plan, G, sx, sy, sz, ex, ey, ez, pos, ioff, vd, m, nnd, addL);
+    if (rc) {
+      parcel_hip_plan_destroy(plan);
+      return rc;
+    }
+  }
   rc = parcel_hip_plan_solve(plan, xp.data(), bp.data(), itrmax, rtolmax, reshistory, iret);
   parcel_hip_plan_destroy(plan);
   if (rc) return rc;

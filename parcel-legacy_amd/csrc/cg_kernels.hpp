@@ -366,6 +366,37 @@ __global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restr
   }
 }
 
+// alpha = rho0 / <p,q> (reduced from the SpMV's partials); r -= alpha q
+// (pcg.f90:590-620) for a preconditioner applied in a separate step; LB
+// logical owners per workgroup as in k_cg_spmv
+template <class T, int LB = 1>
+__global__ void __launch_bounds__(kNT) k_cg_axpy_r(int n, int it, T* __restrict__ r, const T* __restrict__ q,
+                                                   double* part, CGState* st, int nb_prod, Bat B) {
+  __shared__ double sh[kWaves];
+  st += sys_id();
+  {
+    const int d = st->done_at;
+    if (d != 0 && d <= it) return;
+  }
+  const long so = sys_id() * B.vs;
+  r += so; q += so;
+  part = sys_part(part);
+  CGSlot& cur = st->slot[it & 1];
+  const double gamma = reduce_part(part + PART_GAMMA * kNBMax, nb_prod, sh);
+  const double alpha = cur.rho0 / gamma;  // pcg.f90:590-600
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur.alpha = alpha;
+  const int nb = gridDim.x * LB;
+  const int o0 = tile_owner() * LB;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+    for (int j = 0; j < LB; ++j) {
+      const long i = (base + o0 + j) * kNT + threadIdx.x;
+      if (i < n) r[i] = fma(-(T)alpha, q[i], r[i]);
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kNT) k_spmv(int n, SellArgsT<T> A, const T* __restrict__ x, T* __restrict__ y,
                                               Bat B) {

@@ -123,7 +123,7 @@ struct CGState {
   CGSlot slot[2];
 };
 
-enum PreconKind { PRECON_NONE = 0, PRECON_PJ = 1, PRECON_BJ = 2 };
+enum PreconKind { PRECON_NONE = 0, PRECON_PJ = 1, PRECON_BJ = 2, PRECON_RAS = 3 };
 enum BJMode { BJ_ILU0 = 0, BJ_DILU = 1 };
 
 struct Precon {
@@ -159,6 +159,15 @@ struct Plan {
   double inner_rtol = 1e-4;         // mixed-precision inner PCG tolerance
   int inner_itrmax = 10000;
   DevBuf<char> solver_state;        // device-resident scalars of the other solvers
+  // restricted additive Schwarz (PRECON_RAS): the block D-ILU of the
+  // expanded block-diagonal matrix (one block per extended box) lives in the
+  // plan ras_plan; ras_src[p] is the row gathered into expanded row p,
+  // ras_dst[p] the row it is scattered back to (-1: an overlap row, dropped)
+  parcel_hip_plan* ras_owner = nullptr;
+  Plan* ras_plan = nullptr;
+  int ras_n = 0;
+  DevBuf<int> ras_src, ras_dst;
+  DevBuf<double> ras_r, ras_s;
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
   DevBuf<double> hist;      // residual history
@@ -196,6 +205,8 @@ struct CGVecs {
   double* hist = nullptr;    // device residual history (hist_cap entries)
   int hist_cap = 0;
   void (*precon)(Plan&, const T*, T*, const CGFuseT<T>*) = nullptr;
+  // the same without a fused update, returning at once when *skip != 0
+  void (*precon_skip)(Plan&, const T*, T*, const int*) = nullptr;
 };
 template <class T>
 int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rtol, double* hist_host, int* niter,

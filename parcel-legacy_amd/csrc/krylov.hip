@@ -203,6 +203,7 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   int enq = 0;  // K1 launches enqueued
   int chunk = 8;
   const bool bj = P.M.kind == PRECON_BJ;
+  const bool ras = P.M.kind == PRECON_RAS;
   const T* dinv = V.dinv;
   while (enq < itrmax + 1) {
     const int k = std::min(chunk, itrmax + 1 - enq);
@@ -217,7 +218,18 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
       hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
       if (Tm.on) Tm.spmv.push_back({ea, eb, it});
       if (it > itrmax) break;  // the last K1 only closes the loop
-      if (!bj) {
+      if (ras) {
+        // r -= alpha q; s = M^-1 r (gather, block solves, scatter); dots
+        hipLaunchKernelGGL((k_cg_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, kOne);
+        hipEvent_t ec0 = Tm.on ? Tm.rec(st) : nullptr;
+        V.precon_skip(P, V.r, V.s, &S->done_at);
+        hipEvent_t ec = Tm.on ? Tm.rec(st) : nullptr;
+        if (Tm.on) Tm.prec.push_back({ec0, ec, it});
+        hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                           (const CGState*)S, kOne);
+        launches += 5;
+        if (Tm.on) Tm.vec.push_back({ec, Tm.rec(st), it});
+      } else if (!bj) {
         hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, dinv, V.s, part, S,
                            nb, kOne);
         launches++;
@@ -342,6 +354,7 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   V.hist = P.hist.p;
   V.hist_cap = P.hist_cap;
   V.precon = [](Plan& Q, const double* r, double* s, const CGFuse* f) { precon_apply(Q, r, s, f); };
+  V.precon_skip = [](Plan& Q, const double* r, double* s, const int* skip) { precon_apply(Q, r, s, nullptr, skip); };
   return cg_run<double>(P, V, x, b, itrmax, rtol, hist_host, niter, iret);
 }
 

@@ -84,6 +84,7 @@ int mixed_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, do
   V.hist = nullptr;
   V.hist_cap = 0;
   V.precon = apply32;
+  V.precon_skip = [](Plan& Q, const float* r, float* s, const int* skip) { precon_apply32(Q, r, s, nullptr, skip); };
   if (P.solver_state.n < 2 * sizeof(double)) P.solver_state.alloc(2 * sizeof(double));
   double* dst = reinterpret_cast<double*>(P.solver_state.p);
   double hs[2];

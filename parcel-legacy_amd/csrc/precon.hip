@@ -837,10 +837,50 @@ const double* dinv_of<double>(const Precon& M) { return M.dinv.p; }
 template <>
 const float* dinv_of<float>(const Precon& M) { return M.dinv32.p; }
 
+// restricted additive Schwarz: expanded row p takes r[src[p]]; after the
+// block solve, s[dst[p]] = z[p] for the owned rows (dst >= 0)
+__global__ void __launch_bounds__(kNT) k_ras_gather(int ne, const int* __restrict__ src, const double* __restrict__ r,
+                                                    double* __restrict__ re, const int* skip) {
+  if (skip && *skip) return;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ne; p += (long)gridDim.x * blockDim.x)
+    re[p] = r[src[p]];
+}
+__global__ void __launch_bounds__(kNT) k_ras_scatter(int ne, const int* __restrict__ dst, const double* __restrict__ ze,
+                                                     double* __restrict__ s, const int* skip) {
+  if (skip && *skip) return;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ne; p += (long)gridDim.x * blockDim.x) {
+    const int d = dst[p];
+    if (d >= 0) s[d] = ze[p];
+  }
+}
+
+template <class T>
+void precon_apply_t(Plan& P, const T* r, T* s, const CGFuseT<T>* fuse, const int* skip);
+
+template <class T>
+void ras_apply(Plan& P, const T* r, T* s, const int* skip) {
+  if constexpr (!std::is_same<T, double>::value) {
+    (void)P; (void)r; (void)s; (void)skip;
+    throw hip_error("additive Schwarz runs in FP64 only", PARCEL_HIP_ERR_UNSUPPORTED);
+  } else {
+    const int ne = P.ras_n;
+    const dim3 g((unsigned)std::min<long>(((long)ne + kNT - 1) / kNT, 4096)), b(kNT);
+    hipLaunchKernelGGL(k_ras_gather, g, b, 0, P.stream, ne, (const int*)P.ras_src.p, r, P.ras_r.p, skip);
+    precon_apply_t<double>(*P.ras_plan, P.ras_r.p, P.ras_s.p, nullptr, skip);
+    hipLaunchKernelGGL(k_ras_scatter, g, b, 0, P.stream, ne, (const int*)P.ras_dst.p, (const double*)P.ras_s.p, s,
+                       skip);
+  }
+}
+
 template <class T>
 void precon_apply_t(Plan& P, const T* r, T* s, const CGFuseT<T>* fuse, const int* skip) {
   Precon& M = P.M;
   const int n = P.A.n;
+  if (M.kind == PRECON_RAS) {
+    if (fuse) throw hip_error("additive Schwarz has no fused residual update", PARCEL_HIP_ERR_ARG);
+    ras_apply<T>(P, r, s, skip);
+    return;
+  }
   if (M.kind == PRECON_NONE || M.kind == PRECON_PJ) {
     if (fuse) throw hip_error("fused update needs the block-Jacobi preconditioner", PARCEL_HIP_ERR_ARG);
     int grid = tiled_nb(n);

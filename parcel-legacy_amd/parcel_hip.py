@@ -351,7 +351,7 @@ def poisson3d_ddm(gnx, gny, gnz, margin=1):
 
 
 def dcg_ddm(x, b, grid, margin, val_dia, ioff_dia, ioff_grids, ipreflag=PRECON_BJ, itrmax=1000, rtolmax=1e-8,
-            independ_nvec=1):
+            independ_nvec=1, addL=0, iflagAS=0):
     """parcel_dcg_ddm (src/ddm_kernel.f90:2893-2986): x, b in the padded
     layout; returns (hist[:niter], niter, iret)."""
     nnd = int(ioff_dia.shape[0])
@@ -362,7 +362,7 @@ def dcg_ddm(x, b, grid, margin, val_dia, ioff_dia, ioff_grids, ipreflag=PRECON_B
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
     _check(load().parcel_hip_dcg_ddm(0, _ptr(x), _ptr(b), m, nnd, _ptr(ioff_dia), _ptr(val_dia), 0, None, margin,
-                                     int(ng.shape[0]), _ptr(nd), _ptr(ng), _ptr(ioff_grids), 1, ipreflag, 0, 0,
+                                     int(ng.shape[0]), _ptr(nd), _ptr(ng), _ptr(ioff_grids), 1, ipreflag, addL, iflagAS,
                                      ctypes.byref(it), rtolmax, _ptr(hist), 0, -1, 1, independ_nvec, 2000,
                                      ctypes.byref(iret)))
     return hist[: it.value].copy(), it.value, iret.value

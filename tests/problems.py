@@ -85,3 +85,62 @@ def ddm_box_crs(d):
     A.sort_indices()
     return (A.indptr.astype(np.int32) + 1, A.indices.astype(np.int32) + 1, A.data.copy(), perm,
             np.array(starts, np.int32))
+
+
+def ddm_ras(d, addL):
+    """Restricted additive Schwarz of the GPU build (init_AS_DDM,
+    src/src_DDM/precon_ddm_GPU.f90:370-643, one rank): the boxes of
+    ddm_box_split extended by addL cells on every side facing another box;
+    expanded row q = a cell of an extended box (boxes in box order, cells in
+    z, y, x order).  Returns (src, dst, rp_e, col_e, val_e, starts_e) in the
+    box-major numbering of ddm_box_crs: src[q] = the row gathered, dst[q] =
+    the row written back (-1 for overlap cells), the expanded matrix keeps
+    the couplings inside each extended box (1-based CSR)."""
+    nx, ny, nz = d["grid"]
+    mg = d["margin"]
+    mx, my = nx + 2 * mg, ny + 2 * mg
+    m = d["m"]
+    val = d["val_dia"].reshape(7, m)
+    ioff = d["ioff_dia"]
+    ex, ey, ez = ddm_box_split(nx), ddm_box_split(ny), ddm_box_split(nz)
+    _, _, _, perm, _ = ddm_box_crs(d)
+    pos = -np.ones(m, np.int64)
+    pos[perm] = np.arange(perm.size)
+    lo = lambda e, k: e[k] - addL if k > 0 else e[k]
+    hi = lambda e, k: e[k + 1] + addL if k < len(e) - 2 else e[k + 1]
+    src, dst, starts, rows, cols, vals = [], [], [], [], [], []
+    for bz in range(len(ez) - 1):
+        for by in range(len(ey) - 1):
+            for bx in range(len(ex) - 1):
+                base = len(src)
+                starts.append(base)
+                cells = []
+                for jz in range(lo(ez, bz), hi(ez, bz)):
+                    for jy in range(lo(ey, by), hi(ey, by)):
+                        for jx in range(lo(ex, bx), hi(ex, bx)):
+                            j = (jx + mg) + mx * (jy + mg) + mx * my * (jz + mg)
+                            own = (ex[bx] <= jx < ex[bx + 1] and ey[by] <= jy < ey[by + 1] and
+                                   ez[bz] <= jz < ez[bz + 1])
+                            src.append(pos[j])
+                            dst.append(pos[j] if own else -1)
+                            cells.append(j)
+                local = {j: base + q for q, j in enumerate(cells)}
+                for q, j in enumerate(cells):
+                    ent = []
+                    for ip in range(7):
+                        jn = j + ioff[ip]
+                        v = val[ip, j]
+                        if jn not in local or (v == 0.0 and jn != j):
+                            continue
+                        ent.append((local[jn], v))
+                    for c, v in sorted(ent):
+                        rows.append(base + q)
+                        cols.append(c)
+                        vals.append(v)
+    ne = len(src)
+    starts.append(ne)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(ne, ne))
+    A.sort_indices()
+    return (np.array(src, np.int32), np.array(dst, np.int32), A.indptr.astype(np.int32) + 1,
+            A.indices.astype(np.int32) + 1, A.data.copy(), np.array(starts, np.int32))
+

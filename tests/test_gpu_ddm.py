@@ -11,7 +11,7 @@ import pytest
 
 import oracle as orc
 import parcel_hip as ph
-from problems import ddm_box_crs
+from problems import ddm_box_crs, ddm_ras
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -55,9 +55,30 @@ def test_dcg_ddm_reference_gpu_golden():
     assert rel.max() <= 1e-12
 
 
-def test_dcg_ddm_rejects_neighbour_ranks_and_as():
+@pytest.mark.parametrize("grid,addL", [((16, 16, 16), 1), ((16, 16, 16), 2), ((13, 10, 9), 1), ((24, 8, 1), 3)])
+def test_dcg_ddm_additive_schwarz_matches_oracle(grid, addL):
+    # ipreflag 3: restricted additive Schwarz over the boxes extended by addL
+    print("dcg_ddm AS", grid, addL, flush=True)
+    d = ph.poisson3d_ddm(*grid)
+    x = np.zeros(d["m"])
+    hist, niter, iret = ph.dcg_ddm(x, d["b"], grid, 1, d["val_dia"], d["ioff_dia"], d["ioff_grids"], ipreflag=3,
+                                   itrmax=2000, rtolmax=1e-9, addL=addL)
+    rp, col, val, perm, starts = ddm_box_crs(d)
+    src, dst, rpe, cole, vale, st = ddm_ras(d, addL)
+    M = orc.RasPrecon(perm.size, src, dst, rpe, cole, vale, st)
+    xo, ho, no, io = orc.pcg_factor(rp, col, val, d["b"][perm].copy(), M, itrmax=2000, rtol=1e-9)
+    assert (niter, iret) == (no, io) and iret == 0
+    assert np.array_equal(hist, ho)
+    assert np.array_equal(x[perm], xo)
+
+
+def test_dcg_ddm_rejects_bad_overlap():
     d = ph.poisson3d_ddm(8, 8, 8)
     x = np.zeros(d["m"])
+    # 8 cells make two boxes of 4: an overlap of 5 reaches past the grid
     with pytest.raises(ph.ParcelError) as e:
-        ph.dcg_ddm(x, d["b"], (8, 8, 8), 1, d["val_dia"], d["ioff_dia"], d["ioff_grids"], ipreflag=3)
+        ph.dcg_ddm(x, d["b"], (8, 8, 8), 1, d["val_dia"], d["ioff_dia"], d["ioff_grids"], ipreflag=3, addL=5)
     assert e.value.code == ph.ERR_UNSUPPORTED
+    with pytest.raises(ph.ParcelError) as e:
+        ph.dcg_ddm(x, d["b"], (8, 8, 8), 1, d["val_dia"], d["ioff_dia"], d["ioff_grids"], ipreflag=3, addL=-1)
+    assert e.value.code == ph.ERR_ARG
