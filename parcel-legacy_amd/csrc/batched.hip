@@ -131,29 +131,34 @@ __device__ __forceinline__ bool sys_done(const CGState* st, int sys, int it) {
   return d != 0 && d <= it;
 }
 
-// Registers of one grid line in flight (the loads of line y + ahead are
-// issued before line y's scan, so memory latency overlaps the recurrence).
-// A line takes 4*E values per lane; up to 128 VGPRs of them stay in flight.
-template <class T, int E>
+// Registers of one grid line in flight.  The loads of line t + ahead are
+// issued as soon as line t is done with its registers, so each line's load
+// latency overlaps the recurrences of the lines before it.  q stays in its
+// load type until use: a conversion at load time would wait for the load.
+// A line takes E (3 sizeof(T) + sizeof(TQ)) / 4 VGPRs per lane; up to about
+// 160 of them stay in flight.
+template <class T, class TQ, int E>
 constexpr int lines_ahead() {
-  return 4 * E * (int)sizeof(T) / 4 >= 128 ? 1 : (4 * E * (int)sizeof(T) / 4 >= 64 ? 2 : 4);
+  return E * (3 * (int)sizeof(T) + (int)sizeof(TQ)) / 4 >= 96 ? 1
+         : (E * (3 * (int)sizeof(T) + (int)sizeof(TQ)) / 4 >= 48 ? 2 : 4);
 }
-template <class T, int E>
+template <class T, class TQ, int E>
 struct LineRegs {
-  T a[E], p[E], q[E], i[E];
+  T a[E], p[E], i[E];
+  TQ q[E];
 };
 
 // The lane's E cells x0 .. x0+E-1 of one line; MIRROR stores them in reverse
 // (slot k holds cell x0+E-1-k), which the backward sweep uses.
 template <class T, class TQ, int E, bool FULL, bool MIRROR>
-__device__ __forceinline__ void load_line(LineRegs<T, E>& L, const T* __restrict__ a, const T* __restrict__ p,
+__device__ __forceinline__ void load_line(LineRegs<T, TQ, E>& L, const T* __restrict__ a, const T* __restrict__ p,
                                           const TQ* __restrict__ q, const T* __restrict__ inv, long row, int x0,
                                           int nx) {
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int k = MIRROR ? E - 1 - j : j;
     if (FULL || x0 + j < nx) {
-      L.a[k] = a[row + x0 + j]; L.p[k] = p[row + x0 + j]; L.q[k] = (T)q[row + x0 + j]; L.i[k] = inv[row + x0 + j];
+      L.a[k] = a[row + x0 + j]; L.p[k] = p[row + x0 + j]; L.q[k] = q[row + x0 + j]; L.i[k] = inv[row + x0 + j];
     } else {
       L.a[k] = 0; L.p[k] = 0; L.q[k] = 0; L.i[k] = 0;
     }
@@ -199,65 +204,75 @@ __device__ __forceinline__ T scan_maps(T A, T C) {
   return dpp<0x138, 0xf>((T)0, C);
 }
 
-// One sweep over the lines (forward: y = 0 .. ny-1, lane cells in increasing
-// x; backward: y = ny-1 .. 0, lane cells mirrored so the recurrence still
-// runs in lane order).  Map of a cell: v = a v_prev + c with
+// One line of a sweep: v = a v_prev + c along the lane's cells, with
 // c = fma(-p, carry, q) * inv, carry = the value of the same cell on the
 // previous line, p = ls of that previous line for the backward sweep.
 template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
+__device__ __forceinline__ void sweep_line(const LineRegs<T, TQ, E>& L, T (&carry)[E], T (&pn)[E], int x0, int nx,
+                                           TO* __restrict__ orow) {
+  T c[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int x = BWD ? x0 + E - 1 - k : x0 + k;
+    if (FULL || x < nx) {
+      // forward: p is this line's ls; backward: the previous (upper) line's
+      const T pk = BWD ? pn[k] : L.p[k];
+      c[k] = fma(-pk, carry[k], (T)L.q[k]) * L.i[k];
+      if (BWD) pn[k] = L.p[k];
+    } else {
+      c[k] = 0;
+      if (BWD) pn[k] = 0;
+    }
+  }
+  T A = 1, C = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k) { C = fma(L.a[k], C, c[k]); A = L.a[k] * A; }
+  T v = scan_maps(A, C);
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int x = BWD ? x0 + E - 1 - k : x0 + k;
+    v = fma(L.a[k], v, c[k]);
+    carry[k] = v;
+    if (FULL || x < nx) orow[x] = (TO)v;
+  }
+}
+
+// One sweep over the lines (forward: y = 0 .. ny-1, lane cells in increasing
+// x; backward: y = ny-1 .. 0, lane cells mirrored so the recurrence still
+// runs in lane order).  The main loop takes kAhead lines per trip without
+// branches (the prefetch of a line past the end reloads the last line) and
+// with a scheduling barrier between lines, so the compiler's wait before a
+// line's first use counts only that line's loads; the last ny mod kAhead
+// lines follow from the registers.
+template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
 __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, const T* __restrict__ ls,
                                       const TQ* __restrict__ q, const T* __restrict__ inv, TO* __restrict__ out) {
-  constexpr int kAhead = lines_ahead<T, E>();
+  constexpr int kAhead = lines_ahead<T, TQ, E>();
   const int lane = threadIdx.x;
   const int x0 = BWD ? 64 * E - (lane + 1) * E : lane * E;
+  auto row_of = [&](int t) -> long { return (long)(BWD ? ny - 1 - t : t) * nx; };
   T carry[E], pn[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) { carry[k] = 0; pn[k] = 0; }
-  LineRegs<T, E> buf[kAhead];
+  LineRegs<T, TQ, E> buf[kAhead];
 #pragma unroll
-  for (int d = 0; d < kAhead; ++d) {
-    const int y = BWD ? ny - 1 - d : d;
-    if (d < ny) load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, (long)y * nx, x0, nx);
-  }
-  for (int t0 = 0; t0 < ny; t0 += kAhead) {
+  for (int d = 0; d < kAhead; ++d)
+    load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(min(d, ny - 1)), x0, nx);
+  int t0 = 0;
+  for (; t0 + kAhead <= ny; t0 += kAhead) {
 #pragma unroll
     for (int d = 0; d < kAhead; ++d) {
       const int t = t0 + d;
-      if (t >= ny) break;
-      const int y = BWD ? ny - 1 - t : t;
-      const long row = (long)y * nx;
-      T am[E], c[E];
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        const int x = BWD ? x0 + E - 1 - k : x0 + k;
-        if (FULL || x < nx) {
-          am[k] = buf[d].a[k];
-          // forward: p is this line's ls; backward: the previous (upper) line's
-          const T pk = BWD ? pn[k] : buf[d].p[k];
-          c[k] = fma(-pk, carry[k], buf[d].q[k]) * buf[d].i[k];
-          if (BWD) pn[k] = buf[d].p[k];
-        } else {
-          am[k] = 0; c[k] = 0;
-          if (BWD) pn[k] = 0;
-        }
-      }
-      if (t + kAhead < ny) {
-        const int yn = BWD ? y - kAhead : y + kAhead;
-        load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, (long)yn * nx, x0, nx);
-      }
-      T A = 1, C = 0;
-#pragma unroll
-      for (int k = 0; k < E; ++k) { C = fma(am[k], C, c[k]); A = am[k] * A; }
-      T v = scan_maps(A, C);
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        const int x = BWD ? x0 + E - 1 - k : x0 + k;
-        v = fma(am[k], v, c[k]);
-        carry[k] = v;
-        if (FULL || x < nx) out[row + x] = (TO)v;
-      }
+      sweep_line<T, TQ, TO, E, FULL, BWD>(buf[d], carry, pn, x0, nx, out + row_of(t));
+      load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(min(t + kAhead, ny - 1)), x0, nx);
+      // keep the next line's work (its q conversions above all) from being
+      // hoisted into this one, where it would wait on loads just issued
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
+#pragma unroll
+  for (int d = 0; d < kAhead - 1; ++d)
+    if (t0 + d < ny) sweep_line<T, TQ, TO, E, FULL, BWD>(buf[d], carry, pn, x0, nx, out + row_of(t0 + d));
 }
 
 // z = (L L^T)^-1 r for every system: forward sweep into w, then the
