@@ -406,8 +406,8 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
     ap_total = learned["apply_ms"] * learned["apply_calls"]
     sp_total = learned["spmv_ms"] * learned["spmv_calls"]
     if ap_total >= sp_total:
-        kname, ms, by = ("k_lscan<float,double,8,FULL> (FP32 factor apply on FP64 vectors)", learned["apply_ms"],
-                         lscan_bytes(n, nsys, 4, 8))
+        kname, ms, by = ("k_lscan<float,double,2,4,FULL> (FP32 factor apply on FP64 vectors, 4 wavefronts per "
+                         "line)", learned["apply_ms"], lscan_bytes(n, nsys, 4, 8))
     else:
         if learned.get("stencil5"):
             kname, ms, by = ("k_cg_spmv<double,4,St5> (batched, five-point stencil)", learned["spmv_ms"],

@@ -100,20 +100,34 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
   }
 }
 
-/* The GPU's wavefront scan, lane by lane.  Forward sweep: lane l owns cells
- * x = l*E + k (k = 0 .. E-1); backward sweep: lane l owns cells
- * x = 64E - 1 - (l*E + k), so the right-to-left recurrence runs in lane
- * order too.  Cells past nx are padding with a = c = 0.
+/* Wavefronts per line and cells per lane of the GPU's factor apply for lines
+ * of nx cells: E1 = the smallest power of two with 64 E1 >= nx (the cells per
+ * lane of a single wavefront); lines of E1 >= 8 are split over 4 wavefronts
+ * (E = E1 / 4), E1 = 4 over 2 (E = 2), shorter lines take one wavefront. */
+static void lscan_layout(int nx, int* E, int* NW) {
+  const int e1 = lanes_e(nx);
+  *NW = e1 >= 8 ? 4 : (e1 >= 4 ? 2 : 1);
+  *E = e1 / *NW;
+}
+
+/* The GPU's wavefront scan, lane by lane.  A line is padded to WT = 64 E NW
+ * cells in "slot order" u = 0 .. WT-1: forward x = u, backward
+ * x = WT - 1 - u, so the right-to-left recurrence also runs in slot order.
+ * Wavefront w owns slots [64 E w, 64 E (w+1)), its lane l the E slots
+ * 64 E w + l E + k (k = 0 .. E-1).  Cells past nx are padding with
+ * a = c = 0.
  * A cell's map: v = a v_prev + c, c = fma(-p, carry, q) * inv (forward:
  * p = ls, carry = w of the same cell on line y-1, q = r; backward: p = ls of
- * line y+1, carry = z of the same cell on line y+1, q = w).  The lane's map
- * (A, C) is composed in k order (C = fma(a, C, c); A = a*A), then a scan in
- * lane order: shifts 1, 2, 4, 8 inside rows of 16 lanes, then lane 16r-1
- * into rows 1 and 3, then lane 31 into rows 2 and 3; a lane that receives
- * nothing in a step combines with the identity (1, 0); each step combines
- * C = fma(A, Cp, C), A = A*Ap from the values before the step.  The value
- * entering lane l is C[l-1] (0 for lane 0), and the lane re-applies its maps
- * v = fma(a, v, c) in k order. */
+ * line y+1, carry = z of the same cell on line y+1, q = w).  In each
+ * wavefront the lane's map (A, C) is composed in k order (C = fma(a, C, c);
+ * A = a*A), then scanned in lane order: shifts 1, 2, 4, 8 inside rows of 16
+ * lanes, then lane 16r-1 into rows 1 and 3, then lane 31 into rows 2 and 3;
+ * a lane that receives nothing in a step combines with the identity (1, 0);
+ * each step combines C = fma(A, Cp, C), A = A*Ap from the values before the
+ * step.  The value entering lane l: in wavefront 0, C[l-1] (0 for lane 0);
+ * in wavefront w > 0, fma(A[l-1], vin, C[l-1]) with (A, C)[-1] = (1, 0) and
+ * vin = the value of wavefront w-1's last slot on the same line.  The lane
+ * then re-applies its maps v = fma(a, v, c) in k order. */
 #define ORACLE_LSCAN(NAME, T, FMA)                                                              \
   static void NAME##_scan(T* A, T* C) {                                                         \
     T A2[64], C2[64];                                                                           \
@@ -136,20 +150,21 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
       memcpy(C, C2, sizeof(C2));                                                                \
     }                                                                                           \
   }                                                                                             \
-  static void NAME##_sweep(int nx, int ny, int E, int bwd, const T* a_, const T* ls, const T* q, \
+  static void NAME##_sweep(int nx, int ny, int bwd, const T* a_, const T* ls, const T* q,       \
                            const T* inv, T* out) {                                              \
-    const int W = 64 * E;                                                                       \
-    T* carry = (T*)calloc((size_t)W, sizeof(T));                                                \
-    T* pn = (T*)calloc((size_t)W, sizeof(T));                                                   \
-    T* a = (T*)calloc((size_t)W, sizeof(T));                                                    \
-    T* c = (T*)calloc((size_t)W, sizeof(T));                                                    \
+    int E, NW;                                                                                  \
+    lscan_layout(nx, &E, &NW);                                                                  \
+    const int W = 64 * E, WT = NW * W;                                                          \
+    T* carry = (T*)calloc((size_t)WT, sizeof(T));                                               \
+    T* pn = (T*)calloc((size_t)WT, sizeof(T));                                                  \
+    T* a = (T*)calloc((size_t)WT, sizeof(T));                                                   \
+    T* c = (T*)calloc((size_t)WT, sizeof(T));                                                   \
     T A[64], C[64];                                                                             \
     for (int t = 0; t < ny; ++t) {                                                              \
       const int y = bwd ? ny - 1 - t : t;                                                       \
       const long row = (long)y * nx;                                                            \
-      /* slot u = l*E + k in lane order; cell x(u) */                                           \
-      for (int u = 0; u < W; ++u) {                                                             \
-        const int x = bwd ? W - 1 - u : u;                                                      \
+      for (int u = 0; u < WT; ++u) {                                                            \
+        const int x = bwd ? WT - 1 - u : u;                                                     \
         if (x < nx) {                                                                           \
           a[u] = a_[row + x];                                                                   \
           const T p = bwd ? pn[u] : ls[row + x];                                                \
@@ -160,21 +175,36 @@ void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F3
           if (bwd) pn[u] = 0;                                                                   \
         }                                                                                       \
       }                                                                                         \
-      for (int l = 0; l < 64; ++l) {                                                            \
-        T Al = 1, Cl = 0;                                                                       \
-        for (int k = 0; k < E; ++k) { const int u = l * E + k; Cl = FMA(a[u], Cl, c[u]); Al = a[u] * Al; } \
-        A[l] = Al; C[l] = Cl;                                                                   \
-      }                                                                                         \
-      NAME##_scan(A, C);                                                                        \
-      for (int l = 0; l < 64; ++l) {                                                            \
-        T v = l > 0 ? C[l - 1] : (T)0;                                                          \
-        for (int k = 0; k < E; ++k) {                                                           \
-          const int u = l * E + k;                                                              \
-          const int x = bwd ? W - 1 - u : u;                                                    \
-          v = FMA(a[u], v, c[u]);                                                               \
-          carry[u] = v;                                                                         \
-          if (x < nx) out[row + x] = v;                                                         \
+      T vin = 0;                                                                                \
+      for (int w = 0; w < NW; ++w) {                                                            \
+        const int u0 = w * W;                                                                   \
+        for (int l = 0; l < 64; ++l) {                                                          \
+          T Al = 1, Cl = 0;                                                                     \
+          for (int k = 0; k < E; ++k) {                                                         \
+            const int u = u0 + l * E + k;                                                       \
+            Cl = FMA(a[u], Cl, c[u]);                                                           \
+            Al = a[u] * Al;                                                                     \
+          }                                                                                     \
+          A[l] = Al; C[l] = Cl;                                                                 \
         }                                                                                       \
+        NAME##_scan(A, C);                                                                      \
+        T vlast = 0;                                                                            \
+        for (int l = 0; l < 64; ++l) {                                                          \
+          T v;                                                                                  \
+          if (w == 0)                                                                           \
+            v = l > 0 ? C[l - 1] : (T)0;                                                        \
+          else                                                                                  \
+            v = FMA(l > 0 ? A[l - 1] : (T)1, vin, l > 0 ? C[l - 1] : (T)0);                     \
+          for (int k = 0; k < E; ++k) {                                                         \
+            const int u = u0 + l * E + k;                                                       \
+            const int x = bwd ? WT - 1 - u : u;                                                 \
+            v = FMA(a[u], v, c[u]);                                                             \
+            carry[u] = v;                                                                       \
+            if (x < nx) out[row + x] = v;                                                       \
+          }                                                                                     \
+          vlast = v;                                                                            \
+        }                                                                                       \
+        vin = vlast;                                                                            \
       }                                                                                         \
     }                                                                                           \
     free(carry); free(pn); free(a); free(c);                                                    \
@@ -187,13 +217,12 @@ ORACLE_LSCAN(lscan32, float, fmaf)
 void oracle_lscan_apply_strips(int nx, int ny, int nstrip, const double* F, const double* r, double* z) {
   const long n = (long)nx * ny;
   double* w = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
-  const int E = lanes_e(nx);
   for (int k = 0; k < nstrip; ++k) {
     const int y0 = (int)((long)ny * k / nstrip), y1 = (int)((long)ny * (k + 1) / nstrip);
     if (y1 <= y0) continue;
     const long o = (long)y0 * nx;
-    lscan64_sweep(nx, y1 - y0, E, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
-    lscan64_sweep(nx, y1 - y0, E, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
+    lscan64_sweep(nx, y1 - y0, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
+    lscan64_sweep(nx, y1 - y0, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
   }
   free(w);
 }
@@ -201,13 +230,12 @@ void oracle_lscan_apply_strips(int nx, int ny, int nstrip, const double* F, cons
 void oracle_lscan_apply32_strips(int nx, int ny, int nstrip, const float* F, const float* r, float* z) {
   const long n = (long)nx * ny;
   float* w = (float*)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
-  const int E = lanes_e(nx);
   for (int k = 0; k < nstrip; ++k) {
     const int y0 = (int)((long)ny * k / nstrip), y1 = (int)((long)ny * (k + 1) / nstrip);
     if (y1 <= y0) continue;
     const long o = (long)y0 * nx;
-    lscan32_sweep(nx, y1 - y0, E, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
-    lscan32_sweep(nx, y1 - y0, E, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
+    lscan32_sweep(nx, y1 - y0, 0, F + 2 * n + o, F + o, r + o, F + n + o, w + o);
+    lscan32_sweep(nx, y1 - y0, 1, F + 3 * n + o, F + o, w + o, F + n + o, z + o);
   }
   free(w);
 }

@@ -35,6 +35,7 @@ struct Batch {
   DevBuf<double> L64;            // 3*nsys*n: Ls, Lw, Ld
   DevBuf<uint16_t> Lbf;          // 3*nsys*n bf16
   DevBuf<double> F64;            // 4*nsys*n: ls, 1/Ld, a_fwd, a_bwd
+  DevBuf<double> sink;           // 1024: the factor apply's filler and padding stores (never read)
   DevBuf<float> F32;
   DevBuf<double> r, q, s, p0, p1, w, xb, bb;
   DevBuf<float> b32, d32, r32, q32, s32, p032, p132, w32;

@@ -13,13 +13,16 @@
 //     w_x = a_x w_{x-1} + c_x,  a_x = -Lw_x / Ld_x,
 //     c_x = (r_x - Ls_x w_x^{(y-1)}) / Ld_x
 // is a first-order linear recurrence along the line once line y-1 is known,
-// i.e. a scan of affine maps.  One wavefront per system walks the lines; lane
-// l owns E = nx/64 consecutive cells, composes its E maps, then an inclusive
-// 64-lane scan (Hillis-Steele, shuffles 1..32) gives each lane the value
-// entering its cells, and the lane re-applies its maps.  The previous line
-// stays in registers (each lane reads only the cells it owns), so the
-// recurrence never waits on memory.  The backward solve L^T z = w runs the
-// lines top-down with the mirrored (suffix) scan.
+// i.e. a scan of affine maps.  A workgroup of up to 4 wavefronts per strip of
+// lines walks the lines; each wavefront owns a piece of every line, its lane
+// l E consecutive cells: the lane composes its E maps, an inclusive 64-lane
+// scan (DPP row shifts and broadcasts) gives each lane the value entering
+// its cells from the piece's start, the piece's entering value comes from
+// the wavefront before (an LDS ring, the wavefronts pipelined a few lines
+// apart), and the lane re-applies its maps.  The previous line's values a
+// lane needs are its own (registers), and later lines' loads are issued
+// ahead.  The backward solve L^T z = w runs the lines top-down with the
+// mirrored layout.
 #include <omp.h>
 
 #include <algorithm>
@@ -139,8 +142,8 @@ __device__ __forceinline__ bool sys_done(const CGState* st, int sys, int it) {
 // 160 of them stay in flight.
 template <class T, class TQ, int E>
 constexpr int lines_ahead() {
-  return E * (3 * (int)sizeof(T) + (int)sizeof(TQ)) / 4 >= 96 ? 1
-         : (E * (3 * (int)sizeof(T) + (int)sizeof(TQ)) / 4 >= 48 ? 2 : 4);
+  constexpr int regs = E * (3 * (int)sizeof(T) + (int)sizeof(TQ)) / 4;
+  return regs >= 96 ? 1 : (regs >= 48 ? 2 : (regs >= 24 ? 4 : 8));
 }
 template <class T, class TQ, int E>
 struct LineRegs {
@@ -149,7 +152,8 @@ struct LineRegs {
 };
 
 // The lane's E cells x0 .. x0+E-1 of one line; MIRROR stores them in reverse
-// (slot k holds cell x0+E-1-k), which the backward sweep uses.
+// (slot k holds cell x0+E-1-k), which the backward sweep uses.  Padding cells
+// (x >= nx) load the line's last cell and take zeros (no branch).
 template <class T, class TQ, int E, bool FULL, bool MIRROR>
 __device__ __forceinline__ void load_line(LineRegs<T, TQ, E>& L, const T* __restrict__ a, const T* __restrict__ p,
                                           const TQ* __restrict__ q, const T* __restrict__ inv, long row, int x0,
@@ -157,10 +161,14 @@ __device__ __forceinline__ void load_line(LineRegs<T, TQ, E>& L, const T* __rest
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int k = MIRROR ? E - 1 - j : j;
-    if (FULL || x0 + j < nx) {
+    if (FULL) {
       L.a[k] = a[row + x0 + j]; L.p[k] = p[row + x0 + j]; L.q[k] = q[row + x0 + j]; L.i[k] = inv[row + x0 + j];
     } else {
-      L.a[k] = 0; L.p[k] = 0; L.q[k] = 0; L.i[k] = 0;
+      const bool in = x0 + j < nx;
+      const long c = row + min(x0 + j, nx - 1);
+      const T av = a[c], pv = p[c], iv = inv[c];
+      const TQ qv = q[c];
+      L.a[k] = in ? av : (T)0; L.p[k] = in ? pv : (T)0; L.q[k] = in ? qv : (TQ)0; L.i[k] = in ? iv : (T)0;
     }
   }
 }
@@ -192,24 +200,30 @@ __device__ __forceinline__ void scan_step(T& A, T& C) {
 // Inclusive scan of affine maps over the 64 lanes in lane order: shifts
 // 1, 2, 4, 8 inside each row of 16, then row 0's last map into rows 1 and 3
 // (row_bcast:15) and lane 31's into rows 2 and 3 (row_bcast:31).  Returns
-// the value entering the lane's cells (lane 0: 0).
+// the value entering the lane's cells: the maps of the lanes before it
+// applied to vin when `chained` (lane 0: vin itself), else to 0.
 template <class T>
-__device__ __forceinline__ T scan_maps(T A, T C) {
+__device__ __forceinline__ T scan_maps(T A, T C, bool chained, T vin) {
   scan_step<0x111, 0xf>(A, C);
   scan_step<0x112, 0xf>(A, C);
   scan_step<0x114, 0xf>(A, C);
   scan_step<0x118, 0xf>(A, C);
   scan_step<0x142, 0xa>(A, C);
   scan_step<0x143, 0xc>(A, C);
-  return dpp<0x138, 0xf>((T)0, C);
+  const T Ce = dpp<0x138, 0xf>((T)0, C);
+  const T Cv = fma(dpp<0x138, 0xf>((T)1, A), vin, Ce);
+  return chained ? Cv : Ce;  // a select: no branch inside the line
 }
 
 // One line of a sweep: v = a v_prev + c along the lane's cells, with
 // c = fma(-p, carry, q) * inv, carry = the value of the same cell on the
 // previous line, p = ls of that previous line for the backward sweep.
+// Returns the value of the lane's last cell.  A line that is not `valid`
+// (a pipeline filler) leaves carry and pn as they are (selects, no branch);
+// the caller points its stores at the sink, where padding cells store too.
 template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
-__device__ __forceinline__ void sweep_line(const LineRegs<T, TQ, E>& L, T (&carry)[E], T (&pn)[E], int x0, int nx,
-                                           TO* __restrict__ orow) {
+__device__ __forceinline__ T sweep_line(const LineRegs<T, TQ, E>& L, T (&carry)[E], T (&pn)[E], int x0, int nx,
+                                        TO* orow, TO* sink, bool chained, T vin, bool valid) {
   T c[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
@@ -218,7 +232,7 @@ __device__ __forceinline__ void sweep_line(const LineRegs<T, TQ, E>& L, T (&carr
       // forward: p is this line's ls; backward: the previous (upper) line's
       const T pk = BWD ? pn[k] : L.p[k];
       c[k] = fma(-pk, carry[k], (T)L.q[k]) * L.i[k];
-      if (BWD) pn[k] = L.p[k];
+      if (BWD) pn[k] = valid ? L.p[k] : pn[k];
     } else {
       c[k] = 0;
       if (BWD) pn[k] = 0;
@@ -227,65 +241,110 @@ __device__ __forceinline__ void sweep_line(const LineRegs<T, TQ, E>& L, T (&carr
   T A = 1, C = 0;
 #pragma unroll
   for (int k = 0; k < E; ++k) { C = fma(L.a[k], C, c[k]); A = L.a[k] * A; }
-  T v = scan_maps(A, C);
+  T v = scan_maps(A, C, chained, vin);
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int x = BWD ? x0 + E - 1 - k : x0 + k;
     v = fma(L.a[k], v, c[k]);
-    carry[k] = v;
-    if (FULL || x < nx) orow[x] = (TO)v;
+    carry[k] = valid ? v : carry[k];
+    // a padding cell (x < 64 E NW <= 1024) stores to the sink
+    if (FULL)
+      orow[x] = (TO)v;
+    else
+      *(x < nx ? orow + x : sink + x) = (TO)v;
   }
+  return v;
 }
 
-// One sweep over the lines (forward: y = 0 .. ny-1, lane cells in increasing
-// x; backward: y = ny-1 .. 0, lane cells mirrored so the recurrence still
-// runs in lane order).  The main loop takes kAhead lines per trip without
-// branches (the prefetch of a line past the end reloads the last line) and
-// with a scheduling barrier between lines, so the compiler's wait before a
-// line's first use counts only that line's loads; the last ny mod kAhead
-// lines follow from the registers.
-template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
+// Lines per pipeline step of a multi-wavefront sweep (a multiple of every
+// lines_ahead value).
+constexpr int kLineBlk = 8;
+
+// One sweep over the lines of a strip by the NW wavefronts of the workgroup
+// (forward: y = 0 .. ny-1, cells in increasing x; backward: y = ny-1 .. 0,
+// cells mirrored so the recurrence still runs in lane order).  A line is cut
+// into NW pieces of 64 E cells in recurrence order; wavefront wv owns piece
+// wv and starts each line from the last value of piece wv-1 on the same
+// line.  The wavefronts run a pipeline of kLineBlk-line steps: at step s
+// wavefront wv takes lines (s - wv) kLineBlk + j, j < kLineBlk, and hands
+// every lane's last values to wavefront wv+1 through the LDS ring (two steps
+// deep, slot (s kLineBlk + j) mod 2 kLineBlk), a barrier between steps.
+// Every step runs its kLineBlk lines without a branch, so the compiler's
+// wait before a line's first use counts only that line's loads: lines
+// outside [0, ny) (a wavefront's first wv and last NW-1-wv steps, the end of
+// the last step) are fillers that load the nearest line, store to `sink` (a
+// line of scratch no one reads) and leave the recurrence state alone.  The
+// loads of line t + kAhead are issued as soon as line t is done with its
+// registers, with a scheduling barrier between lines.
+template <class T, class TQ, class TO, int E, int NW, bool FULL, bool BWD>
 __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, const T* __restrict__ ls,
-                                      const TQ* __restrict__ q, const T* __restrict__ inv, TO* __restrict__ out) {
+                                      const TQ* __restrict__ q, const T* __restrict__ inv, TO* __restrict__ out,
+                                      TO* __restrict__ sink, T (*ring)[2 * kLineBlk][64]) {
   constexpr int kAhead = lines_ahead<T, TQ, E>();
-  const int lane = threadIdx.x;
-  const int x0 = BWD ? 64 * E - (lane + 1) * E : lane * E;
-  auto row_of = [&](int t) -> long { return (long)(BWD ? ny - 1 - t : t) * nx; };
+  static_assert(kLineBlk % kAhead == 0, "pipeline steps hold whole prefetch groups");
+  constexpr int W = 64 * E;  // cells per wavefront piece
+  // the wavefront index is uniform in the wavefront: readfirstlane lets the
+  // compiler keep it (and what derives from it) in scalar registers
+  const int wv = NW > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+  const int lane = threadIdx.x & 63;
+  const int x0 = BWD ? NW * W - wv * W - (lane + 1) * E : wv * W + lane * E;
+  const bool chained = wv > 0;
+  auto row_of = [&](int t) -> long {
+    t = min(max(t, 0), ny - 1);
+    return (long)(BWD ? ny - 1 - t : t) * nx;
+  };
   T carry[E], pn[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) { carry[k] = 0; pn[k] = 0; }
   LineRegs<T, TQ, E> buf[kAhead];
+  const int tw = -wv * kLineBlk;  // this wavefront's first line (its fillers first)
+  // issued in line order (the loop's first wait then counts the later
+  // lines' loads, as on every later trip)
 #pragma unroll
-  for (int d = 0; d < kAhead; ++d)
-    load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(min(d, ny - 1)), x0, nx);
-  int t0 = 0;
-  for (; t0 + kAhead <= ny; t0 += kAhead) {
+  for (int d = 0; d < kAhead; ++d) {
+    load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(tw + d), x0, nx);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int nsteps = (ny + kLineBlk - 1) / kLineBlk + NW - 1;
+  for (int step = 0; step < nsteps; ++step) {
+    const int tb = (step - wv) * kLineBlk;
 #pragma unroll
-    for (int d = 0; d < kAhead; ++d) {
-      const int t = t0 + d;
-      sweep_line<T, TQ, TO, E, FULL, BWD>(buf[d], carry, pn, x0, nx, out + row_of(t));
-      load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(min(t + kAhead, ny - 1)), x0, nx);
+    for (int j = 0; j < kLineBlk; ++j) {
+      const int t = tb + j;
+      const bool valid = t >= 0 && t < ny;
+      const int slot = (step * kLineBlk + j) % (2 * kLineBlk);
+      // the piece before this one, written by wavefront wv-1 a step earlier
+      // (wavefront 0 reads a slot too and ignores it)
+      const T vin = NW > 1 ? ring[chained ? wv - 1 : 0][(slot + kLineBlk) % (2 * kLineBlk)][63] : (T)0;
+      // the row offset is computed on every path, so the store target is a
+      // select of two global pointers (no branch, no flat store)
+      const long off = row_of(t);
+      TO* orow = valid ? out + off : sink;
+      const T vl = sweep_line<T, TQ, TO, E, FULL, BWD>(buf[j % kAhead], carry, pn, x0, nx, orow, sink, chained,
+                                                       vin, valid);
+      if (NW > 1) ring[wv][slot][lane] = vl;
+      load_line<T, TQ, E, FULL, BWD>(buf[j % kAhead], a, ls, q, inv, row_of(t + kAhead), x0, nx);
       // keep the next line's work (its q conversions above all) from being
       // hoisted into this one, where it would wait on loads just issued
       __builtin_amdgcn_sched_barrier(0);
     }
+    if (NW > 1) __syncthreads();
   }
-#pragma unroll
-  for (int d = 0; d < kAhead - 1; ++d)
-    if (t0 + d < ny) sweep_line<T, TQ, TO, E, FULL, BWD>(buf[d], carry, pn, x0, nx, out + row_of(t0 + d));
 }
 
 // z = (L L^T)^-1 r for every system: forward sweep into w, then the
-// backward sweep into z, one wavefront per strip of grid lines.  With nstrip
-// strips per system, strip k holds lines [ny k / nstrip, ny (k+1) / nstrip)
-// and drops the factor's couplings to the neighbouring strips (block-diagonal
-// M = diag(L_k L_k^T)); nstrip = 1 is the exact M = L L^T.  T: arithmetic and
-// factor type; TV: the type of r and z (double with T = float: the FP32
-// apply of an FP64 PCG).
-template <class T, class TV, int E, bool FULL>
-__global__ void __launch_bounds__(64) k_lscan(int nx, int ny, int nstrip, long n, long nv, const T* __restrict__ F,
-                                              const TV* __restrict__ r, T* __restrict__ w, TV* __restrict__ z,
-                                              const CGState* st, int it) {
+// backward sweep into z, one workgroup of NW wavefronts per strip of grid
+// lines.  With nstrip strips per system, strip k holds lines
+// [ny k / nstrip, ny (k+1) / nstrip) and drops the factor's couplings to the
+// neighbouring strips (block-diagonal M = diag(L_k L_k^T)); nstrip = 1 is
+// the exact M = L L^T.  T: arithmetic and factor type; TV: the type of r
+// and z (double with T = float: the FP32 apply of an FP64 PCG).
+template <class T, class TV, int E, int NW, bool FULL>
+__global__ void __launch_bounds__(64 * NW) k_lscan(int nx, int ny, int nstrip, long n, long nv,
+                                                   const T* __restrict__ F, const TV* __restrict__ r,
+                                                   T* __restrict__ w, TV* __restrict__ z, const CGState* st, int it,
+                                                   double* __restrict__ sink) {
+  __shared__ T ring[NW][2 * kLineBlk][64];
   const int sys = blockIdx.x / nstrip, strip = blockIdx.x - sys * nstrip;
   if (sys_done(st, sys, it)) return;
   const int y0 = (int)((long)ny * strip / nstrip), y1 = (int)((long)ny * (strip + 1) / nstrip);
@@ -295,10 +354,10 @@ __global__ void __launch_bounds__(64) k_lscan(int nx, int ny, int nstrip, long n
   const T* inv = F + nv + so;
   const T* af = F + 2 * nv + so;
   const T* ab = F + 3 * nv + so;
-  sweep<T, TV, T, E, FULL, false>(nx, y1 - y0, af, ls, r + so, inv, w + so);
+  sweep<T, TV, T, E, NW, FULL, false>(nx, y1 - y0, af, ls, r + so, inv, w + so, (T*)sink, ring);
   // the backward sweep reads w cells written by other lanes
   __syncthreads();
-  sweep<T, T, TV, E, FULL, true>(nx, y1 - y0, ab, ls, w + so, inv, z + so);
+  sweep<T, T, TV, E, NW, FULL, true>(nx, y1 - y0, ab, ls, w + so, inv, z + so, (TV*)sink, ring);
 }
 
 // s = Dinv r (dinv null: s = r), every system
@@ -396,19 +455,22 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
 template <class T, class TV>
 void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TV* z, const CGState* st, int it) {
   const int nx = B.nx;
-  const dim3 g(B.nsys * B.nstrip), b(64);
-#define PARCEL_LSCAN(E)                                                                                          \
-  if (nx == 64 * (E) && ((B.n * sizeof(T)) % 16 == 0))                                                           \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, true>), g, b, 0, B.stream, nx, B.ny, B.nstrip, (long)B.n, B.nv(), F, r, \
-                       w, z, st, it);                                                                            \
-  else                                                                                                           \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, false>), g, b, 0, B.stream, nx, B.ny, B.nstrip, (long)B.n, B.nv(), F,   \
-                       r, w, z, st, it);
-  if (nx <= 64) { PARCEL_LSCAN(1) }
-  else if (nx <= 128) { PARCEL_LSCAN(2) }
-  else if (nx <= 256) { PARCEL_LSCAN(4) }
-  else if (nx <= 512) { PARCEL_LSCAN(8) }
-  else { PARCEL_LSCAN(16) }
+  const dim3 g(B.nsys * B.nstrip);
+  // line layout (the oracle's lscan_layout): E1 = cells per lane of one
+  // wavefront, a power of two with 64 E1 >= nx; E1 >= 8 splits the line
+  // over 4 wavefronts, E1 = 4 over 2
+#define PARCEL_LSCAN(E, NW)                                                                                     \
+  if (nx == 64 * (E) * (NW) && ((B.n * sizeof(T)) % 16 == 0))                                                  \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, true>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,      \
+                       (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);                                       \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, false>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,     \
+                       (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);
+  if (nx <= 64) { PARCEL_LSCAN(1, 1) }
+  else if (nx <= 128) { PARCEL_LSCAN(2, 1) }
+  else if (nx <= 256) { PARCEL_LSCAN(2, 2) }
+  else if (nx <= 512) { PARCEL_LSCAN(2, 4) }
+  else { PARCEL_LSCAN(4, 4) }
 #undef PARCEL_LSCAN
   PARCEL_HIP_CHECK(hipGetLastError());
 }
@@ -760,7 +822,10 @@ int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const in
   S.sk.upload(sk, B.stream);
   S.col.upload(scol, B.stream);
   B.diag_slot.upload(dslot, B.stream);
-  if (B.nx) B.nbr_slot.upload(nslot, B.stream);
+  if (B.nx) {
+    B.nbr_slot.upload(nslot, B.stream);
+    B.sink.alloc(1024);  // one line (nx <= 1024) of the factor apply's filler and padding stores
+  }
   DevBuf<int> didx;
   didx.upload(idx, B.stream);
   // values: CSR (host or device) -> SELL order on the device

@@ -115,7 +115,9 @@ def test_ic0_reproduces_a_on_its_pattern():
     assert abs(D).max() < 1e-12 * abs(A).max()
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 8), (100, 37), (512, 4), (1000, 3), (1, 50)])
+# line layouts: one wavefront (nx <= 128), two (128 < nx <= 256), four
+# (nx > 256: E = 2 or 4 cells per lane, 300 leaves a whole wavefront of padding)
+@pytest.mark.parametrize("nx,ny", [(64, 8), (100, 37), (200, 6), (256, 4), (300, 5), (512, 4), (1000, 3), (1, 50)])
 def test_lscan_solves_llt(nx, ny):
     rp, col, val = five_point(nx, ny)
     n = nx * ny
