@@ -122,9 +122,8 @@ static void lscan_layout(int nx, int* E, int* NW) {
  * wavefront the lane's map (A, C) is composed in k order (C = fma(a, C, c);
  * A = a*A), then scanned in lane order: shifts 1, 2, 4, 8 inside rows of 16
  * lanes, then lane 16r-1 into rows 1 and 3, then lane 31 into rows 2 and 3;
- * a lane that receives nothing in a step combines with the identity (1, 0);
- * each step combines C = fma(A, Cp, C), A = A*Ap from the values before the
- * step.  The value entering lane l: in wavefront 0, C[l-1] (0 for lane 0);
+ * a lane that receives nothing in a step keeps its map; the others combine
+ * C = fma(A, Cp, C), A = A*Ap from the values before the step.  The value entering lane l: in wavefront 0, C[l-1] (0 for lane 0);
  * in wavefront w > 0, fma(A[l-1], vin, C[l-1]) with (A, C)[-1] = (1, 0) and
  * vin = the value of wavefront w-1's last slot on the same line.  The lane
  * then re-applies its maps v = fma(a, v, c) in k order. */
@@ -142,9 +141,13 @@ static void lscan_layout(int nx, int* E, int* NW) {
         } else {                                                                                \
           if (l / 16 >= 2) src = 31;                                                            \
         }                                                                                       \
-        const T Ap = src >= 0 ? A[src] : (T)1, Cp = src >= 0 ? C[src] : (T)0;                   \
-        C2[l] = FMA(A[l], Cp, C[l]);                                                            \
-        A2[l] = A[l] * Ap;                                                                      \
+        if (src >= 0) {                                                                         \
+          C2[l] = FMA(A[l], C[src], C[l]);                                                      \
+          A2[l] = A[l] * A[src];                                                                \
+        } else {                                                                                \
+          C2[l] = C[l];                                                                         \
+          A2[l] = A[l];                                                                         \
+        }                                                                                       \
       }                                                                                         \
       memcpy(A, A2, sizeof(A2));                                                                \
       memcpy(C, C2, sizeof(C2));                                                                \

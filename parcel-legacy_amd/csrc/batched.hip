@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "batch.hpp"
@@ -151,21 +152,31 @@ struct LineRegs {
   TQ q[E];
 };
 
-// The lane's E cells x0 .. x0+E-1 of one line; MIRROR stores them in reverse
-// (slot k holds cell x0+E-1-k), which the backward sweep uses.  Padding cells
-// (x >= nx) load the line's last cell and take zeros (no branch).
+// base + i elements with a 32-bit byte offset (a strip's arrays are far
+// below 4 GiB): the access takes the uniform base in scalar registers and a
+// 32-bit lane offset
+template <class T>
+__device__ __forceinline__ T* at(T* base, unsigned i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return (T*)((B*)base + i * (unsigned)sizeof(T));
+}
+
+// The lane's E cells of one line: elements i0 .. i0+E-1 of the strip's
+// arrays (i0 = line start + x0); MIRROR stores them in reverse (slot k holds
+// cell x0+E-1-k), which the backward sweep uses.  Padding cells (x >= nx)
+// load the line's last cell and take zeros (no branch).
 template <class T, class TQ, int E, bool FULL, bool MIRROR>
 __device__ __forceinline__ void load_line(LineRegs<T, TQ, E>& L, const T* __restrict__ a, const T* __restrict__ p,
-                                          const TQ* __restrict__ q, const T* __restrict__ inv, long row, int x0,
+                                          const TQ* __restrict__ q, const T* __restrict__ inv, unsigned i0, int x0,
                                           int nx) {
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const int k = MIRROR ? E - 1 - j : j;
     if (FULL) {
-      L.a[k] = a[row + x0 + j]; L.p[k] = p[row + x0 + j]; L.q[k] = q[row + x0 + j]; L.i[k] = inv[row + x0 + j];
+      L.a[k] = at(a, i0)[j]; L.p[k] = at(p, i0)[j]; L.q[k] = at(q, i0)[j]; L.i[k] = at(inv, i0)[j];
     } else {
       const bool in = x0 + j < nx;
-      const long c = row + min(x0 + j, nx - 1);
+      const unsigned c = in ? i0 + j : i0 - x0 + nx - 1;
       const T av = a[c], pv = p[c], iv = inv[c];
       const TQ qv = q[c];
       L.a[k] = in ? av : (T)0; L.p[k] = in ? pv : (T)0; L.q[k] = in ? qv : (TQ)0; L.i[k] = in ? iv : (T)0;
@@ -188,28 +199,69 @@ __device__ __forceinline__ double dpp(double old, double v) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Whether lane `lane` receives a map in the scan step with DPP control CTRL
+// (row_shr:n inside rows of 16; row_bcast:15 into rows 1 and 3;
+// row_bcast:31 into rows 2 and 3).
+template <int CTRL>
+__device__ __forceinline__ bool scan_has_source(int lane) {
+  if (CTRL == 0x142) return (lane >> 4) & 1;
+  if (CTRL == 0x143) return lane >= 32;
+  return (lane & 15) >= (CTRL - 0x110);
+}
+
 // One scan step: the lane's map (A, C) after the map (Ap, Cp) moved in by
-// CTRL; lanes that receive nothing get the identity (1, 0).
+// CTRL; a lane that receives nothing keeps its map.
 template <int CTRL, int RM, class T>
 __device__ __forceinline__ void scan_step(T& A, T& C) {
   const T Ap = dpp<CTRL, RM>((T)1, A), Cp = dpp<CTRL, RM>((T)0, C);
-  C = fma(A, Cp, C);
+  const T Cn = fma(A, Cp, C);
   A = A * Ap;
+  C = scan_has_source<CTRL>(threadIdx.x & 63) ? Cn : C;
 }
 
 // Inclusive scan of affine maps over the 64 lanes in lane order: shifts
 // 1, 2, 4, 8 inside each row of 16, then row 0's last map into rows 1 and 3
-// (row_bcast:15) and lane 31's into rows 2 and 3 (row_bcast:31).  Returns
-// the value entering the lane's cells: the maps of the lanes before it
-// applied to vin when `chained` (lane 0: vin itself), else to 0.
+// (row_bcast:15) and lane 31's into rows 2 and 3 (row_bcast:31).
 template <class T>
-__device__ __forceinline__ T scan_maps(T A, T C, bool chained, T vin) {
+__device__ __forceinline__ void scan_inclusive(T& A, T& C) {
   scan_step<0x111, 0xf>(A, C);
   scan_step<0x112, 0xf>(A, C);
   scan_step<0x114, 0xf>(A, C);
   scan_step<0x118, 0xf>(A, C);
   scan_step<0x142, 0xa>(A, C);
   scan_step<0x143, 0xc>(A, C);
+}
+
+// FP32: each step is one FMA and one multiply that take the moved-in operand
+// straight from DPP (C += C(src) A, A *= A(src)); a lane without a source
+// (bound_ctrl off, or its row masked) is not written and keeps its map, as
+// above, with half the dependent instructions.  A DPP read needs two wait
+// states after a VALU write of its register: the order fma, mul, s_nop 0
+// gives each of C and A exactly two between its write and its next read.
+template <>
+__device__ __forceinline__ void scan_inclusive<float>(float& A, float& C) {
+#define PARCEL_SCAN_STEP(CTL)                                           \
+  "v_fmac_f32_dpp %0, %0, %1 " CTL " bank_mask:0xf\n\t"                \
+  "v_mul_f32_dpp %1, %1, %1 " CTL " bank_mask:0xf\n\t"                 \
+  "s_nop 0\n\t"
+  asm volatile(
+      "s_nop 1\n\t"
+      PARCEL_SCAN_STEP("row_shr:1 row_mask:0xf")
+      PARCEL_SCAN_STEP("row_shr:2 row_mask:0xf")
+      PARCEL_SCAN_STEP("row_shr:4 row_mask:0xf")
+      PARCEL_SCAN_STEP("row_shr:8 row_mask:0xf")
+      PARCEL_SCAN_STEP("row_bcast:15 row_mask:0xa")
+      PARCEL_SCAN_STEP("row_bcast:31 row_mask:0xc")
+      "s_nop 0"
+      : "+v"(C), "+v"(A));
+#undef PARCEL_SCAN_STEP
+}
+
+// Returns the value entering the lane's cells: the scanned maps of the lanes
+// before it applied to vin when `chained` (lane 0: vin itself), else to 0.
+template <class T>
+__device__ __forceinline__ T scan_maps(T A, T C, bool chained, T vin) {
+  scan_inclusive(A, C);
   const T Ce = dpp<0x138, 0xf>((T)0, C);
   const T Cv = fma(dpp<0x138, 0xf>((T)1, A), vin, Ce);
   return chained ? Cv : Ce;  // a select: no branch inside the line
@@ -223,7 +275,7 @@ __device__ __forceinline__ T scan_maps(T A, T C, bool chained, T vin) {
 // the caller points its stores at the sink, where padding cells store too.
 template <class T, class TQ, class TO, int E, bool FULL, bool BWD>
 __device__ __forceinline__ T sweep_line(const LineRegs<T, TQ, E>& L, T (&carry)[E], T (&pn)[E], int x0, int nx,
-                                        TO* orow, TO* sink, bool chained, T vin, bool valid) {
+                                        TO* obase, unsigned orow, TO* sink, bool chained, T vin, bool valid) {
   T c[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
@@ -247,11 +299,12 @@ __device__ __forceinline__ T sweep_line(const LineRegs<T, TQ, E>& L, T (&carry)[
     const int x = BWD ? x0 + E - 1 - k : x0 + k;
     v = fma(L.a[k], v, c[k]);
     carry[k] = valid ? v : carry[k];
-    // a padding cell (x < 64 E NW <= 1024) stores to the sink
+    // the cell is element orow + x of obase; a padding cell
+    // (x < 64 E NW <= 1024) stores to the sink
     if (FULL)
-      orow[x] = (TO)v;
+      at(obase, orow + (unsigned)x0)[x - x0] = (TO)v;
     else
-      *(x < nx ? orow + x : sink + x) = (TO)v;
+      *(x < nx ? obase + orow + x : sink + x) = (TO)v;
   }
   return v;
 }
@@ -289,9 +342,10 @@ __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, c
   const int lane = threadIdx.x & 63;
   const int x0 = BWD ? NW * W - wv * W - (lane + 1) * E : wv * W + lane * E;
   const bool chained = wv > 0;
-  auto row_of = [&](int t) -> long {
+  // first element of line t (clamped to the strip), 32-bit
+  auto row_of = [&](int t) -> unsigned {
     t = min(max(t, 0), ny - 1);
-    return (long)(BWD ? ny - 1 - t : t) * nx;
+    return (unsigned)((BWD ? ny - 1 - t : t) * nx);
   };
   T carry[E], pn[E];
 #pragma unroll
@@ -302,7 +356,7 @@ __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, c
   // lines' loads, as on every later trip)
 #pragma unroll
   for (int d = 0; d < kAhead; ++d) {
-    load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(tw + d), x0, nx);
+    load_line<T, TQ, E, FULL, BWD>(buf[d], a, ls, q, inv, row_of(tw + d) + (unsigned)x0, x0, nx);
     __builtin_amdgcn_sched_barrier(0);
   }
   const int nsteps = (ny + kLineBlk - 1) / kLineBlk + NW - 1;
@@ -316,14 +370,14 @@ __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, c
       // the piece before this one, written by wavefront wv-1 a step earlier
       // (wavefront 0 reads a slot too and ignores it)
       const T vin = NW > 1 ? ring[chained ? wv - 1 : 0][(slot + kLineBlk) % (2 * kLineBlk)][63] : (T)0;
-      // the row offset is computed on every path, so the store target is a
-      // select of two global pointers (no branch, no flat store)
-      const long off = row_of(t);
-      TO* orow = valid ? out + off : sink;
-      const T vl = sweep_line<T, TQ, TO, E, FULL, BWD>(buf[j % kAhead], carry, pn, x0, nx, orow, sink, chained,
-                                                       vin, valid);
+      // a filler's stores go to the sink's first line: the store base is a
+      // select of two global pointers and the row a select of two scalars
+      TO* obase = valid ? out : sink;
+      const unsigned orow = valid ? row_of(t) : 0u;
+      const T vl = sweep_line<T, TQ, TO, E, FULL, BWD>(buf[j % kAhead], carry, pn, x0, nx, obase, orow, sink,
+                                                       chained, vin, valid);
       if (NW > 1) ring[wv][slot][lane] = vl;
-      load_line<T, TQ, E, FULL, BWD>(buf[j % kAhead], a, ls, q, inv, row_of(t + kAhead), x0, nx);
+      load_line<T, TQ, E, FULL, BWD>(buf[j % kAhead], a, ls, q, inv, row_of(t + kAhead) + (unsigned)x0, x0, nx);
       // keep the next line's work (its q conversions above all) from being
       // hoisted into this one, where it would wait on loads just issued
       __builtin_amdgcn_sched_barrier(0);
