@@ -67,7 +67,11 @@ def test_batch_ic0_fp64_bit_exact(nx, ny, nsys):
         assert np.array_equal(x[s], xo), s
 
 
-@pytest.mark.parametrize("nx,ny", [(512, 5), (100, 9), (64, 4), (1000, 2)])
+# line layouts: one wavefront (64, 100), two (200 padded, 256), four (300 with
+# a whole wavefront of padding, 512, 1000); strips shorter and longer than the
+# 8-line pipeline step
+@pytest.mark.parametrize("nx,ny", [(512, 5), (100, 9), (64, 4), (1000, 2), (200, 7), (256, 19), (300, 11),
+                                   (512, 37)])
 def test_batch_precon_apply_bit_exact(nx, ny):
     print("batch apply", nx, ny, flush=True)
     nsys = 2
