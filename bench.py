@@ -237,11 +237,12 @@ def run_c4(args):
     }
 
 
-def lscan_bytes(n, nsys, t=4, tv=4):
+def lscan_bytes(n, nsys, t=4, tv=4, tz=None):
     """Factor apply per cell: forward reads r (tv), ls, 1/Ld, a_fwd and writes
-    w; backward reads w, 1/Ld, a_bwd, ls and writes z (tv); factor and w in t
-    bytes: 8t + 2tv (40 B in FP32, 48 B for the FP32 apply on FP64 vectors)."""
-    return (8 * t + 2 * tv) * n * nsys
+    w; backward reads w, 1/Ld, a_bwd, ls and writes z (tz, default tv);
+    factor and w in t bytes: 8t + tv + tz (40 B in FP32, 44 B for the FP32
+    apply on FP64 vectors with z kept in FP32)."""
+    return (8 * t + tv + (tv if tz is None else tz)) * n * nsys
 
 
 def k1_bytes_batch(n, nnz, nsys, t=8):
@@ -251,11 +252,13 @@ def k1_bytes_batch(n, nnz, nsys, t=8):
     return nsys * (t * nnz + 6 * t * n) + 4 * nnz + 4 * (n + 1)
 
 
-def k1_bytes_stencil(n, nsys, t=8):
+def k1_bytes_stencil(n, nsys, t=8, tz=None):
     """Batched fused CG kernel on the five-point stencil form: per system the
     three coefficient arrays (c, e, n; the w and s couplings are the e and n
-    of the neighbouring rows) and the six vector passes of k1_bytes."""
-    return nsys * (3 * t * n + 6 * t * n)
+    of the neighbouring rows) and the six vector passes of k1_bytes, the
+    preconditioned residual z read in tz bytes (default t; 4 when an FP32
+    apply keeps it in FP32)."""
+    return nsys * (3 * t * n + 5 * t * n + (t if tz is None else tz) * n)
 
 
 def run_c5(args):
@@ -406,12 +409,12 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
     ap_total = learned["apply_ms"] * learned["apply_calls"]
     sp_total = learned["spmv_ms"] * learned["spmv_calls"]
     if ap_total >= sp_total:
-        kname, ms, by = ("k_lscan<float,double,2,4,FULL> (FP32 factor apply on FP64 vectors, 4 wavefronts per "
-                         "line)", learned["apply_ms"], lscan_bytes(n, nsys, 4, 8))
+        kname, ms, by = ("k_lscan<float,double,2,4,FULL,float> (FP32 factor apply on FP64 vectors, z kept in "
+                         "FP32, 4 wavefronts per line)", learned["apply_ms"], lscan_bytes(n, nsys, 4, 8, 4))
     else:
         if learned.get("stencil5"):
-            kname, ms, by = ("k_cg_spmv<double,4,St5> (batched, five-point stencil)", learned["spmv_ms"],
-                             k1_bytes_stencil(n, nsys, 8))
+            kname, ms, by = ("k_cg_spmv<double,4,St5,float> (batched, five-point stencil, z in FP32)",
+                             learned["spmv_ms"], k1_bytes_stencil(n, nsys, 8, 4))
         else:
             kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
     ach = by / (ms * 1e-3) / 1e9

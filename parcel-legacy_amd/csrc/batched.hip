@@ -392,11 +392,13 @@ __device__ __forceinline__ void sweep(int nx, int ny, const T* __restrict__ a, c
 // [ny k / nstrip, ny (k+1) / nstrip) and drops the factor's couplings to the
 // neighbouring strips (block-diagonal M = diag(L_k L_k^T)); nstrip = 1 is
 // the exact M = L L^T.  T: arithmetic and factor type; TV: the type of r
-// and z (double with T = float: the FP32 apply of an FP64 PCG).
-template <class T, class TV, int E, int NW, bool FULL>
+// and z (double with T = float: the FP32 apply of an FP64 PCG); TZ: the
+// stored type of z (float there too when the CG keeps z in FP32: its values
+// are FP32 results, widened exactly where they are read).
+template <class T, class TV, int E, int NW, bool FULL, class TZ = TV>
 __global__ void __launch_bounds__(64 * NW) k_lscan(int nx, int ny, int nstrip, long n, long nv,
                                                    const T* __restrict__ F, const TV* __restrict__ r,
-                                                   T* __restrict__ w, TV* __restrict__ z, const CGState* st, int it,
+                                                   T* __restrict__ w, TZ* __restrict__ z, const CGState* st, int it,
                                                    double* __restrict__ sink) {
   __shared__ T ring[NW][2 * kLineBlk][64];
   const int sys = blockIdx.x / nstrip, strip = blockIdx.x - sys * nstrip;
@@ -411,7 +413,7 @@ __global__ void __launch_bounds__(64 * NW) k_lscan(int nx, int ny, int nstrip, l
   sweep<T, TV, T, E, NW, FULL, false>(nx, y1 - y0, af, ls, r + so, inv, w + so, (T*)sink, ring);
   // the backward sweep reads w cells written by other lanes
   __syncthreads();
-  sweep<T, T, TV, E, NW, FULL, true>(nx, y1 - y0, ab, ls, w + so, inv, z + so, (TV*)sink, ring);
+  sweep<T, T, TZ, E, NW, FULL, true>(nx, y1 - y0, ab, ls, w + so, inv, z + so, (TZ*)sink, ring);
 }
 
 // s = Dinv r (dinv null: s = r), every system
@@ -506,8 +508,8 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
 
 // --- host helpers ------------------------------------------------------------------
 
-template <class T, class TV>
-void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TV* z, const CGState* st, int it) {
+template <class T, class TV, class TZ = TV>
+void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TZ* z, const CGState* st, int it) {
   const int nx = B.nx;
   const dim3 g(B.nsys * B.nstrip);
   // line layout (the oracle's lscan_layout): E1 = cells per lane of one
@@ -515,10 +517,10 @@ void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TV* z, const CG
   // over 4 wavefronts, E1 = 4 over 2
 #define PARCEL_LSCAN(E, NW)                                                                                     \
   if (nx == 64 * (E) * (NW) && ((B.n * sizeof(T)) % 16 == 0))                                                  \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, true>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,      \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, true, TZ>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,  \
                        (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);                                       \
   else                                                                                                          \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, false>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,     \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, false, TZ>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip, \
                        (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);
   if (nx <= 64) { PARCEL_LSCAN(1, 1) }
   else if (nx <= 128) { PARCEL_LSCAN(2, 1) }
@@ -572,6 +574,7 @@ struct BVecs {
   const T* F;     // factor apply arrays (IC0 / FACTOR) or null
   const float* Fmix = nullptr;  // FP32 factor apply arrays used on T vectors (FP32 apply in FP64 PCG)
   float* wmix = nullptr;        // its FP32 intermediate vector
+  float* zmix = nullptr;        // its output z = M^-1 r kept in FP32 (the CG widens it where it reads it)
   const T* st5 = nullptr;       // five-point stencil arrays (c, e, n; nv apart) or null
   double* hist;
   int hist_cap;
@@ -579,10 +582,11 @@ struct BVecs {
 
 // One batched PCG: every system from its x to ||r||/||b|| < rtol.  mask (may
 // be null): systems whose OuterState is done take no step.  Returns the
-// final CGState of every system in out.
-template <class T>
-void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, double rtol, const OuterState* mask,
-                  std::vector<CGState>& out, Prof& pf, int& launches) {
+// final CGState of every system in out.  z: the preconditioned residual,
+// V.s or (FP32 apply) V.zmix.
+template <class T, class TZ>
+void batch_cg_loop(Batch& B, const BVecs<T>& V, TZ* z, T* x, const T* b, int itrmax, double rtol,
+                   const OuterState* mask, std::vector<CGState>& out, Prof& pf, int& launches) {
   const int n = B.n, nsys = B.nsys;
   const hipStream_t st = B.stream;
   const int nb = tiled_nb(n);
@@ -601,14 +605,25 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
   PARCEL_HIP_CHECK(hipMemsetAsync(V.p0, 0, sizeof(T) * (size_t)B.nv(), st));
   hipLaunchKernelGGL(k_residual<T>, grid, block, 0, st, n, V.A, (const T*)x, b, V.r, part, bt);
   const bool fac = V.F || V.Fmix;
-  if (V.F)
-    launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, nullptr, 0);
-  else if (V.Fmix)
-    launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, V.s, nullptr, 0);
-  else
-    hipLaunchKernelGGL(k_b_scale<T>, grid, block, 0, st, n, (const T*)V.r, V.dinv, V.s, bt);
-  hipLaunchKernelGGL(k_dot_part<T>, grid, block, 0, st, n, (const T*)V.s, (const T*)V.r, part + PART_RHO0 * kNBMax,
-                     bt);
+  // z = M^-1 r (TZ = float: the FP32 apply's output kept in FP32)
+  auto apply = [&](const CGState* sst, int it) {
+    if constexpr (std::is_same<TZ, T>::value) {
+      if (V.F)
+        launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, z, sst, it);
+      else
+        launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, z, sst, it);
+    } else {
+      launch_lscan<float, T, TZ>(B, V.Fmix, (const T*)V.r, V.wmix, z, sst, it);
+    }
+  };
+  if (fac) {
+    apply(nullptr, 0);
+  } else {
+    if constexpr (std::is_same<TZ, T>::value)
+      hipLaunchKernelGGL(k_b_scale<T>, grid, block, 0, st, n, (const T*)V.r, V.dinv, z, bt);
+  }
+  hipLaunchKernelGGL((k_dot_part<T, TZ>), grid, block, 0, st, n, (const TZ*)z, (const T*)V.r,
+                     part + PART_RHO0 * kNBMax, bt);
   hipLaunchKernelGGL(k_cg_init, dim3(1, nsys), block, 0, st, (const double*)part, nb, S);
   launches += 4;
   if (mask) {
@@ -630,23 +645,24 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
       if (V.st5) {
         const St5<T> A5{V.st5, V.st5 + B.nv(), V.st5 + 2 * B.nv(), B.nx, B.ny};
         if (lb)
-          hipLaunchKernelGGL((k_cg_spmv<T, kLB, St5<T>>), gridl, block, 0, st, n, it, itrmax, rtol, A5, x,
-                             (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+          hipLaunchKernelGGL((k_cg_spmv<T, kLB, St5<T>, TZ>), gridl, block, 0, st, n, it, itrmax, rtol, A5, x,
+                             (const T*)pold, (const TZ*)z, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
         else
-          hipLaunchKernelGGL((k_cg_spmv<T, 1, St5<T>>), grid, block, 0, st, n, it, itrmax, rtol, A5, x,
-                             (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+          hipLaunchKernelGGL((k_cg_spmv<T, 1, St5<T>, TZ>), grid, block, 0, st, n, it, itrmax, rtol, A5, x,
+                             (const T*)pold, (const TZ*)z, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
       } else if (lb)
-        hipLaunchKernelGGL((k_cg_spmv<T, kLB>), gridl, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
-                           (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+        hipLaunchKernelGGL((k_cg_spmv<T, kLB, SellArgsT<T>, TZ>), gridl, block, 0, st, n, it, itrmax, rtol, V.A, x,
+                           (const T*)pold, (const TZ*)z, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
       else
-        hipLaunchKernelGGL((k_cg_spmv<T, 1>), grid, block, 0, st, n, it, itrmax, rtol, V.A, x, (const T*)pold,
-                           (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
+        hipLaunchKernelGGL((k_cg_spmv<T, 1, SellArgsT<T>, TZ>), grid, block, 0, st, n, it, itrmax, rtol, V.A, x,
+                           (const T*)pold, (const TZ*)z, pnew, V.q, part, S, V.hist, V.hist_cap, bt);
       launches++;
       if (pf.on) pf.spmv.push_back({ea, pf.rec(st)});
       if (it > itrmax) break;
       if (!fac) {
-        hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, V.dinv, V.s, part,
-                           S, nb, bt);
+        if constexpr (std::is_same<TZ, T>::value)
+          hipLaunchKernelGGL(k_cg_update_pj<T>, grid, block, 0, st, n, it, V.r, (const T*)V.q, V.dinv, z, part,
+                             S, nb, bt);
         launches++;
       } else {
         if (lb)
@@ -654,16 +670,13 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
         else
           hipLaunchKernelGGL((k_cg_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, bt);
         hipEvent_t eb = pf.rec(st);
-        if (V.F)
-          launch_lscan<T, T>(B, V.F, (const T*)V.r, V.w, V.s, S, it);
-        else
-          launch_lscan<float, T>(B, V.Fmix, (const T*)V.r, V.wmix, V.s, S, it);
+        apply(S, it);
         if (pf.on) pf.apply.push_back({eb, pf.rec(st)});
         if (lb)
-          hipLaunchKernelGGL((k_cg_dot2<T, kLB>), gridl, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
-                             (const CGState*)S, bt);
+          hipLaunchKernelGGL((k_cg_dot2<T, kLB, TZ>), gridl, block, 0, st, n, it, (const TZ*)z, (const T*)V.r,
+                             part, (const CGState*)S, bt);
         else
-          hipLaunchKernelGGL((k_cg_dot2<T, 1>), grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+          hipLaunchKernelGGL((k_cg_dot2<T, 1, TZ>), grid, block, 0, st, n, it, (const TZ*)z, (const T*)V.r, part,
                              (const CGState*)S, bt);
         launches += 3;
       }
@@ -688,6 +701,18 @@ void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, dou
   }
   for (int s = 0; s < nsys; ++s) out[s] = h[s];
   (void)hipHostFree(h);
+}
+
+template <class T>
+void batch_cg_run(Batch& B, const BVecs<T>& V, T* x, const T* b, int itrmax, double rtol, const OuterState* mask,
+                  std::vector<CGState>& out, Prof& pf, int& launches) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (V.Fmix && V.zmix) {
+      batch_cg_loop<T, float>(B, V, V.zmix, x, b, itrmax, rtol, mask, out, pf, launches);
+      return;
+    }
+  }
+  batch_cg_loop<T, T>(B, V, V.s, x, b, itrmax, rtol, mask, out, pf, launches);
 }
 
 void ensure_vectors(Batch& B, bool f32) {
@@ -1005,6 +1030,7 @@ int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, i
       V.F = nullptr;
       V.Fmix = B.F32.p;
       V.wmix = B.w32.p;
+      V.zmix = B.s32.p;  // z stays FP32 (this path has no FP32 CG of its own)
     }
     V.hist = B.hist.p;
     V.hist_cap = B.hist_cap;

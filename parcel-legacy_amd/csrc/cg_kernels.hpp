@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(kNT) k_residual(int n, SellArgsT<T> A, const T
 
 // partials of <x,y> into part[slot] (slot offset already applied by the
 // caller; systems are PART_N*kNBMax apart)
-template <class T>
-__global__ void __launch_bounds__(kNT) k_dot_part(int n, const T* __restrict__ x, const T* __restrict__ y,
+template <class T, class TX = T>
+__global__ void __launch_bounds__(kNT) k_dot_part(int n, const TX* __restrict__ x, const T* __restrict__ y,
                                                   double* part, Bat B) {
   __shared__ double sh[kWaves];
   const long so = sys_id() * B.vs;
@@ -191,11 +191,13 @@ __global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGS
 // + j, j < LB, round by round (tile o + r*nb of every owner o in round r),
 // and writes each owner's partial, so every sum equals the LB = 1 launch's
 // while the partial reduction at the start is paid once per workgroup and
-// LB rows per thread are in flight.
-template <class T, int LB = 1, class MT = SellArgsT<T>>
+// LB rows per thread are in flight.  TZ: the stored type of the
+// preconditioned residual s (float when an FP32 apply produced it: widened
+// at use, exactly).
+template <class T, int LB = 1, class MT = SellArgsT<T>, class TZ = T>
 __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, MT A,
                                                  T* __restrict__ x, const T* __restrict__ p_old,
-                                                 const T* __restrict__ s, T* __restrict__ p_new,
+                                                 const TZ* __restrict__ s, T* __restrict__ p_new,
                                                  T* __restrict__ q, double* part, CGState* st,
                                                  double* hist, int hist_cap, Bat B) {
   __shared__ double sh[kWaves * LB];
@@ -260,9 +262,9 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
       if (i < n) {
         const T po = p_old[i];
         x[i] = fma((T)alpha_prev, po, x[i]);
-        const T pn = fma((T)beta, po, s[i]);
+        const T pn = fma((T)beta, po, (T)s[i]);
         p_new[i] = pn;
-        const T t = mat_row(A, i, [&](long c) { return fma((T)beta, p_old[c], s[c]); });
+        const T t = mat_row(A, i, [&](long c) { return fma((T)beta, p_old[c], (T)s[c]); });
         q[i] = t;
         acc[j] = fma((double)pn, (double)t, acc[j]);
       }
@@ -327,8 +329,8 @@ __global__ void __launch_bounds__(kNT) k_cg_update_pj(int n, int it, T* __restri
 
 // rho1 = <s,r>, rr = <r,r> partials (CG_calc_dot2, pcg.f90:30); LB logical
 // owners per workgroup as in k_cg_spmv
-template <class T, int LB = 1>
-__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const T* __restrict__ s, const T* __restrict__ r,
+template <class T, int LB = 1, class TZ = T>
+__global__ void __launch_bounds__(kNT) k_cg_dot2(int n, int it, const TZ* __restrict__ s, const T* __restrict__ r,
                                                  double* part, const CGState* st, Bat B) {
   __shared__ double sh[kWaves * 2 * LB];
   st += sys_id();
