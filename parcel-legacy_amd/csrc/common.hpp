@@ -48,9 +48,49 @@ struct hip_error : std::runtime_error {
 
 // --- wavefront / workgroup reductions -------------------------------------
 
+// The value of lane (lane ^ O) of a 64-lane wavefront, in registers: DPP
+// quad permutes for O = 1, 2, row shifts by 4 (one up, one down, picked by
+// lane bit 2) for O = 4, a row rotation by 8 for O = 8, and the gfx950 row /
+// half swaps (v_permlane16_swap, v_permlane32_swap) for O = 16, 32.
+template <int O>
+__device__ __forceinline__ unsigned xor_lane(unsigned v) {
+  const unsigned lane = threadIdx.x & 63;
+  if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xf, 0xf, false);  // quad_perm 1,0,3,2
+  if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xf, 0xf, false);  // quad_perm 2,3,0,1
+  if constexpr (O == 4) {
+    const unsigned up = __builtin_amdgcn_update_dpp(0u, v, 0x104, 0xf, 0xf, false);    // row_shl:4
+    const unsigned down = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    return (lane & 4) ? down : up;
+  }
+  if constexpr (O == 8) return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xf, 0xf, false);  // row_ror:8
+  if constexpr (O == 16) {
+    // rows (0, 1, 2, 3) -> first (0, 0, 2, 2), second (1, 1, 3, 3)
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return ((lane >> 4) & 1) ? p[0] : p[1];
+  }
+  if constexpr (O == 32) {
+    // halves (0, 1) -> first (0, 0), second (1, 1)
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return lane >= 32 ? p[0] : p[1];
+  }
+  return v;
+}
+template <int O>
+__device__ __forceinline__ double xor_lane(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = xor_lane<O>((unsigned)u), hi = xor_lane<O>((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Xor butterfly over the 64 lanes, offsets 32 .. 1: after every step lane l
+// holds v[l] + v[l ^ o]; every lane ends with the total (the same bits).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += xor_lane<32>(v);
+  v += xor_lane<16>(v);
+  v += xor_lane<8>(v);
+  v += xor_lane<4>(v);
+  v += xor_lane<2>(v);
+  v += xor_lane<1>(v);
   return v;
 }
 
