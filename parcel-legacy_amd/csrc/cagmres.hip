@@ -372,10 +372,11 @@ __global__ void __launch_bounds__(kNT) k_tsqr(int n, const double* V, long ldv, 
       double d[NC];
 #pragma unroll
       for (int k = i; k < NC; ++k) d[k] = x * P[k];
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1)
-#pragma unroll
-        for (int k = i; k < NC; ++k) d[k] += __shfl_xor(d[k], o, 64);
+      // the xor butterfly (offsets 32..1) in registers
+#define PARCEL_XSTEP(O) \
+  for (int k = i; k < NC; ++k) d[k] += xor_lane<O>(d[k]);
+      PARCEL_XSTEP(32) PARCEL_XSTEP(16) PARCEL_XSTEP(8) PARCEL_XSTEP(4) PARCEL_XSTEP(2) PARCEL_XSTEP(1)
+#undef PARCEL_XSTEP
       const double alpha = __shfl(Rrow[i], i, 64);
       const double s2 = d[i];
       const double nrm = sqrt(fma(alpha, alpha, s2));

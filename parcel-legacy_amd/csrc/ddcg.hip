@@ -26,14 +26,20 @@ struct DDState {
 
 enum { DS_BB = 0, DS_RR = 1, DS_RHO = 2, DS_GAMMA = 3, DS_RHO1 = 4, DS_RR1 = 5, DS_N = 6 };
 
-__device__ __forceinline__ dd shfl_xor_dd(dd v, int o) {
-  return mkdd(__shfl_xor(v.hi, o, 64), __shfl_xor(v.lo, o, 64));
+// the value of lane (lane ^ O), in registers
+template <int O>
+__device__ __forceinline__ dd xor_lane_dd(dd v) {
+  return mkdd(xor_lane<O>(v.hi), xor_lane<O>(v.lo));
 }
 
 // the tiled tree of common.hpp with dd_add(own, other) at every step
 __device__ __forceinline__ dd dd_block_sum(dd v, dd* sh) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = dd_add(v, shfl_xor_dd(v, o));
+  v = dd_add(v, xor_lane_dd<32>(v));
+  v = dd_add(v, xor_lane_dd<16>(v));
+  v = dd_add(v, xor_lane_dd<8>(v));
+  v = dd_add(v, xor_lane_dd<4>(v));
+  v = dd_add(v, xor_lane_dd<2>(v));
+  v = dd_add(v, xor_lane_dd<1>(v));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) sh[w] = v;
