@@ -127,7 +127,7 @@ __global__ void k_b_derive(int n, int nx, long nv, const S* __restrict__ L, T* _
   }
 }
 
-// --- factor solves: z = (L L^T)^-1 r, one wavefront per system -----------------
+// --- factor solves: z = (L L^T)^-1 r, one workgroup per strip of grid lines -----
 
 __device__ __forceinline__ bool sys_done(const CGState* st, int sys, int it) {
   if (!st) return false;
