@@ -312,8 +312,9 @@ int parcel_hip_device_synchronize(void);
  *   FACTOR            M = L L^T from a caller-supplied (or network-predicted)
  *                     lower factor on the same pattern, BF16 values.
  * IC0 and FACTOR need the pattern of a 2D five-point grid in natural order
- * (rows x fastest, grid lines of at most 1024 cells); the factor solves are
- * line recurrences evaluated as 64-lane scans. */
+ * (rows x fastest, grid lines of at most 1024 cells, n < 2^29 per system);
+ * the factor solves are line recurrences evaluated as 64-lane scans, a line
+ * split over up to four wavefronts. */
 #define PARCEL_BATCH_NONE 0
 #define PARCEL_BATCH_JACOBI 1
 #define PARCEL_BATCH_IC0 2

@@ -852,6 +852,9 @@ int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const in
   if ((precon == PARCEL_BATCH_IC0 || precon == PARCEL_BATCH_FACTOR) && !B.nx)
     return api_fail(PARCEL_HIP_ERR_UNSUPPORTED,
                     "batch_create: IC0/FACTOR need a 2D five-point pattern in natural order (lines <= 1024)");
+  // the factor apply addresses a system's cells with 32-bit byte offsets
+  if ((precon == PARCEL_BATCH_IC0 || precon == PARCEL_BATCH_FACTOR) && (long)n >= (1L << 29))
+    return api_fail(PARCEL_HIP_ERR_UNSUPPORTED, "batch_create: IC0/FACTOR systems need n < 2^29");
   B.stats.nx = B.nx;
   B.stats.ny = B.ny;
   // SELL-64 layout of the pattern and the slot -> CSR entry map
