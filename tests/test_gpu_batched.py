@@ -183,10 +183,12 @@ def test_batch_rejects_non_five_point_for_factor():
         assert iret[0] == 0
 
 
-@pytest.mark.parametrize("kind", ["ic0", "bf16"])
-def test_batch_fp32_apply_in_fp64_pcg_bit_exact(kind):
-    print("batch fp32 apply", kind, flush=True)
-    nx, ny, nsys = 512, 12, 2
+# the headline path (FP64 PCG, factor applied in FP32, z kept in FP32) on
+# full four-wavefront lines and on padded two- and four-wavefront lines
+@pytest.mark.parametrize("kind,nx,ny", [("ic0", 512, 12), ("bf16", 512, 12), ("ic0", 300, 10), ("bf16", 200, 9)])
+def test_batch_fp32_apply_in_fp64_pcg_bit_exact(kind, nx, ny):
+    print("batch fp32 apply", kind, nx, ny, flush=True)
+    nsys = 2
     rp, col, vals, b = batch_problem(nx, ny, nsys)
     n = nx * ny
     if kind == "ic0":
