@@ -405,30 +405,36 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
     log("c5 plain FP64 BJ(%d): %.3f s solve (setup %.2f s), its mean %.0f" % (nb, plain["time_to_solution_s"],
                                                                            plain["setup_s_total"],
                                                                            plain["niter_mean"]))
-    # dominant kernel of the learned solve
+    # dominant kernel of the learned solve (the larger total time of the two
+    # big iteration kernels; the other one is reported beside it)
+    lscan_k = ("k_lscan<float,double,2,4,FULL,float> (FP32 factor apply on FP64 vectors, z kept in FP32, 4 "
+               "wavefronts per line, %d strip(s) per system)" % learned["strips"], learned["apply_ms"],
+               lscan_bytes(n, nsys, 4, 8, 4))
+    if learned.get("stencil5"):
+        spmv_k = ("k_cg_spmv<double,4,St5,float> (batched, five-point stencil, z in FP32)", learned["spmv_ms"],
+                  k1_bytes_stencil(n, nsys, 8, 4))
+    else:
+        spmv_k = ("k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8))
     ap_total = learned["apply_ms"] * learned["apply_calls"]
     sp_total = learned["spmv_ms"] * learned["spmv_calls"]
-    if ap_total >= sp_total:
-        kname, ms, by = ("k_lscan<float,double,2,4,FULL,float> (FP32 factor apply on FP64 vectors, z kept in "
-                         "FP32, 4 wavefronts per line)", learned["apply_ms"], lscan_bytes(n, nsys, 4, 8, 4))
-    else:
-        if learned.get("stencil5"):
-            kname, ms, by = ("k_cg_spmv<double,4,St5,float> (batched, five-point stencil, z in FP32)",
-                             learned["spmv_ms"], k1_bytes_stencil(n, nsys, 8, 4))
-        else:
-            kname, ms, by = "k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8)
-    ach = by / (ms * 1e-3) / 1e9
-    traffic = None
+    dom, other = (lscan_k, spmv_k) if ap_total >= sp_total else (spmv_k, lscan_k)
     tf = os.path.join(ROOT, "profiles", "batched_pcg_c5_grf512_traffic.json")
+    tks = {}
     if nsys == 64 and n == 512 * 512 and os.path.exists(tf):
         # HBM bytes per launch of the same kernel on the same workload, from the
         # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy kernel)
-        tk = json.load(open(tf))["kernels"].get(kname)
-        if tk:
-            traffic = tk["traffic_bytes"]
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic, "bytes_per_launch": by,
-                "mean_ms": round(ms, 4)}
+        tks = json.load(open(tf))["kernels"]
+
+    def line(k):
+        kname, ms, by = k
+        ach = by / (ms * 1e-3) / 1e9
+        tk = tks.get(kname)
+        return {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tk["traffic_bytes"] if tk else None,
+                "bytes_per_launch": by, "mean_ms": round(ms, 4)}
+
+    roofline = line(dom)
+    roofline["next_kernel"] = line(other)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as orc
