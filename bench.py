@@ -308,7 +308,7 @@ def run_c5(args):
         res = true_res(B)
         B.set_profile(True)
         x_d.zero()
-        B.solve(x_d, b_d, itr, 1e-10, precision=precision, history=False)
+        _, pniter, _ = B.solve(x_d, b_d, itr, 1e-10, precision=precision, history=False)
         pst = B.stats()
         B.close()
         if net is not None:
@@ -318,7 +318,11 @@ def run_c5(args):
              "true_rel_res_max": res, "device_ms": st["solve_device_ms"],
              "apply_ms": pst["apply_ms"] / max(pst["apply_calls"], 1), "apply_calls": pst["apply_calls"],
              "spmv_ms": pst["spmv_ms"] / max(pst["spmv_calls"], 1), "spmv_calls": pst["spmv_calls"],
-             "stencil5": int(pst["stencil5"])}
+             "stencil5": int(pst["stencil5"]), "fallback_systems": int(st["fallback_systems"]),
+             # system-iterations of the profiled solve: a converged system's
+             # workgroups return at once, so a launch carries only the systems
+             # still iterating
+             "system_iterations": int(np.sum(pniter))}
         if precision == ph.BATCH_REFINE32:
             m["fp32_batched_iterations"] = st["inner_steps"]
             m["fp32_iterations_per_system_mean"] = st["inner_total"] / nsys
@@ -407,14 +411,20 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
                                                                            plain["niter_mean"]))
     # dominant kernel of the learned solve (the larger total time of the two
     # big iteration kernels; the other one is reported beside it)
+    # algorithmic bytes per launch = per-system bytes x the system-iterations
+    # of the profiled solve / launches (converged systems' workgroups return
+    # at once and the last chunk of launches finds every system done)
+    sysit = learned["system_iterations"]
     lscan_k = ("k_lscan<float,double,2,4,FULL,float> (FP32 factor apply on FP64 vectors, z kept in FP32, 4 "
                "wavefronts per line, %d strip(s) per system)" % learned["strips"], learned["apply_ms"],
-               lscan_bytes(n, nsys, 4, 8, 4))
+               lscan_bytes(n, 1, 4, 8, 4) * sysit / max(learned["apply_calls"], 1), learned["apply_calls"])
     if learned.get("stencil5"):
         spmv_k = ("k_cg_spmv<double,4,St5,float> (batched, five-point stencil, z in FP32)", learned["spmv_ms"],
-                  k1_bytes_stencil(n, nsys, 8, 4))
+                  k1_bytes_stencil(n, 1, 8, 4) * sysit / max(learned["spmv_calls"], 1), learned["spmv_calls"])
     else:
-        spmv_k = ("k_cg_spmv<double> (batched)", learned["spmv_ms"], k1_bytes_batch(n, nnz, nsys, 8))
+        spmv_k = ("k_cg_spmv<double> (batched)", learned["spmv_ms"],
+                  (k1_bytes_batch(n, nnz, 1, 8) - 4 * nnz - 4 * (n + 1)) * sysit / max(learned["spmv_calls"], 1)
+                  + 4 * nnz + 4 * (n + 1), learned["spmv_calls"])
     ap_total = learned["apply_ms"] * learned["apply_calls"]
     sp_total = learned["spmv_ms"] * learned["spmv_calls"]
     dom, other = (lscan_k, spmv_k) if ap_total >= sp_total else (spmv_k, lscan_k)
@@ -426,12 +436,13 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
         tks = json.load(open(tf))["kernels"]
 
     def line(k):
-        kname, ms, by = k
+        kname, ms, by, calls = k
         ach = by / (ms * 1e-3) / 1e9
         tk = tks.get(kname)
         return {"bound": "hbm", "kernel": kname, "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tk["traffic_bytes"] if tk else None,
-                "bytes_per_launch": by, "mean_ms": round(ms, 4)}
+                "bytes_per_launch": int(by), "mean_ms": round(ms, 4), "launches": calls,
+                "active_systems_per_launch": round(sysit / max(calls, 1), 2)}
 
     roofline = line(dom)
     roofline["next_kernel"] = line(other)

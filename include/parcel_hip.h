@@ -165,8 +165,14 @@ int parcel_hip_dgmres(int icomm, double* vecx, const double* vecb, int n, int gn
 /* parcel_dcagmres (src/wrapper.f90:1355-1454 GPU build, :2807-2906 CPU build):
  * parcel_dcg's arguments plus CAGMRES_sstep (s, 1..15), CAGMRES_tstep (t >= 1,
  * s*t <= 128), CAGMRES_basis (0 monomial, 1 Newton with Leja-ordered Ritz
- * shifts) and CAGMRES_QRflag (PARCEL_QR_MGS, _CGS, _TSQR, _CHOLESKYQR,
- * _CHOLESKYQR2).  CA-GMRES of src/pcagmres.f90:1557-2517. */
+ * shifts, 2 the same Newton basis with the conjugate-pair step of the
+ * Hessenberg's own recurrence) and CAGMRES_QRflag (PARCEL_QR_MGS, _CGS, _TSQR,
+ * _CHOLESKYQR, _CHOLESKYQR2).  CA-GMRES of src/pcagmres.f90:1557-2517.
+ * Basis 1 is the reference's: for a complex shift pair a +- ib its
+ * calc_NewtonBasisVec (:1519-1530) forms V_i = (A - a) V_{i-1} - b^2 V_{i-2}
+ * while the Hessenberg (calc_NewtonBasis, :586-611) describes + b^2, so with
+ * complex Ritz values the residual estimate drifts from the true residual.
+ * Basis 2 (no counterpart in the reference) forms + b^2. */
 int parcel_hip_dcagmres(int icomm, double* vecx, const double* vecb, int n, int gn,
                         int nnz, int istart, const double* crsA, const int* crsRow_ptr,
                         const int* crsCol, int ipreflag, int ILU_method, int addL,
@@ -341,6 +347,7 @@ typedef struct {
   int spmv_calls;
   int nx, ny;              /* detected five-point grid (0 if none) */
   int stencil5;            /* 1: every system exactly symmetric, the CG SpMV reads c/e/n arrays */
+  int fallback_systems;    /* systems of the last solve re-solved by the fallback */
 } parcel_hip_batch_stats;
 
 /* nsys systems of order n; rowptr/col: 1-based CSR (host), shared; vals:
@@ -368,6 +375,18 @@ int parcel_hip_batch_set_inner(parcel_hip_batch* batch, double inner_rtol, int i
  * factor's couplings between strips dropped; nstrip wavefronts per system.
  * Default 1 (M = L L^T exactly). */
 int parcel_hip_batch_set_strips(parcel_hip_batch* batch, int nstrip);
+/* Fallback of the learned path: after a batched solve, every system whose
+ * solve ended with iret != 0 is solved again, from x = 0, by parcel_dcg's
+ * FP64 PCG (KSP_pCG, src/pcg.f90:224-892) with preconditioner ipreflag
+ * (0..2) / ILU_method / precon_thblock and at most itrmax iterations; that
+ * solve's x, niter, iret and history (its first itrmax entries of the
+ * batch's row) replace the system's.  itrmax <= 0 switches it off.  FACTOR
+ * batches start with (2, ILU(0), 32 blocks, 20000), other batches off.  No
+ * counterpart in the reference. */
+int parcel_hip_batch_set_fallback(parcel_hip_batch* batch, int ipreflag, int ILU_method, int precon_thblock,
+                                  int itrmax);
+/* per system (nsys ints, host): 1 if the last solve fell back */
+int parcel_hip_batch_get_fallback(const parcel_hip_batch* batch, int* fell_back);
 int parcel_hip_batch_set_profile(parcel_hip_batch* batch, int on);
 int parcel_hip_batch_get_stats(const parcel_hip_batch* batch, parcel_hip_batch_stats* stats);
 /* y = A x for every system (x, y: nsys*n, host or device) */
@@ -439,6 +458,25 @@ int parcel_gen_poisson3d_ddm(int gnx, int gny, int gnz, int margin, double* val_
                              int* ioff_grids, double* vecb, double* vecx_exact);
 /* x ~ U[-1,1] from the counter-based stream `seed` (SplitMix64, 53-bit) */
 int parcel_gen_uniform(long n, unsigned long long seed, double lo, double hi, double* x);
+/* 3D 7-point first-order upwind convection-diffusion (finite volumes on cubic
+ * cells, divided by D*h: a diffusion face is -1).  The velocity direction
+ * (vx, vy, vz) is scaled to cell Peclet number max|v_d| h / (2D) = peclet;
+ * with c_d = 2 peclet v_d / max|v| the upwind neighbour gets -(1 + |c_d|),
+ * the downwind one -1, the diagonal 6 + sum |c_d| (Dirichlet walls).  Entry
+ * order b, s, w, c, e, n, t as make_poisson3D_CRS. */
+int parcel_gen_convdiff3d(int gnx, int gny, int gnz, double peclet, double vx, double vy, double vz, int* nnz,
+                          int* rowptr, int* col, double* val);
+/* kappa ~ log-uniform in [kmin, kmax) from the SplitMix64 stream `seed` */
+int parcel_gen_loguniform(long n, unsigned long long seed, double kmin, double kmax, double* kappa);
+/* make_poisson3D_adv_CRS (src/example/problem.f90:620-770) with per-cell
+ * conductivity kappa (x fastest), one rank holding the whole grid; a ghost
+ * cell carries the boundary cell's conductivity.  Pass NULL arrays to query
+ * nnz. */
+int parcel_gen_diffusion3d(int gnx, int gny, int gnz, const double* kappa, int* nnz, int* rowptr, int* col,
+                           double* val);
+/* b = A x in double-double (exact products, dd_add in CRS order) */
+int parcel_gen_spmv_dd(int n, const int* rowptr, const int* col, const double* val, const double* x, double* b_hi,
+                       double* b_lo);
 /* x ~ N(0,1): normal k of stream `seed` is Box-Muller on uniforms 2*(k/2)
  * and 2*(k/2)+1 (cosine for even k, sine for odd k) */
 int parcel_gen_normal(long n, unsigned long long seed, double* x);

@@ -704,6 +704,7 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
   double* ei = (double*)calloc((size_t)m, sizeof(double));
   for (int i = 0; i < s; ++i) B1[IDX(i + 1, i, s1)] = 1.0; /* sub-diagonal 1 (monomial) */
   int iflag_exit = 0, iflag_basis = 0, itr, itr0 = 1, mm = 0, qmat = 0;
+  const int newton = basis == 1 || basis == 2;
   double rnorm, hk_pre = 0.0;
   oracle_spmv(n, rowptr, col, val, x, r);
   for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
@@ -727,17 +728,19 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
     const double a = -1.0 / rnorm;
     for (int i = 0; i < n; ++i) V[i] = a * r[i];       /* v1 = -r/||r|| */
     for (int k = 1; k <= t; ++k) {
-      if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
+      if (newton && iflag_basis) newton_basis(s, B1, er, ei);
       qmat = (k != t);
       for (int i = 1; i <= s; ++i) {
         const double* vi = V + (size_t)(i - 1) * nn;
         double* vn = V + (size_t)i * nn;
         oracle_precon_apply(M, vi, pv);
         oracle_spmv(n, rowptr, col, val, pv, vn);      /* V_{i+1} = A M^-1 V_i */
-        if (iflag_basis && basis == 1) {               /* calc_NewtonBasisVec */
+        if (iflag_basis && newton) {                   /* calc_NewtonBasisVec */
           const double alpha = -B1[IDX(i - 1, i - 1, s1)];
           if (i != 1) {
-            const double beta = B1[IDX(i - 2, i - 1, s1)];
+            /* basis 1: + B1(i-1,i) V_{i-2} as :1519-1530; basis 2: minus it,
+               the recurrence the Hessenberg update (:879-1051) assumes */
+            const double beta = basis == 2 ? -B1[IDX(i - 2, i - 1, s1)] : B1[IDX(i - 2, i - 1, s1)];
             const double* vp = V + (size_t)(i - 2) * nn;
             for (int q = 0; q < n; ++q) vn[q] = fma(beta, vp[q], fma(alpha, vi[q], vn[q]));
           } else {
@@ -784,7 +787,7 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val, 
         if (itr0 == itrmax) { iflag_exit = 1; break; }
       }
       if (iflag_exit) break;
-      if (basis == 1) {
+      if (newton) {
         iflag_basis = 1;
         const int sk = s * k;
         for (int j = 0; j < sk; ++j)

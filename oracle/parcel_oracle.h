@@ -107,8 +107,9 @@ int oracle_gmres(int n, const int* rowptr, const int* col, const double* val,
                  int gsflag, const oracle_red* red, double* hist, int* niter, int* iret);
 
 /* KSP_pCAGMRES (src/pcagmres.f90:1557-2517), CA-GMRES(s,t) with right
- * preconditioning; t = 1.  basis: 0 monomial, 1 Newton with Leja-ordered
- * Ritz shifts.  qrflag: 1 MGS, 2 CGS, 3 TSQR (CholeskyQR2 when n <= max(s+1, 64)),
+ * preconditioning.  basis: 0 monomial, 1 Newton with Leja-ordered
+ * Ritz shifts (the reference's conjugate-pair step, + B1(i-1,i) V_{i-2}),
+ * 2 the same with - B1(i-1,i) V_{i-2} (consistent with the Hessenberg).  qrflag: 1 MGS, 2 CGS, 3 TSQR (CholeskyQR2 when n <= max(s+1, 64)),
  * 4 CholeskyQR, 5 CholeskyQR2.  hist must
  * hold itrmax + s entries (the reference writes reshistory(itr+i) up to
  * itrmax + s when a cycle starts at itr = itrmax).  Returns -4 for t != 1

@@ -44,6 +44,13 @@ struct Batch {
   DevBuf<OuterState> ost;        // nsys
   DevBuf<double> hist;
   int hist_cap = 0;
+  PinnedBuf<CGState> hstate;     // host mirror of st, polled between chunks of iterations
+  std::vector<long> hsptr;       // host copy of A.sptr (fallback: CSR values from SELL slots)
+  // fallback of the learned path: systems whose solve ends with iret != 0
+  // are solved again, from x = 0, by parcel_dcg's FP64 PCG with this
+  // preconditioner (fb_itrmax <= 0: off)
+  int fb_ipreflag = 2, fb_ilu = 0, fb_blocks = 32, fb_itrmax = 0;
+  std::vector<int> fell_back;    // per system, last solve
   double inner_rtol = 1e-4;
   int inner_itrmax = 10000;
   bool profile = false;

@@ -631,8 +631,8 @@ void batch_cg_loop(Batch& B, const BVecs<T>& V, TZ* z, T* x, const T* b, int itr
     launches++;
   }
   PARCEL_HIP_CHECK(hipGetLastError());
-  CGState* h = nullptr;
-  PARCEL_HIP_CHECK(hipHostMalloc((void**)&h, sizeof(CGState) * nsys, hipHostMallocDefault));
+  B.hstate.ensure((size_t)nsys);
+  CGState* h = B.hstate.p;
   int enq = 0, chunk = 8;
   bool all_done = false;
   while (enq < itrmax + 1) {
@@ -694,13 +694,9 @@ void batch_cg_loop(Batch& B, const BVecs<T>& V, TZ* z, T* x, const T* b, int itr
     PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState) * nsys, hipMemcpyDeviceToHost, st));
     PARCEL_HIP_CHECK(hipStreamSynchronize(st));
     for (int s = 0; s < nsys; ++s)
-      if (!h[s].done_at) {
-        (void)hipHostFree(h);
-        throw hip_error("batched CG loop did not terminate", PARCEL_HIP_ERR_HIP);
-      }
+      if (!h[s].done_at) throw hip_error("batched CG loop did not terminate", PARCEL_HIP_ERR_HIP);
   }
   for (int s = 0; s < nsys; ++s) out[s] = h[s];
-  (void)hipHostFree(h);
 }
 
 template <class T>
@@ -831,6 +827,8 @@ int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const in
   if (precon < PARCEL_BATCH_NONE || precon > PARCEL_BATCH_FACTOR)
     return api_fail(PARCEL_HIP_ERR_ARG, "batch_create: unknown preconditioner");
   B.nsys = nsys; B.n = n; B.nnz = nnz; B.precon = precon;
+  // a learned (FACTOR) batch falls back to FP64 PCG + block-Jacobi ILU(0)
+  if (precon == PARCEL_BATCH_FACTOR) B.fb_itrmax = 20000;
   B.rowptr.resize(n + 1);
   B.col.resize(nnz);
   if (rowptr[0] != 1 || rowptr[n] != nnz + 1)
@@ -901,6 +899,7 @@ int batch_create(Batch& B, int nsys, int n, int nnz, const int* rowptr, const in
       }
     }
   S.sptr.upload(sptr, B.stream);
+  B.hsptr = sptr;
   S.sk.upload(sk, B.stream);
   S.col.upload(scol, B.stream);
   B.diag_slot.upload(dslot, B.stream);
@@ -993,6 +992,48 @@ void ensure_hist(Batch& B, int cap) {
   }
 }
 
+// The learned path's fallback for system s: its CSR values are read back
+// from the SELL slots, a single-system plan with the fallback
+// preconditioner is set up, and parcel_dcg's FP64 PCG solves it from x = 0
+// (src/pcg.f90:224-892 with x0 = 0), writing x in place on the device.
+void batch_fallback(Batch& B, int s, double* dx, const double* db, double rtol, int& nit, int& ret,
+                    std::vector<double>& hist) {
+  const int n = B.n, nnz = B.nnz;
+  const long ent = B.A.entries;
+  std::vector<double> sell((size_t)ent);
+  PARCEL_HIP_CHECK(hipMemcpyAsync(sell.data(), B.A.val.p + (long)s * ent, sizeof(double) * ent,
+                                  hipMemcpyDeviceToHost, B.stream));
+  PARCEL_HIP_CHECK(hipMemsetAsync(dx + (long)s * n, 0, sizeof(double) * n, B.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  std::vector<int> rp1(n + 1), col1(nnz);
+  std::vector<double> val(nnz);
+  for (int i = 0; i < n; ++i) {
+    rp1[i] = B.rowptr[i] + 1;
+    for (int e = B.rowptr[i], j = 0; e < B.rowptr[i + 1]; ++e, ++j) {
+      val[e] = sell[B.hsptr[i / 64] + (long)j * 64 + i % 64];
+      col1[e] = B.col[e] + 1;
+    }
+  }
+  rp1[n] = nnz + 1;
+  parcel_hip_param prm;
+  parcel_hip_default_param(&prm);
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = B.fb_ipreflag;
+  prm.ILU_method = B.fb_ilu;
+  prm.precon_thblock = B.fb_blocks;
+  parcel_hip_plan* P = nullptr;
+  int rc = parcel_hip_plan_create(&P, n, n, nnz, 1, val.data(), rp1.data(), col1.data(), &prm);
+  if (rc != PARCEL_HIP_OK) throw hip_error(std::string("fallback setup: ") + parcel_hip_last_error(), rc);
+  int it = B.fb_itrmax;
+  hist.assign((size_t)it, 0.0);
+  rc = parcel_hip_plan_solve(P, dx + (long)s * n, db + (long)s * n, &it, rtol, hist.data(), &ret);
+  const std::string msg = rc == PARCEL_HIP_OK ? "" : parcel_hip_last_error();
+  parcel_hip_plan_destroy(P);
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  if (rc != PARCEL_HIP_OK) throw hip_error("fallback solve: " + msg, rc);
+  nit = it;
+}
+
 int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, int precision, double* hist_host,
                 int* niter, int* iret) {
   if (!x || !b) return api_fail(PARCEL_HIP_ERR_ARG, "batch_solve: null vector");
@@ -1079,6 +1120,18 @@ int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, i
     PARCEL_HIP_CHECK(hipGetLastError());
     for (int s = 0; s < nsys; ++s) { nit[s] = hos[s].niter; ret[s] = hos[s].iret; }
   }
+  // fallback: systems the batched solve did not converge
+  B.fell_back.assign(B.nsys, 0);
+  std::vector<std::vector<double>> fb_hist(B.nsys);
+  int nfb = 0;
+  if (B.fb_itrmax > 0)
+    for (int s = 0; s < B.nsys; ++s)
+      if (ret[s] != 0) {
+        batch_fallback(B, s, dx, db, rtol, nit[s], ret[s], fb_hist[s]);
+        B.fell_back[s] = 1;
+        ++nfb;
+      }
+  B.stats.fallback_systems = nfb;
   PARCEL_HIP_CHECK(hipEventRecord(ev1, st));
   if (!xdev) PARCEL_HIP_CHECK(hipMemcpyAsync(x, dx, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
@@ -1097,6 +1150,15 @@ int batch_solve(Batch& B, double* x, const double* b, int itrmax, double rtol, i
     PARCEL_HIP_CHECK(hipMemcpy(h.data(), B.hist.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < B.nsys; ++s)
       std::memcpy(hist_host + (long)s * itrmax, h.data() + (long)s * B.hist_cap, sizeof(double) * itrmax);
+    // a fallen-back system's row: the fallback solve's history (its first
+    // itrmax entries, zero-padded)
+    for (int s = 0; s < B.nsys; ++s)
+      if (B.fell_back[s]) {
+        double* row = hist_host + (long)s * itrmax;
+        std::memset(row, 0, sizeof(double) * itrmax);
+        const size_t k = std::min((size_t)itrmax, (size_t)std::max(nit[s], 0));
+        std::memcpy(row, fb_hist[s].data(), sizeof(double) * k);
+      }
   }
   B.stats.solve_device_ms = ms;
   B.stats.solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1201,6 +1263,24 @@ int parcel_hip_batch_set_strips(parcel_hip_batch* h, int nstrip) {
 int parcel_hip_batch_set_profile(parcel_hip_batch* h, int on) {
   if (!h) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_profile: null handle");
   h->B.profile = on != 0;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_set_fallback(parcel_hip_batch* h, int ipreflag, int ILU_method, int precon_thblock,
+                                  int itrmax) {
+  if (!h) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_fallback: null handle");
+  if (ipreflag < 0 || ipreflag > 2 || ILU_method < 0 || ILU_method > 2)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_fallback: ipreflag 0..2 and ILU_method 0..2");
+  h->B.fb_ipreflag = ipreflag;
+  h->B.fb_ilu = ILU_method;
+  h->B.fb_blocks = precon_thblock;
+  h->B.fb_itrmax = itrmax > 0 ? itrmax : 0;
+  return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_get_fallback(const parcel_hip_batch* h, int* fell_back) {
+  if (!h || !fell_back) return api_fail(PARCEL_HIP_ERR_ARG, "batch_get_fallback: null argument");
+  for (int s = 0; s < h->B.nsys; ++s) fell_back[s] = s < (int)h->B.fell_back.size() ? h->B.fell_back[s] : 0;
   return PARCEL_HIP_OK;
 }
 

@@ -548,6 +548,7 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
   const hipStream_t st = P.stream;
   const int s = P.prm.CAGMRES_sstep, t = P.prm.CAGMRES_tstep;
   const int basis = P.prm.CAGMRES_basis, qrflag = P.prm.CAGMRES_QRflag;
+  const bool newton = basis == 1 || basis == 2;
   if (s < 1 || s + 1 > kMaxCols) throw hip_error("CAGMRES_sstep must be in 1..15", PARCEL_HIP_ERR_ARG);
   if (t < 1 || s * t > kMaxM) throw hip_error("CAGMRES_tstep must be >= 1 with s*t <= 128", PARCEL_HIP_ERR_ARG);
   if (qrflag < PARCEL_QR_MGS || qrflag > PARCEL_QR_CHOLESKYQR2)
@@ -724,18 +725,22 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
     launches += 3;
     bool converged = false;
     for (int k = 1; k <= t; ++k) {
-      if (basis == 1 && iflag_basis) newton_basis(s, B1, er, ei);
+      if (newton && iflag_basis) newton_basis(s, B1, er, ei);
       qmat = (k != t);
       for (int i = 1; i <= s; ++i) {
         const double* vi = V + (long)(i - 1) * ldv;
         precon_apply(P, vi, pv, nullptr);
         int mode = 0;
         double alpha = 0.0, beta = 0.0;
-        if (iflag_basis && basis == 1) {
+        if (iflag_basis && newton) {
           alpha = -B1[(i - 1) + (size_t)(i - 1) * nc];
           mode = 1;
           if (i != 1) {
-            beta = B1[(i - 2) + (size_t)(i - 1) * nc];
+            // basis 1 adds B1(i-1,i) V_{i-2} as calc_NewtonBasisVec does
+            // (pcagmres.f90:1519-1530); basis 2 subtracts it, the recurrence
+            // A V_{i-1} = V_i + B1(i,i) V_{i-1} + B1(i-1,i) V_{i-2} that the
+            // Hessenberg update assumes
+            beta = basis == 2 ? -B1[(i - 2) + (size_t)(i - 1) * nc] : B1[(i - 2) + (size_t)(i - 1) * nc];
             mode = 2;
           }
         }
@@ -794,7 +799,7 @@ int cagmres_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, 
         }
       }
       if (iflag_exit) break;
-      if (basis == 1) {
+      if (newton) {
         iflag_basis = true;
         const int sk = s * k;
         for (int j = 0; j < sk; ++j)

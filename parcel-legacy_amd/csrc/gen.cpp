@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "dd.hpp"
 #include "parcel_hip.h"
 
 namespace {
@@ -198,6 +199,116 @@ int parcel_gen_uniform(long n, unsigned long long seed, double lo, double hi, do
   const double w = hi - lo;
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < n; ++i) x[i] = lo + w * u01(seed, (uint64_t)i);
+  return PARCEL_HIP_OK;
+}
+
+// 3D 7-point first-order upwind convection-diffusion, finite volumes on
+// cubic cells, divided by D*h so that a diffusion face is -1.  The velocity
+// direction (vx, vy, vz) is scaled so that the cell Peclet number
+// Pe = max|v_d| h / (2 D) equals `peclet`; the convective number of
+// direction d is then c_d = 2 Pe v_d / max|v|.  Upwind: for v_d > 0 the
+// neighbour at -d gets -(1 + c_d) and the one at +d gets -1 (mirrored for
+// v_d < 0); the diagonal is 6 + sum |c_d| on every row (Dirichlet walls:
+// couplings to cells outside the grid are dropped, the diagonal is kept).
+// Entry order b, s, w, c, e, n, t as make_poisson3D_CRS
+// (problem.f90:448-618).
+int parcel_gen_convdiff3d(int gnx, int gny, int gnz, double peclet, double vx, double vy, double vz, int* nnz,
+                          int* rowptr, int* col, double* val) {
+  const double vm = std::fmax(std::fabs(vx), std::fmax(std::fabs(vy), std::fabs(vz)));
+  if (!(vm > 0.0) || !(peclet >= 0.0)) return PARCEL_HIP_ERR_ARG;
+  int rc = stencil3d(gnx, gny, gnz, -1.0, 6.0, nnz, rowptr, col, val);
+  if (rc || !rowptr || !col || !val) return rc;
+  const double c[3] = {2.0 * peclet * vx / vm, 2.0 * peclet * vy / vm, 2.0 * peclet * vz / vm};
+  // coefficient of the neighbour at -d and +d, and the diagonal
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = c[d] > 0.0 ? -(1.0 + c[d]) : -1.0;
+    hi[d] = c[d] < 0.0 ? -(1.0 - c[d]) : -1.0;
+  }
+  const double diag = 6.0 + ((std::fabs(c[0]) + std::fabs(c[1])) + std::fabs(c[2]));
+  const long n = (long)gnx * gny * gnz, pxy = (long)gnx * gny;
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < n; ++r) {
+    const long row = r + 1;
+    for (int e = rowptr[r] - 1; e < rowptr[r + 1] - 1; ++e) {
+      const long dc = (long)col[e] - row;
+      double v = diag;
+      if (dc == -pxy) v = lo[2];
+      else if (dc == -gnx) v = lo[1];
+      else if (dc == -1) v = lo[0];
+      else if (dc == 1) v = hi[0];
+      else if (dc == gnx) v = hi[1];
+      else if (dc == pxy) v = hi[2];
+      val[e] = v;
+    }
+  }
+  return PARCEL_HIP_OK;
+}
+
+// kappa_i = exp(ln kmin + (ln kmax - ln kmin) u_i), u_i the SplitMix64 stream
+// `seed`: log-uniform in [kmin, kmax).
+int parcel_gen_loguniform(long n, unsigned long long seed, double kmin, double kmax, double* kappa) {
+  if (n < 0 || !kappa || !(kmin > 0.0) || !(kmax >= kmin)) return PARCEL_HIP_ERR_ARG;
+  const double a = std::log(kmin), w = std::log(kmax) - a;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) kappa[i] = std::exp(a + w * u01(seed, (uint64_t)i));
+  return PARCEL_HIP_OK;
+}
+
+// make_poisson3D_adv_CRS (problem.f90:620-770) for one rank holding the
+// whole grid: dx = 1/gnx (likewise y, z), face coefficient
+// -dy*dz / (dx*0.5*(1/k_c + 1/k_nb)), diagonal -(cw + ce + cs + cn + cb + ct)
+// summed in that order, entries b, s, w, c, e, n, t.  The reference leaves
+// its ghost layer ramda(0,..), ramda(gnx+1,..) to the caller; here a ghost
+// carries the boundary cell's own conductivity (homogeneous Dirichlet).
+int parcel_gen_diffusion3d(int gnx, int gny, int gnz, const double* kappa, int* nnz, int* rowptr, int* col,
+                           double* val) {
+  if (!kappa && rowptr) return PARCEL_HIP_ERR_ARG;
+  int rc = stencil3d(gnx, gny, gnz, -1.0, 6.0, nnz, rowptr, col, val);
+  if (rc || !rowptr || !col || !val) return rc;
+  const double dx = 1.0 / gnx, dy = 1.0 / gny, dz = 1.0 / gnz;
+  const long pxy = (long)gnx * gny;
+#pragma omp parallel for schedule(static)
+  for (int iz = 0; iz < gnz; ++iz)
+    for (int iy = 0; iy < gny; ++iy)
+      for (int ix = 0; ix < gnx; ++ix) {
+        const long r = ix + (long)iy * gnx + (long)iz * pxy;
+        const double kc = kappa[r];
+        const double kw = ix > 0 ? kappa[r - 1] : kc, ke = ix < gnx - 1 ? kappa[r + 1] : kc;
+        const double ks = iy > 0 ? kappa[r - gnx] : kc, kn = iy < gny - 1 ? kappa[r + gnx] : kc;
+        const double kb = iz > 0 ? kappa[r - pxy] : kc, kt = iz < gnz - 1 ? kappa[r + pxy] : kc;
+        const double cw = -(dy * dz / (dx * 0.5 * (1.0 / kc + 1.0 / kw)));
+        const double ce = -(dy * dz / (dx * 0.5 * (1.0 / kc + 1.0 / ke)));
+        const double cs = -(dx * dz / (dy * 0.5 * (1.0 / kc + 1.0 / ks)));
+        const double cn = -(dx * dz / (dy * 0.5 * (1.0 / kc + 1.0 / kn)));
+        const double cb = -(dx * dy / (dz * 0.5 * (1.0 / kc + 1.0 / kb)));
+        const double ct = -(dx * dy / (dz * 0.5 * (1.0 / kc + 1.0 / kt)));
+        const double cc = -(((((cw + ce) + cs) + cn) + cb) + ct);
+        long e = rowptr[r] - 1;
+        if (iz > 0) val[e++] = cb;
+        if (iy > 0) val[e++] = cs;
+        if (ix > 0) val[e++] = cw;
+        val[e++] = cc;
+        if (ix < gnx - 1) val[e++] = ce;
+        if (iy < gny - 1) val[e++] = cn;
+        if (iz < gnz - 1) val[e++] = ct;
+      }
+  return PARCEL_HIP_OK;
+}
+
+// b = A x in double-double: every product val*x exact (Dekker's product of
+// two doubles), the row's products added in CRS order with dd_add.
+int parcel_gen_spmv_dd(int n, const int* rowptr, const int* col, const double* val, const double* x, double* b_hi,
+                       double* b_lo) {
+  if (n < 0 || !rowptr || !col || !val || !x || !b_hi || !b_lo) return PARCEL_HIP_ERR_ARG;
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < n; ++r) {
+    parcel::dd acc = parcel::mkdd(0.0, 0.0);
+    for (int e = rowptr[r] - 1; e < rowptr[r + 1] - 1; ++e)
+      acc = parcel::dd_add(acc, parcel::dd_mul(parcel::mkdd(val[e], 0.0), parcel::mkdd(x[col[e] - 1], 0.0)));
+    b_hi[r] = acc.hi;
+    b_lo[r] = acc.lo;
+  }
   return PARCEL_HIP_OK;
 }
 

@@ -43,6 +43,27 @@ struct DevBuf {
   }
 };
 
+// Owning page-locked host buffer (device-to-host mirrors of small state).
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    PARCEL_HIP_CHECK(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr; n = 0;
+  }
+};
+
 // Sliced ELLPACK with slice height 64 (one wavefront), resident in HBM.
 // Row i lives in slice i/64, lane i%64; its j-th stored entry (in the CSR
 // order of the row) is at sptr[slice] + j*64 + lane; rows shorter than the

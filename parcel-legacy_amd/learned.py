@@ -12,8 +12,9 @@ Training (on the device, through the library's network kernels):
   2. Adam on minibatches of the pool, loss ||Lh Lh^T xh - bh||^2 / ||bh||^2
      (Jacobi-scaled system, Lh the network's factor).
 Use: `solve_learned` predicts every test system's factor with one batched
-inference and runs the batched FP64-refinement / FP32-PCG solve; the plain
-baseline is parcel_dcg's FP64 block-Jacobi ILU(0) PCG, one system at a time.
+inference and runs the batched FP64 PCG with the factor applied in FP32; a
+system that does not converge falls back to the plain FP64 block-Jacobi
+ILU(0) PCG.
 
     python learned.py train --out weights/mpnn_grf512.npy [--systems 2048]
 """
@@ -66,18 +67,27 @@ def load_weights(path=WEIGHTS):
     return p
 
 
-def solve_learned(net, rp, col, vals, b, x=None, rtol=1e-10, inner_rtol=1e-4, outer_itrmax=50, profile=False):
-    """Factor prediction + batched mixed solve; returns (x, niter, iret, stats, seconds)."""
+def solve_learned(net, rp, col, vals, b, x=None, rtol=1e-10, itrmax=20000, precision=ph.BATCH_APPLY32,
+                  inner_rtol=1e-4, profile=False, fallback_blocks=32):
+    """Factor prediction + batched solve (FP64 PCG with the factor applied in
+    FP32 by default; precision 32: FP64 refinement around FP32 PCG, itrmax
+    then counts corrections).  A system that ends with iret != 0 is solved
+    again by the plain FP64 PCG + block-Jacobi ILU(0) (fallback_blocks
+    blocks), which then judges its convergence.  Returns (x, niter, iret,
+    stats, seconds); stats["fell_back"] lists the systems that fell back."""
     nsys, n = b.shape
     with ph.Batch(rp, col, vals, ph.BATCH_FACTOR) as B:
         B.set_inner(inner_rtol, 20000)
         B.set_profile(profile)
+        B.set_fallback(ph.PRECON_BJ, ph.ILU0, fallback_blocks, 20000)
         x = np.zeros_like(b) if x is None else x
         t0 = time.perf_counter()
         net.factor(B)
-        _, niter, iret = B.solve(x, b, outer_itrmax, rtol, precision=32, history=False)
+        _, niter, iret = B.solve(x, b, itrmax, rtol, precision=precision, history=False)
         dt = time.perf_counter() - t0
-        return x, niter, iret, B.stats(), dt
+        st = B.stats()
+        st["fell_back"] = [int(s) for s in np.flatnonzero(B.fell_back())]
+        return x, niter, iret, st, dt
 
 
 def train(out, n_systems=N_TRAIN, chunk=64, steps=3000, minibatch=4, lr=1e-3, seed=0, log=print, nx=NX, ny=NY,

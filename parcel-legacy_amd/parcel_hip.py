@@ -119,6 +119,10 @@ SYMBOLS = {
     "parcel_gen_poisson3d_spd": (_I, [_I, _I, _I, _IP, _P, _P, _P]),
     "parcel_gen_uniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
     "parcel_gen_normal": (_I, [ctypes.c_long, ctypes.c_ulonglong, _P]),
+    "parcel_gen_convdiff3d": (_I, [_I, _I, _I, _D, _D, _D, _D, _IP, _P, _P, _P]),
+    "parcel_gen_loguniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
+    "parcel_gen_diffusion3d": (_I, [_I, _I, _I, _P, _IP, _P, _P, _P]),
+    "parcel_gen_spmv_dd": (_I, [_I, _P, _P, _P, _P, _P, _P]),
     "parcel_gen_poisson3d_ddm": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "parcel_hip_dcg_ddm": (_I, [_I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D,
                                 _P, _I, _I, _I, _I, _I, _IP]),
@@ -130,6 +134,8 @@ SYMBOLS = {
     "parcel_hip_batch_solve": (_I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
     "parcel_hip_batch_set_inner": (_I, [_P, _D, _I]),
     "parcel_hip_batch_set_strips": (_I, [_P, _I]),
+    "parcel_hip_batch_set_fallback": (_I, [_P, _I, _I, _I, _I]),
+    "parcel_hip_batch_get_fallback": (_I, [_P, _P]),
     "parcel_hip_batch_set_profile": (_I, [_P, _I]),
     "parcel_hip_batch_get_stats": (_I, [_P, _P]),
     "parcel_hip_batch_spmv": (_I, [_P, _P, _P]),
@@ -171,6 +177,7 @@ class BatchStats(ctypes.Structure):
         ("nx", ctypes.c_int),
         ("ny", ctypes.c_int),
         ("stencil5", ctypes.c_int),
+        ("fallback_systems", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -218,7 +225,8 @@ class DeviceArray:
             self.dtype, self.shape = host.dtype, host.shape
         else:
             self.dtype, self.shape = np.dtype(dtype), (int(n),)
-        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
         p = ctypes.c_void_p()
         _check(lib.parcel_hip_malloc(ctypes.byref(p), self.nbytes))
         self.ptr = p
@@ -263,6 +271,53 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def _size(a):
+    return int(np.prod(a.shape))
+
+
+def _as(a, dtype, size=None, name="array"):
+    """An input the library only reads: a DeviceArray of exactly `dtype` (and
+    `size` elements), or anything numpy converts, copied to a C-contiguous
+    array of `dtype` when it is not one already."""
+    dtype = np.dtype(dtype)
+    if isinstance(a, DeviceArray):
+        if a.dtype != dtype:
+            raise ValueError("%s: device array of %s, expected %s" % (name, a.dtype, dtype))
+    else:
+        a = np.ascontiguousarray(a, dtype=dtype)
+    if size is not None and _size(a) != size:
+        raise ValueError("%s: %d elements, expected %d" % (name, _size(a), size))
+    return a
+
+
+def _io(a, dtype, size=None, name="array"):
+    """An array the library writes in place: it must already be a C-contiguous
+    numpy array (or DeviceArray) of exactly `dtype` and `size` elements, since
+    a converted copy would not return the result."""
+    dtype = np.dtype(dtype)
+    if isinstance(a, DeviceArray):
+        ok = a.dtype == dtype
+    else:
+        ok = isinstance(a, np.ndarray) and a.dtype == dtype and a.flags["C_CONTIGUOUS"] and a.flags["WRITEABLE"]
+    if not ok:
+        raise ValueError("%s: expected a writeable C-contiguous %s array" % (name, dtype))
+    if size is not None and _size(a) != size:
+        raise ValueError("%s: %d elements, expected %d" % (name, _size(a), size))
+    return a
+
+
+def _csr(rowptr, col, val):
+    """Validated 1-based CSR inputs: (rowptr int32, col int32, val float64, n, nnz)."""
+    rowptr = _as(rowptr, np.int32, name="rowptr")
+    n = _size(rowptr) - 1
+    if n < 0:
+        raise ValueError("rowptr: needs n+1 >= 1 entries")
+    col = _as(col, np.int32, name="col")
+    nnz = _size(col)
+    val = _as(val, np.float64, nnz, name="val")
+    return rowptr, col, val, n, nnz
+
+
 def default_param(**kw):
     p = Param()
     load().parcel_hip_default_param(ctypes.byref(p))
@@ -304,6 +359,52 @@ def poisson3d_spd(gnx, gny, gnz):
     c = ctypes.c_int(0)
     _check(lib.parcel_gen_poisson3d_spd(gnx, gny, gnz, ctypes.byref(c), _ptr(rp), _ptr(col), _ptr(val)))
     return rp, col, val
+
+
+def convdiff3d(gnx, gny, gnz, peclet=0.5, vel=(1.0, 0.5, 0.25)):
+    """3D 7-point first-order upwind convection-diffusion, cell Peclet
+    max|v| h / (2D) = peclet, divided by D*h (1-based CSR)."""
+    lib = load()
+    nnz = _gen_csr(lib.parcel_gen_convdiff3d, gnx, gny, gnz, peclet, *vel)
+    n = gnx * gny * gnz
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    c = ctypes.c_int(0)
+    _check(lib.parcel_gen_convdiff3d(gnx, gny, gnz, peclet, *vel, ctypes.byref(c), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def loguniform(n, seed, kmin=1e-4, kmax=1e4):
+    k = np.empty(n, np.float64)
+    _check(load().parcel_gen_loguniform(n, seed, kmin, kmax, _ptr(k)))
+    return k
+
+
+def diffusion3d(gnx, gny, gnz, kappa):
+    """make_poisson3D_adv_CRS (src/example/problem.f90:620-770), harmonic-mean
+    faces of the per-cell conductivity kappa (1-based CSR)."""
+    lib = load()
+    kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+    if kappa.shape != (gnx * gny * gnz,):
+        raise ValueError("kappa must hold gnx*gny*gnz values")
+    nnz = ctypes.c_int(0)
+    _check(lib.parcel_gen_diffusion3d(gnx, gny, gnz, None, ctypes.byref(nnz), None, None, None))
+    n = gnx * gny * gnz
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    _check(lib.parcel_gen_diffusion3d(gnx, gny, gnz, _ptr(kappa), ctypes.byref(nnz), _ptr(rp), _ptr(col), _ptr(val)))
+    return rp, col, val
+
+
+def spmv_dd(rowptr, col, val, x):
+    """b = A x in double-double on the host: (b_hi, b_lo)."""
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    x = _as(x, np.float64, n, "x")
+    bh, bl = np.empty(n), np.empty(n)
+    _check(load().parcel_gen_spmv_dd(n, _ptr(rowptr), _ptr(col), _ptr(val), _ptr(x), _ptr(bh), _ptr(bl)))
+    return bh, bl
 
 
 def poisson3d_rhs_parcel(gnx, gny, gnz):
@@ -354,8 +455,13 @@ def dcg_ddm(x, b, grid, margin, val_dia, ioff_dia, ioff_grids, ipreflag=PRECON_B
             independ_nvec=1, addL=0, iflagAS=0):
     """parcel_dcg_ddm (src/ddm_kernel.f90:2893-2986): x, b in the padded
     layout; returns (hist[:niter], niter, iret)."""
-    nnd = int(ioff_dia.shape[0])
-    m = int(b.shape[0])
+    ioff_dia = _as(ioff_dia, np.int32, name="ioff_dia")
+    nnd = _size(ioff_dia)
+    b = _as(b, np.float64, name="b")
+    m = _size(b)
+    x = _io(x, np.float64, m, "x")
+    val_dia = _as(val_dia, np.float64, nnd * m, "val_dia")
+    ioff_grids = _as(ioff_grids, np.int32, nnd * len(grid), "ioff_grids")
     ng = np.ascontiguousarray(grid, dtype=np.int32)
     nd = np.ones(3, np.int32)
     hist = np.zeros(max(itrmax, 1))
@@ -421,8 +527,8 @@ def dcg(x, b, rowptr, col, val, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=10
         _entry="parcel_hip_dcg"):
     """parcel_dcg: solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    b, x = _as(b, np.float64, n, "b"), _io(x, np.float64, n, "x")
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -444,8 +550,8 @@ def dgmres(x, b, rowptr, col, val, GMRES_m=4, GMRES_GSflag=1, ipreflag=PRECON_NO
            Neuman_itr=1, nBlock=2000):
     """parcel_dgmres: restarted GMRES(m); solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    b, x = _as(b, np.float64, n, "b"), _io(x, np.float64, n, "x")
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -461,8 +567,8 @@ def dcagmres(x, b, rowptr, col, val, CAGMRES_sstep=4, CAGMRES_tstep=1, CAGMRES_b
              addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000):
     """parcel_dcagmres: CA-GMRES(s,t); solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    b, x = _as(b, np.float64, n, "b"), _io(x, np.float64, n, "x")
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -490,8 +596,8 @@ def dcbcg(x, b, rowptr, col, val, CBCG_kstep=4, CBCG_Eigenflag=2, power_method_i
           Neuman_itr=1, nBlock=2000):
     """parcel_dcbcg: k-step Chebyshev-basis CG; solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    b, x = _as(b, np.float64, n, "b"), _io(x, np.float64, n, "x")
     hist = np.zeros(max(itrmax, 1) + 1, np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -508,8 +614,8 @@ def dcg_mixed(x, b, rowptr, col, val, inner_rtol=1e-4, inner_itrmax=10000, ipref
               iovlflag=0, Neuman_itr=1, nBlock=2000):
     """FP64 refinement around FP32 PCG; solves in place into x; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    b, x = _as(b, np.float64, n, "b"), _io(x, np.float64, n, "x")
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -525,8 +631,10 @@ def ddcg(xh, xl, bh, bl, rowptr, col, vh, vl, ipreflag=PRECON_NONE, ILU_method=I
          PRECISION_b=2, PRECISION_x=2, PRECISION_PRECONDITION=2):
     """parcel_ddcg: double-double PCG, x (hi, lo) in place; returns (hist[:niter], niter, iret)."""
     lib = load()
-    n = int(rowptr.shape[0]) - 1
-    nnz = int(col.shape[0])
+    rowptr, col, vh, n, nnz = _csr(rowptr, col, vh)
+    vl = None if vl is None else _as(vl, np.float64, nnz, "vl")
+    bh, bl = _as(bh, np.float64, n, "bh"), None if bl is None else _as(bl, np.float64, n, "bl")
+    xh, xl = _io(xh, np.float64, n, "xh"), None if xl is None else _io(xl, np.float64, n, "xl")
     hist = np.zeros(max(itrmax, 1), np.float64)
     it = ctypes.c_int(itrmax)
     iret = ctypes.c_int(-1)
@@ -544,8 +652,7 @@ class Plan:
     def __init__(self, rowptr, col, val, param=None, **kw):
         lib = load()
         self.lib = lib
-        self.n = int(rowptr.shape[0]) - 1
-        nnz = int(col.shape[0])
+        rowptr, col, val, self.n, nnz = _csr(rowptr, col, val)
         p = param if param is not None else default_param(**kw)
         h = ctypes.c_void_p()
         _check(lib.parcel_hip_plan_create(ctypes.byref(h), self.n, self.n, nnz, 1, _ptr(val), _ptr(rowptr),
@@ -553,6 +660,7 @@ class Plan:
         self.h = h
 
     def solve(self, x, b, itrmax, rtolmax):
+        b, x = _as(b, np.float64, self.n, "b"), _io(x, np.float64, self.n, "x")
         hist = np.zeros(max(itrmax, 1), np.float64)
         it = ctypes.c_int(itrmax)
         iret = ctypes.c_int(-1)
@@ -567,9 +675,11 @@ class Plan:
         _check(self.lib.parcel_hip_plan_set_profile(self.h, 1 if on else 0))
 
     def spmv(self, x, y):
+        x, y = _as(x, np.float64, self.n, "x"), _io(y, np.float64, self.n, "y")
         _check(self.lib.parcel_hip_plan_spmv(self.h, _ptr(x), _ptr(y)))
 
     def precon(self, r, s):
+        r, s = _as(r, np.float64, self.n, "r"), _io(s, np.float64, self.n, "s")
         _check(self.lib.parcel_hip_plan_precon(self.h, _ptr(r), _ptr(s)))
 
     def stats(self):
@@ -601,8 +711,10 @@ class Plan:
 
 
 def dot(x, y):
+    n = _size(x)
+    x, y = _as(x, np.float64, n, "x"), _as(y, np.float64, n, "y")
     r = ctypes.c_double(0.0)
-    _check(load().parcel_hip_dot(int(x.shape[0]), _ptr(x), _ptr(y), ctypes.byref(r)))
+    _check(load().parcel_hip_dot(n, _ptr(x), _ptr(y), ctypes.byref(r)))
     return r.value
 
 
@@ -613,10 +725,12 @@ class Batch:
     def __init__(self, rowptr, col, vals, precon=BATCH_JACOBI, nsys=None):
         lib = load()
         self.lib = lib
-        self.n = int(rowptr.shape[0]) - 1
-        nnz = int(col.shape[0])
+        rowptr = _as(rowptr, np.int32, name="rowptr")
+        col = _as(col, np.int32, name="col")
+        self.n, nnz = _size(rowptr) - 1, _size(col)
         if nsys is None:
-            nsys = int(vals.shape[0]) if len(vals.shape) == 2 else int(np.prod(vals.shape)) // nnz
+            nsys = int(vals.shape[0]) if len(vals.shape) == 2 else _size(vals) // max(nnz, 1)
+        vals = _as(vals, np.float64, nsys * nnz, "vals")
         self.nsys = nsys
         h = ctypes.c_void_p()
         _check(lib.parcel_hip_batch_create(ctypes.byref(h), nsys, self.n, nnz, _ptr(rowptr), _ptr(col), _ptr(vals),
@@ -625,10 +739,14 @@ class Batch:
 
     def set_factor(self, Ls, Lw, Ld):
         """bf16 bit patterns (uint16 arrays of nsys*n, host or DeviceArray)."""
+        nv = self.nsys * self.n
+        Ls, Lw, Ld = (_as(a, np.uint16, nv, nm) for a, nm in ((Ls, "Ls"), (Lw, "Lw"), (Ld, "Ld")))
         _check(self.lib.parcel_hip_batch_set_factor(self.h, _ptr(Ls), _ptr(Lw), _ptr(Ld)))
 
     def solve(self, x, b, itrmax, rtol, precision=64, history=True):
         """x, b: (nsys, n) host arrays or DeviceArrays; returns (hist, niter, iret)."""
+        nv = self.nsys * self.n
+        b, x = _as(b, np.float64, nv, "b"), _io(x, np.float64, nv, "x")
         hist = np.zeros((self.nsys, max(itrmax, 1))) if history else None
         niter = np.zeros(self.nsys, np.int32)
         iret = np.zeros(self.nsys, np.int32)
@@ -645,12 +763,25 @@ class Batch:
     def set_profile(self, on):
         _check(self.lib.parcel_hip_batch_set_profile(self.h, 1 if on else 0))
 
+    def set_fallback(self, ipreflag=PRECON_BJ, ILU_method=ILU0, precon_thblock=32, itrmax=20000):
+        """Re-solve systems that end with iret != 0 by parcel_dcg's FP64 PCG
+        (itrmax <= 0: off; FACTOR batches start with these defaults)."""
+        _check(self.lib.parcel_hip_batch_set_fallback(self.h, ipreflag, ILU_method, precon_thblock, itrmax))
+
+    def fell_back(self):
+        """Per system: True if the last solve fell back."""
+        f = np.zeros(self.nsys, np.int32)
+        _check(self.lib.parcel_hip_batch_get_fallback(self.h, _ptr(f)))
+        return f.astype(bool)
+
     def stats(self):
         st = BatchStats()
         _check(self.lib.parcel_hip_batch_get_stats(self.h, ctypes.byref(st)))
         return st.as_dict()
 
     def spmv(self, x, y):
+        nv = self.nsys * self.n
+        x, y = _as(x, np.float64, nv, "x"), _io(y, np.float64, nv, "y")
         _check(self.lib.parcel_hip_batch_spmv(self.h, _ptr(x), _ptr(y)))
 
     def factor(self):
@@ -660,8 +791,10 @@ class Batch:
         return tuple(out)
 
     def precon(self, r, z):
-        fn = self.lib.parcel_hip_batch_precon_apply32 if np.dtype(r.dtype) == np.float32 else \
-            self.lib.parcel_hip_batch_precon_apply
+        f32 = np.dtype(r.dtype) == np.float32
+        fn = self.lib.parcel_hip_batch_precon_apply32 if f32 else self.lib.parcel_hip_batch_precon_apply
+        dt, nv = (np.float32 if f32 else np.float64), self.nsys * self.n
+        r, z = _as(r, dt, nv, "r"), _io(z, dt, nv, "z")
         _check(fn(self.h, _ptr(r), _ptr(z)))
 
     def close(self):
@@ -733,6 +866,8 @@ class MpnnTrainer:
         self.count = 0
 
     def add(self, batch, x, b):
+        nv = batch.nsys * batch.n
+        x, b = _as(x, np.float64, nv, "x"), _as(b, np.float64, nv, "b")
         _check(self.lib.parcel_hip_mpnn_trainer_add(self.h, batch.h, _ptr(x), _ptr(b)))
         self.count += batch.nsys
 

@@ -265,3 +265,55 @@ def test_batch_stencil_spmv_bit_exact(nx, ny, nsys, sym):
         assert np.array_equal(hist[s, :no], ho), s
         assert np.array_equal(x[s], xo), s
 
+
+
+def _fallback_problem():
+    nx, ny, nsys = 64, 32, 3
+    vals = []
+    for s in range(nsys):
+        rp, col, v = ph.diffusion2d(nx, ny, np.exp(ph.grf2d(64, 32, 8.0, 2.0, 900 + s)))
+        vals.append(v)
+    b = np.stack([ph.normal(nx * ny, 950 + s) for s in range(nsys)])
+    return nx, rp, col, np.ascontiguousarray(np.stack(vals)), np.ascontiguousarray(b)
+
+
+def test_learned_fallback_bit_exact():
+    # systems 0 and 2 get a useless factor (M = I) and a budget of 150
+    # iterations, so they end with iret = 1 and fall back to FP64 PCG +
+    # block-Jacobi ILU(0); system 1 gets its IC(0) factor and converges
+    print("learned fallback", flush=True)
+    nx, rp, col, vals, b = _fallback_problem()
+    nsys, n = b.shape
+    one = orc.f32_to_bf16(np.ones(n, np.float32))
+    zero = np.zeros(n, np.uint16)
+    fac = [(zero, zero, one)] * nsys
+    ic = [orc.f32_to_bf16(a.astype(np.float32)) for a in orc.ic0_5pt(rp, vals[1], nx)]
+    fac[1] = tuple(ic)
+    Ls, Lw, Ld = (np.ascontiguousarray(np.concatenate([f[k] for f in fac])) for k in range(3))
+    itrmax = 150
+    with ph.Batch(rp, col, vals, ph.BATCH_FACTOR) as B:
+        B.set_factor(Ls, Lw, Ld)
+        B.set_fallback(ph.PRECON_BJ, ph.ILU0, 4, 5000)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, itrmax, 1e-10, precision=ph.BATCH_APPLY32)
+        fb = B.fell_back()
+        assert B.stats()["fallback_systems"] == 2
+        # the same batch with the fallback off reports the failures as they are
+        B.set_fallback(itrmax=0)
+        x2 = np.zeros_like(b)
+        _, n2, i2 = B.solve(x2, b, itrmax, 1e-10, precision=ph.BATCH_APPLY32)
+        assert not B.fell_back().any() and list(i2) == [1, 0, 1] and list(n2[[0, 2]]) == [itrmax, itrmax]
+    assert list(fb) == [True, False, True]
+    for s in (0, 2):
+        xo, ho, no, io = orc.pcg(rp, col, vals[s], b[s], ipreflag=2, ilu_method=0, nblocks=4, itrmax=5000,
+                                 rtol=1e-10)
+        assert (niter[s], iret[s]) == (no, io) and io == 0, s
+        k = min(no, itrmax)
+        assert np.array_equal(hist[s, :k], ho[:k]) and np.all(hist[s, k:] == 0)
+        assert np.array_equal(x[s], xo), s
+    F64, F32 = orc.derive(*fac[1], nx)
+    M = orc.FactorPrecon(F64, F32, nx, apply32=True)
+    xo, ho, no, io = orc.pcg_factor(rp, col, vals[1], b[1], M, itrmax=itrmax, rtol=1e-10)
+    assert (niter[1], iret[1]) == (no, io) and io == 0
+    assert np.array_equal(hist[1, :no], ho) and np.array_equal(x[1], xo)
+    assert np.array_equal(x2[1], xo)
