@@ -30,6 +30,7 @@ means behind `roofline`.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload c5|c4|both]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -161,10 +162,13 @@ def run_c4(args):
                       "kernels": kernels}
         plan.close()
 
-    head = min(methods, key=lambda k: methods[k]["time_to_solution_s"])
+    # the configured method (FP64 refinement around FP32 PCG + FP32 block-
+    # Jacobi ILU(0)) is the line's method whether or not it is the fastest;
+    # its plain comparator is FP64 PCG with the same preconditioner
+    head = "mixed_bj" if "mixed_bj" in methods else min(methods, key=lambda k: methods[k]["time_to_solution_s"])
     H = methods[head]
-    plain = [k for k in methods if not k.startswith("mixed_")]
-    best_plain = min(plain, key=lambda k: methods[k]["time_to_solution_s"]) if plain else None
+    hb = head[len("mixed_"):] if head.startswith("mixed_") else head
+    best_plain = hb if hb in methods else None
     # dominant kernel of the headline method: the one with most time per solve
     dom_name, dom = max(((k, v) for k, v in H["kernels"].items() if "bytes" in v),
                         key=lambda kv: kv[1]["mean_ms"] * kv[1]["calls_per_solve"])
@@ -231,10 +235,190 @@ def run_c4(args):
         "speedup_vs_plain_fp64": (round(methods[best_plain]["time_to_solution_s"] / H["time_to_solution_s"], 3)
                                   if best_plain else None),
         "plain_fp64_reference": best_plain,
+        "fastest_method": min(methods, key=lambda k: methods[k]["time_to_solution_s"]),
         "methods": methods,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+
+
+def spmv_bytes(n, nnz):
+    """FP64 CSR SpMV: values and columns 12 B/nnz, row pointers, x read, y written."""
+    return 12 * nnz + 4 * (n + 1) + 16 * n
+
+
+def gmres_step_bytes(n, nnz, nnz_l, nnz_u, j):
+    """Inner step j (0-based) of right-preconditioned GMRES with MGS: z = M^-1 v_j
+    (the block-Jacobi ILU(0) apply), w = A z, j+1 MGS passes (read v_i, read
+    and write w: 24 B each), the norm of w (8 B) and the scaling into v_{j+1}
+    (read w, write v: 16 B)."""
+    return ilu0_apply_bytes(n, nnz_l, nnz_u) + spmv_bytes(n, nnz) + 24 * n * (j + 1) + 24 * n
+
+
+def run_c2(args):
+    """3D upwind convection-diffusion 128^3 (cell Peclet 0.5), BJ ILU(0) 64
+    blocks, rtol 1e-9: GMRES(30) (MGS) and CA-GMRES(s=8, t=4) with the Newton
+    basis from the Leja-ordered Ritz values of the 32 x 32 Hessenberg, TSQR."""
+    import numpy as np
+    import parcel_hip as ph
+
+    g = args.c2_grid
+    n = g ** 3
+    t0 = time.perf_counter()
+    rp, col, val = ph.convdiff3d(g, g, g, 0.5, (1.0, 0.5, 0.25))
+    nnz = int(col.shape[0])
+    D = ph.DeviceArray
+    xt_d, b_d, x_d = D(ph.uniform(n, 1)), D(n=n), D(n=n)
+    log("c2: generated %d^3 n=%d nnz=%d in %.2fs" % (g, n, nnz, time.perf_counter() - t0))
+    methods = {}
+    variants = {"gmres30": dict(solver=3, GMRES_m=30, GMRES_GSflag=1),
+                "cagmres_s8t4_newton": dict(solver=4, CAGMRES_sstep=8, CAGMRES_tstep=4, CAGMRES_basis=2,
+                                            CAGMRES_QRflag=3),
+                "cagmres_s8t4_newton_reference_step": dict(solver=4, CAGMRES_sstep=8, CAGMRES_tstep=4,
+                                                           CAGMRES_basis=1, CAGMRES_QRflag=3)}
+    apply_ms = None
+    for name, kw in variants.items():
+        t = time.perf_counter()
+        plan = ph.Plan(rp, col, val, ipreflag=2, ILU_method=0, precon_thblock=64, profile=0, **kw)
+        setup = time.perf_counter() - t
+        if not methods:
+            plan.spmv(xt_d, b_d)
+        times = []
+        for k in range(args.warmup + args.steps):
+            x_d.zero()
+            ph.load().parcel_hip_device_synchronize()
+            t = time.perf_counter()
+            hist, niter, iret = plan.solve(x_d, b_d, 5000, 1e-9)
+            if k >= args.warmup:
+                times.append(time.perf_counter() - t)
+        r_d = D(n=n)
+        plan.spmv(x_d, r_d)
+        bb = b_d.numpy()
+        res = float(np.linalg.norm(bb - r_d.numpy()) / np.linalg.norm(bb))
+        st = plan.stats()
+        if apply_ms is None:
+            apply_ms = plan.time_kernel(2, 20)
+            spmv_ms = plan.time_kernel(1, 20)
+            nl, nu = st["bj_nnz_l"], st["bj_nnz_u"]
+        methods[name] = {"time_to_solution_s": float(np.mean(times)), "niter": niter, "iret": iret,
+                         "final_estimate": float(hist[-1]), "true_rel_res": res, "setup_s": setup,
+                         "device_ms": st["solve_device_ms"]}
+        log("c2 %s: %.4f s, niter %d, true res %.2e" % (name, methods[name]["time_to_solution_s"], niter, res))
+        plan.close()
+    # GMRES(30)'s algorithmic bytes over its solve: inner step j of every cycle
+    G = methods["gmres30"]
+    tot = sum(gmres_step_bytes(n, nnz, nl, nu, i % 30) for i in range(G["niter"]))
+    solve_gbs = tot / G["time_to_solution_s"] / 1e9
+    ab = ilu0_apply_bytes(n, nl, nu)
+    ap_gbs = ab / (apply_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_bj_apply (block-Jacobi ILU(0), 64 blocks: forward and backward "
+                "level-scheduled sweeps, one workgroup per block)", "achieved": round(ap_gbs, 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ap_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                "bytes_per_launch": ab, "mean_ms": round(apply_ms, 4),
+                "solve_level": {"gmres30_algorithmic_bytes": tot, "achieved": round(solve_gbs, 1),
+                                "frac": round(solve_gbs / PEAK_HBM_GBS, 4)},
+                "fused_cg_spmv_kernel_ms": round(spmv_ms, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        bh = b_d.numpy()
+        t = time.perf_counter()
+        _, _, no, _ = orc.gmres(rp, col, val, bh, ipreflag=2, ilu_method=0, nblocks=64, itrmax=5000, rtol=1e-9, m=30,
+                                gsflag=1)
+        tt = time.perf_counter() - t
+        tset, titer = orc.last_times() if hasattr(orc, "last_times") else (0.0, tt)
+        cpu = {"value": round(tt, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": "oracle GMRES(30) + BJ ILU(0) 64 blocks, the whole C2 solve (%d iterations, "
+                         "factorisation included) once" % no}
+        log("c2 cpu baseline %.2f s" % tt)
+    return {"workload": "3D 7-point first-order upwind convection-diffusion %d^3 (n=%d, nnz=%d), cell Peclet "
+                        "max|v| h / (2D) = 0.5, velocity direction (1, 0.5, 0.25), x_true~U[-1,1] seed 1, "
+                        "b = A x_true, x0 = 0, rtol 1e-9" % (g, n, nnz),
+            "method": "GMRES(30) (MGS) and CA-GMRES(s=8, t=4, Newton basis, TSQR), BJ ILU(0) 64 blocks",
+            "value": G["time_to_solution_s"], "methods": methods, "roofline": roofline, "cpu_baseline": cpu}
+
+
+def ddcg_iter_bytes(n, nnz, nnz_l, nnz_u):
+    """One double-double PCG iteration with an FP64 matrix and double-double
+    block-Jacobi ILU(0) factors: dd SpMV (12 B/nnz, x and y as hi/lo pairs:
+    32n), the dd factor apply (20 B per factor entry: hi, lo, column; row
+    data, r in, s out, the backward read-modify-write and the diagonal as
+    pairs: 4(n+1)*2 + 96n) and the dd vector passes of the iteration (x, r, p,
+    q, s updates and the two dot products: 176n)."""
+    return 12 * nnz + 4 * (n + 1) + 32 * n + 20 * (nnz_l + nnz_u) + 8 * (n + 1) + 96 * n + 176 * n
+
+
+def run_c3(args):
+    """3D harmonic-mean diffusion 96^3, log-uniform conductivity in [1e-4, 1e4]
+    (seed 2), dd PCG + BJ ILU(0) 32 blocks to rtol 1e-20."""
+    import numpy as np
+    import parcel_hip as ph
+
+    g = args.c3_grid
+    n = g ** 3
+    t0 = time.perf_counter()
+    rp, col, val = ph.diffusion3d(g, g, g, ph.loguniform(n, 2, 1e-4, 1e4))
+    nnz = int(col.shape[0])
+    bh, bl = ph.spmv_dd(rp, col, val, ph.uniform(n, 102))
+    D = ph.DeviceArray
+    bh_d, bl_d, xh_d, xl_d = D(bh), D(bl), D(n=n), D(n=n)
+    log("c3: generated %d^3 n=%d nnz=%d in %.2fs" % (g, n, nnz, time.perf_counter() - t0))
+    p = ph.default_param(solver=1, ipreflag=2, ILU_method=0, precon_thblock=32)
+    lib = ph.load()
+    h = ctypes.c_void_p()
+    t = time.perf_counter()
+    ph._check(lib.parcel_hip_plan_create_dd(ctypes.byref(h), n, n, nnz, 1, ph._ptr(val), None, ph._ptr(rp),
+                                            ph._ptr(col), ctypes.byref(p), 1, 2))
+    setup = time.perf_counter() - t
+    times = []
+    hist = np.zeros(20000)
+    for k in range(args.warmup + args.steps):
+        xh_d.zero()
+        xl_d.zero()
+        lib.parcel_hip_device_synchronize()
+        it, iret = ctypes.c_int(20000), ctypes.c_int(-1)
+        t = time.perf_counter()
+        ph._check(lib.parcel_hip_plan_solve_dd(h, xh_d.ptr, xl_d.ptr, bh_d.ptr, bl_d.ptr, ctypes.byref(it), 1e-20,
+                                               ph._ptr(hist), 2, 2, ctypes.byref(iret)))
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t)
+    st = ph.Stats()
+    ph._check(lib.parcel_hip_plan_stats(h, ctypes.byref(st)))
+    nl, nu = st.bj_nnz_l, st.bj_nnz_u
+    lib.parcel_hip_plan_destroy(h)
+    niter = it.value
+    tt = float(np.mean(times))
+    by = ddcg_iter_bytes(n, nnz, nl, nu) * niter
+    gbs = by / tt / 1e9
+    log("c3 ddcg: %.4f s, niter %d, final %.3e" % (tt, niter, hist[niter - 1]))
+    roofline = {"bound": "hbm", "kernel": "whole dd-PCG iteration (dd SpMV, dd block-Jacobi ILU(0) apply, dd "
+                "vector updates and reductions), algorithmic bytes x iterations / solve time",
+                "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                "traffic": None, "bytes_per_iteration": ddcg_iter_bytes(n, nnz, nl, nu)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
+        k = args.c3_cpu_iters
+        t = time.perf_counter()
+        orc.ddcg(rp, col, val, None, bh, bl, ipreflag=2, ilu_method=0, nblocks=32, prec=2, itrmax=k, rtol=1e-30)
+        tk = time.perf_counter() - t
+        t = time.perf_counter()
+        orc.ddcg(rp, col, val, None, bh, bl, ipreflag=2, ilu_method=0, nblocks=32, prec=2, itrmax=1, rtol=1e-30)
+        t1 = time.perf_counter() - t
+        per_it = (tk - t1) / (k - 1)
+        cpu = {"value": round(t1 + per_it * (niter - 1), 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": "oracle dd-PCG + dd BJ ILU(0) 32 blocks on the same system: %d iterations timed (%.4f s per "
+                         "iteration after a 1-iteration run with the set-up, %.2f s), scaled to the GPU's %d "
+                         "iterations" % (k, per_it, t1, niter)}
+        log("c3 cpu baseline %.4f s/it" % per_it)
+    return {"workload": "3D 7-point harmonic-mean diffusion %d^3 (n=%d, nnz=%d), kappa log-uniform in [1e-4, 1e4] "
+                        "seed 2, x_true~U[-1,1] seed 102, b = A x_true in double-double, x0 = 0, rtol 1e-20 on "
+                        "rnorm_hi / bnorm_hi" % (g, n, nnz),
+            "method": "double-double PCG, FP64 matrix, double-double BJ ILU(0) 32 blocks",
+            "value": tt, "niter": niter, "setup_s": setup, "final_estimate": float(hist[niter - 1]),
+            "roofline": roofline, "cpu_baseline": cpu}
 
 
 def lscan_bytes(n, nsys, t=4, tv=4, tz=None):
@@ -285,11 +469,14 @@ def run_c5(args):
         B.spmv(x_d, y)
         return float(max(np.linalg.norm(b[s] - y[s]) / np.linalg.norm(b[s]) for s in range(nsys)))
 
-    def batched(name, kind, precision, params=None, steps=args.steps, warmup=args.warmup, strips=1):
+    def batched(name, kind, precision, params=None, steps=args.steps, warmup=args.warmup, strips=1,
+                ic0_blocks=None):
         t = time.perf_counter()
         B = ph.Batch(rp, col, vals_d, kind, nsys=nsys)
         B.set_inner(args.inner_rtol, 20000)
-        if kind != ph.BATCH_JACOBI:
+        if ic0_blocks is not None:
+            B.set_ic0_blocks(ic0_blocks)
+        elif kind != ph.BATCH_JACOBI:
             B.set_strips(strips)
         setup = time.perf_counter() - t
         net = ph.Mpnn(params) if params is not None else None
@@ -353,6 +540,11 @@ def run_c5(args):
     ns = min(ssweep, key=ssweep.get)
     learned = batched("learned_apply32", ph.BATCH_FACTOR, ph.BATCH_APPLY32, params=p, strips=ns)
     learned["strip_sweep_s"] = ssweep
+    # the fair plain comparator: plain FP64 PCG with block-Jacobi IC(0) (=
+    # block-Jacobi ILU(0) of these symmetric systems in exact arithmetic),
+    # the same 64 systems in the same batched launches, timed like the headline
+    batched("plain_fp64_bj_batched", ph.BATCH_IC0, ph.BATCH_FP64, ic0_blocks=args.c5_bj_blocks)
+    methods["plain_fp64_bj_batched"]["blocks"] = args.c5_bj_blocks
     if args.c5_ablations:
         ablations(ns)
     return finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights)
@@ -450,16 +642,23 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
     if not args.no_cpu_baseline:
         import oracle as orc
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(os.cpu_count() or 1, 16))))
-        k = args.cpu_iters
-        t = time.perf_counter()
-        orc.pcg(rp, col, vals[0], b[0], ipreflag=2, ilu_method=0, nblocks=nb, itrmax=k, rtol=1e-30)
-        tset, titer = orc.last_times()
-        per_it = titer / k
-        total_its = int(np.sum(its))
-        cpu = {"value": round(per_it * total_its + tset * nsys, 3), "unit": "s", "cores": threads, "kind": "port",
-               "sample": "oracle FP64 PCG + BJ ILU(0) (%d blocks) on test system 0 for %d iterations "
-                         "(%.4f s/iteration, setup %.2f s), scaled to the plain GPU runs' %d iterations and %d "
-                         "setups over the %d systems" % (nb, k, per_it, tset, total_its, nsys, nsys)}
+        ks = min(args.cpu_systems, nsys)
+        runs = []
+        for rep in range(1 + args.cpu_reps):  # one warm-up run, then the timed ones
+            t = time.perf_counter()
+            cits = []
+            for s in range(ks):
+                _, _, no, io = orc.pcg(rp, col, vals[s], b[s], ipreflag=2, ilu_method=0, nblocks=args.c5_bj_blocks,
+                                       itrmax=20000, rtol=1e-10)
+                cits.append(no)
+            if rep:
+                runs.append(time.perf_counter() - t)
+        med = float(np.median(runs))
+        cpu = {"value": round(med * nsys / ks, 3), "unit": "s", "cores": threads, "kind": "port",
+               "sample": "oracle FP64 PCG + BJ ILU(0) (%d blocks; set-up included) solving test systems 0..%d to "
+                         "1e-10 (%s iterations): median of %d runs %.3f s (runs %s), times %d/%d for the 64-system "
+                         "workload" % (args.c5_bj_blocks, ks - 1, cits, len(runs), med,
+                                       [round(v, 3) for v in runs], nsys, ks)}
     meta = {}
     mp = os.path.splitext(weights)[0] + ".json"
     if os.path.exists(mp):
@@ -473,9 +672,19 @@ def finish_c5(args, rp, col, vals, b, n, nnz, nsys, p, methods, learned, weights
                   "PCG (64 systems per launch), factor solves on %d strip(s) of grid lines per system "
                   "(M = diag(L_k L_k^T)); time includes the network inference" % learned["strips"],
         "headline": "learned_apply32",
-        "speedup_vs_plain_fp64": round(plain["time_to_solution_s"] / learned["time_to_solution_s"], 3),
-        "plain_fp64_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
-                                "one system at a time, solve time only (factorisation excluded)" % nb,
+        "speedup_vs_plain_fp64": round(methods["plain_fp64_bj_batched"]["time_to_solution_s"] /
+                                       learned["time_to_solution_s"], 3),
+        "plain_fp64_reference": "plain_fp64_bj_batched: FP64 PCG + block-Jacobi IC(0) with %d blocks of 16 lines "
+                                "(block-Jacobi ILU(0) of these symmetric systems in exact arithmetic; its "
+                                "iteration counts equal parcel_dcg's), all 64 systems in the same batched "
+                                "launches as the headline" % args.c5_bj_blocks,
+        "speedup_vs_ic0_fp64": (round(methods["ic0_fp64"]["time_to_solution_s"] / learned["time_to_solution_s"], 3)
+                                if "ic0_fp64" in methods else None),
+        "speedup_vs_parcel_dcg_one_system_at_a_time": round(plain["time_to_solution_s"] /
+                                                            learned["time_to_solution_s"], 3),
+        "parcel_dcg_reference": "plain_fp64_bj: parcel_dcg FP64 PCG + BJ ILU(0) %d blocks (fastest of the sweep), "
+                                "one system at a time as the reference API solves them, solve time only "
+                                "(factorisation excluded)" % nb,
         "methods": methods,
         "training": meta,
         "roofline": roofline,
@@ -488,7 +697,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c5", "c4", "both"], default="both")
+    ap.add_argument("--workload", choices=["c5", "c4", "c2", "c3", "both", "all"], default="all",
+                    help="both: C5 and C4; all: C5, C4, C2 and C3")
+    ap.add_argument("--c2-grid", type=int, default=128)
+    ap.add_argument("--c3-grid", type=int, default=96)
+    ap.add_argument("--c3-cpu-iters", type=int, default=60)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--methods", default="bj,pj,mixed_bj,mixed_pj")
     ap.add_argument("--c5-systems", type=int, default=64)
@@ -502,6 +715,10 @@ def main():
     ap.add_argument("--weights", default=os.path.join(ROOT, "parcel-legacy_amd", "weights", "mpnn_grf512.npy"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-systems", type=int, default=4, help="C5 CPU baseline: systems solved to convergence")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="C5 CPU baseline: timed runs (median), after a warm-up")
+    ap.add_argument("--c5-bj-blocks", type=int, default=32,
+                    help="blocks of the batched block-Jacobi comparator (a divisor of 512)")
     ap.add_argument("--calib", action="store_true",
                     help="also run one plain 2 GiB device-to-device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -512,24 +729,40 @@ def main():
     import parcel_hip as ph
 
     ph.init(0)
-    c5 = c4 = None
-    if args.workload in ("c5", "both"):
+    c5 = c4 = c2 = c3 = None
+    if args.workload in ("c5", "both", "all"):
         if os.path.exists(args.weights):
             c5 = run_c5(args)
         else:
             log("c5 skipped: no trained weights at %s" % args.weights)
-    if args.workload in ("c4", "both"):
+    if args.workload in ("c4", "both", "all"):
         c4 = run_c4(args)
+    if args.workload in ("c2", "all"):
+        c2 = run_c2(args)
+    if args.workload in ("c3", "all"):
+        c3 = run_c3(args)
     H = c5 if c5 is not None else c4
+    if H is None:
+        H = {"workload": (c2 or c3)["workload"], "method": (c2 or c3)["method"], "headline": None,
+             "speedup_vs_plain_fp64": None, "plain_fp64_reference": None, "methods": (c2 or {}).get("methods"),
+             "value": (c2 or c3)["value"], "dtype": "f64" if c2 else "dd", "roofline": (c2 or c3)["roofline"],
+             "cpu_baseline": (c2 or c3)["cpu_baseline"]}
     cfg = {"workload": H["workload"], "method": H["method"], "headline": H["headline"],
            "speedup_vs_plain_fp64": H["speedup_vs_plain_fp64"], "plain_fp64_reference": H["plain_fp64_reference"],
            "methods": H["methods"], "parallelism": "1 GPU"}
     if c5 is not None:
         cfg["training"] = c5["training"]
+        for k in ("speedup_vs_ic0_fp64", "speedup_vs_parcel_dcg_one_system_at_a_time", "parcel_dcg_reference"):
+            cfg[k] = c5[k]
         if c4 is not None:
             cfg["c4_mixed_vs_plain"] = {k: c4[k] for k in ("workload", "method", "headline", "value",
                                                             "speedup_vs_plain_fp64", "plain_fp64_reference",
-                                                            "methods", "roofline", "cpu_baseline")}
+                                                            "fastest_method", "methods", "roofline",
+                                                            "cpu_baseline")}
+    if c2 is not None and H is not c2:
+        cfg["c2_convdiff_gmres"] = c2
+    if c3 is not None:
+        cfg["c3_ddcg"] = c3
     line = {
         "metric": METRIC,
         "value": round(H["value"], 5),

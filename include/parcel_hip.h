@@ -205,6 +205,40 @@ int parcel_hip_dcbcg(int icomm, double* vecx, const double* vecb, int n, int gn,
  * triangular with a non-negative diagonal, written to host memory R. */
 int parcel_hip_tsqr_r(int n, int nc, const double* V, int ldv, double* R);
 
+/* --- DIA format entry points: the matrix as nnd diagonals of n values
+ * (diaA(k + (j-1)*n) is row k of diagonal j, coupling it to global column
+ * k + istart - 1 + offset(j)), src/wrapper.f90: parcel_dcg_dia :635-724 (GPU
+ * build) / :2086-2176 (CPU build), parcel_dbicgstab_dia :454 / :1905,
+ * parcel_dgmres_dia :821 / :2273, parcel_dcbcg_dia :1246 / :2698,
+ * parcel_dcagmres_dia :1457 / :2909; the arguments after nnd are the CRS
+ * entry points'.  The diagonals are converted to CRS as dia2crs does
+ * (src/matrix.f90:7536-7576, zero entries dropped) for the preconditioner
+ * set-up, and the fused CG kernel reads the diagonals themselves (MatVec_dia,
+ * :2309-2379, diagonals in stored order: the same row sums). */
+int parcel_hip_dcg_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                       const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS, int* itrmax,
+                       double rtolmax, double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                       int independ_nvec, int nBlock, int* iret);
+int parcel_hip_dbicgstab_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                             const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                             int iflagAS, int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                             int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock, int* iret);
+int parcel_hip_dgmres_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                          const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS,
+                          int* itrmax, double rtolmax, double* reshistory, int iovlflag, int precon_thblock,
+                          int Neuman_itr, int independ_nvec, int nBlock, int GMRES_m, int GMRES_GSflag, int* iret);
+int parcel_hip_dcagmres_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                            const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                            int iflagAS, int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                            int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock, int CAGMRES_sstep,
+                            int CAGMRES_tstep, int CAGMRES_basis, int CAGMRES_QRflag, int* iret);
+int parcel_hip_dcbcg_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                         const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS, int* itrmax,
+                         double rtolmax, double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, int CBCG_kstep, int CBCG_Eigenflag, int power_method_itrmax,
+                         int CAARNOLDI_sstep, int CAARNOLDI_tstep, int CAARNOLDI_basis, int CAARNOLDI_QRflag,
+                         int* iret);
+
 /* Mixed-precision PCG (no counterpart in the reference; its FP64 PCG is
  * KSP_pCG, src/pcg.f90:224-892): parcel_dcg's arguments plus the inner
  * tolerance and iteration limit.  FP64 refinement x += d, r = b - A x, each
@@ -265,6 +299,10 @@ int parcel_hip_plan_create(parcel_hip_plan** plan, int n, int gn, int nnz, int i
 int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vecb,
                           int* itrmax, double rtolmax, double* reshistory, int* iret);
 int parcel_hip_plan_stats(const parcel_hip_plan* plan, parcel_hip_stats* stats);
+/* a plan from DIA diagonals (the set-up split of PARCEL_KSP_DIA,
+ * src/krylov.f90); arguments as parcel_hip_dcg_dia's */
+int parcel_hip_plan_create_dia(parcel_hip_plan** plan, int n, int gn, int istart, const double* diaA,
+                               const int* offset, int nnd, const parcel_hip_param* prm);
 /* double-double plans (the split of parcel_ddcg): the solver must be CG */
 int parcel_hip_plan_create_dd(parcel_hip_plan** plan, int n, int gn, int nnz, int istart,
                               const double* crsA_hi, const double* crsA_lo,
@@ -375,6 +413,15 @@ int parcel_hip_batch_set_inner(parcel_hip_batch* batch, double inner_rtol, int i
  * factor's couplings between strips dropped; nstrip wavefronts per system.
  * Default 1 (M = L L^T exactly). */
 int parcel_hip_batch_set_strips(parcel_hip_batch* batch, int nstrip);
+/* Block-Jacobi IC(0) for a PARCEL_BATCH_IC0 batch: every system's factor is
+ * recomputed with each of nblocks strips of grid lines (lines
+ * [ny k / nblocks, ny (k+1) / nblocks)) factorised on its own, its
+ * couplings to the strip below dropped, and the factor solves run per strip
+ * (set_strips(nblocks)).  On these symmetric systems this is block-Jacobi
+ * ILU(0) (parcel_dcg ipreflag 2, ILU_method 0, src/matrix.f90:8249-8392) in
+ * exact arithmetic when nblocks divides ny (the strips are then
+ * divide_region's row blocks); the rounding is the factor solves' own. */
+int parcel_hip_batch_set_ic0_blocks(parcel_hip_batch* batch, int nblocks);
 /* Fallback of the learned path: after a batched solve, every system whose
  * solve ended with iret != 0 is solved again, from x = 0, by parcel_dcg's
  * FP64 PCG (KSP_pCG, src/pcg.f90:224-892) with preconditioner ipreflag
@@ -474,6 +521,13 @@ int parcel_gen_loguniform(long n, unsigned long long seed, double kmin, double k
  * nnz. */
 int parcel_gen_diffusion3d(int gnx, int gny, int gnz, const double* kappa, int* nnz, int* rowptr, int* col,
                            double* val);
+/* make_poisson2D_matrix_DIA_get_nnd / _from_CRS (src/example/problem.f90:
+ * 342-446) for one rank holding all rows: the distinct column offsets of
+ * the CSR matrix in ascending order (*nnd of them; pass offset = NULL to
+ * query nnd) and diaA (nnd*n, diagonal-major, zero where a row has no entry
+ * on the diagonal). */
+int parcel_gen_crs_to_dia(int n, const int* rowptr, const int* col, const double* val, int* nnd, int* offset,
+                          double* diaA);
 /* b = A x in double-double (exact products, dd_add in CRS order) */
 int parcel_gen_spmv_dd(int n, const int* rowptr, const int* col, const double* val, const double* x, double* b_hi,
                        double* b_lo);

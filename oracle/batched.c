@@ -51,6 +51,41 @@ int oracle_ic0_5pt(int n, int nx, const int* rowptr, const double* val, double* 
   return 0;
 }
 
+/* Block-Jacobi IC(0): the grid's ny lines cut into nstrip strips (strip k =
+ * lines [ny k / nstrip, ny (k+1) / nstrip)), each strip's diagonal block
+ * factorised on its own: the first line of a strip drops its couplings to
+ * the strip below (Ls = 0), as the block-Jacobi set-up keeps in-block
+ * entries only (src/matrix.f90:8249-8392).  For an SPD matrix this is the
+ * block-Jacobi ILU(0) preconditioner in exact arithmetic (ILU(0) = L D L^T,
+ * IC(0) = (L D^1/2)(L D^1/2)^T on the same pattern). */
+int oracle_ic0_5pt_blocks(int n, int nx, int nstrip, const int* rowptr, const double* val, double* L) {
+  double* Ls = L;
+  double* Lw = L + n;
+  double* Ld = L + 2 * (long)n;
+  const int ny = n / nx;
+  int strip = 0, first = 0;  /* current strip and its first line */
+  for (int i = 0; i < n; ++i) {
+    const int line = i / nx;
+    while (strip + 1 < nstrip && line >= (int)((long)ny * (strip + 1) / nstrip)) {
+      ++strip;
+      first = (int)((long)ny * strip / nstrip);
+    }
+    int e = rowptr[i] - 1;
+    double ls = 0.0, lw = 0.0;
+    if (i >= nx) {
+      if (line > first) ls = val[e] / Ld[i - nx];
+      ++e;
+    }
+    if (i % nx > 0) { lw = val[e] / Ld[i - 1]; ++e; }
+    const double t = fma(-lw, lw, fma(-ls, ls, val[e]));
+    if (!(t > 0.0)) return -2;
+    Ls[i] = ls;
+    Lw[i] = lw;
+    Ld[i] = sqrt(t);
+  }
+  return 0;
+}
+
 float oracle_bf16_to_f32(uint16_t h) {
   uint32_t u = (uint32_t)h << 16;
   float f;

@@ -71,6 +71,7 @@ def load():
         "oracle_leja": (None, [_I, _P, _P]),
         "oracle_sym_pinv": (None, [_I, _P, _I, _P, _I]),
         "oracle_precon_create_ras": (_P, [_I, _I, _P, _P, _P, _P, _P, _I, _P, _IP]),
+        "oracle_precon_create_as": (_P, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _IP]),
         "oracle_tsqr_r": (_I, [_I, _I, _P, _I, _P]),
         "oracle_house_r": (None, [_I, _I, _P, _I, _P]),
         "oracle_tsqr_waves": (_I, [_I]),
@@ -137,6 +138,31 @@ class RasPrecon:
     def __del__(self):
         if getattr(self, "h", None):
             load().oracle_precon_free(self.h)
+
+
+class AsPrecon:
+    """Additive Schwarz of the CRS path at one rank (AdditiveSchwarz_crs):
+    divide_region blocks extended by addL rows, D-ILU per extended block,
+    iflagAS 1 BASIC, 2 RESTRICT, 3 INTERPOLATE, 4 NONE."""
+
+    def __init__(self, rowptr, col, val, ilu_method=1, nblocks=1, addL=1, iflagAS=2, independ_nvec=1):
+        self._keep = (rowptr, col, val)
+        self.n = rowptr.shape[0] - 1
+        err = ctypes.c_int(0)
+        self.h = load().oracle_precon_create_as(ilu_method, nblocks, addL, iflagAS, independ_nvec, self.n,
+                                                _p(rowptr), _p(col), _p(val), ctypes.byref(err))
+        if not self.h:
+            raise RuntimeError("oracle additive Schwarz setup failed: %d" % err.value)
+
+    def apply(self, r):
+        s = np.empty(self.n, np.float64)
+        load().oracle_precon_apply(self.h, _p(np.ascontiguousarray(r, dtype=np.float64)), _p(s))
+        return s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().oracle_precon_free(self.h)
+            self.h = None
 
 
 class Precon:
@@ -431,6 +457,7 @@ def _sig_batched(lib):
         "oracle_pcg_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _RP, _P, _IP, _IP]),
         "oracle_pcg_mixed_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _RP, _P, _IP, _IP, _IP]),
         "oracle_ic0_5pt": (_I, [_I, _I, _P, _P, _P]),
+        "oracle_ic0_5pt_blocks": (_I, [_I, _I, _I, _P, _P, _P]),
         "oracle_derive": (None, [_I, _I, _I, _P, _P, _P]),
         "oracle_precon_create_factor": (_P, [_I, _I, _P, _P]),
         "oracle_lscan_apply": (None, [_I, _I, _P, _P, _P]),
@@ -454,11 +481,15 @@ def _blib():
     return lib
 
 
-def ic0_5pt(rowptr, val, nx):
-    """IC(0) factor (Ls, Lw, Ld) of a five-point matrix in natural order."""
+def ic0_5pt(rowptr, val, nx, nblocks=1):
+    """IC(0) factor (Ls, Lw, Ld) of a five-point matrix in natural order;
+    nblocks > 1: block-Jacobi IC(0) over strips of grid lines."""
     n = rowptr.shape[0] - 1
     L = np.zeros(3 * n)
-    rc = _blib().oracle_ic0_5pt(n, nx, _p(rowptr), _p(val), _p(L))
+    if nblocks > 1:
+        rc = _blib().oracle_ic0_5pt_blocks(n, nx, nblocks, _p(rowptr), _p(val), _p(L))
+    else:
+        rc = _blib().oracle_ic0_5pt(n, nx, _p(rowptr), _p(val), _p(L))
     if rc != 0:
         raise RuntimeError("IC(0) breakdown")
     return L[:n].copy(), L[n:2 * n].copy(), L[2 * n:].copy()

@@ -7,6 +7,8 @@
 
 struct oracle_precon {
   int ipreflag, ilu_method, nblocks, independ_nvec, n;
+  int base; /* D-ILU arrays and vectors hold row k (1-based) at k - 1 - base
+               (0; an additive-Schwarz region: its first row - 1) */
   int* blo; /* 1-based first row of each block */
   int* bhi; /* 1-based last row of each block */
   /* point Jacobi and D-ILU */
@@ -32,7 +34,22 @@ struct oracle_precon {
   int* ras_dst;
   double* ras_re;
   double* ras_ze;
+  /* ORACLE_PRECON_AS: additive Schwarz over divide_region blocks extended by
+     addL rows; region t is a D-ILU precon (base = its first row - 1) */
+  int addL, iflagAS;
+  oracle_precon** as_reg;
+  int* olo; /* own rows of block t (1-based, inclusive) */
+  int* ohi;
+  double** as_in;
+  double** as_out;
 };
+
+/* D-ILU factor and L/U ranges of one block (rows lo..hi, 1-based) */
+int oracle_dilu_block(oracle_precon* p, int lo, int hi);
+void oracle_dilu_apply_block(const oracle_precon* p, int lo, int hi, const double* r, double* s);
+/* additive Schwarz (as.c) */
+void oracle_as_apply(const oracle_precon* p, const double* r, double* s);
+void oracle_as_free(oracle_precon* p);
 
 /* factor solves of batched.c */
 void oracle_lscan_apply(int nx, int ny, const double* F, const double* r, double* z);

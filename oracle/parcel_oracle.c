@@ -195,7 +195,7 @@ static int dilu_block(oracle_precon* p, int lo, int hi) {
       break;
     }
   }
-  for (int k = lo; k <= hi; ++k) p->dinv[k - 1] = 1.0 / D[k - lo];
+  for (int k = lo; k <= hi; ++k) p->dinv[k - 1 - p->base] = 1.0 / D[k - lo];
   free(D);
   return 0;
 }
@@ -218,7 +218,7 @@ void oracle_dilu_ranges(oracle_precon* p, int lo, int hi) {
       if (cl[j] < lo) continue;
       if (ist <= cl[j]) { le = j - 1; break; }
     }
-    p->lstart[k - 1] = ls; p->lend[k - 1] = le;
+    p->lstart[k - 1 - p->base] = ls; p->lend[k - 1 - p->base] = le;
   }
   for (int k = hi; k >= lo; --k) {
     int ist = lo + ((k - lo) / nvec) * nvec;
@@ -232,7 +232,7 @@ void oracle_dilu_ranges(oracle_precon* p, int lo, int hi) {
       if (ien < cl[j]) continue;
       ue = j + 1; break;
     }
-    p->ustart[k - 1] = us; p->uend[k - 1] = ue;
+    p->ustart[k - 1 - p->base] = us; p->uend[k - 1 - p->base] = ue;
   }
 }
 
@@ -243,18 +243,24 @@ void oracle_dilu_ranges(oracle_precon* p, int lo, int hi) {
 static void dilu_apply_block(const oracle_precon* p, int lo, int hi,
                              const double* r, double* s) {
   const int* cl = p->col; const double* a = p->val;
+  const int o = 1 + p->base; /* row / column k lives at index k - o */
   for (int k = lo; k <= hi; ++k) {
-    double zz = r[k - 1];
-    for (int j = p->lstart[k - 1]; j <= p->lend[k - 1]; ++j)
-      zz = fma(-s[cl[j] - 1], a[j], zz);
-    s[k - 1] = p->dinv[k - 1] * zz;
+    double zz = r[k - o];
+    for (int j = p->lstart[k - o]; j <= p->lend[k - o]; ++j)
+      zz = fma(-s[cl[j] - o], a[j], zz);
+    s[k - o] = p->dinv[k - o] * zz;
   }
   for (int k = hi; k >= lo; --k) {
     double zz = 0.0;
-    for (int j = p->ustart[k - 1]; j >= p->uend[k - 1]; --j)
-      zz = fma(s[cl[j] - 1], a[j], zz);
-    s[k - 1] = fma(-p->dinv[k - 1], zz, s[k - 1]);
+    for (int j = p->ustart[k - o]; j >= p->uend[k - o]; --j)
+      zz = fma(s[cl[j] - o], a[j], zz);
+    s[k - o] = fma(-p->dinv[k - o], zz, s[k - o]);
   }
+}
+
+int oracle_dilu_block(oracle_precon* p, int lo, int hi) { return dilu_block(p, lo, hi); }
+void oracle_dilu_apply_block(const oracle_precon* p, int lo, int hi, const double* r, double* s) {
+  dilu_apply_block(p, lo, hi, r, s);
 }
 
 /* get_ILU_LU_crs (src/matrix.f90:8249-8392) + ILU_LU_crs (:7970-8072) +
@@ -449,6 +455,10 @@ static oracle_precon* precon_create(int ipreflag, int ilu_method, int nblocks, c
 
 void oracle_precon_apply(const oracle_precon* p, const double* r, double* s) {
   int n = p->n;
+  if (p->ipreflag == ORACLE_PRECON_AS) {
+    oracle_as_apply(p, r, s);
+    return;
+  }
   if (p->ipreflag == ORACLE_PRECON_RAS) {
     for (int q = 0; q < p->ras_n; ++q) p->ras_re[q] = r[p->ras_src[q]];
     oracle_precon_apply(p->inner, p->ras_re, p->ras_ze);
@@ -561,6 +571,10 @@ oracle_precon* oracle_precon_create_ras(int n, int ne, const int* src, const int
 
 void oracle_precon_free(oracle_precon* p) {
   if (!p) return;
+  if (p->ipreflag == ORACLE_PRECON_AS) {
+    oracle_as_free(p);
+    return;
+  }
   if (p->ipreflag == ORACLE_PRECON_RAS) {
     oracle_precon_free(p->inner);
     free(p->ras_src); free(p->ras_dst); free(p->ras_re); free(p->ras_ze);

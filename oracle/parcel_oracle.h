@@ -172,6 +172,14 @@ int oracle_pcg_mixed(int n, const int* rowptr, const int* col, const double* val
 #define ORACLE_PRECON_FACTOR 9
 /* restricted additive Schwarz over overlapping blocks (see oracle_precon_create_ras) */
 #define ORACLE_PRECON_RAS 10
+/* additive Schwarz of the CRS path (AdditiveSchwarz_crs at one rank, as.c):
+ * divide_region blocks extended by addL rows, D-ILU (ilu_method 1 ILUD,
+ * else the row-sum form) per extended block, iflagAS 1 BASIC, 2 RESTRICT,
+ * 3 INTERPOLATE, 4 NONE; block Jacobi D-ILU when addL == 0 or a block has
+ * fewer than 2 addL rows */
+#define ORACLE_PRECON_AS 11
+oracle_precon* oracle_precon_create_as(int ilu_method, int nblocks, int addL, int iflagAS, int independ_nvec, int n,
+                                       const int* rowptr, const int* col, const double* val, int* err);
 oracle_precon* oracle_precon_create_ras(int n, int ne, const int* src, const int* dst, const int* rp_e,
                                         const int* col_e, const double* val_e, int nblocks, const int* starts,
                                         int* err);
@@ -184,6 +192,8 @@ int oracle_pcg_mixed_with(const oracle_precon* M, int n, const int* rowptr, cons
                           const oracle_red* red, double* hist, int* niter, int* iret, int* inner_total);
 /* IC(0) factor (Ls, Lw, Ld; n each) of a five-point matrix, 1-based CSR */
 int oracle_ic0_5pt(int n, int nx, const int* rowptr, const double* val, double* L);
+/* block-Jacobi IC(0): each of nstrip strips of grid lines factorised alone */
+int oracle_ic0_5pt_blocks(int n, int nx, int nstrip, const int* rowptr, const double* val, double* L);
 /* apply arrays (ls, 1/Ld, a_fwd, a_bwd) from a double (src 0) or bf16
  * (src 1) factor; F64 / F32 may be NULL */
 void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F32);

@@ -758,10 +758,19 @@ void detect_grid(Batch& B) {
 //   Ls_i = a(i,i-nx) / Ld_{i-nx},  Lw_i = a(i,i-1) / Ld_{i-1},
 //   Ld_i = sqrt(fma(-Lw, Lw, fma(-Ls, Ls, a_ii)))
 // (no other fill: rows i and i-1, i and i-nx share no earlier column).
-int ic0_host(const Batch& B, const std::vector<double>& vals, std::vector<double>& L) {
-  const int n = B.n, nx = B.nx, nsys = B.nsys;
+// With nblk > 1 every strip of grid lines [ny k / nblk, ny (k+1) / nblk) is
+// factorised on its own (block-Jacobi IC(0)): the first line of a strip
+// drops its couplings to the strip below, as block-Jacobi set-ups keep
+// in-block entries only.
+int ic0_host(const Batch& B, const std::vector<double>& vals, std::vector<double>& L, int nblk = 1) {
+  const int n = B.n, nx = B.nx, nsys = B.nsys, ny = B.ny;
   const long nv = (long)nsys * n;
   L.assign((size_t)(3 * nv), 0.0);
+  // first[line]: the first line of the line's strip
+  std::vector<int> first(ny > 0 ? ny : 1, 0);
+  for (int k = 0; k < nblk; ++k)
+    for (int l = (int)((long)ny * k / nblk); l < (int)((long)ny * (k + 1) / nblk); ++l)
+      first[l] = (int)((long)ny * k / nblk);
   int bad = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
   for (int s = 0; s < nsys; ++s) {
@@ -773,7 +782,10 @@ int ic0_host(const Batch& B, const std::vector<double>& vals, std::vector<double
       const int e0 = B.rowptr[i];
       int e = e0;
       double ls = 0.0, lw = 0.0;
-      if (i >= nx) { ls = v[e] / Ld[i - nx]; ++e; }
+      if (i >= nx) {
+        if (i / nx > first[i / nx]) ls = v[e] / Ld[i - nx];
+        ++e;
+      }
       if (i % nx > 0) { lw = v[e] / Ld[i - 1]; ++e; }
       const double t = std::fma(-lw, lw, std::fma(-ls, ls, v[e]));
       if (!(t > 0.0)) { bad = 1; break; }
@@ -992,6 +1004,22 @@ void ensure_hist(Batch& B, int cap) {
   }
 }
 
+// CSR values of systems [s0, s0 + cnt) read back from their SELL slots.
+void host_values(Batch& B, int s0, int cnt, std::vector<double>& hv) {
+  const long ent = B.A.entries;
+  std::vector<double> sell((size_t)ent * cnt);
+  PARCEL_HIP_CHECK(hipMemcpyAsync(sell.data(), B.A.val.p + (long)s0 * ent, sizeof(double) * ent * cnt,
+                                  hipMemcpyDeviceToHost, B.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
+  hv.resize((size_t)B.nnz * cnt);
+  const int n = B.n;
+#pragma omp parallel for schedule(static)
+  for (int s = 0; s < cnt; ++s)
+    for (int i = 0; i < n; ++i)
+      for (int e = B.rowptr[i], j = 0; e < B.rowptr[i + 1]; ++e, ++j)
+        hv[(size_t)s * B.nnz + e] = sell[(size_t)s * ent + B.hsptr[i / 64] + (long)j * 64 + i % 64];
+}
+
 // The learned path's fallback for system s: its CSR values are read back
 // from the SELL slots, a single-system plan with the fallback
 // preconditioner is set up, and parcel_dcg's FP64 PCG solves it from x = 0
@@ -999,20 +1027,14 @@ void ensure_hist(Batch& B, int cap) {
 void batch_fallback(Batch& B, int s, double* dx, const double* db, double rtol, int& nit, int& ret,
                     std::vector<double>& hist) {
   const int n = B.n, nnz = B.nnz;
-  const long ent = B.A.entries;
-  std::vector<double> sell((size_t)ent);
-  PARCEL_HIP_CHECK(hipMemcpyAsync(sell.data(), B.A.val.p + (long)s * ent, sizeof(double) * ent,
-                                  hipMemcpyDeviceToHost, B.stream));
+  std::vector<double> val;
+  host_values(B, s, 1, val);
   PARCEL_HIP_CHECK(hipMemsetAsync(dx + (long)s * n, 0, sizeof(double) * n, B.stream));
   PARCEL_HIP_CHECK(hipStreamSynchronize(B.stream));
   std::vector<int> rp1(n + 1), col1(nnz);
-  std::vector<double> val(nnz);
   for (int i = 0; i < n; ++i) {
     rp1[i] = B.rowptr[i] + 1;
-    for (int e = B.rowptr[i], j = 0; e < B.rowptr[i + 1]; ++e, ++j) {
-      val[e] = sell[B.hsptr[i / 64] + (long)j * 64 + i % 64];
-      col1[e] = B.col[e] + 1;
-    }
+    for (int e = B.rowptr[i]; e < B.rowptr[i + 1]; ++e) col1[e] = B.col[e] + 1;
   }
   rp1[n] = nnz + 1;
   parcel_hip_param prm;
@@ -1264,6 +1286,26 @@ int parcel_hip_batch_set_profile(parcel_hip_batch* h, int on) {
   if (!h) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_profile: null handle");
   h->B.profile = on != 0;
   return PARCEL_HIP_OK;
+}
+
+int parcel_hip_batch_set_ic0_blocks(parcel_hip_batch* h, int nblocks) {
+  PARCEL_API_BEGIN
+  if (!h || nblocks < 1) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_ic0_blocks: bad argument");
+  Batch& B = h->B;
+  if (B.precon != PARCEL_BATCH_IC0)
+    return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_ic0_blocks: the batch was not created with PARCEL_BATCH_IC0");
+  if (nblocks > B.ny) return api_fail(PARCEL_HIP_ERR_ARG, "batch_set_ic0_blocks: more blocks than grid lines");
+  PARCEL_HIP_CHECK(hipSetDevice(B.device));
+  std::vector<double> hv, L;
+  host_values(B, 0, B.nsys, hv);
+  const int rc = ic0_host(B, hv, L, nblocks);
+  if (rc != PARCEL_HIP_OK) return api_fail(rc, "batch_set_ic0_blocks: IC(0) breakdown (non-positive pivot)");
+  B.L64.upload(L, B.stream);
+  B.factor_src = 1;
+  derive_factor(B);
+  B.nstrip = nblocks;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
 }
 
 int parcel_hip_batch_set_fallback(parcel_hip_batch* h, int ipreflag, int ILU_method, int precon_thblock,

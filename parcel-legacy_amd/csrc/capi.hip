@@ -5,6 +5,7 @@
 // plan_solve + plan_destroy, the split KSP_CRS performs (src/krylov.f90:447-735).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -148,6 +149,8 @@ void parcel_hip_default_param(parcel_hip_param* prm) {
 
 namespace {
 
+int as_build(parcel_hip_plan* plan, const HostCSR& A);
+
 // Plan set-up shared by the FP64 and the double-double entry points:
 // dd_prec = 0 for FP64, else PRECISION_PRECONDITION of a dd plan; crsA_lo
 // (dd only) may be null (PRECISION_A = 1).
@@ -210,8 +213,17 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
     if (P.prm.solver == PARCEL_SOLVER_CBCG && (P.prm.CBCG_kstep < 1 || P.prm.CBCG_kstep > 8))
       throw hip_error("CBCG_kstep must be in 1..8", PARCEL_HIP_ERR_ARG);
     if (block_starts) P.block_starts = *block_starts;
+    // additive Schwarz of the CRS path: the blocks' D-ILU runs in an
+    // expanded plan (as_build); this plan's own set-up then has nothing to do
+    const bool as = !dd_prec && !block_starts && P.prm.ipreflag == PARCEL_PRECON_ADDITIVESCHWARZ;
+    if (as) P.prm.ipreflag = PARCEL_PRECON_NONE;
     if (dd_prec) precon_setup_dd(P, dd_prec);
     else precon_setup(P);
+    if (as) {
+      P.prm.ipreflag = PARCEL_PRECON_ADDITIVESCHWARZ;
+      int arc = as_build(h, P.hA);
+      if (arc != PARCEL_HIP_OK) throw hip_error(g_last_error, arc);
+    }
     P.stats.precon_setup_s = P.M.setup_seconds;
     P.stats.bj_blocks = P.M.nblocks;
     P.stats.bj_levels_fwd = P.M.fwd.nlev;
@@ -231,6 +243,120 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
   *out = h;
   return PARCEL_HIP_OK;
   PARCEL_API_END
+}
+
+// Additive Schwarz of the CRS path at one rank (get_DILU_DInv_AS_crs,
+// src/matrix.f90:3809-4375; AdditiveSchwarz_crs :4749-4858): the nblocks
+// blocks of divide_region, each extended by addL rows on both sides within
+// [1, n] (get_AS_region :3146-3218), D-ILU factorised on their own rows and
+// columns (DILU_DInv_AS_crs :3428-3480: ILU_method 1 ILUD, any other value
+// the row-sum form).  Expanded row q is row src[q] of its extended block;
+// the expanded block-diagonal matrix's block D-ILU runs in a second plan.
+// With addL = 0, n < addL or a block of fewer than 2 addL rows
+// (check_AS_region :3220-3287) the reference falls back to block Jacobi
+// with the D-ILU factor: this plan then becomes that block Jacobi.
+// iflagAS 1 BASIC / 2 RESTRICT gather r on every expanded row, 3
+// INTERPOLATE / 4 NONE only on owned rows (0 on the overlap); 2 / 4 keep
+// the owned rows' results, 1 / 3 add the two results of an overlap row
+// (AS_Local :5804-5904: s = 0, then lower overlaps, owned rows, upper
+// overlaps, so a row's sum is (0 + first) + second).
+int as_build(parcel_hip_plan* plan, const HostCSR& A) {
+  Plan& P = plan->P;
+  const int n = A.n, addL = P.prm.addL, flag = P.prm.iflagAS;
+  if (addL < 0) return fail(PARCEL_HIP_ERR_ARG, "additive Schwarz: addL must be >= 0");
+  if (flag < 1 || flag > 4) return fail(PARCEL_HIP_ERR_ARG, "additive Schwarz: iflagAS must be 1..4");
+  int nblk = P.prm.precon_thblock;
+  if (nblk < 1) {
+    hipDeviceProp_t prop;
+    PARCEL_HIP_CHECK(hipGetDeviceProperties(&prop, P.device));
+    nblk = prop.multiProcessorCount;
+  }
+  nblk = std::max(1, std::min(nblk, n));
+  const int dilu = P.prm.ILU_method == PARCEL_ILUD ? PARCEL_ILUD : PARCEL_MLUD;
+  std::vector<int> lo(nblk), hi(nblk);
+  bool ok = addL > 0 && n >= addL;
+  for (int t = 0; t < nblk; ++t) {
+    divide_region0(t, nblk, n, &lo[t], &hi[t]);
+    if ((hi[t] - lo[t]) / 2 < addL) ok = false;
+  }
+  if (!ok) {
+    // block Jacobi with the D-ILU factor the reference computes
+    P.prm.ipreflag = PARCEL_PRECON_BLOCKJACOBI;
+    P.prm.ILU_method = dilu;
+    P.prm.precon_thblock = nblk;
+    P.hA = A;
+    precon_setup(P);
+    return PARCEL_HIP_OK;
+  }
+  std::vector<int> src, starts, rp(1, 1), col, oa(n, -1), ob(n, -1);
+  std::vector<unsigned char> own;
+  std::vector<double> val;
+  for (int t = 0; t < nblk; ++t) {
+    const int a = std::max(lo[t] - addL, 0), b = std::min(hi[t] + addL, n);
+    const int base = (int)src.size();
+    starts.push_back(base);
+    for (int k = a; k < b; ++k) {
+      const int q = (int)src.size();
+      const bool mine = lo[t] <= k && k < hi[t];
+      src.push_back(k);
+      own.push_back(mine ? 1 : 0);
+      for (int j = A.rowptr[k]; j < A.rowptr[k + 1]; ++j) {
+        const int c = A.col[j];
+        if (c < a || c >= b) continue;
+        col.push_back(c - a + base + 1);
+        val.push_back(A.val[j]);
+      }
+      rp.push_back((int)col.size() + 1);
+      // AS_Local's phases: a lower overlap row (k < lo[t]) is added first,
+      // the owned row second, an upper overlap row (k >= hi[t]) last
+      if (mine) {
+        if (oa[k] < 0) oa[k] = q;
+        else ob[k] = q;
+      } else if (k < lo[t]) {
+        oa[k] = q;  // before the owner's contribution
+      } else {
+        ob[k] = q;  // after it
+      }
+    }
+  }
+  const int ne = (int)src.size();
+  starts.push_back(ne);
+  std::vector<int> gsrc(ne), dst(ne);
+  for (int q = 0; q < ne; ++q) {
+    gsrc[q] = (own[q] || flag == 1 || flag == 2) ? src[q] : -1;
+    dst[q] = own[q] ? src[q] : -1;
+  }
+  parcel_hip_param prm;
+  unset_param(prm);
+  prm.solver = PARCEL_SOLVER_CG;
+  prm.ipreflag = PARCEL_PRECON_BLOCKJACOBI;
+  prm.ILU_method = dilu;
+  prm.independ_nvec = P.prm.independ_nvec;
+  prm.precon_thblock = nblk;
+  prm.profile = 0;
+  parcel_hip_plan* aux = nullptr;
+  int rc = create_impl(&aux, ne, ne, (int)col.size(), 1, val.data(), nullptr, rp.data(), col.data(), &prm, 0,
+                       &starts);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(aux->P.stream);
+  (void)hipStreamDestroy(aux->P.stream);
+  aux->P.stream = P.stream;
+  P.ras_owner = aux;
+  P.ras_plan = &aux->P;
+  P.ras_n = ne;
+  P.ras_src.upload(gsrc, P.stream);
+  P.ras_dst.upload(dst, P.stream);
+  if (flag == 1 || flag == 3) {
+    P.ras_add = true;
+    P.ras_oa.upload(oa, P.stream);
+    P.ras_ob.upload(ob, P.stream);
+  }
+  P.ras_r.alloc((size_t)ne);
+  P.ras_s.alloc((size_t)ne);
+  P.M.kind = PRECON_RAS;
+  P.M.nblocks = nblk;
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  return PARCEL_HIP_OK;
 }
 
 }  // namespace
@@ -400,6 +526,148 @@ int parcel_hip_plan_solve(parcel_hip_plan* plan, double* vecx, const double* vec
   *iret = ret;
   return PARCEL_HIP_OK;
   PARCEL_API_END
+}
+
+// dia2crs (src/matrix.f90:7536-7576): row k takes diagonal j in stored
+// order at column k + offset(j) (global, 1-based), dropping zero entries
+// (minReal8 = 0, src/type.f90:445).  The plan keeps the diagonals too: its
+// fused CG kernel reads them (MatVec_dia order, the same sums).
+int parcel_hip_plan_create_dia(parcel_hip_plan** out, int n, int gn, int istart, const double* diaA, const int* offset,
+                               int nnd, const parcel_hip_param* prm) {
+  PARCEL_API_BEGIN
+  if (!out) return fail(PARCEL_HIP_ERR_ARG, "plan pointer is NULL");
+  *out = nullptr;
+  if (n < 1 || nnd < 1 || !diaA || !offset) return fail(PARCEL_HIP_ERR_ARG, "invalid DIA matrix arguments");
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::vector<int> off;
+  std::vector<double> dv;
+  fetch(offset, (size_t)nnd, off);
+  fetch(diaA, (size_t)nnd * n, dv);
+  std::vector<int> rp(n + 1), col;
+  std::vector<double> val;
+  col.reserve((size_t)n * nnd);
+  val.reserve((size_t)n * nnd);
+  for (int k = 1; k <= n; ++k) {
+    rp[k - 1] = (int)col.size() + 1;
+    const long row = (long)k - 1 + istart;
+    for (int j = 0; j < nnd; ++j) {
+      const double a = dv[(size_t)j * n + (k - 1)];
+      if (std::fabs(a) > 0.0) {
+        col.push_back((int)(row + off[j]));
+        val.push_back(a);
+      }
+    }
+  }
+  rp[n] = (int)col.size() + 1;
+  rc = create_impl(out, n, gn, (int)col.size(), istart, val.data(), nullptr, rp.data(), col.data(), prm, 0);
+  if (rc) return rc;
+  Plan& P = (*out)->P;
+  P.dia_nnd = nnd;
+  P.dia_val.alloc((size_t)nnd * n);
+  P.dia_val.upload(dv.data(), dv.size(), P.stream);
+  P.dia_off.alloc((size_t)nnd);
+  P.dia_off.upload(off.data(), off.size(), P.stream);
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+static int one_shot_dia(parcel_hip_param& prm, double* vecx, const double* vecb, int n, int gn, int istart,
+                        const double* diaA, const int* offset, int nnd, int* itrmax, double rtolmax,
+                        double* reshistory, int* iret) {
+  parcel_hip_plan* plan = nullptr;
+  int rc = parcel_hip_plan_create_dia(&plan, n, gn, istart, diaA, offset, nnd, &prm);
+  if (rc) return rc;
+  rc = parcel_hip_plan_solve(plan, vecx, vecb, itrmax, rtolmax, reshistory, iret);
+  parcel_hip_plan_destroy(plan);
+  return rc;
+}
+
+static void common_param(parcel_hip_param& prm, int solver, int ipreflag, int ILU_method, int addL, int iflagAS,
+                         int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock) {
+  unset_param(prm);
+  prm.solver = solver;
+  prm.ipreflag = ipreflag;
+  prm.ILU_method = ILU_method;
+  prm.addL = addL;
+  prm.iflagAS = iflagAS;
+  prm.iovlflag = iovlflag;
+  prm.precon_thblock = precon_thblock;
+  prm.Neuman_itr = Neuman_itr;
+  prm.independ_nvec = independ_nvec;
+  prm.nBlock = nBlock;
+}
+
+int parcel_hip_dcg_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                       const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS, int* itrmax,
+                       double rtolmax, double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                       int independ_nvec, int nBlock, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
+}
+
+int parcel_hip_dbicgstab_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                             const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                             int iflagAS, int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                             int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_BICG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
+}
+
+int parcel_hip_dgmres_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                          const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS,
+                          int* itrmax, double rtolmax, double* reshistory, int iovlflag, int precon_thblock,
+                          int Neuman_itr, int independ_nvec, int nBlock, int GMRES_m, int GMRES_GSflag, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_GMRES, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.GMRES_m = GMRES_m;
+  prm.GMRES_GSflag = GMRES_GSflag;
+  return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
+}
+
+int parcel_hip_dcagmres_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                            const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                            int iflagAS, int* itrmax, double rtolmax, double* reshistory, int iovlflag,
+                            int precon_thblock, int Neuman_itr, int independ_nvec, int nBlock, int CAGMRES_sstep,
+                            int CAGMRES_tstep, int CAGMRES_basis, int CAGMRES_QRflag, int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CAGMRES, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock,
+               Neuman_itr, independ_nvec, nBlock);
+  prm.CAGMRES_sstep = CAGMRES_sstep;
+  prm.CAGMRES_tstep = CAGMRES_tstep;
+  prm.CAGMRES_basis = CAGMRES_basis;
+  prm.CAGMRES_QRflag = CAGMRES_QRflag;
+  return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
+}
+
+int parcel_hip_dcbcg_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                         const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS, int* itrmax,
+                         double rtolmax, double* reshistory, int iovlflag, int precon_thblock, int Neuman_itr,
+                         int independ_nvec, int nBlock, int CBCG_kstep, int CBCG_Eigenflag, int power_method_itrmax,
+                         int CAARNOLDI_sstep, int CAARNOLDI_tstep, int CAARNOLDI_basis, int CAARNOLDI_QRflag,
+                         int* iret) {
+  (void)icomm;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CBCG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.CBCG_kstep = CBCG_kstep;
+  prm.CBCG_Eigenflag = CBCG_Eigenflag;
+  prm.power_method_itrmax = power_method_itrmax;
+  prm.CAARNOLDI_sstep = CAARNOLDI_sstep;
+  prm.CAARNOLDI_tstep = CAARNOLDI_tstep;
+  prm.CAARNOLDI_basis = CAARNOLDI_basis;
+  prm.CAARNOLDI_QRflag = CAARNOLDI_QRflag;
+  return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
 }
 
 static int one_shot(parcel_hip_param& prm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,

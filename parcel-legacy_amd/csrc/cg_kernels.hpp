@@ -72,6 +72,33 @@ __device__ __forceinline__ T mat_row(const SellArgsT<T>& A, long i, G g) {
 // a(i+nx,i); one system's arrays are vs apart.  A row sums its entries in
 // the CSR order s, w, c, e, n (sell_row's order for that pattern), so the
 // product is bitwise the SELL product when the matrix is exactly symmetric.
+// DIA: nnd diagonals of n values each, diagonal j of row i at val[j*n + i],
+// coupling row i to column i + off[j] (MatVec_dia, src/matrix.f90:2309-2379:
+// every diagonal in stored order; entries whose column falls outside
+// [0, n) are skipped).  A zero entry adds fma(0, x, t) = t, so the row sum
+// equals the CRS row's without the zeros (dia2crs drops them).
+template <class T>
+struct DiaArgsT {
+  const T* val;
+  const int* off;
+  int nnd;
+  long n;
+};
+template <class T, class G>
+__device__ __forceinline__ T mat_row(const DiaArgsT<T>& A, long i, G g) {
+  T t = 0;
+  for (int j = 0; j < A.nnd; ++j) {
+    const long c = i + A.off[j];
+    if (c >= 0 && c < A.n) t = fma(A.val[(long)j * A.n + i], g(c), t);
+  }
+  return t;
+}
+template <class T>
+__device__ __forceinline__ DiaArgsT<T> sys_matrix(DiaArgsT<T> A, const Bat& B) {
+  A.val += sys_id() * B.as;
+  return A;
+}
+
 template <class T>
 struct St5 {
   const T* c;

@@ -296,6 +296,37 @@ int parcel_gen_diffusion3d(int gnx, int gny, int gnz, const double* kappa, int* 
   return PARCEL_HIP_OK;
 }
 
+int parcel_gen_crs_to_dia(int n, const int* rowptr, const int* col, const double* val, int* nnd, int* offset,
+                          double* diaA) {
+  if (n < 1 || !rowptr || !col || !nnd) return PARCEL_HIP_ERR_ARG;
+  // the offsets present, ascending (make_poisson2D_matrix_DIA_get_nnd)
+  std::vector<char> seen(2 * (size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int e = rowptr[i] - 1; e < rowptr[i + 1] - 1; ++e) {
+      const long d = (long)(col[e] - 1) - i;
+      if (d <= -n || d >= n) return PARCEL_HIP_ERR_ARG;
+      seen[(size_t)(d + n)] = 1;
+    }
+  std::vector<int> offs;
+  for (long d = -(long)n; d <= n; ++d)
+    if (seen[(size_t)(d + n)]) offs.push_back((int)d);
+  *nnd = (int)offs.size();
+  if (!offset) return PARCEL_HIP_OK;
+  if (!val || !diaA) return PARCEL_HIP_ERR_ARG;
+  std::vector<int> slot(2 * (size_t)n + 1, -1);
+  for (size_t j = 0; j < offs.size(); ++j) {
+    offset[j] = offs[j];
+    slot[(size_t)(offs[j] + n)] = (int)j;
+  }
+  for (long k = 0; k < (long)offs.size() * n; ++k) diaA[k] = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int e = rowptr[i] - 1; e < rowptr[i + 1] - 1; ++e) {
+      const long d = (long)(col[e] - 1) - i;
+      diaA[(long)slot[(size_t)(d + n)] * n + i] = val[e];
+    }
+  return PARCEL_HIP_OK;
+}
+
 // b = A x in double-double: every product val*x exact (Dekker's product of
 // two doubles), the row's products added in CRS order with dd_add.
 int parcel_gen_spmv_dd(int n, const int* rowptr, const int* col, const double* val, const double* x, double* b_hi,

@@ -184,11 +184,23 @@ struct Plan {
   // expanded block-diagonal matrix (one block per extended box) lives in the
   // plan ras_plan; ras_src[p] is the row gathered into expanded row p,
   // ras_dst[p] the row it is scattered back to (-1: an overlap row, dropped)
+  // (ras_src[p] = -1: the expanded row starts from 0).  With ras_add the
+  // result is gathered instead: s[k] = (0 + z[ras_oa[k]]) + z[ras_ob[k]]
+  // (the second term only where ras_ob[k] >= 0; CRS additive Schwarz with
+  // iflagAS BASIC / INTERPOLATE)
   parcel_hip_plan* ras_owner = nullptr;
   Plan* ras_plan = nullptr;
   int ras_n = 0;
   DevBuf<int> ras_src, ras_dst;
   DevBuf<double> ras_r, ras_s;
+  bool ras_add = false;
+  DevBuf<int> ras_oa, ras_ob;
+  // DIA storage of the matrix (plans created from diagonals): the fused CG
+  // kernel reads these instead of the SELL slots (dia_nnd = 0: none)
+  int dia_nnd = 0;
+  DevBuf<double> dia_val;   // dia_nnd * n, diagonal-major
+  DevBuf<int> dia_off;      // dia_nnd
+  PinnedBuf<CGState> hstate;  // host mirror of cg, polled between chunks of iterations
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
   DevBuf<double> hist;      // residual history
@@ -235,6 +247,8 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
 
 // setup helpers (precon.hip)
 void precon_setup(Plan& P);
+// divide_region (src/common.f90:125-182): 0-based rows [lo, hi) of block `rank` of `nproc`
+void divide_region0(int rank, int nproc, int n, int* lo, int* hi);
 // skip: if non-null, the kernels return at once when *skip != 0 (a solver
 // whose loop has already ended on the device).
 void precon_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip = nullptr);

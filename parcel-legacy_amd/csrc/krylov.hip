@@ -22,6 +22,7 @@
 // With block Jacobi, K2 is the fused sweep kernel (precon.hip) followed by
 // k_cg_dot2(i).  K1 reads p_new of neighbour rows as fma(beta, p_old[c], s[c]),
 // the same rounding the owning row uses, so p never needs its own pass.
+#include <type_traits>
 #include <omp.h>
 
 #include <algorithm>
@@ -197,8 +198,8 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   launches += 4;
   PARCEL_HIP_CHECK(hipGetLastError());
 
-  CGState* h = nullptr;
-  PARCEL_HIP_CHECK(hipHostMalloc((void**)&h, sizeof(CGState), hipHostMallocDefault));
+  P.hstate.ensure(1);
+  CGState* h = P.hstate.p;
   h->done_at = 0;
   int enq = 0;  // K1 launches enqueued
   int chunk = 8;
@@ -212,8 +213,18 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
       T* pold = (it - 1) & 1 ? V.p1 : V.p0;
       T* pnew = it & 1 ? V.p1 : V.p0;
       hipEvent_t ea = Tm.on ? Tm.rec(st) : nullptr;
-      hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, A, x, (const T*)pold,
-                         (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
+      bool dia = false;
+      if constexpr (std::is_same<T, double>::value) {
+        if (P.dia_nnd > 0) {
+          const DiaArgsT<double> Ad{P.dia_val.p, P.dia_off.p, P.dia_nnd, (long)n};
+          hipLaunchKernelGGL((k_cg_spmv<T, 1, DiaArgsT<double>>), grid, block, 0, st, n, it, itrmax, rtol, Ad, x,
+                             (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
+          dia = true;
+        }
+      }
+      if (!dia)
+        hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, A, x, (const T*)pold,
+                           (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
       launches++;
       hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
       if (Tm.on) Tm.spmv.push_back({ea, eb, it});
@@ -261,10 +272,7 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   PARCEL_HIP_CHECK(hipEventRecord(ev1, st));
   PARCEL_HIP_CHECK(hipMemcpyAsync(h, S, sizeof(CGState), hipMemcpyDeviceToHost, st));
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
-  if (!h->done_at) {
-    (void)hipHostFree(h);
-    throw hip_error("CG loop did not terminate", PARCEL_HIP_ERR_HIP);
-  }
+  if (!h->done_at) throw hip_error("CG loop did not terminate", PARCEL_HIP_ERR_HIP);
   const int nit = h->niter;
   const int ret = h->iret;
   if (nit > 0 && hist_host)
@@ -291,7 +299,6 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   }
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
-  (void)hipHostFree(h);
   *niter = nit;
   *iret = ret;
   return PARCEL_HIP_OK;

@@ -44,6 +44,8 @@ int apply_k(int k) {
 // divide_region's split.
 void block_ranges(const Plan& P, int nblk, int n, std::vector<int>& blo, std::vector<int>& bhi);
 
+}  // namespace
+
 // divide_region (src/common.f90:125-182), 0-based half-open result.
 void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   if (n == 1) { *lo = 0; *hi = 1; return; }
@@ -53,6 +55,8 @@ void divide_region0(int rank, int nproc, int n, int* lo, int* hi) {
   else { l = 1 + iblock * rank + over; h = l + iblock - 1; }
   *lo = l - 1; *hi = h;
 }
+
+namespace {
 
 void block_ranges(const Plan& P, int nblk, int n, std::vector<int>& blo, std::vector<int>& bhi) {
   if (!P.block_starts.empty()) {
@@ -842,8 +846,22 @@ const float* dinv_of<float>(const Precon& M) { return M.dinv32.p; }
 __global__ void __launch_bounds__(kNT) k_ras_gather(int ne, const int* __restrict__ src, const double* __restrict__ r,
                                                     double* __restrict__ re, const int* skip) {
   if (skip && *skip) return;
-  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ne; p += (long)gridDim.x * blockDim.x)
-    re[p] = r[src[p]];
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < ne; p += (long)gridDim.x * blockDim.x) {
+    const int q = src[p];
+    re[p] = q >= 0 ? r[q] : 0.0;
+  }
+}
+// additive output: s[k] = (0 + ze[oa[k]]) + ze[ob[k]] (second term where ob >= 0)
+__global__ void __launch_bounds__(kNT) k_as_add(int n, const int* __restrict__ oa, const int* __restrict__ ob,
+                                                const double* __restrict__ ze, double* __restrict__ s,
+                                                const int* skip) {
+  if (skip && *skip) return;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    double v = 0.0 + ze[oa[k]];
+    const int b = ob[k];
+    if (b >= 0) v = v + ze[b];
+    s[k] = v;
+  }
 }
 __global__ void __launch_bounds__(kNT) k_ras_scatter(int ne, const int* __restrict__ dst, const double* __restrict__ ze,
                                                      double* __restrict__ s, const int* skip) {
@@ -867,8 +885,15 @@ void ras_apply(Plan& P, const T* r, T* s, const int* skip) {
     const dim3 g((unsigned)std::min<long>(((long)ne + kNT - 1) / kNT, 4096)), b(kNT);
     hipLaunchKernelGGL(k_ras_gather, g, b, 0, P.stream, ne, (const int*)P.ras_src.p, r, P.ras_r.p, skip);
     precon_apply_t<double>(*P.ras_plan, P.ras_r.p, P.ras_s.p, nullptr, skip);
-    hipLaunchKernelGGL(k_ras_scatter, g, b, 0, P.stream, ne, (const int*)P.ras_dst.p, (const double*)P.ras_s.p, s,
-                       skip);
+    if (P.ras_add) {
+      const int n = P.A.n;
+      const dim3 gn((unsigned)std::min<long>(((long)n + kNT - 1) / kNT, 4096));
+      hipLaunchKernelGGL(k_as_add, gn, b, 0, P.stream, n, (const int*)P.ras_oa.p, (const int*)P.ras_ob.p,
+                         (const double*)P.ras_s.p, s, skip);
+    } else {
+      hipLaunchKernelGGL(k_ras_scatter, g, b, 0, P.stream, ne, (const int*)P.ras_dst.p, (const double*)P.ras_s.p, s,
+                         skip);
+    }
   }
 }
 

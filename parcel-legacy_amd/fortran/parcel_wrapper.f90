@@ -1,9 +1,11 @@
 ! parcel_wrapper.f90 -- Fortran module PARCEL_wrapper over libparcel_hip.so.
 !
-! Keeps the procedure names and argument lists of the reference module
+! Keeps the procedure names and argument lists of the reference modules
 ! (src/wrapper.f90): parcel_dcg (:544-632 GPU build, :1995-2083 CPU build),
-! parcel_dbicgstab (:362, :1813), parcel_dgmres (:727, :2179), and
-! PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22).  Each generic
+! parcel_dbicgstab (:362, :1813), parcel_dgmres (:727, :2179),
+! parcel_dcagmres (:1355, :2807), parcel_dcbcg (:1139); parcel_ddcg
+! (src/ddwrapper.f90:184); parcel_dcg_ddm (src/src_DDM/ddm_kernel.f90:2266);
+! and PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22).  Each generic
 ! name takes either host arrays (the CPU build's real*8 / integer arrays) or
 ! device pointers (the GPU build's type(c_ptr)); both forward to the C ABI of
 ! include/parcel_hip.h.  A library error stops the program with its message,
@@ -12,7 +14,8 @@ module PARCEL_wrapper
   use iso_c_binding
   implicit none
   private
-  public :: PARCEL_Init, PARCEL_Free, parcel_dcg, parcel_dbicgstab, parcel_dgmres
+  public :: PARCEL_Init, PARCEL_Free, parcel_dcg, parcel_dbicgstab, parcel_dgmres, parcel_dcagmres, &
+       parcel_dcbcg, parcel_ddcg, parcel_dcg_ddm
 
   interface
     integer(c_int) function c_init(myrank) bind(c, name="parcel_hip_init")
@@ -47,6 +50,51 @@ module PARCEL_wrapper
       integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
       integer(c_int), value :: GMRES_m, GMRES_GSflag
     end function
+    integer(c_int) function c_dcagmres(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, sstep, tstep, basis, qrflag, iret) bind(c, name="parcel_hip_dcagmres")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, crsA, crsRow_ptr, crsCol, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: sstep, tstep, basis, qrflag
+    end function
+    integer(c_int) function c_dcbcg(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, kstep, eigenflag, pm_itrmax, ca_s, ca_t, ca_basis, ca_qr, iret) &
+         bind(c, name="parcel_hip_dcbcg")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, crsA, crsRow_ptr, crsCol, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: kstep, eigenflag, pm_itrmax, ca_s, ca_t, ca_basis, ca_qr
+    end function
+    integer(c_int) function c_ddcg(icomm, xh, xl, bh, bl, n, gn, nnz, istart, ah, al, crsRow_ptr, crsCol, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+         independ_nvec, nBlock, pa, pb, px, pp, iret) bind(c, name="parcel_hip_ddcg")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: xh, xl, bh, bl, ah, al, crsRow_ptr, crsCol, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, independ_nvec, nBlock, pa, pb, px, pp
+    end function
+    integer(c_int) function c_dcg_ddm(icomm, x, b, nrow, nnd, ioff_dia, val_dia, num_neighbor_ranks, &
+         neighbor_ranks, margin, ncomp_grids, ndiv_grids, num_grids, ioff_grids, div_direc_th, ipreflag, addL, &
+         iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, iret) &
+         bind(c, name="parcel_hip_dcg_ddm")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, nrow, nnd, num_neighbor_ranks, margin, ncomp_grids, div_direc_th
+      type(c_ptr), value :: x, b, ioff_dia, val_dia, neighbor_ranks, ndiv_grids, num_grids, ioff_grids, reshistory
+      integer(c_int), value :: ipreflag, addL, iflagAS
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    end function
   end interface
 
   procedure(c_krylov), bind(c, name="parcel_hip_dcg") :: c_dcg
@@ -60,6 +108,15 @@ module PARCEL_wrapper
   end interface
   interface parcel_dgmres
     module procedure parcel_dgmres_host, parcel_dgmres_dev
+  end interface
+  interface parcel_dcagmres
+    module procedure parcel_dcagmres_host, parcel_dcagmres_dev
+  end interface
+  interface parcel_dcbcg
+    module procedure parcel_dcbcg_host, parcel_dcbcg_dev
+  end interface
+  interface parcel_ddcg
+    module procedure parcel_ddcg_host, parcel_ddcg_dev
   end interface
 
 contains
@@ -191,5 +248,140 @@ contains
          addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, &
          independ_nvec, nBlock, GMRES_m, GMRES_GSflag, iret))
   end subroutine parcel_dgmres_dev
+
+  ! ---- CA-GMRES(s,t) (src/wrapper.f90:1355-1454 GPU build, :2807-2906 CPU build)
+  subroutine parcel_dcagmres_host(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), crsA(nnz), rtolmax
+    integer, target, intent(in) :: crsRow_ptr(n + 1), crsCol(nnz)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag
+    integer, intent(out) :: iret
+    call check(c_dcagmres(icomm, c_loc(vecx), c_loc(vecb), n, gn, nnz, istart, c_loc(crsA), c_loc(crsRow_ptr), &
+         c_loc(crsCol), ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, &
+         CAGMRES_QRflag, iret))
+  end subroutine parcel_dcagmres_host
+
+  subroutine parcel_dcagmres_dev(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, crsA, crsRow_ptr, crsCol
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag
+    integer, intent(out) :: iret
+    call check(c_dcagmres(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, &
+         independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, iret))
+  end subroutine parcel_dcagmres_dev
+
+  ! ---- Chebyshev-basis CG (src/wrapper.f90:1139-1233) ---------------------
+  subroutine parcel_dcbcg_host(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), crsA(nnz), rtolmax
+    integer, target, intent(in) :: crsRow_ptr(n + 1), crsCol(nnz)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CBCG_kstep, CBCG_Eigenflag, power_method_itrmax
+    integer, intent(in) :: CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    integer, intent(out) :: iret
+    call check(c_dcbcg(icomm, c_loc(vecx), c_loc(vecb), n, gn, nnz, istart, c_loc(crsA), c_loc(crsRow_ptr), &
+         c_loc(crsCol), ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+         CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
+  end subroutine parcel_dcbcg_host
+
+  subroutine parcel_dcbcg_dev(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, crsA, crsRow_ptr, crsCol
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CBCG_kstep, CBCG_Eigenflag, power_method_itrmax
+    integer, intent(in) :: CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    integer, intent(out) :: iret
+    call check(c_dcbcg(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, ipreflag, ILU_method, &
+         addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, &
+         independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, &
+         CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
+  end subroutine parcel_dcbcg_dev
+
+  ! ---- double-double CG (src/ddwrapper.f90:184-282) ----------------------
+  subroutine parcel_ddcg_host(icomm, vecx_hi, vecx_lo, vecb_hi, vecb_lo, n, gn, nnz, istart, crsA_hi, crsA_lo, &
+       crsRow_ptr, crsCol, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+       precon_thblock, independ_nvec, nBlock, PRECISION_A, PRECISION_b, PRECISION_x, PRECISION_PRECONDITION, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx_hi(n), vecx_lo(n)
+    real(8), target, intent(in) :: vecb_hi(n), vecb_lo(n), crsA_hi(nnz), crsA_lo(nnz), rtolmax
+    integer, target, intent(in) :: crsRow_ptr(n + 1), crsCol(nnz)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, independ_nvec, nBlock
+    integer, intent(in) :: PRECISION_A, PRECISION_b, PRECISION_x, PRECISION_PRECONDITION
+    integer, intent(out) :: iret
+    call check(c_ddcg(icomm, c_loc(vecx_hi), c_loc(vecx_lo), c_loc(vecb_hi), c_loc(vecb_lo), n, gn, nnz, istart, &
+         c_loc(crsA_hi), c_loc(crsA_lo), c_loc(crsRow_ptr), c_loc(crsCol), ipreflag, ILU_method, addL, iflagAS, &
+         itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, independ_nvec, nBlock, PRECISION_A, &
+         PRECISION_b, PRECISION_x, PRECISION_PRECONDITION, iret))
+  end subroutine parcel_ddcg_host
+
+  subroutine parcel_ddcg_dev(icomm, vecx_hi, vecx_lo, vecb_hi, vecb_lo, n, gn, nnz, istart, crsA_hi, crsA_lo, &
+       crsRow_ptr, crsCol, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+       precon_thblock, independ_nvec, nBlock, PRECISION_A, PRECISION_b, PRECISION_x, PRECISION_PRECONDITION, iret)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx_hi, vecx_lo, vecb_hi, vecb_lo, crsA_hi, crsA_lo, crsRow_ptr, crsCol
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, independ_nvec, nBlock
+    integer, intent(in) :: PRECISION_A, PRECISION_b, PRECISION_x, PRECISION_PRECONDITION
+    integer, intent(out) :: iret
+    call check(c_ddcg(icomm, vecx_hi, vecx_lo, vecb_hi, vecb_lo, n, gn, nnz, istart, crsA_hi, crsA_lo, &
+         crsRow_ptr, crsCol, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, &
+         precon_thblock, independ_nvec, nBlock, PRECISION_A, PRECISION_b, PRECISION_x, PRECISION_PRECONDITION, &
+         iret))
+  end subroutine parcel_ddcg_dev
+
+  ! ---- stencil (DDM) CG (src/src_DDM/ddm_kernel.f90:2266-2360): x, b,
+  ! ioff_dia and val_dia are type(c_ptr) in the reference in both builds
+  subroutine parcel_dcg_ddm(icomm, x, b, nrow, nnd, ioff_dia, val_dia, num_neighbor_ranks, neighbor_ranks, &
+       margin, ncomp_grids, ndiv_grids, num_grids, ioff_grids, div_direc_th, ipreflag, addL, iflagAS, itrmax, &
+       rtolmax, reshistory, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, iret)
+    integer, intent(in) :: icomm, nrow, nnd
+    type(c_ptr) :: x, b, ioff_dia, val_dia
+    integer, intent(in) :: num_neighbor_ranks, margin, ncomp_grids, div_direc_th
+    integer, target, intent(in) :: neighbor_ranks(*)
+    integer, target, intent(in) :: ndiv_grids(ncomp_grids), num_grids(ncomp_grids), ioff_grids(nnd, ncomp_grids)
+    integer :: ipreflag
+    integer, intent(inout) :: addL
+    integer, intent(in) :: iflagAS
+    integer, intent(inout) :: itrmax
+    real(8), intent(inout) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(inout) :: iret
+    call check(c_dcg_ddm(icomm, x, b, nrow, nnd, ioff_dia, val_dia, num_neighbor_ranks, c_loc(neighbor_ranks), &
+         margin, ncomp_grids, c_loc(ndiv_grids), c_loc(num_grids), c_loc(ioff_grids), div_direc_th, ipreflag, &
+         addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, &
+         independ_nvec, nBlock, iret))
+  end subroutine parcel_dcg_ddm
 
 end module PARCEL_wrapper

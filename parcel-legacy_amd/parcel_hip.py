@@ -123,6 +123,18 @@ SYMBOLS = {
     "parcel_gen_loguniform": (_I, [ctypes.c_long, ctypes.c_ulonglong, _D, _D, _P]),
     "parcel_gen_diffusion3d": (_I, [_I, _I, _I, _P, _IP, _P, _P, _P]),
     "parcel_gen_spmv_dd": (_I, [_I, _P, _P, _P, _P, _P, _P]),
+    "parcel_gen_crs_to_dia": (_I, [_I, _P, _P, _P, _IP, _P, _P]),
+    "parcel_hip_plan_create_dia": (_I, [ctypes.POINTER(_P), _I, _I, _I, _P, _P, _I, ctypes.POINTER(Param)]),
+    "parcel_hip_dcg_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I, _I,
+                                _I, _IP]),
+    "parcel_hip_dbicgstab_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I,
+                                      _I, _I, _IP]),
+    "parcel_hip_dgmres_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I, _I,
+                                   _I, _I, _I, _IP]),
+    "parcel_hip_dcagmres_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I,
+                                     _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dcbcg_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I, _I,
+                                  _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_gen_poisson3d_ddm": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "parcel_hip_dcg_ddm": (_I, [_I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D,
                                 _P, _I, _I, _I, _I, _I, _IP]),
@@ -135,6 +147,7 @@ SYMBOLS = {
     "parcel_hip_batch_set_inner": (_I, [_P, _D, _I]),
     "parcel_hip_batch_set_strips": (_I, [_P, _I]),
     "parcel_hip_batch_set_fallback": (_I, [_P, _I, _I, _I, _I]),
+    "parcel_hip_batch_set_ic0_blocks": (_I, [_P, _I]),
     "parcel_hip_batch_get_fallback": (_I, [_P, _P]),
     "parcel_hip_batch_set_profile": (_I, [_P, _I]),
     "parcel_hip_batch_get_stats": (_I, [_P, _P]),
@@ -405,6 +418,47 @@ def spmv_dd(rowptr, col, val, x):
     bh, bl = np.empty(n), np.empty(n)
     _check(load().parcel_gen_spmv_dd(n, _ptr(rowptr), _ptr(col), _ptr(val), _ptr(x), _ptr(bh), _ptr(bl)))
     return bh, bl
+
+
+def crs_to_dia(rowptr, col, val):
+    """(offset int32[nnd] ascending, diaA float64[nnd*n]) of a 1-based CSR matrix
+    (make_poisson2D_matrix_DIA_get_nnd / _from_CRS)."""
+    lib = load()
+    rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+    nnd = ctypes.c_int(0)
+    _check(lib.parcel_gen_crs_to_dia(n, _ptr(rowptr), _ptr(col), _ptr(val), ctypes.byref(nnd), None, None))
+    off = np.zeros(nnd.value, np.int32)
+    dia = np.zeros(nnd.value * n)
+    _check(lib.parcel_gen_crs_to_dia(n, _ptr(rowptr), _ptr(col), _ptr(val), ctypes.byref(nnd), _ptr(off), _ptr(dia)))
+    return off, dia
+
+
+_DIA_SOLVERS = {"cg": "parcel_hip_dcg_dia", "bicgstab": "parcel_hip_dbicgstab_dia", "gmres": "parcel_hip_dgmres_dia",
+                "cagmres": "parcel_hip_dcagmres_dia", "cbcg": "parcel_hip_dcbcg_dia"}
+
+
+def solve_dia(solver, x, b, offset, diaA, ipreflag=PRECON_NONE, ILU_method=ILUD, itrmax=1000, rtolmax=1e-8,
+              precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1, nBlock=2000,
+              extra=()):
+    """parcel_d<solver>_dia: DIA matrix (offset, diaA diagonal-major); extra:
+    the solver's own trailing parameters (GMRES_m, GSflag / CA-GMRES s, t,
+    basis, QR flag / CBCG k, eigenflag, pm itrmax, CA-Arnoldi s, t, basis, QR);
+    solves in place into x; returns (hist[:niter], niter, iret)."""
+    lib = load()
+    offset = _as(offset, np.int32, name="offset")
+    nnd = _size(offset)
+    b = _as(b, np.float64, name="b")
+    n = _size(b)
+    x = _io(x, np.float64, n, "x")
+    diaA = _as(diaA, np.float64, nnd * n, "diaA")
+    hist = np.zeros(max(itrmax, 1) + 256, np.float64)
+    it = ctypes.c_int(itrmax)
+    iret = ctypes.c_int(-1)
+    _check(getattr(lib, _DIA_SOLVERS[solver])(0, _ptr(x), _ptr(b), n, n, 1, _ptr(diaA), _ptr(offset), nnd, ipreflag,
+                                              ILU_method, addL, iflagAS, ctypes.byref(it), rtolmax, _ptr(hist),
+                                              iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock,
+                                              *[int(v) for v in extra], ctypes.byref(iret)))
+    return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
 def poisson3d_rhs_parcel(gnx, gny, gnz):
@@ -762,6 +816,10 @@ class Batch:
 
     def set_profile(self, on):
         _check(self.lib.parcel_hip_batch_set_profile(self.h, 1 if on else 0))
+
+    def set_ic0_blocks(self, nblocks):
+        """IC0 batches: block-Jacobi IC(0) over nblocks strips of grid lines."""
+        _check(self.lib.parcel_hip_batch_set_ic0_blocks(self.h, nblocks))
 
     def set_fallback(self, ipreflag=PRECON_BJ, ILU_method=ILU0, precon_thblock=32, itrmax=20000):
         """Re-solve systems that end with iret != 0 by parcel_dcg's FP64 PCG

@@ -4,6 +4,7 @@ errors (instead of crashing) when no device is present."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -27,17 +28,34 @@ FORTRAN_HEADER = os.path.join(ROOT, "include", "parcel_fortran.h")
 def test_by_reference_symbols_exported():
     text = re.sub(r"/\*.*?\*/", "", open(FORTRAN_HEADER).read(), flags=re.S)
     syms = sorted(set(re.findall(r"\b(parcel_\w+_c_)\s*\(", text)))
-    assert syms == ["parcel_dbicgstab_c_", "parcel_dcg_c_", "parcel_dgmres_c_", "parcel_free_c_", "parcel_init_c_"]
+    assert syms == ["parcel_dbicgstab_c_", "parcel_dcagmres_c_", "parcel_dcbcg_c_", "parcel_dcg_c_",
+                    "parcel_dcg_ddm_c_", "parcel_ddcg_c_", "parcel_dgmres_c_", "parcel_free_c_", "parcel_init_c_"]
     lib = ph.load()
     for s in syms:
         assert hasattr(lib, s), s
+
+
+def test_devicemem_symbols_exported():
+    # the GPU-build drivers' convention: the same names, in their own library
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "parcel_devicemem.h")).read(), flags=re.S)
+    syms = sorted(set(re.findall(r"\b(parcel_\w+_c_)\s*\(", text)) - {"parcel_init_c_", "parcel_free_c_"})
+    assert len(syms) == 7
+    dev = ctypes.CDLL(os.path.join(ROOT, "parcel-legacy_amd", "lib", "libparcel_devicemem.so"))
+    for s in syms:
+        assert hasattr(dev, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "parcel-legacy_amd", "lib",
+                                                                      "libparcel_devicemem.so")],
+                         capture_output=True, text=True)
+    defined = sorted(set(re.findall(r" T (parcel_\w+_c_)", out.stdout)))
+    assert defined == syms
 
 
 def test_fortran_module_built():
     lib_dir = os.path.join(ROOT, "parcel-legacy_amd", "lib")
     assert os.path.exists(os.path.join(lib_dir, "libparcel_fortran.so"))
     mod = open(os.path.join(lib_dir, "parcel_wrapper.mod"), "rb").read().lower()
-    for name in (b"parcel_dcg", b"parcel_dbicgstab", b"parcel_dgmres", b"parcel_init", b"parcel_free"):
+    for name in (b"parcel_dcg", b"parcel_dbicgstab", b"parcel_dgmres", b"parcel_dcagmres", b"parcel_dcbcg",
+                 b"parcel_ddcg", b"parcel_dcg_ddm", b"parcel_init", b"parcel_free"):
         assert name in mod
 
 

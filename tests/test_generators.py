@@ -152,3 +152,20 @@ def test_binding_checks_dtypes_and_sizes():
     # converted call only reaches the library, which then reports no device
     with pytest.raises(ph.ParcelError):
         ph.dcg(np.zeros(n), b.astype(np.float32), rp.astype(np.int64), col.astype(np.int64), val)
+
+
+def test_crs_to_dia_round_trip():
+    import scipy.sparse as sp
+    for rp, col, val in (ph.poisson3d_parcel(5, 4, 3), convdiff3d(4, peclet=1.0)[:3], ph.poisson2d(7, 3, -1.0)):
+        n = rp.shape[0] - 1
+        off, dia = ph.crs_to_dia(rp, col, val)
+        assert np.all(np.diff(off) > 0)
+        D = np.zeros((n, n))
+        for j, o in enumerate(off):
+            for i in range(n):
+                if 0 <= i + o < n:
+                    D[i, i + o] = dia[j * n + i]
+                else:
+                    assert dia[j * n + i] == 0.0
+        A = sp.csr_matrix((val, col - 1, rp - 1), shape=(n, n)).toarray()
+        assert np.array_equal(D, A)

@@ -317,3 +317,27 @@ def test_learned_fallback_bit_exact():
     assert (niter[1], iret[1]) == (no, io) and io == 0
     assert np.array_equal(hist[1, :no], ho) and np.array_equal(x[1], xo)
     assert np.array_equal(x2[1], xo)
+
+
+@pytest.mark.parametrize("nb", [1, 4, 8])
+def test_block_ic0_batch_bit_exact(nb):
+    # block-Jacobi IC(0): the batched form of parcel_dcg's block-Jacobi
+    # ILU(0) for these symmetric systems (exact arithmetic); every system bit
+    # for bit against the oracle PCG with the same strip factor, and its
+    # iteration count next to parcel_dcg's block-Jacobi ILU(0) on the same blocks
+    print("block ic0 batch", nb, flush=True)
+    rp, col, vals, b = batch_problem(96, 32, 3, seed0=1200)
+    nx = 96
+    with ph.Batch(rp, col, vals, ph.BATCH_IC0) as B:
+        B.set_ic0_blocks(nb)
+        x = np.zeros_like(b)
+        hist, niter, iret = B.solve(x, b, 3000, 1e-10)
+    for s in range(3):
+        F64, F32 = orc.derive(*orc.ic0_5pt(rp, vals[s], nx, nblocks=nb), nx)
+        M = orc.FactorPrecon(F64, F32, nx, nstrip=nb)
+        xo, ho, no, io = orc.pcg_factor(rp, col, vals[s], b[s], M, itrmax=3000, rtol=1e-10)
+        assert (niter[s], iret[s]) == (no, io) and io == 0, s
+        assert np.array_equal(hist[s, :no], ho) and np.array_equal(x[s], xo), s
+        _, hb, nbj, _ = orc.pcg(rp, col, vals[s], b[s], ipreflag=2, ilu_method=0, nblocks=nb, itrmax=3000,
+                                rtol=1e-10)
+        assert abs(int(niter[s]) - nbj) <= 2, (s, niter[s], nbj)

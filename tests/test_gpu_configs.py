@@ -82,8 +82,9 @@ def test_c2_cagmres_newton(c2, basis):
     g = GOLD["c2_cagmres_s8t4_basis%d" % basis]
     same(g, hist, niter, iret)
     assert sha(x) == g["x_sha256"]
+    # the same x (hash above); the norms of numpy's BLAS may round differently by host
     res = true_res(rp, col, val, x, b)
-    assert res == g["true_rel_res"]
+    assert abs(res - g["true_rel_res"]) <= 1e-10 * g["true_rel_res"]
     if basis == 2:
         # the consistent conjugate-pair step: the estimate is the true residual
         assert res < c["rtol"]

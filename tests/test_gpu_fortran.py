@@ -1,8 +1,10 @@
 """GPU tests of the Fortran module PARCEL_wrapper and the by-reference C
 entry points: a Fortran program (parcel-legacy_amd/fortran/example_poisson.f90)
-calls parcel_dcg / parcel_dbicgstab / parcel_dgmres exactly as the
-reference's drivers do, and its printed histories and solutions must equal
-the oracle's bit for bit."""
+calls every module solver exactly as the reference's drivers do, and a C
+program in the form of the reference's GPU-build driver
+(parcel-legacy_amd/fortran/example_devicemem.c) passes device arrays as
+devicemem handles through libparcel_devicemem.so; the printed histories and
+solutions must equal the oracle's bit for bit."""
 import ctypes
 import os
 import subprocess
@@ -13,16 +15,18 @@ import pytest
 
 import oracle as orc
 import parcel_hip as ph
+from problems import ddm_box_crs, ddm_ras
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "parcel-legacy_amd", "bin", "example_poisson")
+EXE_DEV = os.path.join(ROOT, "parcel-legacy_amd", "bin", "example_devicemem")
 
 
-def run_fortran(*args):
-    print("example_poisson", *args, flush=True)
-    out = subprocess.run([EXE] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+def run_fortran(*args, exe=EXE):
+    print(os.path.basename(exe), *args, flush=True)
+    out = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = out.stdout.split("\n")
     head = lines[0].split()
@@ -33,15 +37,33 @@ def run_fortran(*args):
     return niter, iret, hist, x1, xn
 
 
-@pytest.mark.parametrize("solver,g,ipre,ilu,nblk,itrmax,rtol,m,gs", [
-    (1, 16, 2, 0, 4, 500, 1e-9, 4, 1),
-    (1, 12, 1, 1, 1, 500, 1e-9, 4, 1),
-    (2, 12, 2, 1, 6, 500, 1e-9, 4, 1),
-    (3, 12, 2, 2, 3, 300, 1e-9, 8, 2),
-    (3, 12, 0, 1, 1, 300, 1e-9, 10, 1),
-])
-def test_fortran_module_matches_oracle(solver, g, ipre, ilu, nblk, itrmax, rtol, m, gs):
-    niter, iret, hist, x1, xn = run_fortran(solver, g, ipre, ilu, nblk, itrmax, rtol, m, gs)
+CASES = [
+    (1, 16, 2, 0, 4, 500, 1e-9, (4, 1)),
+    (1, 12, 1, 1, 1, 500, 1e-9, (4, 1)),
+    (2, 12, 2, 1, 6, 500, 1e-9, (4, 1)),
+    (3, 12, 2, 2, 3, 300, 1e-9, (8, 2)),
+    (3, 12, 0, 1, 1, 300, 1e-9, (10, 1)),
+    (4, 12, 2, 0, 4, 400, 1e-9, (4, 2, 1, 3)),
+    (4, 12, 1, 1, 1, 400, 1e-9, (5, 1, 0, 5)),
+    (5, 12, 2, 1, 3, 400, 1e-9, (4, 1)),
+    (6, 10, 2, 0, 4, 400, 1e-20, ()),
+    (7, 12, 2, 1, 1, 400, 1e-9, (0,)),
+    (7, 12, 3, 1, 1, 400, 1e-9, (1,)),
+]
+
+
+def oracle_case(solver, g, ipre, ilu, nblk, itrmax, rtol, p):
+    """(x1, xn, hist, niter, iret) of the oracle for a driver case."""
+    if solver == 7:
+        d = ph.poisson3d_ddm(g, g, g)
+        rp, col, val, perm, starts = ddm_box_crs(d)
+        if ipre == 3:
+            src, dst, rpe, cole, vale, st = ddm_ras(d, p[0])
+            M = orc.RasPrecon(perm.size, src, dst, rpe, cole, vale, st)
+        else:
+            M = orc.Precon(rp, col, val, ipre, ilu_method=1, starts=starts if ipre == 2 else None)
+        xo, ho, no, io = orc.pcg_factor(rp, col, val, d["b"][perm].copy(), M, itrmax=itrmax, rtol=rtol)
+        return xo[0], xo[-1], ho, no, io
     rp, col, val = ph.poisson3d_parcel(g, g, g)
     b, _ = ph.poisson3d_rhs_parcel(g, g, g)
     kw = dict(ipreflag=ipre, ilu_method=ilu, nblocks=nblk, itrmax=itrmax, rtol=rtol)
@@ -49,11 +71,37 @@ def test_fortran_module_matches_oracle(solver, g, ipre, ilu, nblk, itrmax, rtol,
         xo, ho, no, io = orc.pcg(rp, col, val, b, **kw)
     elif solver == 2:
         xo, ho, no, io = orc.bicgstab(rp, col, val, b, **kw)
+    elif solver == 3:
+        xo, ho, no, io = orc.gmres(rp, col, val, b, m=p[0], gsflag=p[1], **kw)
+    elif solver == 4:
+        xo, ho, no, io = orc.cagmres(rp, col, val, b, s=p[0], t=p[1], basis=p[2], qrflag=p[3], **kw)
+    elif solver == 5:
+        xo, ho, no, io, _ = orc.cbcg(rp, col, val, b, k=p[0], eigflag=p[1], pm_itrmax=100, **kw)
     else:
-        xo, ho, no, io = orc.gmres(rp, col, val, b, m=m, gsflag=gs, **kw)
+        z = np.zeros_like(b)
+        xo, _, ho, no, io = orc.ddcg(rp, col, val, np.zeros_like(val), b, z, prec=2, **kw)
+    return xo[0], xo[-1], ho, no, io
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fortran_module_matches_oracle(case):
+    solver, g, ipre, ilu, nblk, itrmax, rtol, p = case
+    niter, iret, hist, x1, xn = run_fortran(solver, g, ipre, ilu, nblk, itrmax, rtol, *p)
+    o1, on, ho, no, io = oracle_case(*case)
     assert (niter, iret) == (no, io)
     assert np.array_equal(hist, ho)
-    assert x1 == xo[0] and xn == xo[-1]
+    assert x1 == o1 and xn == on
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_devicemem_c_driver_matches_oracle(case):
+    # the reference GPU build's C calling convention: devicemem handles
+    solver, g, ipre, ilu, nblk, itrmax, rtol, p = case
+    niter, iret, hist, x1, xn = run_fortran(solver, g, ipre, ilu, nblk, itrmax, rtol, *p, exe=EXE_DEV)
+    o1, on, ho, no, io = oracle_case(*case)
+    assert (niter, iret) == (no, io)
+    assert np.array_equal(hist, ho)
+    assert x1 == o1 and xn == on
 
 
 def test_by_reference_c_entry_point():
@@ -80,3 +128,35 @@ def test_by_reference_c_entry_point():
                          I(4), ctypes.byref(niter), ctypes.byref(ctypes.c_double(1e-9)), P(hist), I(0), I(4), I(1),
                          I(1), I(2000), I(0), I(1), ctypes.byref(iret))
     assert iret.value == -1
+
+
+def test_by_reference_dcagmres_and_ddcg_entry_points():
+    # the host-array by-reference forms of parcel_dcagmres_c and parcel_ddcg_c
+    lib = ph.load()
+    g = 10
+    rp, col, val = ph.poisson3d_parcel(g, g, g)
+    b, _ = ph.poisson3d_rhs_parcel(g, g, g)
+    n, nnz = g ** 3, int(col.shape[0])
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    D = lambda v: ctypes.byref(ctypes.c_double(v))
+    lib.parcel_init_c_(I(0))
+    x, hist = np.zeros(n), np.zeros(500)
+    niter, iret = ctypes.c_int(400), ctypes.c_int(-7)
+    lib.parcel_dcagmres_c_(I(0), P(x), P(b), I(n), I(n), I(nnz), I(1), P(val), P(rp), P(col), I(2), I(0), I(0),
+                           I(4), ctypes.byref(niter), D(1e-9), P(hist), I(0), I(4), I(1), I(1), I(2000), I(4), I(2),
+                           I(1), I(3), ctypes.byref(iret))
+    xo, ho, no, io = orc.cagmres(rp, col, val, b, ipreflag=2, ilu_method=0, nblocks=4, itrmax=400, rtol=1e-9, s=4,
+                                 t=2, basis=1, qrflag=3)
+    assert (niter.value, iret.value) == (no, io)
+    assert np.array_equal(hist[:no], ho) and np.array_equal(x, xo)
+    xh, xl, hist = np.zeros(n), np.zeros(n), np.zeros(500)
+    niter, iret = ctypes.c_int(400), ctypes.c_int(-7)
+    z, zv = np.zeros(n), np.zeros(nnz)
+    lib.parcel_ddcg_c_(I(0), P(xh), P(xl), P(b), P(z), I(n), I(n), I(nnz), I(1), P(val), P(zv), P(rp), P(col), I(2),
+                       I(1), I(0), I(4), ctypes.byref(niter), D(1e-20), P(hist), I(0), I(5), I(1), I(1), I(2000), I(2),
+                       I(2), I(2), I(2), ctypes.byref(iret))
+    oh, ol, ho, no, io = orc.ddcg(rp, col, val, zv, b, z, ipreflag=2, ilu_method=1, nblocks=5, prec=2, itrmax=400,
+                                  rtol=1e-20)
+    assert (niter.value, iret.value) == (no, io)
+    assert np.array_equal(hist[:no], ho) and np.array_equal(xh, oh) and np.array_equal(xl, ol)

@@ -191,3 +191,32 @@ def test_lscan_strips_solve_block_diagonal(nx, ny, nstrip):
         Lk = sp.csr_matrix(Lk)
         ref[sl] = spl.spsolve_triangular(Lk.T.tocsr(), spl.spsolve_triangular(Lk, r[sl], lower=True), lower=False)
     assert np.max(np.abs(z - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_block_ic0_is_ic0_of_each_diagonal_block():
+    # block-Jacobi IC(0) over 3 strips of grid lines: each strip's factor is
+    # the IC(0) of the strip's diagonal block alone, and L L^T equals the
+    # block-diagonal part of A on the pattern
+    import parcel_hip as ph
+    nx, ny, nb = 12, 9, 3
+    rp, col, val = ph.diffusion2d(nx, ny, np.exp(ph.grf2d(16, 16, 3.0, 1.0, 8)[: nx * ny]))
+    n = nx * ny
+    Ls, Lw, Ld = orc.ic0_5pt(rp, val, nx, nblocks=nb)
+    A = sp.csr_matrix((val, col - 1, rp - 1), shape=(n, n)).toarray()
+    L = np.diag(Ld) + np.diag(Lw[1:], -1) + np.diag(Ls[nx:], -nx)
+    for k in range(nb):
+        a, b = ny * k // nb * nx, ny * (k + 1) // nb * nx
+        Ab = A[a:b, a:b]
+        Ab = Ab.copy()
+        sub = sp.csr_matrix(Ab)
+        sub.sort_indices()
+        S = orc.ic0_5pt(sub.indptr.astype(np.int32) + 1, sub.data.copy(), nx)
+        assert np.array_equal(Ls[a:b][nx:], S[0][nx:]) and np.all(Ls[a:a + nx] == 0)
+        assert np.array_equal(Lw[a:b], S[1]) and np.array_equal(Ld[a:b], S[2])
+    LLt = L @ L.T
+    Abd = np.zeros_like(A)
+    for k in range(nb):
+        a, b = ny * k // nb * nx, ny * (k + 1) // nb * nx
+        Abd[a:b, a:b] = A[a:b, a:b]
+    pattern = A != 0
+    assert np.allclose(LLt[pattern & (Abd != 0)], Abd[pattern & (Abd != 0)], rtol=1e-12, atol=1e-12)
