@@ -313,7 +313,10 @@ int as_build(parcel_hip_plan* plan, const HostCSR& A) {
         if (oa[k] < 0) oa[k] = q;
         else ob[k] = q;
       } else if (k < lo[t]) {
-        oa[k] = q;  // before the owner's contribution
+        // before the owner's contribution, which block t - 1 (processed
+        // earlier) left in oa: it moves to the second term
+        ob[k] = oa[k];
+        oa[k] = q;
       } else {
         ob[k] = q;  // after it
       }

@@ -67,16 +67,18 @@ def test_as_pcg_bit_exact(case, nb, addL, flag):
 
 
 def test_as_large_3d_pcg():
-    # 64^3 Poisson, 64 blocks of 4096 rows, one grid plane of overlap
+    # 64^3 Poisson, 64 blocks of 4096 rows (one grid plane each), half a
+    # plane of overlap (check_AS_region turns a wider overlap than half a
+    # block into block Jacobi)
     say("AS pcg 64^3")
     g = 64
     rp, col, val = ph.poisson3d_spd(g, g, g)
     n = g ** 3
     b = orc.spmv(rp, col, val, ph.uniform(n, 10))
     x = np.zeros(n)
-    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=3, ILU_method=1, precon_thblock=64, addL=g * g,
-                               iflagAS=1, itrmax=1000, rtolmax=1e-10)
-    M = orc.AsPrecon(rp, col, val, ilu_method=1, nblocks=64, addL=g * g, iflagAS=1)
+    hist, niter, iret = ph.dcg(x, b, rp, col, val, ipreflag=3, ILU_method=1, precon_thblock=64,
+                               addL=g * g // 2, iflagAS=1, itrmax=1000, rtolmax=1e-10)
+    M = orc.AsPrecon(rp, col, val, ilu_method=1, nblocks=64, addL=g * g // 2, iflagAS=1)
     xo, ho, no, io = orc.pcg_factor(rp, col, val, b, M, itrmax=1000, rtol=1e-10)
     assert (niter, iret) == (no, io) and iret == 0
     assert np.array_equal(hist, ho) and np.array_equal(x, xo)
