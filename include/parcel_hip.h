@@ -330,6 +330,17 @@ int parcel_hip_plan_precon(parcel_hip_plan* plan, const double* r, double* s);
 /* <x,y> in the library's fixed reduction order (src/vec.f90:1027
  * calc_dot_FP64_CPU; GPU reference src/CUDA/src/cu_common.cu:412 cuda_dot). */
 int parcel_hip_dot(int n, const double* x, const double* y, double* result);
+/* parcel_hip_dot without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  x and y must be
+ * device pointers (n > 0).  The kernels keep their partial sums in a fixed
+ * device array of the library (2049 doubles, 16 KiB per GPU, held until the
+ * process ends), which bounds the partials to 2048 workgroups: every n >= 0
+ * fits.  result: a device pointer receives the value without a wait; a host
+ * pointer receives it after a synchronisation.  Same bits as parcel_hip_dot.
+ * All calls on one GPU share that array: calls that overlap on one GPU (from
+ * several host threads, or streams that run concurrently) race. */
+int parcel_hip_dot_noalloc(int n, const double* x, const double* y, double* result);
 /* Time `reps` launches of the plan's dominant kernel class (1: fused CG SpMV,
  * 2: preconditioner apply) with HIP events on the plan's stream; returns the
  * mean duration in milliseconds. */

@@ -129,6 +129,7 @@ int parcel_hip_init(int myrank) {
   if (e != hipSuccess || count < 1) return fail(PARCEL_HIP_ERR_HIP, "no HIP device available");
   int dev = (myrank < 0 ? 0 : myrank) % count;
   PARCEL_HIP_CHECK(hipSetDevice(dev));
+  noalloc_scratch(dev);  // the allocation-free dot's workspace on this device
   g_device = dev;
   return PARCEL_HIP_OK;
   PARCEL_API_END
@@ -890,6 +891,20 @@ int parcel_hip_dot(int n, const double* x, const double* y, double* result) {
   if (!is_device_ptr(x)) { tx.alloc(n); tx.upload(x, n, nullptr); dx = tx.p; }
   if (!is_device_ptr(y)) { ty.alloc(n); ty.upload(y, n, nullptr); dy = ty.p; }
   *result = dot_device(n, dx, dy, nullptr);
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+int parcel_hip_dot_noalloc(int n, const double* x, const double* y, double* result) {
+  PARCEL_API_BEGIN
+  if (n < 0 || !x || !y || !result) return fail(PARCEL_HIP_ERR_ARG, "invalid argument");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (n > 0 && (!is_device_ptr(x) || !is_device_ptr(y)))
+    return fail(PARCEL_HIP_ERR_ARG, "dot_noalloc: x and y must be device pointers (staging host arrays would allocate)");
+  int dev = 0;
+  PARCEL_HIP_CHECK(hipGetDevice(&dev));
+  dot_noalloc(dev, n, x, y, result, is_device_ptr(result), nullptr);
   return PARCEL_HIP_OK;
   PARCEL_API_END
 }

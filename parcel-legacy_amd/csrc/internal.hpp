@@ -284,6 +284,9 @@ void ensure_vecs(Plan& P, int k);
 void launch_residual(Plan& P, const double* x, const double* b, double* r, double* part);
 void spmv(Plan& P, const double* x, double* y);
 double dot_device(int n, const double* x, const double* y, hipStream_t st);
+// allocation-free dot (krylov.hip): the partials live in a file-scope device array
+double* noalloc_scratch(int dev);
+void dot_noalloc(int dev, int n, const double* x, const double* y, double* out, bool out_dev, hipStream_t st);
 double time_kernel(Plan& P, int which, int reps);
 
 // C-API plumbing shared with the batched entry points (capi.hip)

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "cg_kernels.hpp"
@@ -127,6 +128,53 @@ double dot_device(int n, const double* x, const double* y, hipStream_t st) {
   PARCEL_HIP_CHECK(hipMemcpyAsync(&h, part.p + kNBMax, sizeof(double), hipMemcpyDeviceToHost, st));
   PARCEL_HIP_CHECK(hipStreamSynchronize(st));
   return h;
+}
+
+// --- allocation-free dot -----------------------------------------------------
+// The dot's workspace (kNBMax workgroup partials and the reduced value) as a
+// file-scope device array, so that parcel_hip_dot_noalloc allocates nothing.
+// Every GPU holds its own copy for the life of the process.  In host code
+// `scratch` names a host-side stand-in, never the device copy: the kernels
+// and copies use the address hipGetSymbolAddress returns for the current
+// device (noalloc_scratch).
+constexpr int kScratchDoubles = kNBMax + 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kScratchDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `dev` (which must be the current device),
+// resolved once per device and kept.
+double* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) throw hip_error("device ordinal out of range", PARCEL_HIP_ERR_ARG);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    double* ptr = nullptr;
+    PARCEL_HIP_CHECK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw hip_error("hipGetSymbolAddress returned no address for the dot scratch", PARCEL_HIP_ERR_HIP);
+    g_scratch[dev] = ptr;
+  }
+  return g_scratch[dev];
+}
+
+// <x, y> of device vectors with the partials in `scratch` (the same kernels,
+// launch shape and reduction tree as dot_device, so the same bits).  With a
+// device `out` the value stays on the device and the call does not wait;
+// otherwise it is copied to the host `out` after a stream synchronisation.
+void dot_noalloc(int dev, int n, const double* x, const double* y, double* out, bool out_dev, hipStream_t st) {
+  const int nb = tiled_nb(n);
+  if (nb > kScratchDoubles - 1) throw hip_error("dot_noalloc: more partials than the scratch holds", PARCEL_HIP_ERR_ARG);
+  double* part = noalloc_scratch(dev);
+  hipLaunchKernelGGL(k_dot_part<double>, dim3(nb), dim3(kNT), 0, st, n, x, y, part, kOne);
+  hipLaunchKernelGGL(k_reduce_one, dim3(1), dim3(kNT), 0, st, (const double*)part, nb,
+                     out_dev ? out : part + kNBMax);
+  PARCEL_HIP_CHECK(hipGetLastError());
+  if (out_dev) return;
+  PARCEL_HIP_CHECK(hipMemcpyAsync(out, part + kNBMax, sizeof(double), hipMemcpyDeviceToHost, st));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 namespace {

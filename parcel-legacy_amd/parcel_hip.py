@@ -107,6 +107,7 @@ SYMBOLS = {
     "parcel_hip_plan_spmv": (_I, [_P, _P, _P]),
     "parcel_hip_plan_precon": (_I, [_P, _P, _P]),
     "parcel_hip_dot": (_I, [_I, _P, _P, _DP]),
+    "parcel_hip_dot_noalloc": (_I, [_I, _P, _P, _P]),
     "parcel_hip_plan_time_kernel": (_I, [_P, _I, _I, _DP]),
     "parcel_hip_malloc": (_I, [ctypes.POINTER(_P), ctypes.c_ulong]),
     "parcel_hip_free": (_I, [_P]),
