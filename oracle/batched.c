@@ -86,6 +86,45 @@ int oracle_ic0_5pt_blocks(int n, int nx, int nstrip, const int* rowptr, const do
   return 0;
 }
 
+/* Block-Jacobi IC(0) of a general CRS matrix whose blocks are the strips of
+ * whole grid lines of oracle_ic0_5pt_blocks: a row's couplings are found by
+ * column (i - nx on the line below, in the same strip; i - 1 on the same
+ * line), every other off-diagonal entry is outside the factor's pattern
+ * (couplings between planes of a 3D grid whose strips are single planes).
+ * The pivots follow oracle_ic0_5pt.  This is the factor of the FP32 inner
+ * apply of the mixed solver for grid-line blocks (block ILU(0) of a
+ * symmetric matrix, src/matrix.f90:8249-8392, in exact arithmetic). */
+static double crs_entry(const int* rowptr, const int* col, const double* val, int i, int j) {
+  for (int e = rowptr[i] - 1; e < rowptr[i + 1] - 1; ++e)
+    if (col[e] - 1 == j) return val[e];
+  return 0.0;
+}
+
+int oracle_ic0_grid_blocks(int n, int nx, int nstrip, const int* rowptr, const int* col, const double* val,
+                           double* L) {
+  double* Ls = L;
+  double* Lw = L + n;
+  double* Ld = L + 2 * (long)n;
+  const int ny = n / nx;
+  int strip = 0, first = 0;
+  for (int i = 0; i < n; ++i) {
+    const int line = i / nx;
+    while (strip + 1 < nstrip && line >= (int)((long)ny * (strip + 1) / nstrip)) {
+      ++strip;
+      first = (int)((long)ny * strip / nstrip);
+    }
+    double ls = 0.0, lw = 0.0;
+    if (line > first) ls = crs_entry(rowptr, col, val, i, i - nx) / Ld[i - nx];
+    if (i % nx > 0) lw = crs_entry(rowptr, col, val, i, i - 1) / Ld[i - 1];
+    const double t = fma(-lw, lw, fma(-ls, ls, crs_entry(rowptr, col, val, i, i)));
+    if (!(t > 0.0)) return -2;
+    Ls[i] = ls;
+    Lw[i] = lw;
+    Ld[i] = sqrt(t);
+  }
+  return 0;
+}
+
 float oracle_bf16_to_f32(uint16_t h) {
   uint32_t u = (uint32_t)h << 16;
   float f;

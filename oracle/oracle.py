@@ -458,6 +458,7 @@ def _sig_batched(lib):
         "oracle_pcg_mixed_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _RP, _P, _IP, _IP, _IP]),
         "oracle_ic0_5pt": (_I, [_I, _I, _P, _P, _P]),
         "oracle_ic0_5pt_blocks": (_I, [_I, _I, _I, _P, _P, _P]),
+        "oracle_ic0_grid_blocks": (_I, [_I, _I, _I, _P, _P, _P, _P]),
         "oracle_derive": (None, [_I, _I, _I, _P, _P, _P]),
         "oracle_precon_create_factor": (_P, [_I, _I, _P, _P]),
         "oracle_lscan_apply": (None, [_I, _I, _P, _P, _P]),
@@ -490,6 +491,18 @@ def ic0_5pt(rowptr, val, nx, nblocks=1):
         rc = _blib().oracle_ic0_5pt_blocks(n, nx, nblocks, _p(rowptr), _p(val), _p(L))
     else:
         rc = _blib().oracle_ic0_5pt(n, nx, _p(rowptr), _p(val), _p(L))
+    if rc != 0:
+        raise RuntimeError("IC(0) breakdown")
+    return L[:n].copy(), L[n:2 * n].copy(), L[2 * n:].copy()
+
+
+def ic0_grid(rowptr, col, val, nx, nblocks=1):
+    """Block-Jacobi IC(0) factor (Ls, Lw, Ld) of a CRS matrix over strips of
+    whole grid lines (nx cells per line), couplings found by column: the
+    factor of the mixed solver's FP32 line-scan apply."""
+    n = rowptr.shape[0] - 1
+    L = np.zeros(3 * n)
+    rc = _blib().oracle_ic0_grid_blocks(n, nx, nblocks, _p(rowptr), _p(col), _p(val), _p(L))
     if rc != 0:
         raise RuntimeError("IC(0) breakdown")
     return L[:n].copy(), L[n:2 * n].copy(), L[2 * n:].copy()

@@ -194,6 +194,8 @@ int oracle_pcg_mixed_with(const oracle_precon* M, int n, const int* rowptr, cons
 int oracle_ic0_5pt(int n, int nx, const int* rowptr, const double* val, double* L);
 /* block-Jacobi IC(0): each of nstrip strips of grid lines factorised alone */
 int oracle_ic0_5pt_blocks(int n, int nx, int nstrip, const int* rowptr, const double* val, double* L);
+int oracle_ic0_grid_blocks(int n, int nx, int nstrip, const int* rowptr, const int* col, const double* val,
+                           double* L);
 /* apply arrays (ls, 1/Ld, a_fwd, a_bwd) from a double (src 0) or bf16
  * (src 1) factor; F64 / F32 may be NULL */
 void oracle_derive(int n, int nx, int src, const void* L, double* F64, float* F32);

@@ -508,20 +508,22 @@ __global__ void __launch_bounds__(kNT) k_bo_xadd(int n, double* __restrict__ x, 
 
 // --- host helpers ------------------------------------------------------------------
 
+// nsys systems of ny lines of nx cells, nstrip strips each; F: the apply
+// arrays (nv = nsys n apart); one workgroup per strip
 template <class T, class TV, class TZ = TV>
-void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TZ* z, const CGState* st, int it) {
-  const int nx = B.nx;
-  const dim3 g(B.nsys * B.nstrip);
+void lscan_launch(int nx, int ny, int nstrip, int nsys, long n, long nv, const T* F, const TV* r, T* w, TZ* z,
+                  const CGState* st, int it, double* sink, hipStream_t stream) {
+  const dim3 g(nsys * nstrip);
   // line layout (the oracle's lscan_layout): E1 = cells per lane of one
   // wavefront, a power of two with 64 E1 >= nx; E1 >= 8 splits the line
   // over 4 wavefronts, E1 = 4 over 2
 #define PARCEL_LSCAN(E, NW)                                                                                     \
-  if (nx == 64 * (E) * (NW) && ((B.n * sizeof(T)) % 16 == 0))                                                  \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, true, TZ>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip,  \
-                       (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);                                       \
+  if (nx == 64 * (E) * (NW) && ((n * sizeof(T)) % 16 == 0))                                                    \
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, true, TZ>), g, dim3(64 * (NW)), 0, stream, nx, ny, nstrip, n, nv,  \
+                       F, r, w, z, st, it, sink);                                                              \
   else                                                                                                          \
-    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, false, TZ>), g, dim3(64 * (NW)), 0, B.stream, nx, B.ny, B.nstrip, \
-                       (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p);
+    hipLaunchKernelGGL((k_lscan<T, TV, E, NW, false, TZ>), g, dim3(64 * (NW)), 0, stream, nx, ny, nstrip, n, nv, \
+                       F, r, w, z, st, it, sink);
   if (nx <= 64) { PARCEL_LSCAN(1, 1) }
   else if (nx <= 128) { PARCEL_LSCAN(2, 1) }
   else if (nx <= 256) { PARCEL_LSCAN(2, 2) }
@@ -529,6 +531,11 @@ void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TZ* z, const CG
   else { PARCEL_LSCAN(4, 4) }
 #undef PARCEL_LSCAN
   PARCEL_HIP_CHECK(hipGetLastError());
+}
+
+template <class T, class TV, class TZ = TV>
+void launch_lscan(const Batch& B, const T* F, const TV* r, T* w, TZ* z, const CGState* st, int it) {
+  lscan_launch<T, TV, TZ>(B.nx, B.ny, B.nstrip, B.nsys, (long)B.n, B.nv(), F, r, w, z, st, it, B.sink.p, B.stream);
 }
 
 struct EvPair {
@@ -798,6 +805,11 @@ int ic0_host(const Batch& B, const std::vector<double>& vals, std::vector<double
 }
 
 }  // namespace
+
+void parcel::lscan_apply32(const float* F, const float* r, float* w, float* z, int nx, int ny, int nstrip, long n,
+                           const CGState* st, int it, double* sink, hipStream_t stream) {
+  lscan_launch<float, float, float>(nx, ny, nstrip, 1, n, n, F, r, w, z, st, it, sink, stream);
+}
 
 void parcel::derive_factor(Batch& B) {
   const long nv = B.nv();

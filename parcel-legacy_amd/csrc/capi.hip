@@ -232,6 +232,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
     P.stats.bj_entries = P.M.fwd.entries + P.M.bwd.entries;
     P.stats.bj_nnz_l = P.M.fwd.real_entries;
     P.stats.bj_nnz_u = P.M.bwd.real_entries;
+    P.stats.bj_lscan = P.M.lscan.on ? 1 : 0;
     P.hA = HostCSR();  // setup copy no longer needed
     plan_workspace(P, 1);
     PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));

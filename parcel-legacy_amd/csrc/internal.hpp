@@ -147,6 +147,18 @@ struct CGState {
 enum PreconKind { PRECON_NONE = 0, PRECON_PJ = 1, PRECON_BJ = 2, PRECON_RAS = 3 };
 enum BJMode { BJ_ILU0 = 0, BJ_DILU = 1 };
 
+// FP32 block-Jacobi apply as line scans of the block IC(0) factor (blocks of
+// whole grid lines, symmetric couplings; precon.hip: lscan_setup): the apply
+// arrays ls, 1/Ld, -Lw/Ld, -Lw(i+1)/Ld (each n floats), the forward values w
+// and the sink line of the scans' filler stores.
+struct LscanBJ {
+  bool on = false;
+  int nx = 0, ny = 0, nstrip = 0;
+  DevBuf<float> F;
+  DevBuf<float> w;
+  DevBuf<double> sink;
+};
+
 struct Precon {
   int kind = PRECON_NONE;
   int mode = BJ_ILU0;
@@ -155,6 +167,7 @@ struct Precon {
   DevBuf<double> dinv;  // point Jacobi
   DevBuf<float> dinv32;
   Sweep fwd, bwd;       // block Jacobi
+  LscanBJ lscan;        // FP32 line-scan form of the block factors (mixed solver)
   bool has32 = false;   // FP32 copies of the factors exist
   // double-double factors (precon_setup_dd): Dinv lo parts, apply precision
   DevBuf<double> dinv_lo;
@@ -247,6 +260,11 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
 
 // setup helpers (precon.hip)
 void precon_setup(Plan& P);
+// z = (L L^T)^-1 r in FP32 over nstrip strips of a grid of ny lines of nx
+// cells (batched.hip, k_lscan with one system); w: forward values; a strip
+// returns at once when st says the CG loop ended before iteration it
+void lscan_apply32(const float* F, const float* r, float* w, float* z, int nx, int ny, int nstrip, long n,
+                   const CGState* st, int it, double* sink, hipStream_t stream);
 // divide_region (src/common.f90:125-182): 0-based rows [lo, hi) of block `rank` of `nproc`
 void divide_region0(int rank, int nproc, int n, int* lo, int* hi);
 // skip: if non-null, the kernels return at once when *skip != 0 (a solver

@@ -253,6 +253,7 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
   int chunk = 8;
   const bool bj = P.M.kind == PRECON_BJ;
   const bool ras = P.M.kind == PRECON_RAS;
+  const bool lscan = std::is_same<T, float>::value && bj && P.M.lscan.on;
   const T* dinv = V.dinv;
   while (enq < itrmax + 1) {
     const int k = std::min(chunk, itrmax + 1 - enq);
@@ -277,7 +278,20 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
       hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
       if (Tm.on) Tm.spmv.push_back({ea, eb, it});
       if (it > itrmax) break;  // the last K1 only closes the loop
-      if (ras) {
+      if (lscan) {
+        // r -= alpha q; s = M^-1 r by line scans of the block IC(0) factor; dots
+        hipLaunchKernelGGL((k_cg_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, kOne);
+        hipEvent_t ec0 = Tm.on ? Tm.rec(st) : nullptr;
+        const LscanBJ& G = P.M.lscan;
+        lscan_apply32(G.F.p, (const float*)V.r, G.w.p, (float*)V.s, G.nx, G.ny, G.nstrip, (long)n, S, it, G.sink.p,
+                      st);
+        hipEvent_t ec = Tm.on ? Tm.rec(st) : nullptr;
+        if (Tm.on) Tm.prec.push_back({ec0, ec, it});
+        hipLaunchKernelGGL(k_cg_dot2<T>, grid, block, 0, st, n, it, (const T*)V.s, (const T*)V.r, part,
+                           (const CGState*)S, kOne);
+        launches += 3;
+        if (Tm.on) Tm.vec.push_back({ec, Tm.rec(st), it});
+      } else if (ras) {
         // r -= alpha q; s = M^-1 r (gather, block solves, scatter); dots
         hipLaunchKernelGGL((k_cg_axpy_r<T, 1>), grid, block, 0, st, n, it, V.r, (const T*)V.q, part, S, nb, kOne);
         hipEvent_t ec0 = Tm.on ? Tm.rec(st) : nullptr;

@@ -18,6 +18,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cstring>
 
@@ -770,6 +771,105 @@ SweepArgs<float> sweep_args<float>(const Sweep& S) {
   return a;
 }
 
+// Line-scan form of the FP32 inner apply (mixed-precision solver): when every
+// block-Jacobi block is a strip of whole grid lines of one nx-cell line
+// length, the strips are the equal split [ny k / nblk, ny (k+1) / nblk) of
+// the ny = n / nx lines, the blocks couple only to the previous cell (offset
+// 1, same line) and the line below (offset nx), and those couplings are
+// symmetric bit for bit, the block ILU(0) factor L D L^T equals the block
+// IC(0) factor (L D^1/2)(L D^1/2)^T in exact arithmetic, and the FP32 apply
+// runs it as line scans (batched.hip, k_lscan: one workgroup per strip):
+//   Ls_i = a(i,i-nx) / Ld_{i-nx} (0 on a strip's first line),
+//   Lw_i = a(i,i-1) / Ld_{i-1} (0 on a line's first cell),
+//   Ld_i = sqrt(fma(-Lw, Lw, fma(-Ls, Ls, a_ii))),
+// computed in FP64 from the block factors' pattern and rounded to FP32 in
+// the apply arrays ls, 1/Ld, -Lw/Ld, -Lw(i+1)/Ld.  Returns false (the
+// level-scheduled FP32 kernel runs) for anything else.
+bool lscan_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<BlockSweep>& B,
+                 const std::vector<int>& blo, const std::vector<int>& bhi) {
+  Precon& M = P.M;
+  LscanBJ& G = M.lscan;
+  G = LscanBJ();
+  if (P.prm.solver != PARCEL_SOLVER_CG_MIXED || M.mode != BJ_ILU0) return false;
+  if (P.prm.independ_nvec > 1) return false;
+  const HostCSR& A = P.hA;
+  const int n = A.n, nblk = (int)F.size();
+  // nx: the one dependency offset above 1
+  long nx = 0;
+  for (int b = 0; b < nblk; ++b)
+    for (size_t i = 0; i + 1 < F[b].ptr.size(); ++i)
+      for (int q = F[b].ptr[i]; q < F[b].ptr[i + 1]; ++q) {
+        const long off = (long)i - F[b].dep[q];
+        if (off <= 1 || off == nx) continue;
+        if (nx) return false;  // a second offset (couplings between planes)
+        nx = off;
+      }
+  if (nx < 1 || nx > 1024 || n % nx) return false;
+  for (int b = 0; b < nblk; ++b)  // the upper couplings mirror them
+    for (size_t i = 0; i + 1 < B[b].ptr.size(); ++i)
+      for (int q = B[b].ptr[i]; q < B[b].ptr[i + 1]; ++q) {
+        const long off = (long)B[b].dep[q] - (long)i;
+        if (off != 1 && off != nx) return false;
+      }
+  const int ny = (int)(n / nx);
+  if (nblk > ny) return false;
+  for (int b = 0; b < nblk; ++b)
+    if (blo[b] != (int)((long)ny * b / nblk) * nx || bhi[b] != (int)((long)ny * (b + 1) / nblk) * nx) return false;
+  // a(i, j) of the block entry (i, j), 0.0 when the pattern has none
+  auto entry = [&](int i, int j) -> double {
+    for (int e = A.rowptr[i]; e < A.rowptr[i + 1]; ++e)
+      if (A.col[e] == j) return A.val[e];
+    return 0.0;
+  };
+  std::vector<double> L(3 * (size_t)n, 0.0);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+  for (int b = 0; b < nblk; ++b) {
+    double* Ls = L.data();
+    double* Lw = Ls + n;
+    double* Ld = Lw + n;
+    for (int i = blo[b]; i < bhi[b] && !bad; ++i) {
+      const int x = i % nx;
+      double ls = 0.0, lw = 0.0;
+      if (i - nx >= blo[b]) {
+        const double a = entry(i, i - (int)nx);
+        if (a != entry(i - (int)nx, i) || std::signbit(a) != std::signbit(entry(i - (int)nx, i))) { bad = 1; break; }
+        ls = a / Ld[i - nx];
+      }
+      if (x > 0) {
+        const double a = entry(i, i - 1);
+        if (a != entry(i - 1, i) || std::signbit(a) != std::signbit(entry(i - 1, i))) { bad = 1; break; }
+        lw = a / Ld[i - 1];
+      }
+      const double t = std::fma(-lw, lw, std::fma(-ls, ls, entry(i, i)));
+      if (!(t > 0.0)) { bad = 1; break; }
+      Ls[i] = ls;
+      Lw[i] = lw;
+      Ld[i] = std::sqrt(t);
+    }
+  }
+  if (bad) return false;
+  std::vector<float> Fa(4 * (size_t)n);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const float inv = 1.0f / (float)L[2 * (size_t)n + i];
+    const float lwn = (i % nx) + 1 < nx ? (float)L[(size_t)n + i + 1] : 0.0f;
+    Fa[i] = (float)L[i];
+    Fa[(size_t)n + i] = inv;
+    Fa[2 * (size_t)n + i] = -(float)L[(size_t)n + i] * inv;
+    Fa[3 * (size_t)n + i] = -lwn * inv;
+  }
+  G.F.upload(Fa, P.stream);
+  G.w.alloc((size_t)n);
+  G.sink.alloc(1024);
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  G.nx = (int)nx;
+  G.ny = ny;
+  G.nstrip = nblk;
+  G.on = true;
+  return true;
+}
+
 }  // namespace
 
 void precon_setup(Plan& P) {
@@ -826,6 +926,7 @@ void precon_setup(Plan& P) {
     M.nth = maxw <= 256 ? 256 : maxw <= 512 ? 512 : 1024;
     pack_sweep(F, blo, M.fwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
     pack_sweep(B, blo, M.bwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
+    lscan_setup(P, F, B, blo, bhi);
   } else if (M.kind != PRECON_NONE) {
     throw hip_error("preconditioner not supported on this path", PARCEL_HIP_ERR_UNSUPPORTED);
   }
@@ -911,6 +1012,12 @@ void precon_apply_t(Plan& P, const T* r, T* s, const CGFuseT<T>* fuse, const int
     int grid = tiled_nb(n);
     hipLaunchKernelGGL(k_scale_pj<T>, dim3(grid), dim3(kNT), 0, P.stream, n, r,
                        M.kind == PRECON_PJ ? dinv_of<T>(M) : nullptr, s, skip);
+  } else if (std::is_same<T, float>::value && M.lscan.on) {
+    if (fuse) throw hip_error("the line-scan apply has no fused residual update", PARCEL_HIP_ERR_ARG);
+    if (skip) throw hip_error("the line-scan apply skips through the CG state", PARCEL_HIP_ERR_ARG);
+    const LscanBJ& G = M.lscan;
+    lscan_apply32(G.F.p, (const float*)r, G.w.p, (float*)s, G.nx, G.ny, G.nstrip, (long)n, nullptr, 0, G.sink.p,
+                  P.stream);
   } else {
     const int nb = M.nblocks;
     const int nth = M.nth;  // >= kNT: the fused gamma reduction uses kNT threads

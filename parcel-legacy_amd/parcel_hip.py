@@ -62,6 +62,7 @@ class Stats(ctypes.Structure):
         ("bj_nnz_u", ctypes.c_long),
         ("inner_niter", ctypes.c_int),
         ("final_rel_res", ctypes.c_double),
+        ("bj_lscan", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -13,13 +13,16 @@ without re-running multi-minute CPU solves on the GPU host.
       dd PCG with BJ ILU(0) (32 blocks) to rtol 1e-20.
   C4  3D 7-point Poisson 256^3, x_true ~ U[-1,1] seed 3, b = A x_true:
       FP64 PCG + BJ ILU(0) (256 blocks) to 1e-10, and the FP64 refinement
-      around FP32 PCG + FP32 BJ ILU(0) (inner rtol 1e-4) to 1e-10.
+      around FP32 PCG + FP32 BJ ILU(0) (inner rtol 1e-4) to 1e-10, whose
+      blocks (the grid planes) make the FP32 apply the line-scan form of the
+      block IC(0) factor (= block ILU(0) of the symmetric matrix in exact
+      arithmetic).
 
 Each entry holds the residual history (exact doubles), niter, iret and the
 SHA-256 of the solution's bytes (little-endian float64).  The CPU tests
 re-run a prefix of every entry and check it against this file.
 
-    python tests/golden/make_config_fixtures.py [--only c2,c3,c4]
+    python tests/golden/make_config_fixtures.py [--only c2,c3,c4]   (c4mixed: the mixed entry only)
 """
 import argparse
 import hashlib
@@ -119,11 +122,27 @@ def make_c4(out, log):
                            rtol=c["rtol"])
     out["c4_bj"] = entry(h, ni, ir, x_sha256=sha(x))
     log("c4 bj: %d its %.1fs" % (ni, time.time() - t))
+    make_c4_mixed(out, log, (rp, col, val, b))
+
+
+def c4_mixed_precon(rp, col, val, g=256):
+    """The FP32 inner preconditioner of the C4 mixed solve: the 256 blocks
+    are the grid planes, so block ILU(0) of the symmetric matrix is applied
+    as line scans of the block IC(0) factor."""
+    import oracle as orc
+    nb = C4_MIXED["nblocks"]
+    F64, F32 = orc.derive(*orc.ic0_grid(rp, col, val, g, nblocks=nb), g)
+    return orc.FactorPrecon(F64, F32, g, nstrip=nb)
+
+
+def make_c4_mixed(out, log, prob=None):
+    import oracle as orc
+    rp, col, val, b = prob if prob is not None else c4_problem()
     c = C4_MIXED
     t = time.time()
-    x, h, ni, ir, inner = orc.pcg_mixed(rp, col, val, b, ipreflag=2, ilu_method=0, nblocks=c["nblocks"],
-                                        itrmax=c["itrmax"], rtol=c["rtol"], inner_itrmax=c["inner_itrmax"],
-                                        inner_rtol=c["inner_rtol"])
+    M = c4_mixed_precon(rp, col, val)
+    x, h, ni, ir, inner = orc.pcg_mixed_factor(rp, col, val, b, M, itrmax=c["itrmax"], rtol=c["rtol"],
+                                               inner_itrmax=c["inner_itrmax"], inner_rtol=c["inner_rtol"])
     out["c4_mixed_bj"] = entry(h, ni, ir, x_sha256=sha(x), inner_total=int(inner))
     log("c4 mixed bj: %d outer, %d inner its %.1fs" % (ni, inner, time.time() - t))
 
@@ -135,7 +154,7 @@ def main():
     out = json.load(open(OUT)) if os.path.exists(OUT) else {}
     log = lambda *a: print(*a, flush=True)
     for k in args.only.split(","):
-        {"c2": make_c2, "c3": make_c3, "c4": make_c4}[k](out, log)
+        {"c2": make_c2, "c3": make_c3, "c4": make_c4, "c4mixed": make_c4_mixed}[k](out, log)
         with open(OUT, "w") as f:
             json.dump(out, f, indent=0)
 

@@ -137,6 +137,9 @@ def test_c4_mixed_bj(c4):
         x = np.zeros_like(b)
         hist, niter, iret = plan.solve(x, b, c["itrmax"], c["rtol"])
         st = plan.stats()
+    # the blocks are the grid planes: the FP32 apply is the line-scan form of
+    # the block IC(0) factor (block ILU(0) of the symmetric matrix)
+    assert st["bj_lscan"] == 1
     g = GOLD["c4_mixed_bj"]
     same(g, hist, niter, iret)
     assert st["inner_niter"] == g["inner_total"]

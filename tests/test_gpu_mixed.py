@@ -56,9 +56,20 @@ def test_mixed_bit_exact(case, pname):
         st = plan.stats()
     finally:
         plan.close()
-    xo, ho, no, io, inner = orc.pcg_mixed(rp, col, val, b, ipreflag=kw["ipreflag"],
-                                          ilu_method=kw.get("ILU_method", 1), nblocks=kw.get("precon_thblock", 1),
-                                          itrmax=20, rtol=1e-10, inner_itrmax=5000, inner_rtol=1e-4)
+    nb = kw.get("precon_thblock", 1)
+    if st["bj_lscan"]:
+        # ILU(0) on blocks of whole grid lines of a symmetric matrix: the FP32
+        # inner apply is the block IC(0) factor in line-scan form
+        assert case == "poisson2d_200" and pname == "ilu0_8"
+        nx = 200
+        F64, F32 = orc.derive(*orc.ic0_grid(rp, col, val, nx, nblocks=nb), nx)
+        M = orc.FactorPrecon(F64, F32, nx, nstrip=nb)
+        xo, ho, no, io, inner = orc.pcg_mixed_factor(rp, col, val, b, M, itrmax=20, rtol=1e-10, inner_itrmax=5000,
+                                                     inner_rtol=1e-4)
+    else:
+        xo, ho, no, io, inner = orc.pcg_mixed(rp, col, val, b, ipreflag=kw["ipreflag"],
+                                              ilu_method=kw.get("ILU_method", 1), nblocks=nb, itrmax=20,
+                                              rtol=1e-10, inner_itrmax=5000, inner_rtol=1e-4)
     assert (niter, iret) == (no, io)
     assert st["inner_niter"] == inner
     assert np.array_equal(hist, ho) and np.array_equal(x, xo)

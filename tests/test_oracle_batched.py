@@ -220,3 +220,42 @@ def test_block_ic0_is_ic0_of_each_diagonal_block():
         Abd[a:b, a:b] = A[a:b, a:b]
     pattern = A != 0
     assert np.allclose(LLt[pattern & (Abd != 0)], Abd[pattern & (Abd != 0)], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("nb", [1, 3, 7])
+def test_ic0_grid_equals_ic0_5pt_on_five_point_matrices(nb):
+    # the column-search restatement reproduces the positional one bit for bit
+    import parcel_hip as ph
+    nx, ny = 14, 21
+    rp, col, val = ph.diffusion2d(nx, ny, np.exp(ph.grf2d(32, 32, 4.0, 1.5, 3)[: nx * ny]))
+    a = orc.ic0_grid(rp, col, val, nx, nblocks=nb)
+    b = orc.ic0_5pt(rp, val, nx, nblocks=nb)
+    for u, v in zip(a, b):
+        assert u.tobytes() == v.tobytes()
+
+
+def test_ic0_grid_on_3d_plane_blocks_is_ic0_of_each_plane():
+    # 3D seven-point Poisson, one block per grid plane: the plane couplings
+    # (offset nx ny) are outside every block, and each plane's factor is the
+    # IC(0) of the plane's own five-point matrix; L L^T equals A's in-plane
+    # entries on the pattern
+    import parcel_hip as ph
+    nx, ny, nz = 9, 7, 5
+    rp, col, val = ph.poisson3d_spd(nx, ny, nz)
+    n = nx * ny * nz
+    Ls, Lw, Ld = orc.ic0_grid(rp, col, val, nx, nblocks=nz)
+    A = sp.csr_matrix((val, col - 1, rp - 1), shape=(n, n))
+    for k in range(nz):
+        a, b = k * nx * ny, (k + 1) * nx * ny
+        sub = sp.csr_matrix(A[a:b, a:b])
+        sub.sort_indices()
+        S = orc.ic0_5pt(sub.indptr.astype(np.int32) + 1, sub.data.copy(), nx)
+        assert np.array_equal(Ls[a:b], S[0]) and np.array_equal(Lw[a:b], S[1]) and np.array_equal(Ld[a:b], S[2])
+        m = b - a
+        L = np.diag(Ld[a:b]) + np.diag(Lw[a + 1:b], -1) + np.diag(Ls[a + nx:b], -nx)
+        Ad = sub.toarray()
+        pat = Ad != 0
+        assert np.allclose((L @ L.T)[pat], Ad[pat], rtol=1e-12, atol=1e-12)
+    # a planted mistake: strips one line off the planes mix two planes
+    Ls2, _, _ = orc.ic0_grid(rp, col, val, nx, nblocks=nz + 1)
+    assert not np.array_equal(Ls2, Ls)
