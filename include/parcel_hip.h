@@ -118,6 +118,7 @@ typedef struct parcel_hip_stats {
   int inner_niter;         /* FP32 PCG iterations summed over the corrections */
   double final_rel_res;    /* ||b - A x|| / ||b|| after the last correction (FP64) */
   int bj_lscan;            /* 1: the mixed solver's FP32 block-Jacobi apply runs block IC(0) line scans */
+  int spmv_stencil;        /* 1: the CG kernel reads the matrix as a symmetric 5/7-point stencil (4 arrays) */
 } parcel_hip_stats;
 
 /* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */

@@ -194,6 +194,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
       A.col[j] = c;
     }
     sell_build(P);
+    if (!dd_prec && (P.prm.solver == PARCEL_SOLVER_CG || P.prm.solver == PARCEL_SOLVER_CG_MIXED)) stencil_build(P);
     if (dd_prec && P.prm.solver != PARCEL_SOLVER_CG)
       throw hip_error("the double-double path has CG only", PARCEL_HIP_ERR_UNSUPPORTED);
     if (P.prm.solver != PARCEL_SOLVER_CG && P.prm.solver != PARCEL_SOLVER_BICG &&
@@ -233,6 +234,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
     P.stats.bj_nnz_l = P.M.fwd.real_entries;
     P.stats.bj_nnz_u = P.M.bwd.real_entries;
     P.stats.bj_lscan = P.M.lscan.on ? 1 : 0;
+    P.stats.spmv_stencil = P.st7 ? 1 : 0;
     P.hA = HostCSR();  // setup copy no longer needed
     plan_workspace(P, 1);
     PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));

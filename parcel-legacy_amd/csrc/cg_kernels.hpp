@@ -128,6 +128,44 @@ __device__ __forceinline__ St5<T> sys_matrix(St5<T> A, const Bat& B) {
   return A;
 }
 
+// Seven-point (3D) or five-point (2D) stencil in natural order with
+// symmetric couplings stored once per row, for one system: c = a(i,i),
+// e = a(i,i+1) = a(i+1,i), nn = a(i,i+nx) = a(i+nx,i), t = a(i,i+nxy) =
+// a(i+nxy,i) (nxy = n for a single plane; t is then never read).  A row
+// sums its entries in CSR order b, s, w, c, e, n, t over the neighbours the
+// grid has, which is sell_row's order when every such neighbour is stored:
+// the product equals the SELL product bit for bit.
+template <class T>
+struct St7 {
+  const T* c;
+  const T* e;
+  const T* nn;
+  const T* t;
+  int nx, ny;
+  long nxy, n;
+};
+
+template <class T, class G>
+__device__ __forceinline__ T mat_row(const St7<T>& A, long i, G g) {
+  const long l = i / A.nx;
+  const int x = (int)(i - l * A.nx), y = (int)(l % A.ny);
+  T t = 0;
+  if (i >= A.nxy) t = fma(A.t[i - A.nxy], g(i - A.nxy), t);
+  if (y > 0) t = fma(A.nn[i - A.nx], g(i - A.nx), t);
+  if (x > 0) t = fma(A.e[i - 1], g(i - 1), t);
+  t = fma(A.c[i], g(i), t);
+  if (x < A.nx - 1) t = fma(A.e[i], g(i + 1), t);
+  if (y < A.ny - 1) t = fma(A.nn[i], g(i + A.nx), t);
+  if (i + A.nxy < A.n) t = fma(A.t[i], g(i + A.nxy), t);
+  return t;
+}
+
+template <class T>
+__device__ __forceinline__ St7<T> sys_matrix(St7<T> A, const Bat& B) {
+  (void)B;  // single-system plans only
+  return A;
+}
+
 __device__ __forceinline__ double* sys_part(double* part) { return part + sys_id() * (PART_N * kNBMax); }
 
 // r = b - A x and the partials of <b,b>, <r,r> (calc_res_CRS,

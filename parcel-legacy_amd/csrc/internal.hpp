@@ -213,6 +213,13 @@ struct Plan {
   int dia_nnd = 0;
   DevBuf<double> dia_val;   // dia_nnd * n, diagonal-major
   DevBuf<int> dia_off;      // dia_nnd
+  // symmetric stencil form of the matrix for the fused CG kernel (St7):
+  // c, e, nn, t per row (FP64 and the FP32 copies of the mixed solver)
+  bool st7 = false;
+  int st7_nx = 0, st7_ny = 0;
+  long st7_nxy = 0;
+  DevBuf<double> st7_a[4];
+  DevBuf<float> st7_a32[4];
   PinnedBuf<CGState> hstate;  // host mirror of cg, polled between chunks of iterations
   DevBuf<CGState> cg;       // device-resident solver scalars
   DevBuf<double> part;      // tiled partial sums (PART_N * kNBMax)
@@ -246,6 +253,7 @@ struct SellArgsT;
 template <class T>
 struct CGVecs {
   SellArgsT<T> A;
+  const T* st7[4] = {nullptr, nullptr, nullptr, nullptr};  // the stencil arrays when the plan has them
   T *r = nullptr, *q = nullptr, *s = nullptr, *p0 = nullptr, *p1 = nullptr;
   const T* dinv = nullptr;   // point Jacobi scale, or null
   double* hist = nullptr;    // device residual history (hist_cap entries)
@@ -278,6 +286,8 @@ void precon_apply_dd(Plan& P, const double* rh, const double* rl, double* sh, do
 void precon_setup32(Plan& P);
 void precon_apply32(Plan& P, const float* r, float* s, const CGFuse32* fuse, const int* skip = nullptr);
 void sell_build(Plan& P);
+// the symmetric stencil arrays of a natural-order 5/7-point matrix (krylov.hip)
+void stencil_build(Plan& P);
 
 // solvers and kernels (krylov.hip)
 void plan_workspace(Plan& P, int itrmax);

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -88,6 +89,91 @@ void sell_build(Plan& P) {
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 
+
+// Symmetric stencil form (St7) of a natural-order matrix whose every row i
+// holds exactly its grid neighbours i-nxy, i-nx, i-1, i, i+1, i+nx, i+nxy
+// (those inside the grid, columns ascending) with a(i,j) == a(j,i) bit for
+// bit: nx is the smallest off-diagonal offset above 1, nxy the next one (n
+// itself for a single plane).  Leaves P.st7 false for anything else.
+void stencil_build(Plan& P) {
+  const HostCSR& A = P.hA;
+  const int n = A.n;
+  P.st7 = false;
+  if (n < 4 || !A.val_lo.empty()) return;
+  long o1 = 0, o2 = 0;
+  for (int i = 0; i < n; ++i)
+    for (int e = A.rowptr[i]; e < A.rowptr[i + 1]; ++e) {
+      const long off = (long)A.col[e] - i;
+      if (off <= 1) continue;
+      if (off == o1 || off == o2) continue;
+      if (!o1) o1 = off;
+      else if (!o2) o2 = off;
+      else return;
+    }
+  if (!o1) return;
+  if (o2 && o2 < o1) std::swap(o1, o2);
+  const long nx = o1, nxy = o2 ? o2 : n;
+  if (nxy % nx || n % nxy || nx < 2) return;
+  const int ny = (int)(nxy / nx);
+  std::vector<double> a[4];
+  for (auto& v : a) v.assign((size_t)n, 0.0);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int i = 0; i < n; ++i) {
+    if (bad) continue;
+    const long l = i / nx;
+    const int x = (int)(i - l * nx), y = (int)(l % ny);
+    const long nb[7] = {i - nxy, i - nx, i - 1L, (long)i, i + 1L, i + nx, i + nxy};
+    const bool has[7] = {i >= nxy, y > 0, x > 0, true, x < nx - 1, y < ny - 1, i + nxy < n};
+    int e = A.rowptr[i];
+    for (int k = 0; k < 7; ++k) {
+      if (!has[k]) continue;
+      if (e >= A.rowptr[i + 1] || A.col[e] != nb[k]) { bad = 1; break; }
+      if (k == 3) a[0][i] = A.val[e];
+      else if (k > 3) a[k - 3][i] = A.val[e];  // e, nn, t: the upper couplings
+      ++e;
+    }
+    if (!bad && e != A.rowptr[i + 1]) bad = 1;
+  }
+  if (bad) return;
+  // symmetry bit for bit: a(j, i) for each stored upper coupling a(i, j)
+  auto lower = [&](long j, long i) -> double {
+    for (int e = A.rowptr[j]; e < A.rowptr[j + 1]; ++e)
+      if (A.col[e] == i) return A.val[e];
+    return 0.0;
+  };
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int i = 0; i < n; ++i) {
+    const long l = i / nx;
+    const int x = (int)(i - l * nx), y = (int)(l % ny);
+    if (x < nx - 1) {
+      const double v = lower(i + 1, i);
+      if (std::memcmp(&a[1][i], &v, 8) != 0) bad = 1;
+    }
+    if (y < ny - 1) {
+      const double v = lower(i + nx, i);
+      if (std::memcmp(&a[2][i], &v, 8) != 0) bad = 1;
+    }
+    if (i + nxy < n) {
+      const double v = lower(i + nxy, i);
+      if (std::memcmp(&a[3][i], &v, 8) != 0) bad = 1;
+    }
+  }
+  if (bad) return;
+  for (int k = 0; k < 4; ++k) P.st7_a[k].upload(a[k], P.stream);
+  if (P.prm.solver == PARCEL_SOLVER_CG_MIXED) {
+    for (int k = 0; k < 4; ++k) {
+      std::vector<float> f((size_t)n);
+      for (int i = 0; i < n; ++i) f[i] = (float)a[k][i];
+      P.st7_a32[k].upload(f, P.stream);
+    }
+  }
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  P.st7_nx = (int)nx;
+  P.st7_ny = ny;
+  P.st7_nxy = nxy;
+  P.st7 = true;
+}
 
 void ensure_vecs(Plan& P, int k) {
   const int n = P.A.n;
@@ -271,9 +357,14 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
           dia = true;
         }
       }
-      if (!dia)
+      if (!dia && P.st7) {
+        const St7<T> As{V.st7[0], V.st7[1], V.st7[2], V.st7[3], P.st7_nx, P.st7_ny, P.st7_nxy, (long)n};
+        hipLaunchKernelGGL((k_cg_spmv<T, 1, St7<T>>), grid, block, 0, st, n, it, itrmax, rtol, As, x,
+                           (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
+      } else if (!dia) {
         hipLaunchKernelGGL(k_cg_spmv<T>, grid, block, 0, st, n, it, itrmax, rtol, A, x, (const T*)pold,
                            (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
+      }
       launches++;
       hipEvent_t eb = Tm.on ? Tm.rec(st) : nullptr;
       if (Tm.on) Tm.spmv.push_back({ea, eb, it});
@@ -414,6 +505,7 @@ int cg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, doubl
   plan_workspace(P, itrmax);
   CGVecs<double> V;
   V.A = sell_args(P.A);
+  for (int k = 0; k < 4; ++k) V.st7[k] = P.st7_a[k].p;
   V.r = P.r.p;
   V.q = P.q.p;
   V.s = P.s.p;

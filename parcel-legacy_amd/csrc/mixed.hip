@@ -75,6 +75,7 @@ int mixed_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, do
   float* d32 = P.vec32[1].p;
   CGVecs<float> V;
   V.A = sell_args32(P.A);
+  for (int k = 0; k < 4; ++k) V.st7[k] = P.st7_a32[k].p;
   V.r = P.vec32[2].p;
   V.q = P.vec32[3].p;
   V.s = P.vec32[4].p;

@@ -1134,6 +1134,137 @@ __global__ void __launch_bounds__(256) k_dd_bj(DDSweep F, DDSweep B, const doubl
   dd_sweep_run<false, MODE>(B, nullptr, nullptr, sh, sl, two);
 }
 
+// Register-pipelined double-double sweep for sweeps whose dependencies all
+// sit in the LDS ring (the last W levels, hi and lo rings side by side): the
+// structure of level l+2 and the operand of level l+1 are loaded while level
+// l computes, and a level reads its dependencies from LDS, so the level
+// chain waits on no global round trip.  Every row computes exactly as in
+// dd_sweep_run (entries in stored order up to the first padding entry, the
+// same dd operations), so the results are the same bits.
+template <int K>
+struct DDLevel {
+  int row, m;
+  int dep[K];
+  double vh[K], vl[K];
+  double dh, dl;
+  double ah, al;  // forward: r[row]; backward: the forward value s[row]
+};
+
+template <int K>
+__device__ __forceinline__ void dd_stage_struct(const DDSweep& A, const int4* sm, int base, int l, int l1, int two,
+                                                DDLevel<K>& R) {
+  const int4 mt = sm[min(l, l1 - 1) - base];
+  const int t = min((int)threadIdx.x, mt.x - 1);
+  R.m = mt.x;
+  R.row = A.rows[mt.w + t];
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    R.dep[e] = A.dep[mt.z + e * mt.x + t];
+    R.vh[e] = A.eh[mt.z + e * mt.x + t];
+    R.vl[e] = two ? A.el[mt.z + e * mt.x + t] : 0.0;
+  }
+  R.dh = A.dh[mt.w + t];
+  R.dl = two ? A.dl[mt.w + t] : 0.0;
+}
+
+template <int K>
+__device__ __forceinline__ void dd_stage_vec(DDLevel<K>& R, const double* h, const double* l) {
+  R.ah = h[R.row];
+  R.al = l[R.row];
+}
+
+template <bool FWD, int MODE, int K>
+__device__ __forceinline__ void dd_compute(const DDLevel<K>& C, int l0, int L, int W, int pad, double* rgh,
+                                           double* rgl, double* sh, double* sl) {
+  dd z = (!FWD && MODE == BJ_DILU) ? mkdd(0.0, 0.0) : mkdd(C.ah, C.al);
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    const int c = C.dep[e];
+    if (c == pad) break;
+    const dd prod = dd_mul(mkdd(rgh[-c - 1], rgl[-c - 1]), mkdd(C.vh[e], C.vl[e]));
+    z = (!FWD && MODE == BJ_DILU) ? dd_add(z, prod) : dd_sub(z, prod);
+  }
+  const dd d = mkdd(C.dh, C.dl);
+  dd v;
+  if (MODE == BJ_ILU0) v = dd_div(z, d);
+  else if (FWD) v = dd_mul(z, d);
+  else v = dd_sub(mkdd(C.ah, C.al), dd_mul(d, z));
+  if ((int)threadIdx.x < C.m) {
+    sh[C.row] = v.hi;
+    sl[C.row] = v.lo;
+  }
+  const int slot = ((L - l0) % W) * (int)blockDim.x + (int)threadIdx.x;
+  rgh[slot] = v.hi;
+  rgl[slot] = v.lo;
+}
+
+__device__ __forceinline__ void dd_load_meta(const int4* meta, int4* sm, int c0, int l1) {
+  __syncthreads();
+  const int cnt = min(kMetaLds, l1 - c0);
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) sm[j] = meta[c0 + j];
+  __syncthreads();
+}
+
+template <bool FWD, int MODE, int K>
+__device__ void dd_sweep_fast(const DDSweep& A, int W, const double* ah, const double* al, double* sh, double* sl,
+                              int two, double* rgh, double* rgl, int4* sm) {
+  const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
+  if (l1 <= l0) return;
+  const int zero = W * (int)blockDim.x;  // the ring slot that holds 0 (padding)
+  const int pad = -(1 + zero);
+  int base = l0;
+  if (threadIdx.x == 0) { rgh[zero] = 0.0; rgl[zero] = 0.0; }
+  dd_load_meta(A.meta, sm, base, l1);
+  DDLevel<K> R0, R1, R2;
+  dd_stage_struct<K>(A, sm, base, l0, l1, two, R0);
+  dd_stage_struct<K>(A, sm, base, l0 + 1, l1, two, R1);
+  dd_stage_vec<K>(R0, ah, al);
+  int l = l0;
+  for (; l + 2 < l1; l += 3) {
+    if (l + 4 >= base + kMetaLds) {
+      base = l;
+      dd_load_meta(A.meta, sm, base, l1);
+    }
+    dd_stage_struct<K>(A, sm, base, l + 2, l1, two, R2);
+    dd_stage_vec<K>(R1, ah, al);
+    dd_compute<FWD, MODE, K>(R0, l0, l, W, pad, rgh, rgl, sh, sl);
+    lds_barrier();
+    dd_stage_struct<K>(A, sm, base, l + 3, l1, two, R0);
+    dd_stage_vec<K>(R2, ah, al);
+    dd_compute<FWD, MODE, K>(R1, l0, l + 1, W, pad, rgh, rgl, sh, sl);
+    lds_barrier();
+    dd_stage_struct<K>(A, sm, base, l + 4, l1, two, R1);
+    dd_stage_vec<K>(R0, ah, al);
+    dd_compute<FWD, MODE, K>(R2, l0, l + 2, W, pad, rgh, rgl, sh, sl);
+    lds_barrier();
+  }
+  if (l < l1) {
+    if (l + 1 < l1) dd_stage_vec<K>(R1, ah, al);
+    dd_compute<FWD, MODE, K>(R0, l0, l, W, pad, rgh, rgl, sh, sl);
+    lds_barrier();
+    if (l + 1 < l1) {
+      dd_compute<FWD, MODE, K>(R1, l0, l + 1, W, pad, rgh, rgl, sh, sl);
+      lds_barrier();
+    }
+  }
+}
+
+// One workgroup per block, both sweeps in ring mode; NTH >= the widest level.
+template <int MODE, int NTH, int K>
+__global__ void __launch_bounds__(NTH) k_dd_bj_fast(DDSweep F, DDSweep B, int WF, int WB, const double* rh,
+                                                    const double* rl, double* sh, double* sl, int two,
+                                                    const int* skip) {
+  if (skip && *skip) return;
+  extern __shared__ double dd_ring[];
+  __shared__ int4 smeta[kMetaLds];
+  const int W = max(WF, WB);
+  double* rgh = dd_ring;
+  double* rgl = dd_ring + W * NTH + 1;
+  dd_sweep_fast<true, MODE, K>(F, WF, rh, rl, sh, sl, two, rgh, rgl, smeta);
+  __syncthreads();  // forward values in memory before the backward sweep reads them
+  dd_sweep_fast<false, MODE, K>(B, WB, sh, sl, sh, sl, two, rgh, rgl, smeta);
+}
+
 __global__ void k_dd_pj(int n, const double* __restrict__ rh, const double* __restrict__ rl,
                         const double* __restrict__ dh, const double* __restrict__ dl, double* __restrict__ sh,
                         double* __restrict__ sl, const int* skip) {
@@ -1202,10 +1333,27 @@ void precon_setup_dd(Plan& P, int prec) {
     for (int b = 0; b < nblk; ++b)
       for (const BlockSweep* S : {&F[b], &B[b]})
         for (size_t i = 0; i + 1 < S->ptr.size(); ++i) kmax = std::max(kmax, S->ptr[i + 1] - S->ptr[i]);
-    M.nth = 256;
-    // no LDS ring: every dependency is a global row id
-    pack_sweep(F, blo, M.fwd, 0, kmax, M.nth, P.stream);
-    pack_sweep(B, blo, M.bwd, 0, kmax, M.nth, P.stream);
+    int maxw = 1;
+    for (int b = 0; b < nblk; ++b)
+      for (const BlockSweep* S : {&F[b], &B[b]})
+        for (size_t l = 0; l + 1 < S->lev_start.size(); ++l)
+          maxw = std::max(maxw, S->lev_start[l + 1] - S->lev_start[l]);
+    // the pipelined kernel (k_dd_bj_fast) when every level fits 256 or 512
+    // threads, rows have at most kKR entries and both sweeps keep their
+    // dependencies in the LDS ring; otherwise global row ids for k_dd_bj
+    const int kk = apply_k(kmax);
+    M.nth = maxw <= 256 ? 256 : 512;
+    bool fast = maxw <= 512 && kk <= kKR;
+    if (fast) {
+      pack_sweep(F, blo, M.fwd, kRingBudget, kk, M.nth, P.stream);
+      pack_sweep(B, blo, M.bwd, kRingBudget, kk, M.nth, P.stream);
+      fast = M.fwd.ring > 0 && M.bwd.ring > 0;
+    }
+    if (!fast) {
+      M.nth = 256;
+      pack_sweep(F, blo, M.fwd, 0, kmax, M.nth, P.stream);
+      pack_sweep(B, blo, M.bwd, 0, kmax, M.nth, P.stream);
+    }
   } else if (M.kind != PRECON_NONE) {
     throw hip_error("preconditioner not supported on the double-double path", PARCEL_HIP_ERR_UNSUPPORTED);
   }
@@ -1222,10 +1370,39 @@ void precon_apply_dd(Plan& P, const double* rh, const double* rl, double* sh, do
                        (pj && two) ? (const double*)M.dinv_lo.p : nullptr, sh, sl, skip);
   } else {
     const DDSweep F = dd_sweep(M.fwd), B = dd_sweep(M.bwd);
-    if (M.mode == BJ_ILU0)
+    if (M.fwd.ring > 0 && M.bwd.ring > 0) {
+      const int W = std::max(M.fwd.ring, M.bwd.ring), nth = M.nth;
+      const size_t shm = sizeof(double) * 2 * ((size_t)W * nth + 1);
+      const int kk = std::max(M.fwd.max_k, M.bwd.max_k);
+#define PARCEL_DD_K(NTH, KK)                                                                                     \
+  do {                                                                                                           \
+    if (M.mode == BJ_ILU0)                                                                                       \
+      hipLaunchKernelGGL((k_dd_bj_fast<BJ_ILU0, NTH, KK>), dim3(M.nblocks), dim3(NTH), shm, P.stream, F, B,     \
+                         M.fwd.ring, M.bwd.ring, rh, rl, sh, sl, two, skip);                                     \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_dd_bj_fast<BJ_DILU, NTH, KK>), dim3(M.nblocks), dim3(NTH), shm, P.stream, F, B,     \
+                         M.fwd.ring, M.bwd.ring, rh, rl, sh, sl, two, skip);                                     \
+  } while (0)
+#define PARCEL_DD_NTH(NTH)                                                                                       \
+  do {                                                                                                           \
+    switch (kk) {                                                                                                \
+      case 1: PARCEL_DD_K(NTH, 1); break;                                                                        \
+      case 2: PARCEL_DD_K(NTH, 2); break;                                                                        \
+      case 3: PARCEL_DD_K(NTH, 3); break;                                                                        \
+      case 4: PARCEL_DD_K(NTH, 4); break;                                                                        \
+      case 6: PARCEL_DD_K(NTH, 6); break;                                                                        \
+      default: PARCEL_DD_K(NTH, 8); break;                                                                       \
+    }                                                                                                            \
+  } while (0)
+      if (nth <= 256) PARCEL_DD_NTH(256);
+      else PARCEL_DD_NTH(512);
+#undef PARCEL_DD_NTH
+#undef PARCEL_DD_K
+    } else if (M.mode == BJ_ILU0) {
       hipLaunchKernelGGL(k_dd_bj<BJ_ILU0>, dim3(M.nblocks), dim3(256), 0, P.stream, F, B, rh, rl, sh, sl, two, skip);
-    else
+    } else {
       hipLaunchKernelGGL(k_dd_bj<BJ_DILU>, dim3(M.nblocks), dim3(256), 0, P.stream, F, B, rh, rl, sh, sl, two, skip);
+    }
   }
   PARCEL_HIP_CHECK(hipGetLastError());
 }

@@ -63,6 +63,7 @@ class Stats(ctypes.Structure):
         ("inner_niter", ctypes.c_int),
         ("final_rel_res", ctypes.c_double),
         ("bj_lscan", ctypes.c_int),
+        ("spmv_stencil", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -234,3 +234,34 @@ def test_bicgstab_itrmax_and_edge():
         hist, niter, iret = ph.dbicgstab(x, b, rp, col, val, ipreflag=1, itrmax=itrmax, rtolmax=1e-15)
         xo, ho, no, io = orc.bicgstab(rp, col, val, b, ipreflag=1, itrmax=itrmax, rtol=1e-15)
         assert (niter, iret) == (no, io) and np.array_equal(hist, ho) and np.array_equal(x, xo)
+
+
+def _asym(rp, col, val, r=37):
+    # one coupling changed in its last bit: no longer symmetric bit for bit
+    val = val.copy()
+    for e in range(rp[r] - 1, rp[r + 1] - 1):
+        if col[e] - 1 == r + 1:
+            val[e] = np.nextafter(val[e], 0.0)
+    return rp, col, val
+
+
+@pytest.mark.parametrize("case,want", [("parcel3d_20", 1), ("poisson2d_64", 1), ("random_5000", 0),
+                                       ("spd3d_12x9x7", 1), ("asym_2d", 0)])
+def test_stencil_form_of_the_cg_kernel(case, want):
+    # symmetric 5/7-point matrices in natural order run the fused CG kernel on
+    # the stencil arrays; the histories stay the oracle's bit for bit
+    if case == "spd3d_12x9x7":
+        rp, col, val = ph.poisson3d_spd(12, 9, 7)
+    elif case == "asym_2d":
+        rp, col, val = _asym(*ph.poisson2d(40, 30, -1.0))
+    else:
+        rp, col, val = CASES[case]()
+    n = rp.shape[0] - 1
+    b = ph.uniform(n, 23)
+    with ph.Plan(rp, col, val, ipreflag=1) as P:
+        assert P.stats()["spmv_stencil"] == want
+        x = np.zeros(n)
+        hist, niter, iret = P.solve(x, b, 3000, 1e-10)
+    xo, ho, no, io = orc.pcg(rp, col, val, b, ipreflag=1, itrmax=3000, rtol=1e-10)
+    assert (niter, iret) == (no, io)
+    assert np.array_equal(hist, ho) and np.array_equal(x, xo)
