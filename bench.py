@@ -53,6 +53,12 @@ def k1_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 48 * n
 
 
+def k1_bytes_st7(n, t=8):
+    """Fused CG kernel on the symmetric stencil form: the four coefficient
+    arrays c, e, n, t (4t per row) and the six vector passes (6t)."""
+    return 10 * t * n
+
+
 def ilu0_apply_bytes(n, nnz_l, nnz_u):
     """Block-Jacobi ILU(0) apply (forward + backward sweep): 12 B per factor
     entry, row ids 4n per sweep, r 8n, s written 8n (forward), s read+write
@@ -146,9 +152,15 @@ def run_c4(args):
         pre_ms = st["precon_ms"] / max(st["precon_calls"], 1) if st["precon_calls"] else 0.0
         vec_ms = st["vec_ms"] / max(st["vec_calls"], 1) if st["vec_calls"] else 0.0
         sfx = "_f32" if mixed else ""
-        kb = k1_bytes32(n, nnz) if mixed else k1_bytes(n, nnz)
-        kernels = {"cg_spmv_fused" + sfx: {"mean_ms": k1_ms, "calls_per_solve": st["spmv_calls"], "bytes": kb,
-                                           "GBs": kb / (k1_ms * 1e-3) / 1e9}}
+        if st["spmv_stencil"]:
+            # the symmetric seven-point stencil form (4 coefficient arrays)
+            kb = k1_bytes_st7(n, 4 if mixed else 8)
+            k1name = "cg_spmv_fused_st7" + sfx
+        else:
+            kb = k1_bytes32(n, nnz) if mixed else k1_bytes(n, nnz)
+            k1name = "cg_spmv_fused" + sfx
+        kernels = {k1name: {"mean_ms": k1_ms, "calls_per_solve": st["spmv_calls"], "bytes": kb,
+                            "GBs": kb / (k1_ms * 1e-3) / 1e9}}
         if base == "bj" and st["bj_lscan"]:
             # the FP32 apply on the grid-plane blocks: line scans of the block
             # IC(0) factor (k_lscan, one workgroup per plane), 40 B per cell

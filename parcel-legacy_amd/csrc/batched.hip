@@ -650,7 +650,7 @@ void batch_cg_loop(Batch& B, const BVecs<T>& V, TZ* z, T* x, const T* b, int itr
       T* pnew = it & 1 ? V.p1 : V.p0;
       hipEvent_t ea = pf.rec(st);
       if (V.st5) {
-        const St5<T> A5{V.st5, V.st5 + B.nv(), V.st5 + 2 * B.nv(), B.nx, B.ny};
+        const St5<T> A5{V.st5, V.st5 + B.nv(), V.st5 + 2 * B.nv(), B.nx, B.ny, FastDiv((unsigned)B.nx)};
         if (lb)
           hipLaunchKernelGGL((k_cg_spmv<T, kLB, St5<T>, TZ>), gridl, block, 0, st, n, it, itrmax, rtol, A5, x,
                              (const T*)pold, (const TZ*)z, pnew, V.q, part, S, V.hist, V.hist_cap, bt);

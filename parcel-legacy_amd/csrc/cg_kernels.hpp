@@ -105,11 +105,13 @@ struct St5 {
   const T* e;
   const T* nn;
   int nx, ny;
+  FastDiv dx;  // by nx (a system's rows < 2^31)
 };
 
 template <class T, class G>
 __device__ __forceinline__ T mat_row(const St5<T>& A, long i, G g) {
-  const int x = (int)(i % A.nx), y = (int)(i / A.nx);
+  const unsigned ui = (unsigned)i, yy = A.dx.div(ui);
+  const int x = (int)(ui - yy * (unsigned)A.nx), y = (int)yy;
   T t = 0;
   if (y > 0) t = fma(A.nn[i - A.nx], g(i - A.nx), t);
   if (x > 0) t = fma(A.e[i - 1], g(i - 1), t);
@@ -143,12 +145,13 @@ struct St7 {
   const T* t;
   int nx, ny;
   long nxy, n;
+  FastDiv dx, dy;  // by nx and ny (rows < 2^31)
 };
 
 template <class T, class G>
 __device__ __forceinline__ T mat_row(const St7<T>& A, long i, G g) {
-  const long l = i / A.nx;
-  const int x = (int)(i - l * A.nx), y = (int)(l % A.ny);
+  const unsigned ui = (unsigned)i, l = A.dx.div(ui), q = A.dy.div(l);
+  const int x = (int)(ui - l * (unsigned)A.nx), y = (int)(l - q * (unsigned)A.ny);
   T t = 0;
   if (i >= A.nxy) t = fma(A.t[i - A.nxy], g(i - A.nxy), t);
   if (y > 0) t = fma(A.nn[i - A.nx], g(i - A.nx), t);

@@ -27,6 +27,28 @@ constexpr int kNT = 256;      // threads per workgroup for vector kernels
 constexpr int kWaves = kNT / 64;
 constexpr int kNBMax = 2048;  // max owner workgroups of the tiled layout
 
+// Division of n < 2^31 by a runtime divisor d >= 1 without a division
+// instruction: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1 (Granlund and Montgomery's round-up
+// method; the sum stays below 2^32 because umulhi(n, m) <= n < 2^31).
+struct FastDiv {
+  unsigned m = 1;
+  int s = 0;
+  unsigned d = 1;
+  FastDiv() = default;
+  explicit FastDiv(unsigned dv) : d(dv) {
+    while ((1u << s) < dv) ++s;
+    m = (unsigned)((((unsigned long long)1 << 32) * (((unsigned long long)1 << s) - dv)) / dv + 1);
+  }
+  __host__ __device__ __forceinline__ unsigned div(unsigned n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    return (__umulhi(n, m) + n) >> s;
+#else
+    return (unsigned)((((unsigned long long)n * m >> 32) + n) >> s);
+#endif
+  }
+};
+
 inline int tiled_nb(long n) {
   long t = (n + kNT - 1) / kNT;
   if (t < 1) t = 1;

@@ -358,7 +358,8 @@ int cg_run(Plan& P, const CGVecs<T>& V, T* x, const T* b, int itrmax, double rto
         }
       }
       if (!dia && P.st7) {
-        const St7<T> As{V.st7[0], V.st7[1], V.st7[2], V.st7[3], P.st7_nx, P.st7_ny, P.st7_nxy, (long)n};
+        const St7<T> As{V.st7[0], V.st7[1], V.st7[2], V.st7[3], P.st7_nx, P.st7_ny, P.st7_nxy, (long)n,
+                        FastDiv((unsigned)P.st7_nx), FastDiv((unsigned)P.st7_ny)};
         hipLaunchKernelGGL((k_cg_spmv<T, 1, St7<T>>), grid, block, 0, st, n, it, itrmax, rtol, As, x,
                            (const T*)pold, (const T*)V.s, pnew, V.q, part, S, V.hist, V.hist_cap, kOne);
       } else if (!dia) {

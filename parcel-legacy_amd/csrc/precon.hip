@@ -623,49 +623,43 @@ __device__ __forceinline__ void load_meta(const SweepArgs<T>& A, int4* sm, int c
 }
 
 // Register-pipelined sweep: every row of the sweep has K entries (padded)
-// and every level fits the workgroup.  Level l computes with data loaded
-// while levels l-2 (structure) and l-1 (vector operands) were computed.
+// and every level fits the workgroup.  Level l computes with its structure
+// loaded kPS levels earlier and its vector operands kPV levels earlier
+// (kPS + 1 register sets in rotation, the rotation unrolled so every set is
+// a fixed set of registers): a level waits on a global round trip only when
+// kPV levels of work are shorter than one.
+constexpr int kPS = 4, kPV = 2, kPD = kPS + 1;
+
 template <bool FWD, int MODE, int K, bool RING, bool FUSED, class T>
 __device__ void run_sweep_fast(const SweepArgs<T>& A, const T* rin, T* s, const T* q, T* rout, double alpha,
                                T* ring, int4* sm) {
   const int l0 = A.blk_lev[blockIdx.x], l1 = A.blk_lev[blockIdx.x + 1];
   if (l1 <= l0) return;
-  // metadata window of kMetaLds levels starting at `base`; it is moved
-  // forward before the look-ahead would leave it
+  // metadata window of kMetaLds levels starting at `base`; it moves forward
+  // before the staging of a round would leave it
   int base = l0;
   if (RING && threadIdx.x == 0) ring[A.ring * A.stride] = (T)0;  // zero slot for padding
   load_meta(A, sm, base, l1);  // its barriers also publish the zero slot
-  LevelRegs<K, T> R0, R1, R2;
-  stage_struct<FWD, MODE, K, T>(A, sm, base, l0, l1, R0);
-  stage_struct<FWD, MODE, K, T>(A, sm, base, l0 + 1, l1, R1);
-  stage_vec<FWD, FUSED, K, T>(R0, rin, q, s);
-  int l = l0;
-  for (; l + 2 < l1; l += 3) {
-    if (l + 4 >= base + kMetaLds) {
-      base = l;
+  LevelRegs<K, T> R[kPD];
+#pragma unroll
+  for (int d = 0; d < kPS; ++d) stage_struct<FWD, MODE, K, T>(A, sm, base, l0 + d, l1, R[d]);
+#pragma unroll
+  for (int d = 0; d < kPV; ++d) stage_vec<FWD, FUSED, K, T>(R[d], rin, q, s);
+  for (int l = l0; l < l1; l += kPD) {
+    if (l + kPD - 1 + kPS >= base + kMetaLds) {
+      base = min(l + kPS, l1 - 1);
       load_meta(A, sm, base, l1);
     }
-    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 2, l1, R2);
-    stage_vec<FWD, FUSED, K, T>(R1, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
-    level_barrier(A);
-    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 3, l1, R0);
-    stage_vec<FWD, FUSED, K, T>(R2, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
-    level_barrier(A);
-    stage_struct<FWD, MODE, K, T>(A, sm, base, l + 4, l1, R1);
-    stage_vec<FWD, FUSED, K, T>(R0, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 2, R2, s, rout, alpha, ring);
-    level_barrier(A);
-  }
-  // tail: at most two levels; R0 holds level l, R1 the structure of l+1
-  if (l < l1) {
-    if (l + 1 < l1) stage_vec<FWD, FUSED, K, T>(R1, rin, q, s);
-    compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l, R0, s, rout, alpha, ring);
-    level_barrier(A);
-    if (l + 1 < l1) {
-      compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + 1, R1, s, rout, alpha, ring);
-      level_barrier(A);
+#pragma unroll
+    for (int j = 0; j < kPD; ++j) {
+      // R[j] holds level l + j; the set after it in the rotation takes the
+      // structure of level l + j + kPS (it held level l + j - 1, done)
+      stage_struct<FWD, MODE, K, T>(A, sm, base, l + j + kPS, l1, R[(j + kPS) % kPD]);
+      stage_vec<FWD, FUSED, K, T>(R[(j + kPV) % kPD], rin, q, s);
+      if (l + j < l1) {
+        compute_level<FWD, MODE, K, RING, FUSED, T>(A, sm, base, l0, l + j, R[j], s, rout, alpha, ring);
+        level_barrier(A);
+      }
     }
   }
 }
