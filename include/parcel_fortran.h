@@ -86,6 +86,20 @@ void parcel_dcbcg_c_(const int* icomm, double* vecx, const double* vecb, const i
                      const int* caarnoldi_sstep, const int* caarnoldi_tstep,
                      const int* caarnoldi_basis, const int* caarnoldi_qrflag, int* iret);
 
+/* parcel_dcaarnoldi_c (src/example/c2fortran_kernel.f90:1119-1226): the
+ * eigen outputs of parcel_hip_dcaarnoldi (one eigenpair, the first
+ * component of the eigenvector) before iret */
+void parcel_dcaarnoldi_c_(const int* icomm, double* vecx, const double* vecb, const int* n,
+                          const int* gn, const int* nnz, const int* istart, const double* crsa,
+                          const int* crsrow_ptr, const int* crscol, const int* ipreflag,
+                          const int* ilu_method, const int* addl, const int* iflagas,
+                          const int* iovlflag, const int* precon_thblock, const int* Neuman_itr,
+                          const int* independ_nvec, const int* nBlock, const int* caarnoldi_sstep,
+                          const int* caarnoldi_tstep, const int* caarnoldi_basis,
+                          const int* caarnoldi_qrflag, double* EigenValueReal,
+                          double* EigenValueImag, double* EigenVector, double* EigenErrorNorm,
+                          int* EmaxID, int* EminID, int* iret);
+
 /* parcel_ddcg_c (src/example/c2fortran_kernel.f90:620-708): double-double
  * PCG, hi / lo arrays; Neuman_itr is in its list (not in parcel_ddcg's) and
  * unused */

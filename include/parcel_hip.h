@@ -201,6 +201,48 @@ int parcel_hip_dcbcg(int icomm, double* vecx, const double* vecb, int n, int gn,
                      int CBCG_Eigenflag, int power_method_itrmax, int CAARNOLDI_sstep,
                      int CAARNOLDI_tstep, int CAARNOLDI_basis, int CAARNOLDI_QRflag, int* iret);
 
+/* parcel_dpower (src/wrapper.f90:913-1022) and parcel_dcaarnoldi (:1561-1680),
+ * with their DIA forms (:1025, :1683): power_method_symMatrix
+ * (src/power_method.f90:188-437) from the start vector vecx (all ones when it
+ * is 0), max(1, power_method_itrmax) iterations of x' = y/||y||,
+ * y = A M^-1 x', Emax = <x', y>; parcel_dcaarnoldi runs the same method
+ * (KSP_pCAARNOLDI compiles to power_method_symMatrix,
+ * src/pcagmres.f90:2647-2651) with one iteration, its power_method_itrmax
+ * being left at -1 (src/wrapper.f90:24, src/krylov.f90:32,
+ * src/type.f90:1331-1336).  Outputs one eigenpair: EigenValueReal[0] = Emax,
+ * EigenValueImag[0] = 0, EigenErrorNorm[0] = ||A x - Emax x|| / ||Emax x||,
+ * MaxEigenID = MinEigenID = 1, and the normalised last iterate x: its first
+ * component only for the CRS forms (src/krylov.f90:360-369 copies
+ * size(EigenValueReal) = 1 entries), all n for the DIA forms
+ * (get_EigenPair_KSP, src/EigenProblem.f90:18-46); EigenVector is host or
+ * device memory.  vecx and vecb are left unchanged; itrmax (parcel_dpower)
+ * and iret, which the reference leaves unset, are returned as 0. */
+int parcel_hip_dpower(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                      const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag, int ILU_method,
+                      int addL, int iflagAS, int* itrmax, int iovlflag, int precon_thblock, int Neuman_itr,
+                      int independ_nvec, int nBlock, int power_method_itrmax, double* EigenValueReal,
+                      double* EigenValueImag, double* EigenVector, double* EigenErrorNorm, int* MaxEigenID,
+                      int* MinEigenID, int* iret);
+int parcel_hip_dpower_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                          const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS,
+                          int* itrmax, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                          int nBlock, int power_method_itrmax, double* EigenValueReal, double* EigenValueImag,
+                          double* EigenVector, double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID, int* iret);
+int parcel_hip_dcaarnoldi(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                          const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                          int ILU_method, int addL, int iflagAS, int iovlflag, int precon_thblock, int Neuman_itr,
+                          int independ_nvec, int nBlock, int CAARNOLDI_sstep, int CAARNOLDI_tstep,
+                          int CAARNOLDI_basis, int CAARNOLDI_QRflag, double* EigenValueReal,
+                          double* EigenValueImag, double* EigenVector, double* EigenErrorNorm, int* MaxEigenID,
+                          int* MinEigenID, int* iret);
+int parcel_hip_dcaarnoldi_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                              const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                              int iflagAS, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                              int nBlock, int CAARNOLDI_sstep, int CAARNOLDI_tstep, int CAARNOLDI_basis,
+                              int CAARNOLDI_QRflag, double* EigenValueReal, double* EigenValueImag,
+                              double* EigenVector, double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID,
+                              int* iret);
+
 /* The R factor of TSQR_Ronly_ (src/QRDecomposition.f90:887-1002, the
  * CAGMRES_QRflag = 3 path of QRDecomposition_) for the n x nc column-major
  * matrix V (ld ldv >= n; host or device memory; nc <= 16): nc x nc upper
@@ -321,6 +363,14 @@ int parcel_hip_plan_set_inner(parcel_hip_plan* plan, double inner_rtol, int inne
 /* Switch per-kernel HIP-event timing of later solves on (1) or off (0). */
 int parcel_hip_plan_set_profile(parcel_hip_plan* plan, int on);
 int parcel_hip_plan_destroy(parcel_hip_plan* plan);
+
+/* The power method of parcel_dpower on a created FP64 plan (its matrix and
+ * preconditioner), with the whole eigenvector: EigenVector receives all n
+ * components (host or device); vecx is host or device.  No counterpart in
+ * the reference's API: its CRS driver returns one component. */
+int parcel_hip_plan_eigen(parcel_hip_plan* plan, const double* vecx, int power_method_itrmax,
+                          double* EigenValueReal, double* EigenValueImag, double* EigenVector,
+                          double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID);
 
 /* --- kernels exposed for unit parity tests and measurement --------------- */
 /* y = A x with the plan's matrix (src/matrix.f90:9051 MatVec_crs_FP64_CPU;

@@ -11,6 +11,7 @@
  *   moore_penrose                     src/Moore_Penrose.f90:3-67
  *   getEigenValue                     src/pcbcg.f90:179-196
  *   power_method_symMatrix            src/power_method.f90:188-437
+ *     (oracle_power_emax, oracle_power_eigen)
  *
  * Arithmetic (the HIP path's, bit for bit):
  *   SS_0 = fma(eta, AS_0, -(zeta*r)),
@@ -99,12 +100,12 @@ void oracle_sym_pinv(int n, const double* A, int lda, double* P, int ldp) {
   free(v);
 }
 
-/* power_method_symMatrix (src/power_method.f90:188-437): y = x0 (all ones
- * when ||x0|| = 0); itrmax times: x' = (1/sqrt(<y,y>)) y, y = A M^-1 x',
- * Emax = <x',y>.  Returns Emax (Emin is 0). */
-double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
-                         const double* x0, int itrmax, const oracle_red* red) {
-  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+/* the loop of power_method_symMatrix (src/power_method.f90:290-334): y = x0
+ * (all ones when ||x0|| = 0); itrmax times: x' = (1/sqrt(<y,y>)) y,
+ * y = A M^-1 x', Emax = <x',y>.  Leaves the last y in y and its <y,y> in
+ * *vnorm_out; returns Emax (0 when itrmax < 1). */
+static double power_loop(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                         const double* x0, int itrmax, const oracle_red* red, double* y, double* vnorm_out) {
   double* xp = (double*)calloc((size_t)n, sizeof(double));
   double* z = (double*)malloc(sizeof(double) * (size_t)n);
   memcpy(y, x0, sizeof(double) * (size_t)n);
@@ -121,10 +122,45 @@ double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const
     vnorm = oracle_dot(red, n, y, y);
     emax = oracle_dot(red, n, xp, y);
   }
-  free(y);
   free(xp);
   free(z);
+  *vnorm_out = vnorm;
   return emax;
+}
+
+/* power_method_symMatrix's Emax (src/power_method.f90:188-437), the
+ * eigenvalue bound of KSP_pCBCG; Emin is 0. */
+double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                         const double* x0, int itrmax, const oracle_red* red) {
+  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  double vnorm;
+  const double emax = power_loop(M, n, rowptr, col, val, x0, itrmax, red, y, &vnorm);
+  free(y);
+  return emax;
+}
+
+/* power_method_symMatrix in full (src/power_method.f90:188-437): the loop,
+ * then (:336-398) x = (1/sqrt(<y,y>)) y, y = A x (no preconditioner),
+ * kz = Emax x, z = y - kz, errnorm = sqrt|<z,z>| / sqrt|<kz,kz>|; vec
+ * receives x (EigenVector, :415-420). */
+void oracle_power_eigen(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                        const double* x0, int itrmax, const oracle_red* red, double* emax_out, double* errnorm_out,
+                        double* vec) {
+  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  double* kz = (double*)malloc(sizeof(double) * (size_t)n);
+  double vnorm;
+  const double emax = power_loop(M, n, rowptr, col, val, x0, itrmax, red, y, &vnorm);
+  const double t1 = 1.0 / sqrt(vnorm);
+  for (int i = 0; i < n; ++i) vec[i] = t1 * y[i];
+  oracle_spmv(n, rowptr, col, val, vec, y);
+  for (int i = 0; i < n; ++i) kz[i] = emax * vec[i];
+  const double kx = oracle_dot(red, n, kz, kz);
+  for (int i = 0; i < n; ++i) y[i] = y[i] - kz[i];
+  const double en = oracle_dot(red, n, y, y);
+  *emax_out = emax;
+  *errnorm_out = sqrt(fabs(en)) / sqrt(fabs(kx));
+  free(y);
+  free(kz);
 }
 
 /* S_0 = M^-1 r, AS_0 = A S_0, SS_j and S_{j+1} = M^-1 SS_j (see header) */
@@ -237,9 +273,12 @@ int oracle_cbcg(int n, const int* rowptr, const int* col, const double* val, con
   oracle_precon* M = oracle_precon_create(ipreflag, ilu_method, nblocks, independ_nvec, n, rowptr, col, val, &err);
   if (!M) return err ? err : -1;
   double Emax = 0.0, Emin = 0.0;
-  /* flag 2 (CA-Arnoldi): the reference's GPU build runs the power method
-   * there (KSP_pCAARNOLDI, src/pcagmres.f90:2647-2651) */
-  if (eigflag == 1 || eigflag == 2) Emax = oracle_power_emax(M, n, rowptr, col, val, x, pm_itrmax, red);
+  /* flag 2 (CA-Arnoldi): the reference runs the power method there, its
+   * KSP_pCAARNOLDI being an `#if 1` call of power_method_symMatrix
+   * (src/pcagmres.f90:2647-2651) */
+  /* set_KSP_power_method_itrmax: at least one iteration (src/type.f90:1331-1336) */
+  if (eigflag == 1 || eigflag == 2)
+    Emax = oracle_power_emax(M, n, rowptr, col, val, x, pm_itrmax < 1 ? 1 : pm_itrmax, red);
   if (emax_out) *emax_out = Emax;
   if (emin_out) *emin_out = Emin;
   int rc = oracle_cbcg_with(M, n, rowptr, col, val, b, x, itrmax, rtol, k, Emax, Emin, red, hist, niter, iret);

@@ -76,6 +76,7 @@ def load():
         "oracle_house_r": (None, [_I, _I, _P, _I, _P]),
         "oracle_tsqr_waves": (_I, [_I]),
         "oracle_power_emax": (_D, [_P, _I, _P, _P, _P, _P, _I, _RP]),
+        "oracle_power_eigen": (None, [_P, _I, _P, _P, _P, _P, _I, _RP, _P, _P, _P]),
         "oracle_cbcg": (_I, [_I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _D, _I, _I, _I, _RP, _P, _IP, _IP,
                              ctypes.POINTER(_D), ctypes.POINTER(_D)]),
         "oracle_cbcg_with": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _D, _RP, _P, _IP, _IP]),
@@ -279,6 +280,18 @@ def power_emax(rowptr, col, val, x0, ipreflag=0, ilu_method=1, nblocks=1, indepe
     pre = Precon(rowptr, col, val, ipreflag, ilu_method, nblocks, independ_nvec)
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
     return load().oracle_power_emax(pre.h, n, _p(rowptr), _p(col), _p(val), _p(x0), itrmax, ctypes.byref(red))
+
+
+def power_eigen(rowptr, col, val, x0, ipreflag=0, ilu_method=1, nblocks=1, independ_nvec=1, itrmax=100,
+                red=TILED):
+    """power_method_symMatrix in full: (Emax, ||A x - Emax x|| / ||Emax x||, x)."""
+    n = rowptr.shape[0] - 1
+    pre = Precon(rowptr, col, val, ipreflag, ilu_method, nblocks, independ_nvec)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    e, en, vec = ctypes.c_double(0.0), ctypes.c_double(0.0), np.zeros(n)
+    load().oracle_power_eigen(pre.h, n, _p(rowptr), _p(col), _p(val), _p(x0), itrmax, ctypes.byref(red),
+                              ctypes.byref(e), ctypes.byref(en), _p(vec))
+    return e.value, en.value, vec
 
 
 def tsqr_r(V):

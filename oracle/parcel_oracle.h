@@ -122,7 +122,7 @@ int oracle_cagmres(int n, const int* rowptr, const int* col, const double* val,
 
 /* KSP_pCBCG (src/pcbcg.f90:1182-2025), k-step Chebyshev-basis CG, k in 1..8.
  * eigflag 1 and 2: Emax from the power method (pm_itrmax steps from x0),
- * Emin = 0 (2 is CA-Arnoldi, whose GPU-build body is the power method);
+ * Emin = 0 (2 is CA-Arnoldi, whose compiled body is the power method);
  * other: Emax = 1, Emin = 0.  hist must hold
  * itrmax / k + 1 entries; hist[j-1] = ||r|| / ||b|| at the start of outer
  * step j; niter = min(itrmax, j_last) (entries never written are 0). */
@@ -136,6 +136,12 @@ int oracle_cbcg_with(const oracle_precon* M, int n, const int* rowptr, const int
                      const oracle_red* red, double* hist, int* niter, int* iret);
 double oracle_power_emax(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
                          const double* x0, int itrmax, const oracle_red* red);
+/* power_method_symMatrix in full (src/power_method.f90:188-437), the body of
+ * parcel_dpower and parcel_dcaarnoldi: Emax, the eigenpair error
+ * ||A x - Emax x|| / ||Emax x|| and the normalised eigenvector x (vec, n). */
+void oracle_power_eigen(const oracle_precon* M, int n, const int* rowptr, const int* col, const double* val,
+                        const double* x0, int itrmax, const oracle_red* red, double* emax_out, double* errnorm_out,
+                        double* vec);
 /* TSQR R factor (tsqr.c): R (nc x nc, ld nc, nc <= 16) of V (n x nc, ld ldv)
  * with a non-negative diagonal; oracle_house_r is the second level
  * (House_local_Ronly_omp) on its own; oracle_tsqr_waves the number of

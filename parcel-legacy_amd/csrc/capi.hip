@@ -677,6 +677,63 @@ int parcel_hip_dcbcg_dia(int icomm, double* vecx, const double* vecb, int n, int
   return one_shot_dia(prm, vecx, vecb, n, gn, istart, diaA, offset, nnd, itrmax, rtolmax, reshistory, iret);
 }
 
+// power_method_symMatrix on a created plan (src/power_method.f90:188-437):
+// max(1, itr) iterations from vecx (host or device; left unchanged), then the
+// reference's eigen outputs: one eigenpair, EigenValueReal/Imag(1),
+// EigenErrorNorm(1), MaxEigenID = MinEigenID = 1 (:400-425), and the first
+// nvec components of the normalised eigenvector into evec (host or device).
+static int eigen_run(parcel_hip_plan* plan, const double* vecx, int itr, double* ev_re, double* ev_im, double* evec,
+                     int nvec, double* err, int* maxid, int* minid) {
+  PARCEL_API_BEGIN
+  if (!plan || !vecx) return fail(PARCEL_HIP_ERR_ARG, "eigen: null plan or start vector");
+  Plan& P = plan->P;
+  if (P.M.dd_prec) return fail(PARCEL_HIP_ERR_ARG, "eigen: the power method runs on FP64 plans");
+  PARCEL_HIP_CHECK(hipSetDevice(P.device));
+  const int n = P.A.n;
+  DevBuf<double> d(2 * (size_t)n);
+  const size_t bytes = sizeof(double) * (size_t)n;
+  PARCEL_HIP_CHECK(hipMemcpyAsync(d.p, vecx, bytes, is_device_ptr(vecx) ? hipMemcpyDeviceToDevice
+                                                                         : hipMemcpyHostToDevice, P.stream));
+  double emax = 0.0, en = 0.0;
+  // set_KSP_power_method_itrmax keeps at least one iteration (src/type.f90:1331-1336)
+  power_method_eigen(P, d.p, std::max(1, itr), &emax, &en, d.p + n);
+  if (evec && nvec > 0)
+    PARCEL_HIP_CHECK(hipMemcpyAsync(evec, d.p + n, sizeof(double) * (size_t)std::min(nvec, n),
+                                    is_device_ptr(evec) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                    P.stream));
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  if (ev_re) *ev_re = emax;
+  if (ev_im) *ev_im = 0.0;
+  if (err) *err = en;
+  if (maxid) *maxid = 1;
+  if (minid) *minid = 1;
+  return PARCEL_HIP_OK;
+  PARCEL_API_END
+}
+
+// parcel_dpower / parcel_dcaarnoldi through PARCEL_KSP_CRS / _DIA: the CRS
+// driver copies size(EigenValueReal) = 1 component of the eigenvector
+// (src/krylov.f90:360-369), the DIA driver all n (get_EigenPair_KSP,
+// src/krylov.f90:665-673, src/EigenProblem.f90:18-46).  The reference never
+// sets niter and iret for the power method (src/type.f90:805, :902 carry no
+// initialiser); these return 0.
+static int eigen_one_shot(parcel_hip_param& prm, const double* vecx, int n, int gn, int nnz, int istart,
+                          const double* crsA, const int* crsRow_ptr, const int* crsCol, const double* diaA,
+                          const int* offset, int nnd, int* itrmax, int itr, double* ev_re, double* ev_im,
+                          double* evec, double* err, int* maxid, int* minid, int* iret) {
+  if (!vecx || !iret) return fail(PARCEL_HIP_ERR_ARG, "eigen: null argument");
+  parcel_hip_plan* plan = nullptr;
+  int rc = diaA ? parcel_hip_plan_create_dia(&plan, n, gn, istart, diaA, offset, nnd, &prm)
+                : parcel_hip_plan_create(&plan, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &prm);
+  if (rc) return rc;
+  rc = eigen_run(plan, vecx, itr, ev_re, ev_im, evec, diaA ? n : 1, err, maxid, minid);
+  parcel_hip_plan_destroy(plan);
+  if (rc) return rc;
+  if (itrmax) *itrmax = 0;
+  *iret = 0;
+  return PARCEL_HIP_OK;
+}
+
 static int one_shot(parcel_hip_param& prm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
                     const double* crsA, const int* crsRow_ptr, const int* crsCol, int* itrmax, double rtolmax,
                     double* reshistory, int* iret) {
@@ -1225,6 +1282,87 @@ int parcel_hip_dcg_ddm(int icomm, double* x, const double* b, int nrow, int nnd,
     std::memcpy(x, xb.data(), sizeof(double) * m);
   return PARCEL_HIP_OK;
   PARCEL_API_END
+}
+
+int parcel_hip_dpower(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                      const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag, int ILU_method,
+                      int addL, int iflagAS, int* itrmax, int iovlflag, int precon_thblock, int Neuman_itr,
+                      int independ_nvec, int nBlock, int power_method_itrmax, double* EigenValueReal,
+                      double* EigenValueImag, double* EigenVector, double* EigenErrorNorm, int* MaxEigenID,
+                      int* MinEigenID, int* iret) {
+  (void)icomm; (void)vecb;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.power_method_itrmax = power_method_itrmax;
+  return eigen_one_shot(prm, vecx, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, nullptr, nullptr, 0, itrmax,
+                        power_method_itrmax,
+                        EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret);
+}
+
+int parcel_hip_dpower_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart, const double* diaA,
+                          const int* offset, int nnd, int ipreflag, int ILU_method, int addL, int iflagAS,
+                          int* itrmax, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                          int nBlock, int power_method_itrmax, double* EigenValueReal, double* EigenValueImag,
+                          double* EigenVector, double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID, int* iret) {
+  (void)icomm; (void)vecb;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.power_method_itrmax = power_method_itrmax;
+  return eigen_one_shot(prm, vecx, n, gn, 0, istart, nullptr, nullptr, nullptr, diaA, offset, nnd, itrmax,
+                        power_method_itrmax,
+                        EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret);
+}
+
+// KSP_pCAARNOLDI runs power_method_symMatrix (src/pcagmres.f90:2647-2651).
+// parcel_dcaarnoldi leaves power_method_itrmax at INIT_PARCEL_INPUT's -1
+// (src/wrapper.f90:24, :1561-1680), which SET_KSP_PARAM turns into
+// max(1, -1) = 1 iteration (src/krylov.f90:32, src/type.f90:1331-1336); the
+// CA-Arnoldi parameters are kept in the plan's parameters, unused.
+int parcel_hip_dcaarnoldi(int icomm, double* vecx, const double* vecb, int n, int gn, int nnz, int istart,
+                          const double* crsA, const int* crsRow_ptr, const int* crsCol, int ipreflag,
+                          int ILU_method, int addL, int iflagAS, int iovlflag, int precon_thblock, int Neuman_itr,
+                          int independ_nvec, int nBlock, int CAARNOLDI_sstep, int CAARNOLDI_tstep,
+                          int CAARNOLDI_basis, int CAARNOLDI_QRflag, double* EigenValueReal,
+                          double* EigenValueImag, double* EigenVector, double* EigenErrorNorm, int* MaxEigenID,
+                          int* MinEigenID, int* iret) {
+  (void)icomm; (void)vecb;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.CAARNOLDI_sstep = CAARNOLDI_sstep;
+  prm.CAARNOLDI_tstep = CAARNOLDI_tstep;
+  prm.CAARNOLDI_basis = CAARNOLDI_basis;
+  prm.CAARNOLDI_QRflag = CAARNOLDI_QRflag;
+  return eigen_one_shot(prm, vecx, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, nullptr, nullptr, 0, nullptr, 1,
+                        EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret);
+}
+
+int parcel_hip_dcaarnoldi_dia(int icomm, double* vecx, const double* vecb, int n, int gn, int istart,
+                              const double* diaA, const int* offset, int nnd, int ipreflag, int ILU_method, int addL,
+                              int iflagAS, int iovlflag, int precon_thblock, int Neuman_itr, int independ_nvec,
+                              int nBlock, int CAARNOLDI_sstep, int CAARNOLDI_tstep, int CAARNOLDI_basis,
+                              int CAARNOLDI_QRflag, double* EigenValueReal, double* EigenValueImag,
+                              double* EigenVector, double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID,
+                              int* iret) {
+  (void)icomm; (void)vecb;
+  parcel_hip_param prm;
+  common_param(prm, PARCEL_SOLVER_CG, ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr,
+               independ_nvec, nBlock);
+  prm.CAARNOLDI_sstep = CAARNOLDI_sstep;
+  prm.CAARNOLDI_tstep = CAARNOLDI_tstep;
+  prm.CAARNOLDI_basis = CAARNOLDI_basis;
+  prm.CAARNOLDI_QRflag = CAARNOLDI_QRflag;
+  return eigen_one_shot(prm, vecx, n, gn, 0, istart, nullptr, nullptr, nullptr, diaA, offset, nnd, nullptr, 1,
+                        EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret);
+}
+
+int parcel_hip_plan_eigen(parcel_hip_plan* plan, const double* vecx, int power_method_itrmax,
+                          double* EigenValueReal, double* EigenValueImag, double* EigenVector,
+                          double* EigenErrorNorm, int* MaxEigenID, int* MinEigenID) {
+  return eigen_run(plan, vecx, power_method_itrmax, EigenValueReal, EigenValueImag, EigenVector,
+                   plan ? plan->P.A.n : 0, EigenErrorNorm, MaxEigenID, MinEigenID);
 }
 
 }  // extern "C"

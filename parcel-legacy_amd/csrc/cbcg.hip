@@ -1,8 +1,8 @@
 // cbcg.hip -- Chebyshev-basis CG, KSP_pCBCG (src/pcbcg.f90:1182-2025), with
 // the eigenvalue bounds of getEigenValue (src/pcbcg.f90:179-196): the power
 // method (power_method_symMatrix, src/power_method.f90:188-437) for
-// CBCG_Eigenflag 1 and, as in the reference's GPU build, for 2 (CA-Arnoldi,
-// whose GPU-build body is the power method, src/pcagmres.f90:2647-2651).
+// CBCG_Eigenflag 1 and, as the reference compiles it, for 2 (CA-Arnoldi,
+// whose body is the power method under an `#if 1`, src/pcagmres.f90:2647-2651).
 //
 // One outer step of k-step CBCG:
 //   r   -= AQ alpha                                       (calc_gemv)
@@ -272,8 +272,8 @@ void update(int k, bool upd, dim3 g, hipStream_t st, int n, double* x, const dou
 
 // power_method_symMatrix (src/power_method.f90:188-437): from y = x (or all
 // ones when x = 0), repeat x' = y/||y||, y = A M^-1 x', Emax = <x',y>.
-// Returns Emax; the eigenvector error check after the loop only prints.
-double power_method_emax(Plan& P, const double* x0, int itrmax, int* launches) {
+// Returns Emax; vnorm_out (when given) receives the last <y,y>.
+double power_method_emax(Plan& P, const double* x0, int itrmax, int* launches, double* vnorm_out) {
   const int n = P.A.n;
   const hipStream_t st = P.stream;
   ensure_vecs(P, 3);
@@ -315,7 +315,72 @@ double power_method_emax(Plan& P, const double* x0, int itrmax, int* launches) {
     emax = h[1];
   }
   PARCEL_HIP_CHECK(hipGetLastError());
+  if (vnorm_out) *vnorm_out = vnorm;
   return emax;
+}
+
+namespace {
+// the eigenpair check of power_method_symMatrix after its loop
+// (src/power_method.f90:366-398): with x the normalised iterate and y = A x,
+// kz = Emax x (axpy with beta 0) and z = y - kz (axpy 1, -1); partials of
+// <kz,kz> (slot 0) and <z,z> (slot 1)
+__global__ void __launch_bounds__(kNT) k_pm_err(int n, double emax, const double* __restrict__ x,
+                                                const double* __restrict__ y, double* part) {
+  __shared__ double sh[kWaves];
+  const int nb = gridDim.x;
+  const long ntiles = ((long)n + kNT - 1) / kNT;
+  double a0 = 0.0, a1 = 0.0;
+  for (long t = blockIdx.x; t < ntiles; t += nb) {
+    const long j = t * kNT + threadIdx.x;
+    if (j < n) {
+      const double kz = emax * x[j];
+      const double z = y[j] - kz;
+      a0 = fma(kz, kz, a0);
+      a1 = fma(z, z, a1);
+    }
+  }
+  const double s0 = block_sum(a0, sh);
+  const double s1 = block_sum(a1, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0;
+    part[kNBMax + blockIdx.x] = s1;
+  }
+}
+}  // namespace
+
+// power_method_symMatrix in full (src/power_method.f90:188-437), the body of
+// parcel_dpower and, as the reference compiles it (an `#if 1` branch), of
+// parcel_dcaarnoldi
+// (KSP_pCAARNOLDI, src/pcagmres.f90:2647-2651): the loop of
+// power_method_emax, then x = y / ||y||, y = A x, and the eigenpair error
+// ||y - Emax x|| / ||Emax x||.  vec (device, n) receives x.
+void power_method_eigen(Plan& P, const double* x0, int itrmax, double* emax, double* errnorm, double* vec) {
+  int launches = 0;
+  double vnorm = 0.0;
+  const double e = power_method_emax(P, x0, itrmax, &launches, &vnorm);
+  const int n = P.A.n;
+  const hipStream_t st = P.stream;
+  double* y = P.vec[0].p;
+  const int nb = tiled_nb(n);
+  const dim3 grid(nb), block(kNT);
+  double* part = P.wpart.p;
+  double* out = P.wpart.p + 2 * kNBMax;
+  double h[2];
+  auto fetch = [&](int cnt) {
+    hipLaunchKernelGGL(k_cb_reduce, dim3(cnt), block, 0, st, (const double*)part, nb, out);
+    PARCEL_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    PARCEL_HIP_CHECK(hipStreamSynchronize(st));
+  };
+  // vnorm: the last <y,y> of the loop (of the start vector when itrmax < 1)
+  const double t1 = 1.0 / sqrt(vnorm);
+  hipLaunchKernelGGL(k_pm_scale, grid, block, 0, st, n, t1, (const double*)y, vec);
+  hipLaunchKernelGGL(k_pm_spmv, grid, block, 0, st, n, sell_args(P.A), (const double*)vec, y, (const double*)vec,
+                     part);
+  hipLaunchKernelGGL(k_pm_err, grid, block, 0, st, n, e, (const double*)vec, (const double*)y, part);
+  PARCEL_HIP_CHECK(hipGetLastError());
+  fetch(2);
+  *emax = e;
+  *errnorm = sqrt(fabs(h[1])) / sqrt(fabs(h[0]));
 }
 
 int cbcg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, double* hist_host, int* niter,
@@ -334,10 +399,12 @@ int cbcg_solve(Plan& P, double* x, const double* b, int itrmax, double rtol, dou
   // eigenvalue bounds (getEigenValue, src/pcbcg.f90:179-196, and the guards
   // of KSP_pCBCG, :1500-1512)
   double Emax = 0.0, Emin = 0.0;
-  // flag 2 (CA-Arnoldi) is the power method in the reference's GPU build:
-  // KSP_pCAARNOLDI compiles to power_method_symMatrix (pcagmres.f90:2647-2651)
+  // flag 2 (CA-Arnoldi) is the power method as the reference compiles it:
+  // KSP_pCAARNOLDI's `#if 1` branch calls power_method_symMatrix
+  // (src/pcagmres.f90:2647-2651); SET_KSP_PARAM keeps power_method_itrmax
+  // at least 1 (src/krylov.f90:32, src/type.f90:1331-1336)
   if (P.prm.CBCG_Eigenflag == 1 || P.prm.CBCG_Eigenflag == 2)
-    Emax = power_method_emax(P, x, P.prm.power_method_itrmax, &launches);
+    Emax = power_method_emax(P, x, std::max(1, P.prm.power_method_itrmax), &launches, nullptr);
   if (Emax == Emin) {
     if (Emax < 0.0) Emax = 0.0;
     else Emin = 0.0;

@@ -81,6 +81,23 @@ void parcel_dcbcg_c_(const int* icomm, double* vecx, const double* vecb, const i
   if (rc) *iret = rc;
 }
 
+// parcel_dcaarnoldi_c (src/example/c2fortran_kernel.f90:1119-1226)
+void parcel_dcaarnoldi_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                          const int* nnz, const int* istart, const double* crsa, const int* crsrow_ptr,
+                          const int* crscol, const int* ipreflag, const int* ilu_method, const int* addl,
+                          const int* iflagas, const int* iovlflag, const int* precon_thblock,
+                          const int* Neuman_itr, const int* independ_nvec, const int* nBlock,
+                          const int* caarnoldi_sstep, const int* caarnoldi_tstep, const int* caarnoldi_basis,
+                          const int* caarnoldi_qrflag, double* EigenValueReal, double* EigenValueImag,
+                          double* EigenVector, double* EigenErrorNorm, int* EmaxID, int* EminID, int* iret) {
+  int rc = parcel_hip_dcaarnoldi(*icomm, vecx, vecb, *n, *gn, *nnz, *istart, crsa, crsrow_ptr, crscol, *ipreflag,
+                                 *ilu_method, *addl, *iflagas, *iovlflag, *precon_thblock, *Neuman_itr,
+                                 *independ_nvec, *nBlock, *caarnoldi_sstep, *caarnoldi_tstep, *caarnoldi_basis,
+                                 *caarnoldi_qrflag, EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm,
+                                 EmaxID, EminID, iret);
+  if (rc) *iret = rc;
+}
+
 // parcel_ddcg_c takes Neuman_itr after precon_thblock although parcel_ddcg
 // has no such argument; it is accepted and unused.
 void parcel_ddcg_c_(const int* icomm, double* vecx, double* vecx_lo, const double* vecb, const double* vecb_lo,

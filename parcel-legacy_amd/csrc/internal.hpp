@@ -299,6 +299,9 @@ int gmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double 
                 double* hist_host, int* niter, int* iret);
 int cagmres_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                   double* hist_host, int* niter, int* iret);
+// power_method_symMatrix in full (cbcg.hip): Emax of A M^-1, the eigenpair
+// error ||A x - Emax x|| / ||Emax x|| and the eigenvector x (device, n)
+void power_method_eigen(Plan& P, const double* x0, int itrmax, double* emax, double* errnorm, double* vec);
 int cbcg_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,
                double* hist_host, int* niter, int* iret);
 int mixed_solve(Plan& P, double* x_dev, const double* b_dev, int itrmax, double rtol,

@@ -3,7 +3,8 @@
 ! Keeps the procedure names and argument lists of the reference modules
 ! (src/wrapper.f90): parcel_dcg (:544-632 GPU build, :1995-2083 CPU build),
 ! parcel_dbicgstab (:362, :1813), parcel_dgmres (:727, :2179),
-! parcel_dcagmres (:1355, :2807), parcel_dcbcg (:1139); parcel_ddcg
+! parcel_dcagmres (:1355, :2807), parcel_dcbcg (:1139), parcel_dpower (:913)
+! and parcel_dcaarnoldi (:1561); parcel_ddcg
 ! (src/ddwrapper.f90:184); parcel_dcg_ddm (src/src_DDM/ddm_kernel.f90:2266);
 ! and PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22).  Each generic
 ! name takes either host arrays (the CPU build's real*8 / integer arrays) or
@@ -15,7 +16,7 @@ module PARCEL_wrapper
   implicit none
   private
   public :: PARCEL_Init, PARCEL_Free, parcel_dcg, parcel_dbicgstab, parcel_dgmres, parcel_dcagmres, &
-       parcel_dcbcg, parcel_ddcg, parcel_dcg_ddm
+       parcel_dcbcg, parcel_ddcg, parcel_dcg_ddm, parcel_dpower, parcel_dcaarnoldi
 
   interface
     integer(c_int) function c_init(myrank) bind(c, name="parcel_hip_init")
@@ -73,6 +74,26 @@ module PARCEL_wrapper
       integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
       integer(c_int), value :: kstep, eigenflag, pm_itrmax, ca_s, ca_t, ca_basis, ca_qr
     end function
+    integer(c_int) function c_dpower(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, independ_nvec, &
+         nBlock, pm_itrmax, ev_re, ev_im, ev_vec, ev_err, maxid, minid, iret) bind(c, name="parcel_hip_dpower")
+      import :: c_int, c_ptr
+      integer(c_int), value :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, crsA, crsRow_ptr, crsCol, ev_re, ev_im, ev_vec, ev_err
+      integer(c_int) :: itrmax, maxid, minid, iret
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, pm_itrmax
+    end function
+    integer(c_int) function c_dcaarnoldi(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+         ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+         ca_s, ca_t, ca_basis, ca_qr, ev_re, ev_im, ev_vec, ev_err, maxid, minid, iret) &
+         bind(c, name="parcel_hip_dcaarnoldi")
+      import :: c_int, c_ptr
+      integer(c_int), value :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, crsA, crsRow_ptr, crsCol, ev_re, ev_im, ev_vec, ev_err
+      integer(c_int) :: maxid, minid, iret
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: ca_s, ca_t, ca_basis, ca_qr
+    end function
     integer(c_int) function c_ddcg(icomm, xh, xl, bh, bl, n, gn, nnz, istart, ah, al, crsRow_ptr, crsCol, &
          ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
          independ_nvec, nBlock, pa, pb, px, pp, iret) bind(c, name="parcel_hip_ddcg")
@@ -117,6 +138,12 @@ module PARCEL_wrapper
   end interface
   interface parcel_ddcg
     module procedure parcel_ddcg_host, parcel_ddcg_dev
+  end interface
+  interface parcel_dpower
+    module procedure parcel_dpower_host, parcel_dpower_dev
+  end interface
+  interface parcel_dcaarnoldi
+    module procedure parcel_dcaarnoldi_host, parcel_dcaarnoldi_dev
   end interface
 
 contains
@@ -323,6 +350,81 @@ contains
          independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, &
          CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
   end subroutine parcel_dcbcg_dev
+
+  ! ---- power method / CA-Arnoldi (one eigenpair; the CRS drivers return
+  ! the first component of the eigenvector) ------------------------------
+  subroutine parcel_dpower_host(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, independ_nvec, &
+       nBlock, power_method_itrmax, EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, &
+       MaxEigenID, MinEigenID, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), crsA(nnz)
+    integer, target, intent(in) :: crsRow_ptr(n + 1), crsCol(nnz)
+    integer, intent(inout) :: itrmax
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, power_method_itrmax
+    real(8), target, intent(inout) :: EigenValueReal(*), EigenValueImag(*), EigenVector(*), EigenErrorNorm(*)
+    integer, intent(out) :: MaxEigenID, MinEigenID, iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dpower(icomm, c_loc(vecx), c_loc(vecb), n, gn, nnz, istart, c_loc(crsA), c_loc(crsRow_ptr), &
+         c_loc(crsCol), ipreflag, ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, &
+         independ_nvec, nBlock, power_method_itrmax, c_loc(EigenValueReal), c_loc(EigenValueImag), &
+         c_loc(EigenVector), c_loc(EigenErrorNorm), MaxEigenID, MinEigenID, iret))
+  end subroutine parcel_dpower_host
+
+  subroutine parcel_dpower_dev(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, independ_nvec, &
+       nBlock, power_method_itrmax, EigenValueReal, EigenValueImag, EigenVector, EigenErrorNorm, &
+       MaxEigenID, MinEigenID, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, crsA, crsRow_ptr, crsCol
+    integer, intent(inout) :: itrmax
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, power_method_itrmax
+    real(8), target, intent(inout) :: EigenValueReal(*), EigenValueImag(*), EigenVector(*), EigenErrorNorm(*)
+    integer, intent(out) :: MaxEigenID, MinEigenID, iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dpower(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+         power_method_itrmax, c_loc(EigenValueReal), c_loc(EigenValueImag), c_loc(EigenVector), &
+         c_loc(EigenErrorNorm), MaxEigenID, MinEigenID, iret))
+  end subroutine parcel_dpower_dev
+
+  subroutine parcel_dcaarnoldi_host(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, EigenValueReal, EigenValueImag, &
+       EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), crsA(nnz)
+    integer, target, intent(in) :: crsRow_ptr(n + 1), crsCol(nnz)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    real(8), target, intent(inout) :: EigenValueReal(*), EigenValueImag(*), EigenVector(*), EigenErrorNorm(*)
+    integer, intent(out) :: MaxEigenID, MinEigenID, iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcaarnoldi(icomm, c_loc(vecx), c_loc(vecb), n, gn, nnz, istart, c_loc(crsA), &
+         c_loc(crsRow_ptr), c_loc(crsCol), ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, &
+         CAARNOLDI_QRflag, c_loc(EigenValueReal), c_loc(EigenValueImag), c_loc(EigenVector), &
+         c_loc(EigenErrorNorm), MaxEigenID, MinEigenID, iret))
+  end subroutine parcel_dcaarnoldi_host
+
+  subroutine parcel_dcaarnoldi_dev(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
+       ipreflag, ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, EigenValueReal, EigenValueImag, &
+       EigenVector, EigenErrorNorm, MaxEigenID, MinEigenID, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, nnz, istart, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, crsA, crsRow_ptr, crsCol
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    real(8), target, intent(inout) :: EigenValueReal(*), EigenValueImag(*), EigenVector(*), EigenErrorNorm(*)
+    integer, intent(out) :: MaxEigenID, MinEigenID, iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcaarnoldi(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, ipreflag, &
+         ILU_method, addL, iflagAS, iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+         CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, c_loc(EigenValueReal), &
+         c_loc(EigenValueImag), c_loc(EigenVector), c_loc(EigenErrorNorm), MaxEigenID, MinEigenID, iret))
+  end subroutine parcel_dcaarnoldi_dev
 
   ! ---- double-double CG (src/ddwrapper.f90:184-282) ----------------------
   subroutine parcel_ddcg_host(icomm, vecx_hi, vecx_lo, vecb_hi, vecb_lo, n, gn, nnz, istart, crsA_hi, crsA_lo, &

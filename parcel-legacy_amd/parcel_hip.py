@@ -105,6 +105,7 @@ SYMBOLS = {
     "parcel_hip_plan_solve": (_I, [_P, _P, _P, _IP, _D, _P, _IP]),
     "parcel_hip_plan_stats": (_I, [_P, ctypes.POINTER(Stats)]),
     "parcel_hip_plan_destroy": (_I, [_P]),
+    "parcel_hip_plan_eigen": (_I, [_P, _P, _I, _P, _P, _P, _P, _IP, _IP]),
     "parcel_hip_plan_set_profile": (_I, [_P, _I]),
     "parcel_hip_plan_spmv": (_I, [_P, _P, _P]),
     "parcel_hip_plan_precon": (_I, [_P, _P, _P]),
@@ -138,6 +139,14 @@ SYMBOLS = {
                                      _I, _I, _I, _I, _I, _I, _IP]),
     "parcel_hip_dcbcg_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _D, _P, _I, _I, _I, _I,
                                   _I, _I, _I, _I, _I, _I, _I, _I, _IP]),
+    "parcel_hip_dpower": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _I, _I, _I, _I, _I,
+                               _I, _P, _P, _P, _P, _IP, _IP, _IP]),
+    "parcel_hip_dpower_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _IP, _I, _I, _I, _I, _I,
+                                   _I, _P, _P, _P, _P, _IP, _IP, _IP]),
+    "parcel_hip_dcaarnoldi": (_I, [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I,
+                                   _I, _I, _I, _I, _P, _P, _P, _P, _IP, _IP, _IP]),
+    "parcel_hip_dcaarnoldi_dia": (_I, [_I, _P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I,
+                                       _I, _I, _I, _I, _P, _P, _P, _P, _IP, _IP, _IP]),
     "parcel_gen_poisson3d_ddm": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "parcel_hip_dcg_ddm": (_I, [_I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _IP, _D,
                                 _P, _I, _I, _I, _I, _I, _IP]),
@@ -464,6 +473,47 @@ def solve_dia(solver, x, b, offset, diaA, ipreflag=PRECON_NONE, ILU_method=ILUD,
     return hist[: max(it.value, 0)].copy(), it.value, iret.value
 
 
+def dpower(x, rowptr=None, col=None, val=None, dia=None, power_method_itrmax=100, ipreflag=PRECON_NONE,
+           ILU_method=ILUD, precon_thblock=-1, independ_nvec=1, addL=0, iflagAS=4, iovlflag=0, Neuman_itr=1,
+           nBlock=2000, caarnoldi=None, fill=0.0):
+    """parcel_dpower (or, with caarnoldi=(s, t, basis, QRflag), parcel_dcaarnoldi,
+    which runs the same power method with one iteration) from the start
+    vector x (all ones when x is 0), on a 1-based CSR matrix or dia=(offset,
+    diaA).  Returns a dict: EigenValueReal, EigenValueImag (lists of one),
+    EigenVector (n; the CRS forms write its first entry only, the rest keep
+    `fill`), EigenErrorNorm (list of one), MaxEigenID, MinEigenID, itrmax
+    (parcel_dpower only), iret; x is left unchanged."""
+    lib = load()
+    if dia is not None:
+        offset, diaA = dia
+        offset = _as(offset, np.int32, name="offset")
+        nnd = _size(offset)
+        x = _as(x, np.float64, name="x")
+        n = _size(x)
+        diaA = _as(diaA, np.float64, nnd * n, "diaA")
+        mat = (1, _ptr(diaA), _ptr(offset), nnd)
+    else:
+        rowptr, col, val, n, nnz = _csr(rowptr, col, val)
+        x = _as(x, np.float64, n, "x")
+        mat = (nnz, 1, _ptr(val), _ptr(rowptr), _ptr(col))
+    er, ei, en = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    vec = np.full(n, fill)
+    maxid, minid, iret, it = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(-1), ctypes.c_int(-1)
+    pre = (ipreflag, ILU_method, addL, iflagAS)
+    common = (iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock)
+    outs = (ctypes.byref(er), ctypes.byref(ei), _ptr(vec), ctypes.byref(en), ctypes.byref(maxid),
+            ctypes.byref(minid), ctypes.byref(iret))
+    if caarnoldi is None:
+        fn = lib.parcel_hip_dpower_dia if dia is not None else lib.parcel_hip_dpower
+        _check(fn(0, _ptr(x), None, n, n, *mat, *pre, ctypes.byref(it), *common, power_method_itrmax, *outs))
+    else:
+        fn = lib.parcel_hip_dcaarnoldi_dia if dia is not None else lib.parcel_hip_dcaarnoldi
+        _check(fn(0, _ptr(x), None, n, n, *mat, *pre, *common, *[int(v) for v in caarnoldi], *outs))
+    return {"EigenValueReal": [er.value], "EigenValueImag": [ei.value], "EigenVector": vec,
+            "EigenErrorNorm": [en.value], "MaxEigenID": maxid.value, "MinEigenID": minid.value,
+            "itrmax": it.value if caarnoldi is None else None, "iret": iret.value}
+
+
 def poisson3d_rhs_parcel(gnx, gny, gnz):
     """make_poisson3D_right_vector (src/example/problem.f90:772-875): (b, x_exact)."""
     n = gnx * gny * gnz
@@ -738,6 +788,19 @@ class Plan:
     def precon(self, r, s):
         r, s = _as(r, np.float64, self.n, "r"), _io(s, np.float64, self.n, "s")
         _check(self.lib.parcel_hip_plan_precon(self.h, _ptr(r), _ptr(s)))
+
+    def eigen(self, x, power_method_itrmax=100, vec=None):
+        """The power method of parcel_dpower on this plan, from x (all ones
+        when x is 0): (Emax, EigenErrorNorm, eigenvector); vec (numpy or
+        DeviceArray of n) receives the eigenvector when given."""
+        x = _as(x, np.float64, self.n, "x")
+        vec = np.zeros(self.n) if vec is None else _io(vec, np.float64, self.n, "vec")
+        er, ei, en = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+        mx, mn = ctypes.c_int(0), ctypes.c_int(0)
+        _check(self.lib.parcel_hip_plan_eigen(self.h, _ptr(x), power_method_itrmax, ctypes.byref(er),
+                                              ctypes.byref(ei), _ptr(vec), ctypes.byref(en), ctypes.byref(mx),
+                                              ctypes.byref(mn)))
+        return er.value, en.value, vec
 
     def stats(self):
         st = Stats()

@@ -15,8 +15,17 @@ mathematics fixes:
 * the power method against numpy's eigenvalues of A M^-1, from a given start
   and from x0 = 0 (the all-ones start);
 * the loop bookkeeping against the DO-loop semantics of :1560-1980
-  (do itr = 1, itrmax/k; niter = min(itrmax, itr); iret = itrmax <= itr).
+  (do itr = 1, itrmax/k; niter = min(itrmax, itr); iret = itrmax <= itr);
+* the full power method of parcel_dpower (Emax, the eigenpair error
+  ||A x - Emax x|| / ||Emax x|| and the eigenvector) against its closed form
+  on diagonal operators, where K steps from x0 give x = A^K x0 / ||A^K x0||
+  and Emax = sum d^(2K-1) / sum d^(2K-2) (exact rationals), and against the
+  closed-form top eigenpair of the reference's 3D Poisson matrix,
+  lambda = -6 - 6 cos(pi/(N+1)), whose value for N = 160 the reference's own
+  eigen run prints (src/example/template_Eigen_cpu/STDOUT: -11.9988577649884).
 """
+import math
+from fractions import Fraction
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -156,3 +165,68 @@ def test_cbcg_caarnoldi_flag_is_the_power_method():
     r2 = orc.cbcg(rp, col, val, b, ipreflag=1, itrmax=200, rtol=1e-9, k=4, eigflag=2, pm_itrmax=20)
     assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1]) and r1[2:] == r2[2:]
 
+
+
+def _power_closed_form(d, x0, K):
+    """K power steps on diag(d) from x0 in exact rationals: (Emax, err, x)."""
+    d = [Fraction(v) for v in d]
+    x0 = [Fraction(v) for v in x0]
+    yk1 = [di ** (K - 1) * xi for di, xi in zip(d, x0)]  # A^(K-1) x0
+    yk = [di ** K * xi for di, xi in zip(d, x0)]  # A^K x0
+    emax = sum(a * di * a for a, di in zip(yk1, d)) / sum(a * a for a in yk1)
+    nk = sum(a * a for a in yk)
+    num = sum((di - emax) ** 2 * a * a for di, a in zip(d, yk))
+    err = math.sqrt(num / nk) / math.sqrt(emax * emax)
+    x = np.array([float(a) for a in yk]) / math.sqrt(nk)
+    return float(emax), err, x
+
+
+@pytest.mark.parametrize("d,K", [((1.0, 2.0, 3.0), 5), ((-3.0, 1.0, 2.0, 2.5), 7), ((4.0, -1.0), 1)])
+def test_power_eigen_closed_form_on_diagonal_operators(d, K):
+    n = len(d)
+    rp = np.arange(1, n + 2, dtype=np.int32)
+    col = np.arange(1, n + 1, dtype=np.int32)
+    val = np.array(d)
+    for x0 in (np.array([1.0, 0.5, -2.0, 3.0][:n]), np.zeros(n)):
+        start = x0 if x0.any() else np.ones(n)  # a zero start is replaced by all ones
+        e, err, vec = orc.power_eigen(rp, col, val, x0, ipreflag=0, itrmax=K)
+        ce, cerr, cvec = _power_closed_form(d, start, K)
+        assert abs(e - ce) <= 1e-14 * abs(ce)
+        assert abs(err - cerr) <= 1e-12 * cerr
+        assert np.allclose(vec, cvec, rtol=1e-14, atol=1e-15)
+    # the error is relative to ||Emax x||, not ||A x||: the two differ before convergence
+    e, err, vec = orc.power_eigen(rp, col, val, np.array([1.0, 0.5, -2.0, 3.0][:n]), itrmax=K)
+    ax = np.linalg.norm(val * vec)
+    assert abs(ax - abs(e)) > 1e-6 * ax
+
+
+def test_power_eigen_poisson_top_eigenpair():
+    # the closed form printed by the reference's eigen run on its 160^3 Poisson
+    assert abs(-6.0 - 6.0 * math.cos(math.pi / 161) - (-11.9988577649884)) < 1e-12
+    N = 6
+    rp, col, val = ph.poisson3d_parcel(N, N, N)
+    lam = -6.0 - 6.0 * math.cos(math.pi / (N + 1))
+    rng = np.random.default_rng(11)
+    e, err, vec = orc.power_eigen(rp, col, val, rng.standard_normal(N ** 3), itrmax=2000)
+    assert abs(e - lam) < 1e-12 * abs(lam)
+    assert err < 1e-12
+    # the top mode: sin(k pi i / (N+1)) with k = N in each direction
+    s = np.sin(N * np.pi * np.arange(1, N + 1) / (N + 1))
+    v = np.einsum("i,j,k->kji", s, s, s).ravel()
+    v /= np.linalg.norm(v)
+    assert abs(abs(vec @ v) - 1.0) < 1e-12 and abs(np.linalg.norm(vec) - 1.0) < 1e-14
+
+
+def test_power_eigen_error_with_preconditioner_uses_a():
+    # with M the iteration is on A M^-1, the error check on A alone
+    # (calc_Matvec after the loop, src/power_method.f90:366-398)
+    rp, col, val = ph.poisson3d_spd(5, 4, 3)
+    n = rp.shape[0] - 1
+    rng = np.random.default_rng(4)
+    x0 = rng.standard_normal(n)
+    e, err, vec = orc.power_eigen(rp, col, val, x0, ipreflag=1, itrmax=40)
+    Ad = _dense(rp, col, val)
+    assert abs(e - orc.power_emax(rp, col, val, x0, ipreflag=1, itrmax=40)) == 0.0
+    assert abs(np.linalg.norm(vec) - 1.0) < 1e-14
+    ref = np.linalg.norm(Ad @ vec - e * vec) / np.linalg.norm(e * vec)
+    assert abs(err - ref) < 1e-10 * ref
