@@ -163,6 +163,37 @@ __device__ __forceinline__ T mat_row(const St7<T>& A, long i, G g) {
   return t;
 }
 
+template <class MT>
+struct IsSt7 : std::false_type {};
+template <class T>
+struct IsSt7<St7<T>> : std::true_type {};
+
+// Row i (< A.n) of the stencil form times g = beta p_old + s, mat_row's
+// sum, with every operand loaded before the first product: an absent
+// neighbour reads row i instead (an address inside the arrays) and a select
+// drops its term, so a row is one round of independent loads.  gc: g(i).
+template <class T, class TZ>
+__device__ __forceinline__ T st7_row_g(const St7<T>& A, long i, T beta, const T* __restrict__ po,
+                                       const TZ* __restrict__ s, T gc) {
+  const unsigned ui = (unsigned)i, l = A.dx.div(ui), q = A.dy.div(l);
+  const int x = (int)(ui - l * (unsigned)A.nx), y = (int)(l - q * (unsigned)A.ny);
+  const bool hb = i >= A.nxy, hs = y > 0, hw = x > 0, he = x < A.nx - 1, hn = y < A.ny - 1, ht = i + A.nxy < A.n;
+  const long ib = hb ? i - A.nxy : i, is = hs ? i - A.nx : i, iw = hw ? i - 1 : i;
+  const long ie = he ? i + 1 : i, in = hn ? i + A.nx : i, it = ht ? i + A.nxy : i;
+  const T ab = A.t[ib], as = A.nn[is], aw = A.e[iw], ac = A.c[i], ae = A.e[i], an = A.nn[i], at = A.t[i];
+  const T pb = po[ib], ps = po[is], pw = po[iw], pe = po[ie], pn = po[in], pt = po[it];
+  const T zb = (T)s[ib], zs = (T)s[is], zw = (T)s[iw], ze = (T)s[ie], zn = (T)s[in], zt = (T)s[it];
+  T t = 0;
+  t = hb ? fma(ab, fma(beta, pb, zb), t) : t;
+  t = hs ? fma(as, fma(beta, ps, zs), t) : t;
+  t = hw ? fma(aw, fma(beta, pw, zw), t) : t;
+  t = fma(ac, gc, t);
+  t = he ? fma(ae, fma(beta, pe, ze), t) : t;
+  t = hn ? fma(an, fma(beta, pn, zn), t) : t;
+  t = ht ? fma(at, fma(beta, pt, zt), t) : t;
+  return t;
+}
+
 template <class T>
 __device__ __forceinline__ St7<T> sys_matrix(St7<T> A, const Bat& B) {
   (void)B;  // single-system plans only
@@ -262,6 +293,9 @@ __global__ void __launch_bounds__(kNT) k_cg_init(const double* part, int nb, CGS
 // LB rows per thread are in flight.  TZ: the stored type of the
 // preconditioned residual s (float when an FP32 apply produced it: widened
 // at use, exactly).
+// rounds per trip of the stencil-form loop
+constexpr int kSt7U = 2;
+
 template <class T, int LB = 1, class MT = SellArgsT<T>, class TZ = T>
 __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, double rtol, MT A,
                                                  T* __restrict__ x, const T* __restrict__ p_old,
@@ -323,18 +357,47 @@ __global__ void __launch_bounds__(kNT) k_cg_spmv(int n, int it, int itrmax, doub
   double acc[LB];
 #pragma unroll
   for (int j = 0; j < LB; ++j) acc[j] = 0.0;
-  for (long base = 0; base < ntiles; base += nb) {
+  if constexpr (IsSt7<MT>::value && LB == 1) {
+    // kSt7U rounds per trip, every load of the trip issued before its first
+    // product; each row's values and the partial's order are the loop's below
+    constexpr int U = kSt7U;
+    const long stride = (long)nb * kNT;
+    for (long base = 0; base < ntiles; base += (long)U * nb) {
+      long i[U];
+      T xo[U], po[U], pn[U], t[U];
 #pragma unroll
-    for (int j = 0; j < LB; ++j) {
-      const long i = (base + o0 + j) * kNT + threadIdx.x;
-      if (i < n) {
-        const T po = p_old[i];
-        x[i] = fma((T)alpha_prev, po, x[i]);
-        const T pn = fma((T)beta, po, (T)s[i]);
-        p_new[i] = pn;
-        const T t = mat_row(A, i, [&](long c) { return fma((T)beta, p_old[c], (T)s[c]); });
-        q[i] = t;
-        acc[j] = fma((double)pn, (double)t, acc[j]);
+      for (int u = 0; u < U; ++u) {
+        i[u] = (base + o0) * kNT + threadIdx.x + u * stride;
+        const long ic = i[u] < n ? i[u] : (long)n - 1;
+        po[u] = p_old[ic];
+        xo[u] = x[ic];
+        pn[u] = fma((T)beta, po[u], (T)s[ic]);
+        t[u] = st7_row_g(A, ic, (T)beta, p_old, s, pn[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i[u] < n) {
+          x[i[u]] = fma((T)alpha_prev, po[u], xo[u]);
+          p_new[i[u]] = pn[u];
+          q[i[u]] = t[u];
+          acc[0] = fma((double)pn[u], (double)t[u], acc[0]);
+        }
+      }
+    }
+  } else {
+    for (long base = 0; base < ntiles; base += nb) {
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        const long i = (base + o0 + j) * kNT + threadIdx.x;
+        if (i < n) {
+          const T po = p_old[i];
+          x[i] = fma((T)alpha_prev, po, x[i]);
+          const T pn = fma((T)beta, po, (T)s[i]);
+          p_new[i] = pn;
+          const T t = mat_row(A, i, [&](long c) { return fma((T)beta, p_old[c], (T)s[c]); });
+          q[i] = t;
+          acc[j] = fma((double)pn, (double)t, acc[j]);
+        }
       }
     }
   }
