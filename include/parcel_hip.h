@@ -119,6 +119,8 @@ typedef struct parcel_hip_stats {
   double final_rel_res;    /* ||b - A x|| / ||b|| after the last correction (FP64) */
   int bj_lscan;            /* 1: the mixed solver's FP32 block-Jacobi apply runs block IC(0) line scans */
   int spmv_stencil;        /* 1: the CG kernel reads the matrix as a symmetric 5/7-point stencil (4 arrays) */
+  long sell_entries;       /* stored SELL-64 entries (padding and the long rows' overflow included) */
+  long sell_overflow;      /* entries of long rows kept after the slices (0: none) */
 } parcel_hip_stats;
 
 /* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */

@@ -71,6 +71,12 @@ __device__ __forceinline__ dd dd_row(const DDMat& M, long i, const double* xh, c
     const int c = M.A.col[at];
     if (c >= 0) t = dd_add(dd_mul(mkdd(M.A.val[at], M.vlo ? M.vlo[at] : 0.0), mkdd(xh[c], xl[c])), t);
   }
+  long e0, e1;
+  sell_overflow(M.A, i, e0, e1);
+  for (long e = e0; e < e1; ++e) {
+    const int c = M.A.col[e];
+    t = dd_add(dd_mul(mkdd(M.A.val[e], M.vlo ? M.vlo[e] : 0.0), mkdd(xh[c], xl[c])), t);
+  }
   return t;
 }
 

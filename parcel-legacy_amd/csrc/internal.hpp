@@ -81,6 +81,15 @@ struct DevSELL {
   DevBuf<double> val;    // entries
   DevBuf<float> val32;   // FP32 copy (mixed-precision inner solve)
   DevBuf<double> val_lo; // lo parts of a double-double matrix (empty: none)
+  // long rows: a slice is as wide as its rows after the few longest, whose
+  // remaining entries follow the slices in col/val (entries counts them);
+  // so[slice] = index of the slice's first overflow row (-1: none),
+  // omask[slice] = its lanes holding one, optr = the rows' entry ranges.
+  // Empty when no slice has one.
+  long ovf_entries = 0;
+  DevBuf<int> so;
+  DevBuf<unsigned long long> omask;
+  DevBuf<long> optr;
 };
 
 // 0-based CSR copy on the host (setup work: factorisations, level analysis).

@@ -14,12 +14,35 @@ struct SellArgsT {
   const int* sk;
   const int* col;
   const T* val;
+  // long rows (DevSELL): null when the matrix has none
+  const int* so = nullptr;
+  const unsigned long long* omask = nullptr;
+  const long* optr = nullptr;
 };
 using SellArgs = SellArgsT<double>;
 
-inline SellArgs sell_args(const DevSELL& S) { return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p}; }
+inline SellArgs sell_args(const DevSELL& S) {
+  return SellArgs{S.sptr.p, S.sk.p, S.col.p, S.val.p, S.so.p, S.omask.p, S.optr.p};
+}
 inline SellArgsT<float> sell_args32(const DevSELL& S) {
-  return SellArgsT<float>{S.sptr.p, S.sk.p, S.col.p, S.val32.p};
+  return SellArgsT<float>{S.sptr.p, S.sk.p, S.col.p, S.val32.p, S.so.p, S.omask.p, S.optr.p};
+}
+
+// The overflow entries [e0, e1) of row i (e0 == e1: none): a long row's
+// entries after the first sk[slice], stored after the slices.
+template <class T>
+__device__ __forceinline__ void sell_overflow(const SellArgsT<T>& A, long i, long& e0, long& e1) {
+  e0 = e1 = 0;
+  if (!A.so) return;
+  const long s = i >> 6;
+  const int ob = A.so[s];
+  if (ob < 0) return;
+  const unsigned long long m = A.omask[s];
+  const int lane = (int)(i & 63);
+  if (!((m >> lane) & 1ull)) return;
+  const int r = ob + __popcll(m & ((1ull << lane) - 1ull));
+  e0 = A.optr[r];
+  e1 = A.optr[r + 1];
 }
 
 // Row i of the SELL matrix times the vector produced by `g(c)`, entries in
@@ -35,6 +58,9 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& A, long i, G g) {
     const T v = A.val[base + (long)j * 64];
     if (c >= 0) t = fma(v, g(c), t);
   }
+  long e0, e1;
+  sell_overflow(A, i, e0, e1);
+  for (long e = e0; e < e1; ++e) t = fma(A.val[e], g(A.col[e]), t);
   return t;
 }
 

@@ -37,6 +37,31 @@
 namespace parcel {
 
 // --- SELL-64 build (host) ----------------------------------------------------
+// Width of a slice and its overflow rows: the slice is as wide as its
+// (m+1)-th longest row and its m longest rows (m <= kOvfMax) keep the rest of
+// their entries in the overflow area, m chosen to minimise the stored
+// entries; overflow only where it saves at least 8 slice columns.
+constexpr int kOvfMax = 16;
+static int slice_split(const std::vector<int>& len, std::vector<int>& ovf_lanes) {
+  std::vector<int> ord(len.size());
+  for (size_t l = 0; l < len.size(); ++l) ord[l] = (int)l;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return len[a] > len[b]; });
+  const long full = 64L * len[ord[0]];
+  long best = full, extra = 0;
+  int bm = 0;
+  for (int m = 1; m <= kOvfMax && m < (int)ord.size(); ++m) {
+    extra += len[ord[m - 1]];
+    const long w = len[ord[m]];
+    const long cost = 64 * w + (extra - m * w) + 2L * m;
+    if (cost < best) { best = cost; bm = m; }
+  }
+  ovf_lanes.clear();
+  if (bm == 0 || full - best < 64L * 8) return len[ord[0]];
+  for (int m = 0; m < bm; ++m) ovf_lanes.push_back(ord[m]);
+  std::sort(ovf_lanes.begin(), ovf_lanes.end());
+  return len[ord[bm]];
+}
+
 void sell_build(Plan& P) {
   const HostCSR& A = P.hA;
   const int n = A.n;
@@ -47,15 +72,33 @@ void sell_build(Plan& P) {
   S.nslices = ns;
   std::vector<long> sptr(ns + 1, 0);
   std::vector<int> sk(ns, 0);
+  std::vector<int> so(ns, -1);
+  std::vector<unsigned long long> omask(ns, 0ull);
+  std::vector<long> orow;  // overflow rows, by slice then lane
+  std::vector<int> len(64), lanes;
   for (long s = 0; s < ns; ++s) {
-    int k = 0;
-    const long r1 = std::min((long)n, s * 64 + 64);
-    for (long r = s * 64; r < r1; ++r) k = std::max(k, A.rowptr[r + 1] - A.rowptr[r]);
-    sk[s] = k;
-    sptr[s + 1] = sptr[s] + 64L * k;
+    const long r0 = s * 64, r1 = std::min((long)n, r0 + 64);
+    len.assign((size_t)(r1 - r0), 0);
+    for (long r = r0; r < r1; ++r) len[(size_t)(r - r0)] = A.rowptr[r + 1] - A.rowptr[r];
+    sk[s] = slice_split(len, lanes);
+    if (!lanes.empty()) {
+      so[s] = (int)orow.size();
+      for (int l : lanes) {
+        omask[s] |= 1ull << l;
+        orow.push_back(r0 + l);
+      }
+    }
+    sptr[s + 1] = sptr[s] + 64L * sk[s];
   }
-  const long tot = sptr[ns];
+  const long main = sptr[ns];
+  std::vector<long> optr(orow.size() + 1, main);
+  for (size_t k = 0; k < orow.size(); ++k) {
+    const long r = orow[k];
+    optr[k + 1] = optr[k] + (A.rowptr[r + 1] - A.rowptr[r]) - sk[r >> 6];
+  }
+  const long tot = optr.back();
   S.entries = tot;
+  S.ovf_entries = tot - main;
   std::vector<int> col((size_t)tot);
   std::vector<double> val((size_t)tot);
   const bool lo = !A.val_lo.empty();
@@ -81,11 +124,31 @@ void sell_build(Plan& P) {
       }
     }
   }
+  // an overflow row's entries after the first sk of its slice, in row order
+  for (size_t q = 0; q < orow.size(); ++q) {
+    const long r = orow[q];
+    const int k = sk[r >> 6];
+    for (long e = optr[q]; e < optr[q + 1]; ++e) {
+      const long src = A.rowptr[r] + k + (e - optr[q]);
+      col[(size_t)e] = A.col[src];
+      val[(size_t)e] = A.val[src];
+      if (lo) vlo[(size_t)e] = A.val_lo[src];
+    }
+  }
   S.sptr.upload(sptr, P.stream);
   S.sk.upload(sk, P.stream);
   S.col.upload(col, P.stream);
   S.val.upload(val, P.stream);
   if (lo) S.val_lo.upload(vlo, P.stream);
+  if (!orow.empty()) {
+    S.so.upload(so, P.stream);
+    S.omask.upload(omask, P.stream);
+    S.optr.upload(optr, P.stream);
+  } else {
+    S.so.release();
+    S.omask.release();
+    S.optr.release();
+  }
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 

@@ -64,6 +64,8 @@ class Stats(ctypes.Structure):
         ("final_rel_res", ctypes.c_double),
         ("bj_lscan", ctypes.c_int),
         ("spmv_stencil", ctypes.c_int),
+        ("sell_entries", ctypes.c_long),
+        ("sell_overflow", ctypes.c_long),
     ]
 
     def as_dict(self):
