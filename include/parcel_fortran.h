@@ -126,6 +126,51 @@ void parcel_dcg_ddm_c_(const int* icomm, double* vecx, const double* vecb, const
                        const int* precon_thblock, const int* Neuman_itr,
                        const int* independ_nvec, const int* nBlock, int* iret);
 
+/* DIA forms (src/example/c2fortran_kernel.f90:1228-2091): the CRS forms'
+ * arguments with (n, gn, istart, diaa, offset, nnd) in place of
+ * (n, gn, nnz, istart, crsa, crsrow_ptr, crscol); parcel_dcaarnoldi_dia_c
+ * has no Neuman_itr. */
+void parcel_dcg_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                       const int* istart, const double* diaa, const int* offset, const int* nnd,
+                       const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas, int* niter,
+                       const double* rtolmax, double* reshistory, const int* iovlflag, const int* precon_thblock,
+                       const int* Neuman_itr, const int* independ_nvec, const int* nBlock, int* iret);
+void parcel_dbicgstab_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                             const int* istart, const double* diaa, const int* offset, const int* nnd,
+                             const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas,
+                             int* niter, const double* rtolmax, double* reshistory, const int* iovlflag,
+                             const int* precon_thblock, const int* Neuman_itr, const int* independ_nvec,
+                             const int* nBlock, int* iret);
+void parcel_dgmres_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                          const int* istart, const double* diaa, const int* offset, const int* nnd,
+                          const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas,
+                          int* niter, const double* rtolmax, double* reshistory, const int* iovlflag,
+                          const int* precon_thblock, const int* Neuman_itr, const int* independ_nvec,
+                          const int* nBlock, const int* restart, const int* gmres_gsflag, int* iret);
+void parcel_dcagmres_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                            const int* istart, const double* diaa, const int* offset, const int* nnd,
+                            const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas,
+                            int* niter, const double* rtolmax, double* reshistory, const int* iovlflag,
+                            const int* precon_thblock, const int* Neuman_itr, const int* independ_nvec,
+                            const int* nBlock, const int* cagmres_sstep, const int* cagmres_tstep,
+                            const int* cagmres_basis, const int* cagmres_qrflag, int* iret);
+void parcel_dcbcg_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                         const int* istart, const double* diaa, const int* offset, const int* nnd,
+                         const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas,
+                         int* niter, const double* rtolmax, double* reshistory, const int* iovlflag,
+                         const int* precon_thblock, const int* Neuman_itr, const int* independ_nvec,
+                         const int* nBlock, const int* cbcg_kstep, const int* cbcg_eigenflag,
+                         const int* power_method_itrmax, const int* caarnoldi_sstep, const int* caarnoldi_tstep,
+                         const int* caarnoldi_basis, const int* caarnoldi_qrflag, int* iret);
+void parcel_dcaarnoldi_dia_c_(const int* icomm, double* vecx, const double* vecb, const int* n, const int* gn,
+                              const int* istart, const double* diaa, const int* offset, const int* nnd,
+                              const int* ipreflag, const int* ilu_method, const int* addl, const int* iflagas,
+                              const int* iovlflag, const int* precon_thblock, const int* independ_nvec,
+                              const int* nBlock, const int* caarnoldi_sstep, const int* caarnoldi_tstep,
+                              const int* caarnoldi_basis, const int* caarnoldi_qrflag, double* EigenValueReal,
+                              double* EigenValueImag, double* EigenVector, double* EigenErrorNorm, int* EmaxID,
+                              int* EminID, int* iret);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,9 @@
 !   6 parcel_ddcg (double-double, lo parts zero; x printed as hi),
 !   7 parcel_dcg_ddm (the same grid as the ghost-padded stencil of
 !     src/example/ddm_setup.f90, margin 1; x printed at the first and last
-!     interior cells; ILU_method and nblocks unused, p1 = addL)
+!     interior cells; ILU_method and nblocks unused, p1 = addL),
+!   8 parcel_dcg_dia, 9 parcel_dgmres_dia (p1 = m, p2 = GS flag): the same
+!     matrix as diagonals (make_poisson2D_matrix_DIA_from_CRS's layout)
 program example_poisson
   use iso_c_binding
   use PARCEL_wrapper
@@ -28,6 +30,12 @@ program example_poisson
       integer(c_int), value :: gnx, gny, gnz
       type(c_ptr), value :: b, xe
     end function
+    integer(c_int) function crs_to_dia(n, rp, col, val, nnd, offset, dia) bind(c, name="parcel_gen_crs_to_dia")
+      import :: c_int, c_ptr
+      integer(c_int), value :: n
+      type(c_ptr), value :: rp, col, val, offset, dia
+      integer(c_int) :: nnd
+    end function
     integer(c_int) function gen_ddm(gnx, gny, gnz, margin, val_dia, ioff_dia, ioff_grids, b, xe) &
          bind(c, name="parcel_gen_poisson3d_ddm")
       import :: c_int, c_ptr
@@ -35,12 +43,12 @@ program example_poisson
       type(c_ptr), value :: val_dia, ioff_dia, ioff_grids, b, xe
     end function
   end interface
-  integer :: solver, g, ipreflag, ilu, nblk, itrmax, n, nnz, iret, i, rc, k, m, addl
+  integer :: solver, g, ipreflag, ilu, nblk, itrmax, n, nnz, iret, i, rc, k, m, addl, nnd
   integer :: p(4)
   real(8) :: rtol
   character(len=32) :: arg
-  integer, allocatable, target :: rp(:), col(:), ioff(:), iog(:, :)
-  real(8), allocatable, target :: val(:), vlo(:), b(:), blo(:), xe(:), x(:), xlo(:), hist(:), vd(:)
+  integer, allocatable, target :: rp(:), col(:), ioff(:), iog(:, :), doff(:)
+  real(8), allocatable, target :: val(:), vlo(:), b(:), blo(:), xe(:), x(:), xlo(:), hist(:), vd(:), dval(:)
   integer, target :: nbr(1), ndiv(3), ngrid(3)
 
   call get_command_argument(1, arg); read (arg, *) solver
@@ -90,6 +98,12 @@ program example_poisson
   xlo = 0.0d0
   vlo = 0.0d0
   blo = 0.0d0
+  if (solver == 8 .or. solver == 9) then
+    nnd = 0
+    rc = crs_to_dia(n, c_loc(rp), c_loc(col), c_loc(val), nnd, c_null_ptr, c_null_ptr)
+    allocate (doff(nnd), dval(nnd * n))
+    rc = crs_to_dia(n, c_loc(rp), c_loc(col), c_loc(val), nnd, c_loc(doff), c_loc(dval))
+  end if
   select case (solver)
   case (1)
     call parcel_dcg(0, x, b, n, n, nnz, 1, val, rp, col, ipreflag, ilu, 0, 4, itrmax, rtol, hist, &
@@ -109,6 +123,12 @@ program example_poisson
   case (6)
     call parcel_ddcg(0, x, xlo, b, blo, n, n, nnz, 1, val, vlo, rp, col, ipreflag, ilu, 0, 4, itrmax, rtol, &
          hist, 0, nblk, 1, 2000, 2, 2, 2, 2, iret)
+  case (8)
+    call parcel_dcg_dia(0, x, b, n, n, 1, dval, doff, nnd, ipreflag, ilu, 0, 4, itrmax, rtol, hist, &
+         0, nblk, 1, 1, 2000, iret)
+  case (9)
+    call parcel_dgmres_dia(0, x, b, n, n, 1, dval, doff, nnd, ipreflag, ilu, 0, 4, itrmax, rtol, hist, &
+         0, nblk, 1, 1, 2000, p(1), p(2), iret)
   end select
   call PARCEL_Free()
   write (*, '(a,i0,a,i0)') 'niter ', itrmax, ' iret ', iret

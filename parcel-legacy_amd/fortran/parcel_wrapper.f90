@@ -4,7 +4,8 @@
 ! (src/wrapper.f90): parcel_dcg (:544-632 GPU build, :1995-2083 CPU build),
 ! parcel_dbicgstab (:362, :1813), parcel_dgmres (:727, :2179),
 ! parcel_dcagmres (:1355, :2807), parcel_dcbcg (:1139), parcel_dpower (:913)
-! and parcel_dcaarnoldi (:1561); parcel_ddcg
+! and parcel_dcaarnoldi (:1561), the DIA forms of the first five (:635, :454,
+! :821, :1457, :1246); parcel_ddcg
 ! (src/ddwrapper.f90:184); parcel_dcg_ddm (src/src_DDM/ddm_kernel.f90:2266);
 ! and PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22).  Each generic
 ! name takes either host arrays (the CPU build's real*8 / integer arrays) or
@@ -16,7 +17,8 @@ module PARCEL_wrapper
   implicit none
   private
   public :: PARCEL_Init, PARCEL_Free, parcel_dcg, parcel_dbicgstab, parcel_dgmres, parcel_dcagmres, &
-       parcel_dcbcg, parcel_ddcg, parcel_dcg_ddm, parcel_dpower, parcel_dcaarnoldi
+       parcel_dcbcg, parcel_ddcg, parcel_dcg_ddm, parcel_dpower, parcel_dcaarnoldi, &
+       parcel_dcg_dia, parcel_dbicgstab_dia, parcel_dgmres_dia, parcel_dcagmres_dia, parcel_dcbcg_dia
 
   interface
     integer(c_int) function c_init(myrank) bind(c, name="parcel_hip_init")
@@ -73,6 +75,67 @@ module PARCEL_wrapper
       real(c_double), value :: rtolmax
       integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
       integer(c_int), value :: kstep, eigenflag, pm_itrmax, ca_s, ca_t, ca_basis, ca_qr
+    end function
+    integer(c_int) function c_dcg_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, iret) &
+         bind(c, name="parcel_hip_dcg_dia")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, diaA, offset, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    end function
+    integer(c_int) function c_dbicgstab_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, iret) &
+         bind(c, name="parcel_hip_dbicgstab_dia")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, diaA, offset, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    end function
+    integer(c_int) function c_dgmres_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, GMRES_m, GMRES_GSflag, iret) &
+         bind(c, name="parcel_hip_dgmres_dia")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, diaA, offset, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: GMRES_m, GMRES_GSflag
+    end function
+    integer(c_int) function c_dcagmres_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, &
+         CAGMRES_basis, CAGMRES_QRflag, iret) &
+         bind(c, name="parcel_hip_dcagmres_dia")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, diaA, offset, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag
+    end function
+    integer(c_int) function c_dcbcg_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+         ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, &
+         precon_thblock, Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, &
+         power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, &
+         iret) &
+         bind(c, name="parcel_hip_dcbcg_dia")
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+      type(c_ptr), value :: vecx, vecb, diaA, offset, reshistory
+      integer(c_int) :: itrmax, iret
+      real(c_double), value :: rtolmax
+      integer(c_int), value :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+      integer(c_int), value :: CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
     end function
     integer(c_int) function c_dpower(icomm, vecx, vecb, n, gn, nnz, istart, crsA, crsRow_ptr, crsCol, &
          ipreflag, ILU_method, addL, iflagAS, itrmax, iovlflag, precon_thblock, Neuman_itr, independ_nvec, &
@@ -138,6 +201,21 @@ module PARCEL_wrapper
   end interface
   interface parcel_ddcg
     module procedure parcel_ddcg_host, parcel_ddcg_dev
+  end interface
+  interface parcel_dcg_dia
+    module procedure parcel_dcg_dia_host, parcel_dcg_dia_dev
+  end interface
+  interface parcel_dbicgstab_dia
+    module procedure parcel_dbicgstab_dia_host, parcel_dbicgstab_dia_dev
+  end interface
+  interface parcel_dgmres_dia
+    module procedure parcel_dgmres_dia_host, parcel_dgmres_dia_dev
+  end interface
+  interface parcel_dcagmres_dia
+    module procedure parcel_dcagmres_dia_host, parcel_dcagmres_dia_dev
+  end interface
+  interface parcel_dcbcg_dia
+    module procedure parcel_dcbcg_dia_host, parcel_dcbcg_dia_dev
   end interface
   interface parcel_dpower
     module procedure parcel_dpower_host, parcel_dpower_dev
@@ -350,6 +428,190 @@ contains
          independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, &
          CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
   end subroutine parcel_dcbcg_dev
+
+  ! ---- DIA forms (src/wrapper.f90:454, :635, :821, :1246, :1457): the CRS
+  ! procedures' arguments with (n, gn, istart, diaA, offset, nnd)
+
+  subroutine parcel_dcg_dia_host(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), diaA(nnd * n), rtolmax
+    integer, target, intent(in) :: offset(nnd)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcg_dia(icomm, c_loc(vecx), c_loc(vecb), n, gn, istart, c_loc(diaA), &
+         c_loc(offset), nnd, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, &
+         c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, iret))
+  end subroutine parcel_dcg_dia_host
+
+  subroutine parcel_dcg_dia_dev(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, diaA, offset
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcg_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, ILU_method, &
+         addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, &
+         independ_nvec, nBlock, iret))
+  end subroutine parcel_dcg_dia_dev
+
+  subroutine parcel_dbicgstab_dia_host(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), diaA(nnd * n), rtolmax
+    integer, target, intent(in) :: offset(nnd)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dbicgstab_dia(icomm, c_loc(vecx), c_loc(vecb), n, gn, istart, c_loc(diaA), &
+         c_loc(offset), nnd, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, &
+         c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, iret))
+  end subroutine parcel_dbicgstab_dia_host
+
+  subroutine parcel_dbicgstab_dia_dev(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, diaA, offset
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dbicgstab_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, iret))
+  end subroutine parcel_dbicgstab_dia_dev
+
+  subroutine parcel_dgmres_dia_host(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, GMRES_m, GMRES_GSflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), diaA(nnd * n), rtolmax
+    integer, target, intent(in) :: offset(nnd)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: GMRES_m, GMRES_GSflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dgmres_dia(icomm, c_loc(vecx), c_loc(vecb), n, gn, istart, c_loc(diaA), &
+         c_loc(offset), nnd, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, &
+         c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, GMRES_m, &
+         GMRES_GSflag, iret))
+  end subroutine parcel_dgmres_dia_host
+
+  subroutine parcel_dgmres_dia_dev(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, GMRES_m, GMRES_GSflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, diaA, offset
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: GMRES_m, GMRES_GSflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dgmres_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, GMRES_m, GMRES_GSflag, iret))
+  end subroutine parcel_dgmres_dia_dev
+
+  subroutine parcel_dcagmres_dia_host(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, &
+       ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, &
+       CAGMRES_QRflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), diaA(nnd * n), rtolmax
+    integer, target, intent(in) :: offset(nnd)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcagmres_dia(icomm, c_loc(vecx), c_loc(vecb), n, gn, istart, c_loc(diaA), &
+         c_loc(offset), nnd, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, &
+         c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+         CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag, iret))
+  end subroutine parcel_dcagmres_dia_host
+
+  subroutine parcel_dcagmres_dia_dev(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, &
+       CAGMRES_QRflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, diaA, offset
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, CAGMRES_QRflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcagmres_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, CAGMRES_sstep, CAGMRES_tstep, CAGMRES_basis, &
+         CAGMRES_QRflag, iret))
+  end subroutine parcel_dcagmres_dia_dev
+
+  subroutine parcel_dcbcg_dia_host(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    real(8), target, intent(inout) :: vecx(n)
+    real(8), target, intent(in) :: vecb(n), diaA(nnd * n), rtolmax
+    integer, target, intent(in) :: offset(nnd)
+    integer, intent(inout) :: itrmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcbcg_dia(icomm, c_loc(vecx), c_loc(vecb), n, gn, istart, c_loc(diaA), &
+         c_loc(offset), nnd, ipreflag, ILU_method, addL, iflagAS, itrmax, rtolmax, &
+         c_loc(reshistory), iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock, &
+         CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep, &
+         CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
+  end subroutine parcel_dcbcg_dia_host
+
+  subroutine parcel_dcbcg_dia_dev(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+       ILU_method, addL, iflagAS, itrmax, rtolmax, reshistory, iovlflag, precon_thblock, &
+       Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+       CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret, machineflag)
+    integer, intent(in) :: icomm, n, gn, istart, nnd, ipreflag, ILU_method, addL, iflagAS
+    type(c_ptr), intent(in) :: vecx, vecb, diaA, offset
+    integer, intent(inout) :: itrmax
+    real(8), intent(in) :: rtolmax
+    real(8), target, intent(inout) :: reshistory(*)
+    integer, intent(in) :: iovlflag, precon_thblock, Neuman_itr, independ_nvec, nBlock
+    integer, intent(in) :: CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag
+    integer, intent(out) :: iret
+    integer, optional, intent(in) :: machineflag
+    call check(c_dcbcg_dia(icomm, vecx, vecb, n, gn, istart, diaA, offset, nnd, ipreflag, &
+         ILU_method, addL, iflagAS, itrmax, rtolmax, c_loc(reshistory), iovlflag, precon_thblock, &
+         Neuman_itr, independ_nvec, nBlock, CBCG_kstep, CBCG_Eigenflag, power_method_itrmax, &
+         CAARNOLDI_sstep, CAARNOLDI_tstep, CAARNOLDI_basis, CAARNOLDI_QRflag, iret))
+  end subroutine parcel_dcbcg_dia_dev
 
   ! ---- power method / CA-Arnoldi (one eigenpair; the CRS drivers return
   ! the first component of the eigenvector) ------------------------------

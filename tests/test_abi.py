@@ -28,8 +28,10 @@ FORTRAN_HEADER = os.path.join(ROOT, "include", "parcel_fortran.h")
 def test_by_reference_symbols_exported():
     text = re.sub(r"/\*.*?\*/", "", open(FORTRAN_HEADER).read(), flags=re.S)
     syms = sorted(set(re.findall(r"\b(parcel_\w+_c_)\s*\(", text)))
-    assert syms == ["parcel_dbicgstab_c_", "parcel_dcaarnoldi_c_", "parcel_dcagmres_c_", "parcel_dcbcg_c_", "parcel_dcg_c_",
-                    "parcel_dcg_ddm_c_", "parcel_ddcg_c_", "parcel_dgmres_c_", "parcel_free_c_", "parcel_init_c_"]
+    assert syms == ["parcel_dbicgstab_c_", "parcel_dbicgstab_dia_c_", "parcel_dcaarnoldi_c_",
+                    "parcel_dcaarnoldi_dia_c_", "parcel_dcagmres_c_", "parcel_dcagmres_dia_c_", "parcel_dcbcg_c_",
+                    "parcel_dcbcg_dia_c_", "parcel_dcg_c_", "parcel_dcg_ddm_c_", "parcel_dcg_dia_c_", "parcel_ddcg_c_",
+                    "parcel_dgmres_c_", "parcel_dgmres_dia_c_", "parcel_free_c_", "parcel_init_c_"]
     lib = ph.load()
     for s in syms:
         assert hasattr(lib, s), s
@@ -56,7 +58,8 @@ def test_fortran_module_built():
     mod = open(os.path.join(lib_dir, "parcel_wrapper.mod"), "rb").read().lower()
     for name in (b"parcel_dcg", b"parcel_dbicgstab", b"parcel_dgmres", b"parcel_dcagmres", b"parcel_dcbcg",
                  b"parcel_ddcg", b"parcel_dcg_ddm", b"parcel_dpower", b"parcel_dcaarnoldi", b"parcel_init",
-                 b"parcel_free"):
+                 b"parcel_free", b"parcel_dcg_dia", b"parcel_dbicgstab_dia", b"parcel_dgmres_dia",
+                 b"parcel_dcagmres_dia", b"parcel_dcbcg_dia"):
         assert name in mod
 
 

@@ -67,11 +67,11 @@ def oracle_case(solver, g, ipre, ilu, nblk, itrmax, rtol, p):
     rp, col, val = ph.poisson3d_parcel(g, g, g)
     b, _ = ph.poisson3d_rhs_parcel(g, g, g)
     kw = dict(ipreflag=ipre, ilu_method=ilu, nblocks=nblk, itrmax=itrmax, rtol=rtol)
-    if solver == 1:
+    if solver in (1, 8):
         xo, ho, no, io = orc.pcg(rp, col, val, b, **kw)
     elif solver == 2:
         xo, ho, no, io = orc.bicgstab(rp, col, val, b, **kw)
-    elif solver == 3:
+    elif solver in (3, 9):
         xo, ho, no, io = orc.gmres(rp, col, val, b, m=p[0], gsflag=p[1], **kw)
     elif solver == 4:
         xo, ho, no, io = orc.cagmres(rp, col, val, b, s=p[0], t=p[1], basis=p[2], qrflag=p[3], **kw)
@@ -85,6 +85,24 @@ def oracle_case(solver, g, ipre, ilu, nblk, itrmax, rtol, p):
 
 @pytest.mark.parametrize("case", CASES)
 def test_fortran_module_matches_oracle(case):
+    solver, g, ipre, ilu, nblk, itrmax, rtol, p = case
+    niter, iret, hist, x1, xn = run_fortran(solver, g, ipre, ilu, nblk, itrmax, rtol, *p)
+    o1, on, ho, no, io = oracle_case(*case)
+    assert (niter, iret) == (no, io)
+    assert np.array_equal(hist, ho)
+    assert x1 == o1 and xn == on
+
+
+# the DIA module procedures (diagonals from parcel_gen_crs_to_dia): DIA = CRS bit for bit
+DIA_CASES = [
+    (8, 12, 2, 0, 4, 500, 1e-9, (4, 1)),
+    (8, 12, 1, 1, 1, 500, 1e-9, (4, 1)),
+    (9, 12, 2, 1, 3, 300, 1e-9, (8, 2)),
+]
+
+
+@pytest.mark.parametrize("case", DIA_CASES)
+def test_fortran_module_dia_matches_oracle(case):
     solver, g, ipre, ilu, nblk, itrmax, rtol, p = case
     niter, iret, hist, x1, xn = run_fortran(solver, g, ipre, ilu, nblk, itrmax, rtol, *p)
     o1, on, ho, no, io = oracle_case(*case)
@@ -160,3 +178,53 @@ def test_by_reference_dcagmres_and_ddcg_entry_points():
                                   rtol=1e-20)
     assert (niter.value, iret.value) == (no, io)
     assert np.array_equal(hist[:no], ho) and np.array_equal(xh, oh) and np.array_equal(xl, ol)
+
+
+def test_by_reference_dia_entry_points():
+    # the DIA forms (c2fortran_kernel.f90:1228-2091) against the by-value DIA
+    # entry points and the oracle
+    lib = ph.load()
+    g = 10
+    rp, col, val = ph.poisson3d_parcel(g, g, g)
+    b, _ = ph.poisson3d_rhs_parcel(g, g, g)
+    n = g ** 3
+    off, dia = ph.crs_to_dia(rp, col, val)
+    nnd = int(off.size)
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    D = lambda v: ctypes.byref(ctypes.c_double(v))
+    lib.parcel_init_c_(I(0))
+    common = lambda: (I(2), I(0), I(0), I(4))
+    tail = lambda: (I(0), I(4), I(1), I(1), I(2000))
+    x, hist = np.zeros(n), np.zeros(500)
+    niter, iret = ctypes.c_int(400), ctypes.c_int(-7)
+    lib.parcel_dcg_dia_c_(I(0), P(x), P(b), I(n), I(n), I(1), P(dia), P(off), I(nnd), *common(),
+                          ctypes.byref(niter), D(1e-9), P(hist), *tail(), ctypes.byref(iret))
+    xo, ho, no, io = orc.pcg(rp, col, val, b, ipreflag=2, ilu_method=0, nblocks=4, itrmax=400, rtol=1e-9)
+    assert (niter.value, iret.value) == (no, io)
+    assert np.array_equal(hist[:no], ho) and np.array_equal(x, xo)
+    for name, extra, solver, kw in (("parcel_dbicgstab_dia_c_", (), "bicgstab", {}),
+                                    ("parcel_dgmres_dia_c_", (6, 1), "gmres", {}),
+                                    ("parcel_dcagmres_dia_c_", (4, 2, 1, 3), "cagmres", {}),
+                                    ("parcel_dcbcg_dia_c_", (4, 1, 30, 4, 10, 1, 3), "cbcg", {})):
+        x, hist = np.zeros(n), np.zeros(500)
+        niter, iret = ctypes.c_int(400), ctypes.c_int(-7)
+        getattr(lib, name)(I(0), P(x), P(b), I(n), I(n), I(1), P(dia), P(off), I(nnd), *common(),
+                           ctypes.byref(niter), D(1e-9), P(hist), *tail(), *[I(v) for v in extra],
+                           ctypes.byref(iret))
+        x2 = np.zeros(n)
+        h2, n2, i2 = ph.solve_dia(solver, x2, b, off, dia, ipreflag=2, ILU_method=0, itrmax=400, rtolmax=1e-9,
+                                  precon_thblock=4, extra=extra)
+        assert (niter.value, iret.value) == (n2, i2), name
+        assert np.array_equal(hist[:n2], h2) and np.array_equal(x, x2), name
+    # CA-Arnoldi: one power step, every component of the eigenvector (DIA)
+    ev = [np.zeros(40) for _ in range(2)] + [np.zeros(40 * n), np.zeros(40)]
+    mx, mn, iret = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(-7)
+    x0 = np.random.default_rng(1).standard_normal(n)
+    lib.parcel_dcaarnoldi_dia_c_(I(0), P(x0), P(b), I(n), I(n), I(1), P(dia), P(off), I(nnd), *common(), I(0),
+                                 I(4), I(1), I(2000), I(4), I(10), I(1), I(3), P(ev[0]), P(ev[1]), P(ev[2]),
+                                 P(ev[3]), ctypes.byref(mx), ctypes.byref(mn), ctypes.byref(iret))
+    e, en, vec = orc.power_eigen(rp, col, val, x0, ipreflag=2, ilu_method=0, nblocks=4, itrmax=1)
+    assert (mx.value, mn.value, iret.value) == (1, 1, 0)
+    assert ev[0][0] == e and ev[1][0] == 0.0 and ev[3][0] == en
+    assert np.array_equal(ev[2][:n], vec)
