@@ -66,6 +66,15 @@ def ilu0_apply_bytes(n, nnz_l, nnz_u):
     return 12 * (nnz_l + nnz_u) + 8 * n + 8 * n + 8 * n + 16 * n + 8 * n
 
 
+def bj_lines_bytes(n, dilu=False, fused=True):
+    """Block-Jacobi apply in line-wavefront form (k_bj_lines): forward r 8n
+    (and the fused update's q 8n, r written 8n), two couplings 16n, s written
+    8n (D-ILU: the scale 8n); backward s read 8n, two couplings 16n, the scale
+    8n, s written 8n."""
+    fwd = 8 * n + 16 * n + 8 * n + (16 * n if fused else 0) + (8 * n if dilu else 0)
+    return fwd + 40 * n
+
+
 def k1_bytes32(n, nnz):
     """FP32 fused CG kernel: values+columns 8 B/nnz, row pointers 4(n+1),
     x read/write 8n, p_old 4n, s 4n, p_new 4n, q 4n."""
@@ -168,6 +177,12 @@ def run_c4(args):
             kernels["bj_ic0_lscan_f32"] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"],
                                            "bytes": pb, "GBs": pb / (pre_ms * 1e-3) / 1e9,
                                            "strips": 256, "lines_per_strip": g}
+        elif base == "bj" and st["bj_lines"]:
+            # one plane per block: the apply runs as line wavefronts (k_bj_lines)
+            pb = bj_lines_bytes(n)
+            kernels["bj_ilu0_apply"] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"],
+                                        "bytes": pb, "GBs": pb / (pre_ms * 1e-3) / 1e9,
+                                        "form": "line wavefronts (k_bj_lines), 88 B per row"}
         elif base == "bj":
             pb = (ilu0_apply_bytes32 if mixed else ilu0_apply_bytes)(n, st["bj_nnz_l"], st["bj_nnz_u"])
             kernels["bj_ilu0_apply" + sfx] = {"mean_ms": pre_ms, "calls_per_solve": st["precon_calls"],

@@ -121,6 +121,8 @@ typedef struct parcel_hip_stats {
   int spmv_stencil;        /* 1: the CG kernel reads the matrix as a symmetric 5/7-point stencil (4 arrays) */
   long sell_entries;       /* stored SELL-64 entries (padding and the long rows' overflow included) */
   long sell_overflow;      /* entries of long rows kept after the slices (0: none) */
+  int bj_lines;            /* 1: the FP64 block-Jacobi apply runs as line wavefronts (blocks of whole
+                              grid lines with 5-point couplings inside the block) */
 } parcel_hip_stats;
 
 /* --- library life cycle: PARCEL_Init / PARCEL_Free (src/c_krylov_wrapper.f90:6-22) */

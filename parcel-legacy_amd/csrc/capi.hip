@@ -237,6 +237,7 @@ int create_impl(parcel_hip_plan** out, int n, int gn, int nnz, int istart, const
     P.stats.spmv_stencil = P.st7 ? 1 : 0;
     P.stats.sell_entries = P.A.entries;
     P.stats.sell_overflow = P.A.ovf_entries;
+    P.stats.bj_lines = P.M.line.on ? 1 : 0;
     P.hA = HostCSR();  // setup copy no longer needed
     plan_workspace(P, 1);
     PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));

@@ -168,6 +168,24 @@ struct LscanBJ {
   DevBuf<double> sink;
 };
 
+// Line-wavefront form of the FP64 block-Jacobi apply: every block is whole
+// grid lines of nx cells whose factors couple a row only to the neighbouring
+// cell of its line and to the same cell of the neighbouring line.  The
+// couplings (two per row, in the sweep's stored order) and the scales are
+// kept per block in anti-diagonal order of the sweep (precon.hip,
+// line_doff), so the rows one step of the wavefront visits are contiguous.
+struct LineBJ {
+  bool on = false;
+  int nx = 0;
+  int waves = 1;          // wavefronts per workgroup: lines per block <= 64 * waves
+  int fswap = 0;          // 1: a forward row's first stored coupling is the cell neighbour
+  int bswap = 0;          // the same for the backward sweep
+  DevBuf<int> bstart;     // nblocks + 1 block starts
+  DevBuf<double> fc, bc;  // 2 couplings per row
+  DevBuf<double> fd, bd;  // forward scale (D-ILU), backward scale
+  DevBuf<double> sink;    // 64 doubles per block for the stores of idle lanes
+};
+
 struct Precon {
   int kind = PRECON_NONE;
   int mode = BJ_ILU0;
@@ -177,6 +195,7 @@ struct Precon {
   DevBuf<float> dinv32;
   Sweep fwd, bwd;       // block Jacobi
   LscanBJ lscan;        // FP32 line-scan form of the block factors (mixed solver)
+  LineBJ line;          // line-wavefront form of the FP64 apply
   bool has32 = false;   // FP32 copies of the factors exist
   // double-double factors (precon_setup_dd): Dinv lo parts, apply precision
   DevBuf<double> dinv_lo;

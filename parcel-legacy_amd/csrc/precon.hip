@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 
 #include "dd.hpp"
@@ -765,6 +766,372 @@ SweepArgs<float> sweep_args<float>(const Sweep& S) {
   return a;
 }
 
+// --- line-wavefront form of the FP64 apply ------------------------------------
+//
+// A block of nl grid lines of nx cells whose factor rows couple only to the
+// cell neighbour (same line, offset 1) and the line neighbour (same cell,
+// offset nx) is swept as a wavefront: thread Lp owns sweep line Lp and
+// computes cell ip = t - Lp - lag(w) at step t (forward: line Lp, cell ip;
+// backward: line nl-1-Lp, cell nx-1-ip).  Both dependencies were computed at
+// step t-1: the cell neighbour by the thread itself, the line neighbour by
+// the lane below (a DPP wave shift).  Lane 0 of wavefront w takes its line
+// neighbour from the last line of wavefront w-1 through LDS; wavefront w runs
+// lag(w) = w kLG steps behind, so that value is kLG + 1 steps old and one LDS
+// barrier per kLG steps publishes it.  Every row sums its couplings in the
+// stored order with the same fma sequence as the level-scheduled kernels, so
+// the values are bit-identical to theirs.
+//
+// Memory: the rows of one step are one anti-diagonal of the block, nx - 1
+// apart in the natural order of the vectors.  The factor arrays are stored in
+// anti-diagonal order (one coalesced load per step); the vectors go through
+// LDS in groups of kLG steps: each line of a group is kLG consecutive cells,
+// which the wavefront loads and stores as 64-byte segments (lane 8j + c: line
+// 8k + j, cell c, k = 0..7) and the owning lane reads and writes in its slot
+// row.  The fused update r -= alpha q is elementwise and is applied (and
+// stored) while a group's inputs are staged.
+constexpr int kLG = 8;
+static_assert(kLG == 8, "the staging moves 8 lines x 8 cells per instruction");
+constexpr int kLS = kLG + 1;  // doubles per line of a wavefront's slots (padding against bank conflicts)
+
+// Entries on the anti-diagonals 0..d-1 of an nl x nx block (anti-diagonal
+// d' holds min(d', nl-1) - max(0, d'-nx+1) + 1 cells); the cell of sweep line
+// Lp on anti-diagonal d sits at line_doff(d) + Lp - max(0, d - nx + 1).
+__host__ __device__ inline long line_doff(int d, int nl, int nx) {
+  const long a = d < nl ? d : nl;
+  const long over = d > nx ? d - nx : 0;
+  return a * (a + 1) / 2 + (long)(d > nl ? d - nl : 0) * nl - over * (over + 1) / 2;
+}
+
+struct LineArgs {
+  const int* bstart;
+  const double* fc;
+  const double* fd;
+  const double* bc;
+  const double* bd;
+  double* sink;  // 64 doubles per block: the stores of lanes off the block's cells
+  int nx;
+  int fswap, bswap;
+};
+
+// v of the lane below (DPP wave_shr:1); lane 0 receives `first`.
+__device__ __forceinline__ double from_lane_below(double first, double v) {
+  const long long o = __double_as_longlong(first), x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Orders the LDS accesses of the lanes of one wavefront (no s_barrier: the
+// slots are private to the wavefront).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Factor operands of one step, loaded kLG steps ahead.
+struct LineOps {
+  double c0, c1;  // the row's couplings in stored order
+  double d;       // scale
+};
+
+template <bool FWD, int MODE, bool FUSED, int W>
+__device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, const double* rin, double* s,
+                                           const double* q, double* rout, double alpha, double* hand,
+                                           double* slots) {
+  constexpr bool kScale = !(FWD && MODE == BJ_ILU0);
+  const int nx = A.nx;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wb = (nl + 63) >> 6;  // wavefronts with lines; the others idle through the barriers
+  const int nsteps = nx + nl - 1 + (wb - 1) * kLG;
+  const double* cc = FWD ? A.fc : A.bc;
+  const double* dv = FWD ? A.fd : A.bd;
+  double* sink = A.sink + (long)blockIdx.x * 64 + lane;
+  const bool swap = FWD ? A.fswap : A.bswap;
+  const int Lp = tid;
+  const bool lact = Lp < nl;
+  const int lag = w * kLG;
+  double* ws = slots + w * 64 * kLS;  // this wavefront's slots: [line][step of the group]
+  double* my = ws + lane * kLS;
+  // staging lanes: lane 8j + c moves cell c of the group for local line 8k + j
+  const int sj = lane >> 3, sc = lane & 7;
+  auto stage_row = [&](int t1, int k, bool& v) -> long {
+    const int Lq = 64 * w + 8 * k + sj;
+    const int ip = t1 + sc - Lq - lag;
+    v = Lq < nl && (unsigned)ip < (unsigned)nx;
+    return v ? (long)lo + (long)(FWD ? Lq : nl - 1 - Lq) * nx + (FWD ? ip : nx - 1 - ip) : (long)lo;
+  };
+  double va[8], vb[8];
+  auto stage_load = [&](int t1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      bool v;
+      const long row = stage_row(t1, k, v);
+      va[k] = FWD ? rin[row] : s[row];
+      if (FUSED) vb[k] = q[row];
+    }
+  };
+  auto stage_fill = [&](int t1) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      bool v;
+      const long row = stage_row(t1, k, v);
+      double z0 = va[k];
+      if (FUSED) {
+        z0 = fma(-alpha, vb[k], va[k]);
+        *(v ? rout + row : sink) = z0;
+      }
+      ws[(8 * k + sj) * kLS + sc] = z0;
+    }
+  };
+  auto stage_store = [&](int t0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      bool v;
+      const long row = stage_row(t0, k, v);
+      *(v ? s + row : sink) = ws[(8 * k + sj) * kLS + sc];
+    }
+  };
+  // factor loads: anti-diagonal dl = t - lag of the next load and the entry
+  // offset of sweep line 0 on it (wave-uniform, advanced one step per load)
+  int dl = -lag;
+  long offl = 0;
+  auto load = [&](int t, LineOps& o) {
+    const int ip = t - Lp - lag;
+    const bool on = lact && (unsigned)ip < (unsigned)nx;
+    const long e = on ? (long)lo + offl + Lp : (long)lo;
+    const double2 cv = reinterpret_cast<const double2*>(cc)[e];
+    o.c0 = cv.x;
+    o.c1 = cv.y;
+    if (kScale) o.d = dv[e];
+    if (dl >= 0) offl += min(dl, nl - 1) - max(0, dl - nx + 1) + 1 - (dl >= nx - 1 ? 1 : 0);
+    ++dl;
+  };
+  double yprev = 0.0;  // the thread's value of the previous step (its cell neighbour)
+  auto step = [&](int t, int u, const LineOps& o) {
+    const int ip = t - Lp - lag;
+    const bool on = lact && (unsigned)ip < (unsigned)nx;
+    double hv = 0.0;
+    if (W > 1) hv = hand[(w > 0 ? w - 1 : 0) * nx + (on ? ip : 0)];
+    const double yd = from_lane_below(hv, yprev);  // line neighbour
+    const bool hd = Lp > 0, hc = ip > 0;
+    const double v1 = swap ? yprev : yd, v2 = swap ? yd : yprev;
+    const bool h1 = swap ? hc : hd, h2 = swap ? hd : hc;
+    const double a = my[u];  // forward: r after the fused update; backward: the forward value
+    double y;
+    if (FWD) {
+      double z = a;
+      z = h1 ? fma(-v1, o.c0, z) : z;
+      z = h2 ? fma(-v2, o.c1, z) : z;
+      y = MODE == BJ_ILU0 ? z : o.d * z;
+    } else if (MODE == BJ_ILU0) {
+      double z = a;
+      z = h1 ? fma(-v1, o.c0, z) : z;
+      z = h2 ? fma(-v2, o.c1, z) : z;
+      y = z / o.d;
+    } else {
+      double z = 0.0;
+      z = h1 ? fma(v1, o.c0, z) : z;
+      z = h2 ? fma(v2, o.c1, z) : z;
+      y = fma(-o.d, z, a);
+    }
+    my[u] = y;
+    if (W > 1 && lane == 63 && w < wb - 1 && on) hand[w * nx + ip] = y;
+    yprev = on ? y : yprev;
+  };
+  LineOps R[kLG];
+  stage_load(0);
+  stage_fill(0);
+#pragma unroll
+  for (int u = 0; u < kLG; ++u) load(u, R[u]);
+  for (int t0 = 0; t0 < nsteps; t0 += kLG) {
+    stage_load(t0 + kLG);  // the next group's vectors, in flight during this group
+    wave_lds_sync();
+#pragma unroll
+    for (int u = 0; u < kLG; ++u) {
+      step(t0 + u, u, R[u]);
+      load(t0 + u + kLG, R[u]);
+    }
+    wave_lds_sync();
+    stage_store(t0);
+    wave_lds_sync();
+    stage_fill(t0 + kLG);
+    if (W > 1) lds_barrier();
+  }
+}
+
+// One workgroup of 64 W threads per block: forward then backward sweep.  The
+// fused residual update computes alpha as k_bj_apply does, with gamma summed
+// in the same order (partials j, j + kNT, ... per virtual thread j < kNT,
+// wavefront sums, then the kWaves sums in order).  Dynamic LDS: the slots
+// (64 W kLS doubles), then the hand-over lines ((W - 1) nx doubles).
+template <int MODE, int W>
+__global__ void __launch_bounds__(64 * W) k_bj_lines(LineArgs A, const double* rin, double* s, CGFuse fuse,
+                                                     const int* skip) {
+  if (skip && *skip) return;
+  extern __shared__ double lds_lines[];
+  __shared__ double sh[kWaves];
+  double alpha = 0.0;
+  const double* q = nullptr;
+  double* rout = nullptr;
+  if (fuse.st) {
+    const int d = fuse.st->done_at;
+    if (d != 0 && d <= fuse.it) return;
+    CGSlot* sl = &fuse.st->slot[fuse.it & 1];
+    for (int vw = threadIdx.x >> 6; vw < kWaves; vw += W) {
+      double a = 0.0;
+      for (int j = vw * 64 + (threadIdx.x & 63); j < fuse.nb_prod; j += kNT) a = a + fuse.part_gamma[j];
+      a = wave_sum(a);
+      if ((threadIdx.x & 63) == 0) sh[vw] = a;
+    }
+    __syncthreads();
+    double gamma = sh[0];
+    for (int k = 1; k < kWaves; ++k) gamma = gamma + sh[k];
+    alpha = sl->rho0 / gamma;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sl->alpha = alpha;
+    q = fuse.q;
+    rout = fuse.r;
+    rin = fuse.r;
+  }
+  double* slots = lds_lines;
+  double* hand = lds_lines + 64 * W * kLS;
+  const int lo = A.bstart[blockIdx.x];
+  const int nl = (A.bstart[blockIdx.x + 1] - lo) / A.nx;
+  if (q) line_sweep<true, MODE, true, W>(A, lo, nl, rin, s, q, rout, alpha, hand, slots);
+  else line_sweep<true, MODE, false, W>(A, lo, nl, rin, s, nullptr, nullptr, 0.0, hand, slots);
+  __syncthreads();  // forward values in memory before other threads' backward rows read them
+  line_sweep<false, MODE, false, W>(A, lo, nl, nullptr, s, nullptr, nullptr, 0.0, hand, slots);
+}
+
+constexpr size_t kLineLdsMax = 64 * 1024 - 64;  // dynamic LDS of k_bj_lines (slots + hand-over)
+
+// Set up the line-wavefront form when the block factors have its pattern
+// (FP64 applies only; the level-scheduled sweeps stay for the other paths).
+// PARCEL_HIP_BJ_LEVELS=1 in the environment keeps the level-scheduled apply.
+bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<BlockSweep>& B,
+                const std::vector<int>& blo, const std::vector<int>& bhi) {
+  Precon& M = P.M;
+  LineBJ& G = M.line;
+  G = LineBJ();
+  if (const char* e = std::getenv("PARCEL_HIP_BJ_LEVELS"))
+    if (e[0] && e[0] != '0') return false;
+  const int n = P.hA.n, nblk = (int)F.size();
+  // nx: the one forward dependency offset above 1
+  long nx = 0;
+  for (int b = 0; b < nblk; ++b)
+    for (size_t i = 0; i + 1 < F[b].ptr.size(); ++i)
+      for (int q = F[b].ptr[i]; q < F[b].ptr[i + 1]; ++q) {
+        const long off = (long)i - F[b].dep[q];
+        if (off <= 1 || off == nx) continue;
+        if (nx) return false;
+        nx = off;
+      }
+  if (nx < 2) return false;
+  int maxnl = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const long len = (long)bhi[b] - blo[b];
+    if (len <= 0 || blo[b] % nx || len % nx) return false;
+    maxnl = std::max(maxnl, (int)(len / nx));
+  }
+  if (maxnl > 512) return false;
+  int W = 1;
+  while (64 * W < maxnl) W *= 2;
+  if (sizeof(double) * ((size_t)64 * W * kLS + (size_t)(W - 1) * nx) > kLineLdsMax) return false;
+  // stored order of the two couplings, from the first row that has both
+  auto order = [&](const std::vector<BlockSweep>& S, bool fwd) -> int {
+    for (int b = 0; b < nblk; ++b) {
+      const int nl = (bhi[b] - blo[b]) / (int)nx;
+      if (nl < 2) continue;
+      const int i = fwd ? (int)nx + 1 : (nl - 2) * (int)nx + (int)nx - 2;  // both neighbours exist
+      if (S[b].ptr[i + 1] - S[b].ptr[i] != 2) return -1;
+      const int first = S[b].dep[S[b].ptr[i]];
+      return (first == (fwd ? i - 1 : i + 1)) ? 1 : 0;
+    }
+    return 0;
+  };
+  const int fswap = order(F, true), bswap = order(B, false);
+  if (fswap < 0 || bswap < 0) return false;
+  std::vector<double> fc(2 * (size_t)n, 0.0), bc(2 * (size_t)n, 0.0), fd(n, 0.0), bd(n, 0.0);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+  for (int b = 0; b < nblk; ++b) {
+    const int lo = blo[b], nl = (bhi[b] - lo) / (int)nx, X = (int)nx;
+    for (int L = 0; L < nl && !bad; ++L)
+      for (int x = 0; x < X; ++x) {
+        const int i = L * X + x;
+        for (int dir = 0; dir < 2; ++dir) {
+          const BlockSweep& S = dir == 0 ? F[b] : B[b];
+          const bool fwd = dir == 0;
+          const int cn = fwd ? i - 1 : i + 1, ln = fwd ? i - X : i + X;
+          const bool has_c = fwd ? x > 0 : x < X - 1, has_l = fwd ? L > 0 : L < nl - 1;
+          if (S.ptr[i + 1] - S.ptr[i] != (int)has_c + (int)has_l) { bad = 1; break; }
+          double vc = 0.0, vl = 0.0;
+          int pc = -1, pl = -1;
+          for (int q = S.ptr[i]; q < S.ptr[i + 1]; ++q) {
+            const int dp = S.dep[q];
+            if (has_c && dp == cn) { vc = S.val[q]; pc = q; }
+            else if (has_l && dp == ln) { vl = S.val[q]; pl = q; }
+            else { bad = 1; }
+          }
+          const int sw = fwd ? fswap : bswap;
+          if (bad || (pc >= 0 && pl >= 0 && (pc < pl) != (sw == 1))) { bad = 1; break; }
+          // sweep coordinates and the anti-diagonal entry
+          const int ip = fwd ? x : X - 1 - x, Lp = fwd ? L : nl - 1 - L, dg = ip + Lp;
+          const long e = (long)lo + line_doff(dg, nl, X) + Lp - std::max(0, dg - X + 1);
+          std::vector<double>& cv = fwd ? fc : bc;
+          cv[2 * (size_t)e] = sw ? vc : vl;
+          cv[2 * (size_t)e + 1] = sw ? vl : vc;
+          (fwd ? fd : bd)[(size_t)e] = S.diag[i];
+        }
+        if (bad) break;
+      }
+  }
+  if (bad) return false;
+  std::vector<int> bs(nblk + 1);
+  for (int b = 0; b < nblk; ++b) bs[b] = blo[b];
+  bs[nblk] = bhi[nblk - 1];
+  for (int b = 0; b + 1 < nblk; ++b)
+    if (bhi[b] != blo[b + 1]) return false;
+  G.bstart.upload(bs, P.stream);
+  G.fc.upload(fc, P.stream);
+  G.bc.upload(bc, P.stream);
+  G.fd.upload(fd, P.stream);
+  G.bd.upload(bd, P.stream);
+  G.sink.alloc((size_t)64 * nblk);
+  PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
+  G.nx = (int)nx;
+  G.waves = W;
+  G.fswap = fswap;
+  G.bswap = bswap;
+  G.on = true;
+  return true;
+}
+
+void line_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
+  const Precon& M = P.M;
+  const LineBJ& G = M.line;
+  LineArgs A{G.bstart.p, G.fc.p, G.fd.p, G.bc.p, G.bd.p, G.sink.p, G.nx, G.fswap, G.bswap};
+  CGFuse f;
+  if (fuse) f = *fuse;
+  const size_t shm = sizeof(double) * ((size_t)64 * G.waves * kLS + (size_t)(G.waves - 1) * G.nx);
+  const dim3 grid(M.nblocks);
+#define PARCEL_LINE_K(W)                                                                                       \
+  do {                                                                                                         \
+    if (M.mode == BJ_ILU0)                                                                                     \
+      hipLaunchKernelGGL((k_bj_lines<BJ_ILU0, W>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);      \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_bj_lines<BJ_DILU, W>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);      \
+  } while (0)
+  switch (G.waves) {
+    case 1: PARCEL_LINE_K(1); break;
+    case 2: PARCEL_LINE_K(2); break;
+    case 4: PARCEL_LINE_K(4); break;
+    default: PARCEL_LINE_K(8); break;
+  }
+#undef PARCEL_LINE_K
+}
+
 // Line-scan form of the FP32 inner apply (mixed-precision solver): when every
 // block-Jacobi block is a strip of whole grid lines of one nx-cell line
 // length, the strips are the equal split [ny k / nblk, ny (k+1) / nblk) of
@@ -921,6 +1288,7 @@ void precon_setup(Plan& P) {
     pack_sweep(F, blo, M.fwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
     pack_sweep(B, blo, M.bwd, kRingBudget, apply_k(kmax), M.nth, P.stream);
     lscan_setup(P, F, B, blo, bhi);
+    if (P.prm.solver != PARCEL_SOLVER_CG_MIXED) line_setup(P, F, B, blo, bhi);
   } else if (M.kind != PRECON_NONE) {
     throw hip_error("preconditioner not supported on this path", PARCEL_HIP_ERR_UNSUPPORTED);
   }
@@ -1012,6 +1380,8 @@ void precon_apply_t(Plan& P, const T* r, T* s, const CGFuseT<T>* fuse, const int
     const LscanBJ& G = M.lscan;
     lscan_apply32(G.F.p, (const float*)r, G.w.p, (float*)s, G.nx, G.ny, G.nstrip, (long)n, nullptr, 0, G.sink.p,
                   P.stream);
+  } else if (std::is_same<T, double>::value && M.line.on) {
+    if constexpr (std::is_same<T, double>::value) line_apply(P, r, s, fuse, skip);
   } else {
     const int nb = M.nblocks;
     const int nth = M.nth;  // >= kNT: the fused gamma reduction uses kNT threads

@@ -66,6 +66,7 @@ class Stats(ctypes.Structure):
         ("spmv_stencil", ctypes.c_int),
         ("sell_entries", ctypes.c_long),
         ("sell_overflow", ctypes.c_long),
+        ("bj_lines", ctypes.c_int),
     ]
 
     def as_dict(self):
