@@ -178,8 +178,6 @@ struct LineBJ {
   bool on = false;
   int nx = 0;
   int waves = 1;          // wavefronts per workgroup: lines per block <= 64 * waves
-  int fswap = 0;          // 1: a forward row's first stored coupling is the cell neighbour
-  int bswap = 0;          // the same for the backward sweep
   DevBuf<int> bstart;     // nblocks + 1 block starts
   DevBuf<double> fc, bc;  // 2 couplings per row
   DevBuf<double> fd, bd;  // forward scale (D-ILU), backward scale

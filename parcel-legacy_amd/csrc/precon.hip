@@ -810,7 +810,6 @@ struct LineArgs {
   const double* bd;
   double* sink;  // 64 doubles per block: the stores of lanes off the block's cells
   int nx;
-  int fswap, bswap;
 };
 
 // v of the lane below (DPP wave_shr:1); lane 0 receives `first`.
@@ -835,11 +834,19 @@ struct LineOps {
   double d;       // scale
 };
 
+// The couplings' stored order is fixed by the factor: the forward rows of
+// both factors and the backward rows of D-ILU hold the line neighbour first
+// (ILU(0) keeps the CRS order of the block, D-ILU walks its upper part
+// downward); ILU(0)'s backward rows hold the cell neighbour first.
+template <bool FWD, int MODE>
+constexpr bool line_cell_first() { return !FWD && MODE == BJ_ILU0; }
+
 template <bool FWD, int MODE, bool FUSED, int W>
 __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, const double* rin, double* s,
                                            const double* q, double* rout, double alpha, double* hand,
                                            double* slots) {
   constexpr bool kScale = !(FWD && MODE == BJ_ILU0);
+  constexpr bool kCellFirst = line_cell_first<FWD, MODE>();
   const int nx = A.nx;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -848,50 +855,48 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
   const double* cc = FWD ? A.fc : A.bc;
   const double* dv = FWD ? A.fd : A.bd;
   double* sink = A.sink + (long)blockIdx.x * 64 + lane;
-  const bool swap = FWD ? A.fswap : A.bswap;
   const int Lp = tid;
   const bool lact = Lp < nl;
   const int lag = w * kLG;
   double* ws = slots + w * 64 * kLS;  // this wavefront's slots: [line][step of the group]
   double* my = ws + lane * kLS;
-  // staging lanes: lane 8j + c moves cell c of the group for local line 8k + j
+  // staging lanes: lane 8j + c moves cell c of the group for local line 8k + j;
+  // rows of the current group (stores) and of the next (loads), -1 off the block
   const int sj = lane >> 3, sc = lane & 7;
-  auto stage_row = [&](int t1, int k, bool& v) -> long {
-    const int Lq = 64 * w + 8 * k + sj;
-    const int ip = t1 + sc - Lq - lag;
-    v = Lq < nl && (unsigned)ip < (unsigned)nx;
-    return v ? (long)lo + (long)(FWD ? Lq : nl - 1 - Lq) * nx + (FWD ? ip : nx - 1 - ip) : (long)lo;
-  };
-  double va[8], vb[8];
-  auto stage_load = [&](int t1) {
+  int rcur[8], rnext[8];
+  auto stage_rows = [&](int t1) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      bool v;
-      const long row = stage_row(t1, k, v);
+      const int Lq = 64 * w + 8 * k + sj;
+      const int ip = t1 + sc - Lq - lag;
+      const bool v = Lq < nl && (unsigned)ip < (unsigned)nx;
+      rnext[k] = v ? lo + (FWD ? Lq : nl - 1 - Lq) * nx + (FWD ? ip : nx - 1 - ip) : -1;
+    }
+  };
+  double va[8], vb[8];
+  auto stage_load = [&]() {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const long row = rnext[k] >= 0 ? rnext[k] : lo;
       va[k] = FWD ? rin[row] : s[row];
       if (FUSED) vb[k] = q[row];
     }
   };
-  auto stage_fill = [&](int t1) {
+  auto stage_fill = [&]() {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      bool v;
-      const long row = stage_row(t1, k, v);
       double z0 = va[k];
       if (FUSED) {
         z0 = fma(-alpha, vb[k], va[k]);
-        *(v ? rout + row : sink) = z0;
+        *(rnext[k] >= 0 ? rout + rnext[k] : sink) = z0;
       }
       ws[(8 * k + sj) * kLS + sc] = z0;
+      rcur[k] = rnext[k];
     }
   };
-  auto stage_store = [&](int t0) {
+  auto stage_store = [&]() {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      bool v;
-      const long row = stage_row(t0, k, v);
-      *(v ? s + row : sink) = ws[(8 * k + sj) * kLS + sc];
-    }
+    for (int k = 0; k < 8; ++k) *(rcur[k] >= 0 ? s + rcur[k] : sink) = ws[(8 * k + sj) * kLS + sc];
   };
   // factor loads: anti-diagonal dl = t - lag of the next load and the entry
   // offset of sweep line 0 on it (wave-uniform, advanced one step per load)
@@ -908,57 +913,73 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
     if (dl >= 0) offl += min(dl, nl - 1) - max(0, dl - nx + 1) + 1 - (dl >= nx - 1 ? 1 : 0);
     ++dl;
   };
-  double yprev = 0.0;  // the thread's value of the previous step (its cell neighbour)
-  auto step = [&](int t, int u, const LineOps& o) {
+  // An absent coupling has coefficient +0 and neighbour value kZero, whose
+  // product term is -0 in the row's fma (fma(-(+0), +0, z) forward and in
+  // ILU(0)'s backward rows, fma(-0, +0, z) in D-ILU's), and z + (-0) == z for
+  // every z, the signed zeros included: the same bits as skipping the term.
+  // The values start as kZero (the cell before a line's first, the line below
+  // the block's first).
+  constexpr double kZero = (!FWD && MODE == BJ_DILU) ? -0.0 : 0.0;
+  double yprev = kZero;  // the thread's value of the previous step (its cell neighbour)
+  auto step = [&](int t, double a, double hv, const LineOps& o) -> double {
     const int ip = t - Lp - lag;
     const bool on = lact && (unsigned)ip < (unsigned)nx;
-    double hv = 0.0;
-    if (W > 1) hv = hand[(w > 0 ? w - 1 : 0) * nx + (on ? ip : 0)];
     const double yd = from_lane_below(hv, yprev);  // line neighbour
-    const bool hd = Lp > 0, hc = ip > 0;
-    const double v1 = swap ? yprev : yd, v2 = swap ? yd : yprev;
-    const bool h1 = swap ? hc : hd, h2 = swap ? hd : hc;
-    const double a = my[u];  // forward: r after the fused update; backward: the forward value
+    const double v1 = kCellFirst ? yprev : yd, v2 = kCellFirst ? yd : yprev;
     double y;
     if (FWD) {
-      double z = a;
-      z = h1 ? fma(-v1, o.c0, z) : z;
-      z = h2 ? fma(-v2, o.c1, z) : z;
+      double z = fma(-v1, o.c0, a);
+      z = fma(-v2, o.c1, z);
       y = MODE == BJ_ILU0 ? z : o.d * z;
     } else if (MODE == BJ_ILU0) {
-      double z = a;
-      z = h1 ? fma(-v1, o.c0, z) : z;
-      z = h2 ? fma(-v2, o.c1, z) : z;
+      double z = fma(-v1, o.c0, a);
+      z = fma(-v2, o.c1, z);
       y = z / o.d;
     } else {
-      double z = 0.0;
-      z = h1 ? fma(v1, o.c0, z) : z;
-      z = h2 ? fma(v2, o.c1, z) : z;
+      double z = fma(v1, o.c0, 0.0);
+      z = fma(v2, o.c1, z);
       y = fma(-o.d, z, a);
     }
-    my[u] = y;
     if (W > 1 && lane == 63 && w < wb - 1 && on) hand[w * nx + ip] = y;
     yprev = on ? y : yprev;
+    return y;
   };
   LineOps R[kLG];
-  stage_load(0);
-  stage_fill(0);
-#pragma unroll
-  for (int u = 0; u < kLG; ++u) load(u, R[u]);
-  for (int t0 = 0; t0 < nsteps; t0 += kLG) {
-    stage_load(t0 + kLG);  // the next group's vectors, in flight during this group
+  // one group of kLG steps: the next group's vectors are in flight during it
+  auto group = [&](int t0) {
+    stage_rows(t0 + kLG);
+    stage_load();
     wave_lds_sync();
+    // the group's inputs and hand-over values, read before its first step
+    // (the hand-over lines were written in earlier groups)
+    double av[kLG], hv[kLG];
 #pragma unroll
     for (int u = 0; u < kLG; ++u) {
-      step(t0 + u, u, R[u]);
+      av[u] = my[u];
+      const int ip = t0 + u - Lp - lag;
+      hv[u] = (W > 1 && w > 0) ? hand[(w - 1) * nx + min(max(ip, 0), nx - 1)] : kZero;
+    }
+#pragma unroll
+    for (int u = 0; u < kLG; ++u) {
+      my[u] = step(t0 + u, av[u], hv[u], R[u]);
       load(t0 + u + kLG, R[u]);
     }
     wave_lds_sync();
-    stage_store(t0);
+    stage_store();
     wave_lds_sync();
-    stage_fill(t0 + kLG);
+    stage_fill();
     if (W > 1) lds_barrier();
-  }
+  };
+  stage_rows(0);
+  stage_load();
+  stage_fill();
+#pragma unroll
+  for (int u = 0; u < kLG; ++u) load(u, R[u]);
+  // the first group runs outside the loop, so that the loop is entered with
+  // the memory operations of a whole group in flight on both of its paths
+  // (the compiler's wait counts then match the steady state)
+  group(0);
+  for (int t0 = kLG; t0 < nsteps; t0 += kLG) group(t0);
 }
 
 // One workgroup of 64 W threads per block: forward then backward sweep.  The
@@ -1051,7 +1072,8 @@ bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<Blo
     return 0;
   };
   const int fswap = order(F, true), bswap = order(B, false);
-  if (fswap < 0 || bswap < 0) return false;
+  // the kernel's fixed order (line_cell_first): anything else stays level-scheduled
+  if (fswap != 0 || bswap != (M.mode == BJ_ILU0 ? 1 : 0)) return false;
   std::vector<double> fc(2 * (size_t)n, 0.0), bc(2 * (size_t)n, 0.0), fd(n, 0.0), bd(n, 0.0);
   int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
@@ -1102,8 +1124,6 @@ bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<Blo
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
   G.nx = (int)nx;
   G.waves = W;
-  G.fswap = fswap;
-  G.bswap = bswap;
   G.on = true;
   return true;
 }
@@ -1111,7 +1131,7 @@ bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<Blo
 void line_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
   const Precon& M = P.M;
   const LineBJ& G = M.line;
-  LineArgs A{G.bstart.p, G.fc.p, G.fd.p, G.bc.p, G.bd.p, G.sink.p, G.nx, G.fswap, G.bswap};
+  LineArgs A{G.bstart.p, G.fc.p, G.fd.p, G.bc.p, G.bd.p, G.sink.p, G.nx};
   CGFuse f;
   if (fuse) f = *fuse;
   const size_t shm = sizeof(double) * ((size_t)64 * G.waves * kLS + (size_t)(G.waves - 1) * G.nx);

@@ -74,7 +74,8 @@ def test_apply_equals_level_scheduled(case, ilu):
         assert Q.stats()["bj_lines"] == 0
         sl = np.zeros(n)
         Q.precon(r, sl)
-    assert np.array_equal(s, sl) and np.array_equal(s2, s)
+    # the same bits, signed zeros included
+    assert np.array_equal(s.view(np.uint64), sl.view(np.uint64)) and np.array_equal(s2.view(np.uint64), s.view(np.uint64))
 
 
 @pytest.mark.parametrize("ilu", [0, 1, 2])
