@@ -169,17 +169,21 @@ struct LscanBJ {
 };
 
 // Line-wavefront form of the FP64 block-Jacobi apply: every block is whole
-// grid lines of nx cells whose factors couple a row only to the neighbouring
-// cell of its line and to the same cell of the neighbouring line.  The
+// grid lines of nx cells (whole planes of ny lines, or lines of one plane)
+// whose factors couple a row only to the neighbouring cell of its line, the
+// same cell of the neighbouring line and of the neighbouring plane.  The
 // couplings (two per row, in the sweep's stored order) and the scales are
 // kept per block in anti-diagonal order of the sweep (precon.hip,
 // line_doff), so the rows one step of the wavefront visits are contiguous.
 struct LineBJ {
   bool on = false;
   int nx = 0;
-  int waves = 1;          // wavefronts per workgroup: lines per block <= 64 * waves
+  int ny = 0;             // lines per plane when blocks hold several planes (0: one plane per block)
+  int planes = 1;         // most planes in a block
+  int waves = 1;          // wavefronts per workgroup: ceil(lines per plane / 64) x planes, rounded up to 2^k
   DevBuf<int> bstart;     // nblocks + 1 block starts
-  DevBuf<double> fc, bc;  // 2 couplings per row
+  DevBuf<double> fc, bc;  // line and cell couplings per row
+  DevBuf<double> fp, bp;  // plane coupling per row (several planes per block)
   DevBuf<double> fd, bd;  // forward scale (D-ILU), backward scale
   DevBuf<double> sink;    // 64 doubles per block for the stores of idle lanes
 };

@@ -768,34 +768,44 @@ SweepArgs<float> sweep_args<float>(const Sweep& S) {
 
 // --- line-wavefront form of the FP64 apply ------------------------------------
 //
-// A block of nl grid lines of nx cells whose factor rows couple only to the
-// cell neighbour (same line, offset 1) and the line neighbour (same cell,
-// offset nx) is swept as a wavefront: thread Lp owns sweep line Lp and
-// computes cell ip = t - Lp - lag(w) at step t (forward: line Lp, cell ip;
-// backward: line nl-1-Lp, cell nx-1-ip).  Both dependencies were computed at
-// step t-1: the cell neighbour by the thread itself, the line neighbour by
-// the lane below (a DPP wave shift).  Lane 0 of wavefront w takes its line
-// neighbour from the last line of wavefront w-1 through LDS; wavefront w runs
-// lag(w) = w kLG steps behind, so that value is kLG + 1 steps old and one LDS
-// barrier per kLG steps publishes it.  Every row sums its couplings in the
-// stored order with the same fma sequence as the level-scheduled kernels, so
-// the values are bit-identical to theirs.
+// A block of P grid planes of ny lines of nx cells (P = 1: a strip of lines
+// of one plane, ny its line count) whose factor rows couple only to the cell
+// neighbour (offset 1), the line neighbour (offset nx) and the plane
+// neighbour (offset nx ny) is swept as a wavefront.  In sweep coordinates
+// (forward: cell i, line j, plane k; backward: nx-1-i, ny-1-j, P-1-k)
+// wavefront w = k WP + jw (WP = ceil(ny / 64) wavefronts per plane) owns
+// lines j = 64 jw + lane of plane k, and a thread computes cell
+// i = t - j - lag(w) at step t, lag(w) = 8 jw + 9 k.  Then the cell neighbour
+// was computed at step t-1 by the thread itself, the line neighbour at t-1 by
+// the lane below (a DPP wave shift) or, for lane 0, at t-9 by lane 63 of the
+// wavefront before (through an LDS line indexed by cell), and the plane
+// neighbour at t-9 by the same lane of wavefront w - WP (through an LDS ring
+// of the last kLR steps of every line).  One LDS barrier per kLG = 8 steps
+// publishes both: a value written at step t-9 or earlier was written in an
+// earlier group.  Every row sums its couplings in the stored order with the
+// same fma sequence as the level-scheduled kernels, so the values are
+// bit-identical to theirs.
 //
-// Memory: the rows of one step are one anti-diagonal of the block, nx - 1
-// apart in the natural order of the vectors.  The factor arrays are stored in
-// anti-diagonal order (one coalesced load per step); the vectors go through
-// LDS in groups of kLG steps: each line of a group is kLG consecutive cells,
-// which the wavefront loads and stores as 64-byte segments (lane 8j + c: line
-// 8k + j, cell c, k = 0..7) and the owning lane reads and writes in its slot
-// row.  The fused update r -= alpha q is elementwise and is applied (and
-// stored) while a group's inputs are staged.
+// Memory: the rows of one step of a wavefront are one anti-diagonal of its
+// lines, nx - 1 apart in the natural order of the vectors.  The factor
+// arrays are stored per wavefront in anti-diagonal order (one coalesced load
+// per step); the vectors go through LDS in groups of kLG steps: each line of
+// a group is kLG consecutive cells, which the wavefront loads and stores as
+// 64-byte segments (lane 8j + c: line 8k + j, cell c, k = 0..7) and the
+// owning lane reads and writes in its slot row.  The fused update r -= alpha q
+// is elementwise and is applied (and stored) while a group's inputs are staged.
 constexpr int kLG = 8;
 static_assert(kLG == 8, "the staging moves 8 lines x 8 cells per instruction");
 constexpr int kLS = kLG + 1;  // doubles per line of a wavefront's slots (padding against bank conflicts)
+constexpr int kLPlag = kLG + 1;  // steps between a line and the same line of the next plane
+// Ring depth of the plane hand-over: the value of step s is read at s + 9, in
+// group floor((s + 9) / 8); it may be overwritten from the next group on,
+// at s + kLR >= 8 (floor((s + 9) / 8) + 1), which holds for every s at 17.
+constexpr int kLR = 17;
 
-// Entries on the anti-diagonals 0..d-1 of an nl x nx block (anti-diagonal
-// d' holds min(d', nl-1) - max(0, d'-nx+1) + 1 cells); the cell of sweep line
-// Lp on anti-diagonal d sits at line_doff(d) + Lp - max(0, d - nx + 1).
+// Entries on the anti-diagonals 0..d-1 of an nl x nx sub-block (anti-diagonal
+// d' holds min(d', nl-1) - max(0, d'-nx+1) + 1 cells); the cell of line l on
+// anti-diagonal d sits at line_doff(d) + l - max(0, d - nx + 1).
 __host__ __device__ inline long line_doff(int d, int nl, int nx) {
   const long a = d < nl ? d : nl;
   const long over = d > nx ? d - nx : 0;
@@ -804,12 +814,15 @@ __host__ __device__ inline long line_doff(int d, int nl, int nx) {
 
 struct LineArgs {
   const int* bstart;
-  const double* fc;
+  const double* fc;  // per row: line and cell couplings, in the sweep's stored order
+  const double* fp;  // per row: plane coupling (P > 1)
   const double* fd;
   const double* bc;
+  const double* bp;
   const double* bd;
   double* sink;  // 64 doubles per block: the stores of lanes off the block's cells
   int nx;
+  int ny;        // lines per plane (0: every block is one plane of its own line count)
 };
 
 // v of the lane below (DPP wave_shr:1); lane 0 receives `first`.
@@ -830,36 +843,54 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Factor operands of one step, loaded kLG steps ahead.
 struct LineOps {
-  double c0, c1;  // the row's couplings in stored order
+  double c0, c1;  // the row's line and cell couplings in stored order
+  double cp;      // plane coupling
   double d;       // scale
 };
 
 // The couplings' stored order is fixed by the factor: the forward rows of
-// both factors and the backward rows of D-ILU hold the line neighbour first
-// (ILU(0) keeps the CRS order of the block, D-ILU walks its upper part
-// downward); ILU(0)'s backward rows hold the cell neighbour first.
+// both factors and the backward rows of D-ILU hold the plane neighbour
+// first, then the line neighbour, then the cell neighbour (ILU(0) keeps the
+// CRS order of the block, D-ILU walks its upper part downward); ILU(0)'s
+// backward rows hold them the other way round.
 template <bool FWD, int MODE>
 constexpr bool line_cell_first() { return !FWD && MODE == BJ_ILU0; }
 
-template <bool FWD, int MODE, bool FUSED, int W>
-__device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, const double* rin, double* s,
-                                           const double* q, double* rout, double alpha, double* hand,
-                                           double* slots) {
+// Dynamic LDS of k_bj_lines, in doubles: slots, line hand-over, plane ring.
+__host__ __device__ inline size_t line_lds_doubles(int W, int nx, int planes) {
+  return (size_t)64 * W * kLS + (size_t)W * nx + (planes > 1 ? (size_t)64 * W * kLR : 0);
+}
+
+template <bool FWD, int MODE, bool FUSED, bool PL, int W>
+__device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, int np, const double* rin,
+                                           double* s, const double* q, double* rout, double alpha,
+                                           double* lds) {
   constexpr bool kScale = !(FWD && MODE == BJ_ILU0);
   constexpr bool kCellFirst = line_cell_first<FWD, MODE>();
   const int nx = A.nx;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wb = (nl + 63) >> 6;  // wavefronts with lines; the others idle through the barriers
-  const int nsteps = nx + nl - 1 + (wb - 1) * kLG;
+  const int WP = (ny + 63) >> 6;             // wavefronts per plane
+  const int kp = w / WP, jw = w - kp * WP;   // this wavefront's plane and line group
+  const bool wact = kp < np;                 // the others idle through the barriers
+  const int j = 64 * jw + lane;
+  const bool lact = wact && j < ny;
+  const int nlw = wact ? min(64, ny - 64 * jw) : 1;  // lines of this wavefront
+  const int lag = 8 * jw + kLPlag * kp;
+  const int nsteps = nx + ny - 1 + 8 * (WP - 1) + kLPlag * (np - 1);
   const double* cc = FWD ? A.fc : A.bc;
+  const double* cpl = FWD ? A.fp : A.bp;
   const double* dv = FWD ? A.fd : A.bd;
   double* sink = A.sink + (long)blockIdx.x * 64 + lane;
-  const int Lp = tid;
-  const bool lact = Lp < nl;
-  const int lag = w * kLG;
-  double* ws = slots + w * 64 * kLS;  // this wavefront's slots: [line][step of the group]
+  // sweep line (j, kp) -> first row of the line; the wavefront's sub-block of entries
+  auto line_row = [&](int jj) -> int {
+    return lo + ((FWD ? kp : np - 1 - kp) * ny + (FWD ? jj : ny - 1 - jj)) * nx;
+  };
+  const long sub = (long)lo + (long)(kp * ny + 64 * jw) * nx;
+  double* ws = lds + w * 64 * kLS;  // this wavefront's slots: [line][step of the group]
   double* my = ws + lane * kLS;
+  double* hand = lds + 64 * W * kLS;  // [wavefront][cell]: lane 63's values
+  double* ring = hand + W * nx;       // [thread][step mod kLR]: every line's values (P > 1)
   // staging lanes: lane 8j + c moves cell c of the group for local line 8k + j;
   // rows of the current group (stores) and of the next (loads), -1 off the block
   const int sj = lane >> 3, sc = lane & 7;
@@ -867,10 +898,10 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
   auto stage_rows = [&](int t1) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int Lq = 64 * w + 8 * k + sj;
-      const int ip = t1 + sc - Lq - lag;
-      const bool v = Lq < nl && (unsigned)ip < (unsigned)nx;
-      rnext[k] = v ? lo + (FWD ? Lq : nl - 1 - Lq) * nx + (FWD ? ip : nx - 1 - ip) : -1;
+      const int jj = 64 * jw + 8 * k + sj;
+      const int ip = t1 + sc - jj - lag;
+      const bool v = wact && jj < ny && (unsigned)ip < (unsigned)nx;
+      rnext[k] = v ? line_row(jj) + (FWD ? ip : nx - 1 - ip) : -1;
     }
   };
   double va[8], vb[8];
@@ -898,19 +929,21 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
 #pragma unroll
     for (int k = 0; k < 8; ++k) *(rcur[k] >= 0 ? s + rcur[k] : sink) = ws[(8 * k + sj) * kLS + sc];
   };
-  // factor loads: anti-diagonal dl = t - lag of the next load and the entry
-  // offset of sweep line 0 on it (wave-uniform, advanced one step per load)
-  int dl = -lag;
+  // factor loads: anti-diagonal dl = t - lag - 64 jw of the wavefront's own
+  // lines for the next load and the entry offset of its line 0 on it
+  // (wave-uniform, advanced one step per load)
+  int dl = -lag - 64 * jw;
   long offl = 0;
   auto load = [&](int t, LineOps& o) {
-    const int ip = t - Lp - lag;
+    const int ip = t - j - lag;
     const bool on = lact && (unsigned)ip < (unsigned)nx;
-    const long e = on ? (long)lo + offl + Lp : (long)lo;
+    const long e = on ? sub + offl + lane : (long)lo;
     const double2 cv = reinterpret_cast<const double2*>(cc)[e];
     o.c0 = cv.x;
     o.c1 = cv.y;
+    if (PL) o.cp = cpl[e];
     if (kScale) o.d = dv[e];
-    if (dl >= 0) offl += min(dl, nl - 1) - max(0, dl - nx + 1) + 1 - (dl >= nx - 1 ? 1 : 0);
+    if (dl >= 0) offl += min(dl, nlw - 1) - max(0, dl - nx + 1) + 1 - (dl >= nx - 1 ? 1 : 0);
     ++dl;
   };
   // An absent coupling has coefficient +0 and neighbour value kZero, whose
@@ -918,29 +951,33 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
   // ILU(0)'s backward rows, fma(-0, +0, z) in D-ILU's), and z + (-0) == z for
   // every z, the signed zeros included: the same bits as skipping the term.
   // The values start as kZero (the cell before a line's first, the line below
-  // the block's first).
+  // a plane's first, the plane below the block's first).
   constexpr double kZero = (!FWD && MODE == BJ_DILU) ? -0.0 : 0.0;
   double yprev = kZero;  // the thread's value of the previous step (its cell neighbour)
-  auto step = [&](int t, double a, double hv, const LineOps& o) -> double {
-    const int ip = t - Lp - lag;
+  auto step = [&](int t, double a, double hv, double pv, const LineOps& o) -> double {
+    const int ip = t - j - lag;
     const bool on = lact && (unsigned)ip < (unsigned)nx;
     const double yd = from_lane_below(hv, yprev);  // line neighbour
-    const double v1 = kCellFirst ? yprev : yd, v2 = kCellFirst ? yd : yprev;
     double y;
-    if (FWD) {
-      double z = fma(-v1, o.c0, a);
-      z = fma(-v2, o.c1, z);
-      y = MODE == BJ_ILU0 ? z : o.d * z;
-    } else if (MODE == BJ_ILU0) {
-      double z = fma(-v1, o.c0, a);
-      z = fma(-v2, o.c1, z);
+    if (!FWD && MODE == BJ_DILU) {
+      // plane, line, cell; y = a - d * sum
+      double z = PL ? fma(pv, o.cp, 0.0) : fma(yd, o.c0, 0.0);
+      z = PL ? fma(yd, o.c0, z) : z;
+      z = fma(yprev, o.c1, z);
+      y = fma(-o.d, z, a);
+    } else if (kCellFirst) {
+      double z = fma(-yprev, o.c0, a);
+      z = fma(-yd, o.c1, z);
+      if (PL) z = fma(-pv, o.cp, z);
       y = z / o.d;
     } else {
-      double z = fma(v1, o.c0, 0.0);
-      z = fma(v2, o.c1, z);
-      y = fma(-o.d, z, a);
+      double z = PL ? fma(-pv, o.cp, a) : a;
+      z = fma(-yd, o.c0, z);
+      z = fma(-yprev, o.c1, z);
+      y = MODE == BJ_ILU0 ? z : o.d * z;
     }
-    if (W > 1 && lane == 63 && w < wb - 1 && on) hand[w * nx + ip] = y;
+    if (W > 1 && lane == 63 && jw < WP - 1 && on) hand[w * nx + ip] = y;
+    if (PL) ring[tid * kLR + t % kLR] = y;
     yprev = on ? y : yprev;
     return y;
   };
@@ -951,17 +988,18 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
     stage_load();
     wave_lds_sync();
     // the group's inputs and hand-over values, read before its first step
-    // (the hand-over lines were written in earlier groups)
-    double av[kLG], hv[kLG];
+    // (the hand-over values were written in earlier groups)
+    double av[kLG], hv[kLG], pv[kLG];
 #pragma unroll
     for (int u = 0; u < kLG; ++u) {
       av[u] = my[u];
-      const int ip = t0 + u - Lp - lag;
-      hv[u] = (W > 1 && w > 0) ? hand[(w - 1) * nx + min(max(ip, 0), nx - 1)] : kZero;
+      const int ip = min(max(t0 + u - j - lag, 0), nx - 1);
+      hv[u] = (W > 1 && jw > 0) ? hand[(w - 1) * nx + ip] : kZero;
+      if (PL) pv[u] = (kp > 0) ? ring[(tid - 64 * WP) * kLR + (t0 + u + kLR - kLPlag) % kLR] : kZero;
     }
 #pragma unroll
     for (int u = 0; u < kLG; ++u) {
-      my[u] = step(t0 + u, av[u], hv[u], R[u]);
+      my[u] = step(t0 + u, av[u], hv[u], PL ? pv[u] : kZero, R[u]);
       load(t0 + u + kLG, R[u]);
     }
     wave_lds_sync();
@@ -985,9 +1023,8 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int nl, co
 // One workgroup of 64 W threads per block: forward then backward sweep.  The
 // fused residual update computes alpha as k_bj_apply does, with gamma summed
 // in the same order (partials j, j + kNT, ... per virtual thread j < kNT,
-// wavefront sums, then the kWaves sums in order).  Dynamic LDS: the slots
-// (64 W kLS doubles), then the hand-over lines ((W - 1) nx doubles).
-template <int MODE, int W>
+// wavefront sums, then the kWaves sums in order).
+template <int MODE, int W, bool PL>
 __global__ void __launch_bounds__(64 * W) k_bj_lines(LineArgs A, const double* rin, double* s, CGFuse fuse,
                                                      const int* skip) {
   if (skip && *skip) return;
@@ -1015,17 +1052,17 @@ __global__ void __launch_bounds__(64 * W) k_bj_lines(LineArgs A, const double* r
     rout = fuse.r;
     rin = fuse.r;
   }
-  double* slots = lds_lines;
-  double* hand = lds_lines + 64 * W * kLS;
   const int lo = A.bstart[blockIdx.x];
-  const int nl = (A.bstart[blockIdx.x + 1] - lo) / A.nx;
-  if (q) line_sweep<true, MODE, true, W>(A, lo, nl, rin, s, q, rout, alpha, hand, slots);
-  else line_sweep<true, MODE, false, W>(A, lo, nl, rin, s, nullptr, nullptr, 0.0, hand, slots);
+  const int lines = (A.bstart[blockIdx.x + 1] - lo) / A.nx;
+  const int ny = A.ny > 0 ? A.ny : lines;
+  const int np = lines / ny;
+  if (q) line_sweep<true, MODE, true, PL, W>(A, lo, ny, np, rin, s, q, rout, alpha, lds_lines);
+  else line_sweep<true, MODE, false, PL, W>(A, lo, ny, np, rin, s, nullptr, nullptr, 0.0, lds_lines);
   __syncthreads();  // forward values in memory before other threads' backward rows read them
-  line_sweep<false, MODE, false, W>(A, lo, nl, nullptr, s, nullptr, nullptr, 0.0, hand, slots);
+  line_sweep<false, MODE, false, PL, W>(A, lo, ny, np, nullptr, s, nullptr, nullptr, 0.0, lds_lines);
 }
 
-constexpr size_t kLineLdsMax = 64 * 1024 - 64;  // dynamic LDS of k_bj_lines (slots + hand-over)
+constexpr size_t kLineLdsMax = 64 * 1024 - 64;  // dynamic LDS of k_bj_lines
 
 // Set up the line-wavefront form when the block factors have its pattern
 // (FP64 applies only; the level-scheduled sweeps stay for the other paths).
@@ -1038,91 +1075,104 @@ bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<Blo
   if (const char* e = std::getenv("PARCEL_HIP_BJ_LEVELS"))
     if (e[0] && e[0] != '0') return false;
   const int n = P.hA.n, nblk = (int)F.size();
-  // nx: the one forward dependency offset above 1
-  long nx = 0;
+  // the forward dependency offsets above 1: nx, and nx ny when blocks hold several planes
+  long nx = 0, nxy = 0;
   for (int b = 0; b < nblk; ++b)
     for (size_t i = 0; i + 1 < F[b].ptr.size(); ++i)
       for (int q = F[b].ptr[i]; q < F[b].ptr[i + 1]; ++q) {
         const long off = (long)i - F[b].dep[q];
-        if (off <= 1 || off == nx) continue;
-        if (nx) return false;
-        nx = off;
+        if (off <= 1 || off == nx || off == nxy) continue;
+        if (!nx) nx = off;
+        else if (!nxy) nxy = off;
+        else return false;
       }
-  if (nx < 2) return false;
-  int maxnl = 0;
+  if (nx && nxy && nxy < nx) std::swap(nx, nxy);
+  if (nx < 2 || (nxy && nxy % nx)) return false;
+  const int X = (int)nx;
+  const int NY = nxy ? (int)(nxy / nx) : 0;  // lines per plane (0: one plane per block)
+  for (int b = 0; b + 1 < nblk; ++b)
+    if (bhi[b] != blo[b + 1]) return false;
+  int W = 1, np_max = 1;
   for (int b = 0; b < nblk; ++b) {
     const long len = (long)bhi[b] - blo[b];
-    if (len <= 0 || blo[b] % nx || len % nx) return false;
-    maxnl = std::max(maxnl, (int)(len / nx));
+    const long unit = nxy ? nxy : nx;
+    if (len <= 0 || blo[b] % unit || len % unit) return false;
+    const int lines = (int)(len / nx), ny = NY ? NY : lines, np = lines / ny;
+    const int waves = ((ny + 63) / 64) * np;
+    if (waves > 8) return false;
+    while (W < waves) W *= 2;
+    np_max = std::max(np_max, np);
   }
-  if (maxnl > 512) return false;
-  int W = 1;
-  while (64 * W < maxnl) W *= 2;
-  if (sizeof(double) * ((size_t)64 * W * kLS + (size_t)(W - 1) * nx) > kLineLdsMax) return false;
-  // stored order of the two couplings, from the first row that has both
-  auto order = [&](const std::vector<BlockSweep>& S, bool fwd) -> int {
-    for (int b = 0; b < nblk; ++b) {
-      const int nl = (bhi[b] - blo[b]) / (int)nx;
-      if (nl < 2) continue;
-      const int i = fwd ? (int)nx + 1 : (nl - 2) * (int)nx + (int)nx - 2;  // both neighbours exist
-      if (S[b].ptr[i + 1] - S[b].ptr[i] != 2) return -1;
-      const int first = S[b].dep[S[b].ptr[i]];
-      return (first == (fwd ? i - 1 : i + 1)) ? 1 : 0;
-    }
-    return 0;
-  };
-  const int fswap = order(F, true), bswap = order(B, false);
-  // the kernel's fixed order (line_cell_first): anything else stays level-scheduled
-  if (fswap != 0 || bswap != (M.mode == BJ_ILU0 ? 1 : 0)) return false;
+  const bool PL = np_max > 1;
+  if (sizeof(double) * line_lds_doubles(W, X, np_max) > kLineLdsMax || (PL && W > 4)) return false;
+  const bool ilu0 = M.mode == BJ_ILU0;
   std::vector<double> fc(2 * (size_t)n, 0.0), bc(2 * (size_t)n, 0.0), fd(n, 0.0), bd(n, 0.0);
+  std::vector<double> fp(PL ? n : 0, 0.0), bp(PL ? n : 0, 0.0);
   int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
   for (int b = 0; b < nblk; ++b) {
-    const int lo = blo[b], nl = (bhi[b] - lo) / (int)nx, X = (int)nx;
-    for (int L = 0; L < nl && !bad; ++L)
-      for (int x = 0; x < X; ++x) {
-        const int i = L * X + x;
-        for (int dir = 0; dir < 2; ++dir) {
-          const BlockSweep& S = dir == 0 ? F[b] : B[b];
-          const bool fwd = dir == 0;
-          const int cn = fwd ? i - 1 : i + 1, ln = fwd ? i - X : i + X;
-          const bool has_c = fwd ? x > 0 : x < X - 1, has_l = fwd ? L > 0 : L < nl - 1;
-          if (S.ptr[i + 1] - S.ptr[i] != (int)has_c + (int)has_l) { bad = 1; break; }
-          double vc = 0.0, vl = 0.0;
-          int pc = -1, pl = -1;
-          for (int q = S.ptr[i]; q < S.ptr[i + 1]; ++q) {
-            const int dp = S.dep[q];
-            if (has_c && dp == cn) { vc = S.val[q]; pc = q; }
-            else if (has_l && dp == ln) { vl = S.val[q]; pl = q; }
-            else { bad = 1; }
+    const int lo = blo[b], lines = (bhi[b] - lo) / X, ny = NY ? NY : lines, np = lines / ny;
+    for (int k = 0; k < np && !bad; ++k)
+      for (int jj = 0; jj < ny && !bad; ++jj)
+        for (int x = 0; x < X; ++x) {
+          const int i = (k * ny + jj) * X + x;  // local row
+          for (int dir = 0; dir < 2; ++dir) {
+            const BlockSweep& S = dir == 0 ? F[b] : B[b];
+            const bool fwd = dir == 0;
+            // the neighbours the sweep reads, in the kernel's order
+            const int cn = fwd ? i - 1 : i + 1, ln = fwd ? i - X : i + X, pn = fwd ? i - X * ny : i + X * ny;
+            const bool hc = fwd ? x > 0 : x < X - 1, hl = fwd ? jj > 0 : jj < ny - 1, hp = fwd ? k > 0 : k < np - 1;
+            const bool cell_first = !fwd && ilu0;
+            int want[3], nw = 0;
+            if (cell_first) {
+              if (hc) want[nw++] = cn;
+              if (hl) want[nw++] = ln;
+              if (hp) want[nw++] = pn;
+            } else {
+              if (hp) want[nw++] = pn;
+              if (hl) want[nw++] = ln;
+              if (hc) want[nw++] = cn;
+            }
+            if (S.ptr[i + 1] - S.ptr[i] != nw) { bad = 1; break; }
+            double vc = 0.0, vl = 0.0, vp = 0.0;
+            for (int q = S.ptr[i], m = 0; q < S.ptr[i + 1]; ++q, ++m) {
+              const int dp = S.dep[q];
+              if (dp != want[m]) { bad = 1; break; }
+              (dp == cn ? vc : dp == ln ? vl : vp) = S.val[q];
+            }
+            if (bad) break;
+            // sweep coordinates, the wavefront's sub-block and its anti-diagonal entry
+            const int ip = fwd ? x : X - 1 - x, jp = fwd ? jj : ny - 1 - jj, kq = fwd ? k : np - 1 - k;
+            const int jw = jp / 64, l = jp % 64, nlw = std::min(64, ny - 64 * jw), dg = ip + l;
+            const long e = (long)lo + (long)(kq * ny + 64 * jw) * X + line_doff(dg, nlw, X) + l -
+                           std::max(0, dg - X + 1);
+            std::vector<double>& cv = fwd ? fc : bc;
+            cv[2 * (size_t)e] = cell_first ? vc : vl;
+            cv[2 * (size_t)e + 1] = cell_first ? vl : vc;
+            if (PL) (fwd ? fp : bp)[(size_t)e] = vp;
+            (fwd ? fd : bd)[(size_t)e] = S.diag[i];
           }
-          const int sw = fwd ? fswap : bswap;
-          if (bad || (pc >= 0 && pl >= 0 && (pc < pl) != (sw == 1))) { bad = 1; break; }
-          // sweep coordinates and the anti-diagonal entry
-          const int ip = fwd ? x : X - 1 - x, Lp = fwd ? L : nl - 1 - L, dg = ip + Lp;
-          const long e = (long)lo + line_doff(dg, nl, X) + Lp - std::max(0, dg - X + 1);
-          std::vector<double>& cv = fwd ? fc : bc;
-          cv[2 * (size_t)e] = sw ? vc : vl;
-          cv[2 * (size_t)e + 1] = sw ? vl : vc;
-          (fwd ? fd : bd)[(size_t)e] = S.diag[i];
+          if (bad) break;
         }
-        if (bad) break;
-      }
   }
   if (bad) return false;
   std::vector<int> bs(nblk + 1);
   for (int b = 0; b < nblk; ++b) bs[b] = blo[b];
   bs[nblk] = bhi[nblk - 1];
-  for (int b = 0; b + 1 < nblk; ++b)
-    if (bhi[b] != blo[b + 1]) return false;
   G.bstart.upload(bs, P.stream);
   G.fc.upload(fc, P.stream);
   G.bc.upload(bc, P.stream);
   G.fd.upload(fd, P.stream);
   G.bd.upload(bd, P.stream);
+  if (PL) {
+    G.fp.upload(fp, P.stream);
+    G.bp.upload(bp, P.stream);
+  }
   G.sink.alloc((size_t)64 * nblk);
   PARCEL_HIP_CHECK(hipStreamSynchronize(P.stream));
-  G.nx = (int)nx;
+  G.nx = X;
+  G.ny = NY;
+  G.planes = np_max;
   G.waves = W;
   G.on = true;
   return true;
@@ -1131,23 +1181,29 @@ bool line_setup(Plan& P, const std::vector<BlockSweep>& F, const std::vector<Blo
 void line_apply(Plan& P, const double* r, double* s, const CGFuse* fuse, const int* skip) {
   const Precon& M = P.M;
   const LineBJ& G = M.line;
-  LineArgs A{G.bstart.p, G.fc.p, G.fd.p, G.bc.p, G.bd.p, G.sink.p, G.nx};
+  LineArgs A{G.bstart.p, G.fc.p, G.fp.p, G.fd.p, G.bc.p, G.bp.p, G.bd.p, G.sink.p, G.nx, G.ny};
   CGFuse f;
   if (fuse) f = *fuse;
-  const size_t shm = sizeof(double) * ((size_t)64 * G.waves * kLS + (size_t)(G.waves - 1) * G.nx);
+  const size_t shm = sizeof(double) * line_lds_doubles(G.waves, G.nx, G.planes);
   const dim3 grid(M.nblocks);
-#define PARCEL_LINE_K(W)                                                                                       \
-  do {                                                                                                         \
-    if (M.mode == BJ_ILU0)                                                                                     \
-      hipLaunchKernelGGL((k_bj_lines<BJ_ILU0, W>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);      \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_bj_lines<BJ_DILU, W>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);      \
+#define PARCEL_LINE_K(W, PL)                                                                                    \
+  do {                                                                                                          \
+    if (M.mode == BJ_ILU0)                                                                                      \
+      hipLaunchKernelGGL((k_bj_lines<BJ_ILU0, W, PL>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);   \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_bj_lines<BJ_DILU, W, PL>), grid, dim3(64 * W), shm, P.stream, A, r, s, f, skip);   \
   } while (0)
-  switch (G.waves) {
-    case 1: PARCEL_LINE_K(1); break;
-    case 2: PARCEL_LINE_K(2); break;
-    case 4: PARCEL_LINE_K(4); break;
-    default: PARCEL_LINE_K(8); break;
+  if (G.planes > 1) {
+    // eight wavefronts with the plane ring exceed the LDS budget (line_setup)
+    if (G.waves == 2) PARCEL_LINE_K(2, true);
+    else PARCEL_LINE_K(4, true);
+  } else {
+    switch (G.waves) {
+      case 1: PARCEL_LINE_K(1, false); break;
+      case 2: PARCEL_LINE_K(2, false); break;
+      case 4: PARCEL_LINE_K(4, false); break;
+      default: PARCEL_LINE_K(8, false); break;
+    }
   }
 #undef PARCEL_LINE_K
 }

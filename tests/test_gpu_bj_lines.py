@@ -1,17 +1,18 @@
 """GPU parity of the line-wavefront form of the FP64 block-Jacobi apply
 (precon.hip: line_setup, k_bj_lines) through the C ABI.
 
-When every block is whole grid lines and its ILU(0) / D-ILU factor rows
-couple only to the neighbouring cell of the line and the same cell of the
-neighbouring line (one plane of a 3D 7-point operator, strips of a 2D
-5-point one), the apply runs as a wavefront over the block's lines instead of
-level by level.  Each row still sums its couplings in the stored order of the
+When every block is whole grid lines (strips of one plane, or whole planes)
+and its ILU(0) / D-ILU factor rows couple only to the neighbouring cell of
+the line and the same cell of the neighbouring line and plane (a 3D 7-point
+operator, a 2D 5-point one), the apply runs as a wavefront over the block's
+lines instead of level by level.  Each row still sums its couplings in the stored order of the
 reference sweeps (solve_LU_crs src/matrix.f90:8548-8746, solve_ILU_crs
 :4987-5163), so the solves equal the oracle's sequential ones bit for bit,
 and one apply equals the level-scheduled kernel's
 (PARCEL_HIP_BJ_LEVELS=1) bit for bit.  The cases cover one to eight
 wavefronts per block (16 to 512 lines), symmetric and nonsymmetric
-couplings, the fused CG update (dcg) and the plain apply (BiCGSTAB, GMRES)."""
+couplings, blocks of one plane or strip and of two to four whole planes, the
+fused CG update (dcg) and the plain apply (BiCGSTAB, GMRES)."""
 import os
 import sys
 
@@ -34,7 +35,8 @@ def _init():
     ph.init(0)
 
 
-# (generator, blocks, wavefronts per block)
+# (generator, blocks, wavefronts per block): one plane (or strip) per block,
+# then blocks of 2 to 4 whole planes (the plane neighbour through the LDS ring)
 CASES = {
     "p3d_16_b16": (lambda: ph.poisson3d_spd(16, 16, 16), 16, 1),
     "p3d_48x100x6_b6": (lambda: ph.poisson3d_spd(48, 100, 6), 6, 2),
@@ -42,8 +44,14 @@ CASES = {
     "p2d_40x512_b1": (lambda: ph.poisson2d(40, 512, -1.0), 1, 8),
     "cd3d_20_b20": (lambda: ph.convdiff3d(20, 20, 20), 20, 1),
     "cd3d_12x130x3_b3": (lambda: ph.convdiff3d(12, 130, 3), 3, 4),
+    "p3d_24_b8_planes3": (lambda: ph.poisson3d_spd(24, 24, 24), 8, 4),
+    "p3d_16x100x4_b2_planes2": (lambda: ph.poisson3d_spd(16, 100, 4), 2, 4),
+    "p3d_10x30x16_b4_planes4": (lambda: ph.poisson3d_spd(10, 30, 16), 4, 4),
+    "cd3d_20_b10_planes2": (lambda: ph.convdiff3d(20, 20, 20), 10, 2),
+    "cd3d_12x70x4_b2_planes2": (lambda: ph.convdiff3d(12, 70, 4), 2, 4),
 }
-SYM = ("p3d_16_b16", "p3d_48x100x6_b6", "p2d_100x300_b2", "p2d_40x512_b1")
+SYM = ("p3d_16_b16", "p3d_48x100x6_b6", "p2d_100x300_b2", "p2d_40x512_b1", "p3d_24_b8_planes3",
+       "p3d_16x100x4_b2_planes2", "p3d_10x30x16_b4_planes4")
 
 
 def _levels_plan(rp, col, val, **kw):
@@ -94,7 +102,8 @@ def test_pcg_bit_exact(case, ilu):
     assert np.array_equal(hist, ho) and np.array_equal(x, xo)
 
 
-@pytest.mark.parametrize("case", ["cd3d_20_b20", "cd3d_12x130x3_b3", "p2d_100x300_b2"])
+@pytest.mark.parametrize("case", ["cd3d_20_b20", "cd3d_12x130x3_b3", "p2d_100x300_b2", "cd3d_20_b10_planes2",
+                                  "cd3d_12x70x4_b2_planes2"])
 def test_nonsymmetric_solvers_bit_exact(case):
     gen, nb, _ = CASES[case]
     rp, col, val = gen()
@@ -118,9 +127,9 @@ def test_nonsymmetric_solvers_bit_exact(case):
 
 
 @pytest.mark.parametrize("gen,nb", [
-    (lambda: ph.poisson3d_spd(24, 24, 24), 8),    # blocks of three planes: a coupling at nx * ny
+    (lambda: ph.poisson3d_spd(16, 16, 16), 3),    # blocks that cut a plane
     (lambda: ph.poisson2d(64, 64, -1.0), 7),      # blocks that cut a line
-], ids=["planes3", "cut"])
+], ids=["cut_plane", "cut_line"])
 def test_other_blocks_keep_level_schedule(gen, nb):
     rp, col, val = gen()
     with ph.Plan(rp, col, val, ipreflag=2, ILU_method=0, precon_thblock=nb) as P:
