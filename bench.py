@@ -66,13 +66,15 @@ def ilu0_apply_bytes(n, nnz_l, nnz_u):
     return 12 * (nnz_l + nnz_u) + 8 * n + 8 * n + 8 * n + 16 * n + 8 * n
 
 
-def bj_lines_bytes(n, dilu=False, fused=True):
+def bj_lines_bytes(n, dilu=False, fused=True, planes=False):
     """Block-Jacobi apply in line-wavefront form (k_bj_lines): forward r 8n
-    (and the fused update's q 8n, r written 8n), two couplings 16n, s written
-    8n (D-ILU: the scale 8n); backward s read 8n, two couplings 16n, the scale
-    8n, s written 8n."""
-    fwd = 8 * n + 16 * n + 8 * n + (16 * n if fused else 0) + (8 * n if dilu else 0)
-    return fwd + 40 * n
+    (and the fused update's q 8n, r written 8n), two couplings 16n (a third,
+    the plane neighbour's, 8n when blocks hold several planes), s written 8n
+    (D-ILU: the scale 8n); backward s read 8n, the couplings, the scale 8n, s
+    written 8n."""
+    cpl = 24 * n if planes else 16 * n
+    fwd = 8 * n + cpl + 8 * n + (16 * n if fused else 0) + (8 * n if dilu else 0)
+    return fwd + 24 * n + cpl
 
 
 def k1_bytes32(n, nnz):
@@ -284,12 +286,12 @@ def spmv_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 16 * n
 
 
-def gmres_step_bytes(n, nnz, nnz_l, nnz_u, j):
+def gmres_step_bytes(n, nnz, apply_bytes, j):
     """Inner step j (0-based) of right-preconditioned GMRES with MGS: z = M^-1 v_j
-    (the block-Jacobi ILU(0) apply), w = A z, j+1 MGS passes (read v_i, read
-    and write w: 24 B each), the norm of w (8 B) and the scaling into v_{j+1}
-    (read w, write v: 16 B)."""
-    return ilu0_apply_bytes(n, nnz_l, nnz_u) + spmv_bytes(n, nnz) + 24 * n * (j + 1) + 24 * n
+    (the block-Jacobi ILU(0) apply, apply_bytes), w = A z, j+1 MGS passes (read
+    v_i, read and write w: 24 B each), the norm of w (8 B) and the scaling into
+    v_{j+1} (read w, write v: 16 B)."""
+    return apply_bytes + spmv_bytes(n, nnz) + 24 * n * (j + 1) + 24 * n
 
 
 def run_c2(args):
@@ -336,7 +338,10 @@ def run_c2(args):
         if apply_ms is None:
             apply_ms = plan.time_kernel(2, 20)
             spmv_ms = plan.time_kernel(1, 20)
-            nl, nu = st["bj_nnz_l"], st["bj_nnz_u"]
+            # blocks of two grid planes: the line-wavefront apply (k_bj_lines)
+            lines_form = bool(st["bj_lines"])
+            ab = (bj_lines_bytes(n, fused=False, planes=True) if lines_form
+                  else ilu0_apply_bytes(n, st["bj_nnz_l"], st["bj_nnz_u"]))
         methods[name] = {"time_to_solution_s": float(np.mean(times)), "niter": niter, "iret": iret,
                          "final_estimate": float(hist[-1]), "true_rel_res": res, "setup_s": setup,
                          "device_ms": st["solve_device_ms"]}
@@ -344,12 +349,14 @@ def run_c2(args):
         plan.close()
     # GMRES(30)'s algorithmic bytes over its solve: inner step j of every cycle
     G = methods["gmres30"]
-    tot = sum(gmres_step_bytes(n, nnz, nl, nu, i % 30) for i in range(G["niter"]))
+    tot = sum(gmres_step_bytes(n, nnz, ab, i % 30) for i in range(G["niter"]))
     solve_gbs = tot / G["time_to_solution_s"] / 1e9
-    ab = ilu0_apply_bytes(n, nl, nu)
     ap_gbs = ab / (apply_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_bj_apply (block-Jacobi ILU(0), 64 blocks: forward and backward "
-                "level-scheduled sweeps, one workgroup per block)", "achieved": round(ap_gbs, 1),
+    kname = ("k_bj_lines (block-Jacobi ILU(0), 64 blocks of two grid planes: forward and backward sweeps as "
+             "line wavefronts, one workgroup per block), 88 B per row" if lines_form else
+             "k_bj_apply (block-Jacobi ILU(0), 64 blocks: forward and backward level-scheduled sweeps, one "
+             "workgroup per block)")
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(ap_gbs, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ap_gbs / PEAK_HBM_GBS, 4), "traffic": None,
                 "bytes_per_launch": ab, "mean_ms": round(apply_ms, 4),
                 "solve_level": {"gmres30_algorithmic_bytes": tot, "achieved": round(solve_gbs, 1),
