@@ -856,9 +856,20 @@ struct LineOps {
 template <bool FWD, int MODE>
 constexpr bool line_cell_first() { return !FWD && MODE == BJ_ILU0; }
 
+// Blocks of one plane with at most four wavefronts stage through a 16-cell
+// ring per line instead (kLA doubles with padding): each group loads one
+// aligned 8-cell chunk of every line and stores its newest complete one, so
+// the vector traffic moves whole 64-byte sectors when lines start on one.
+// Row stride 20 doubles (16 ring cells, padding slot 16): a step's lanes read
+// cell (a - lane) mod 16 of their rows, which at this stride fall in distinct
+// banks (at 17 they would all share a bank pair).
+constexpr int kLA = 20;
+__host__ __device__ constexpr bool line_aligned(bool planes, int W) { return !planes && W <= 4; }
+
 // Dynamic LDS of k_bj_lines, in doubles: slots, line hand-over, plane ring.
 __host__ __device__ inline size_t line_lds_doubles(int W, int nx, int planes) {
-  return (size_t)64 * W * kLS + (size_t)W * nx + (planes > 1 ? (size_t)64 * W * kLR : 0);
+  const size_t slots = (size_t)64 * W * (line_aligned(planes > 1, W) ? kLA : kLS);
+  return slots + (size_t)W * nx + (planes > 1 ? (size_t)64 * W * kLR : 0);
 }
 
 template <bool FWD, int MODE, bool FUSED, bool PL, int W>
@@ -887,21 +898,30 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
     return lo + ((FWD ? kp : np - 1 - kp) * ny + (FWD ? jj : ny - 1 - jj)) * nx;
   };
   const long sub = (long)lo + (long)(kp * ny + 64 * jw) * nx;
-  double* ws = lds + w * 64 * kLS;  // this wavefront's slots: [line][step of the group]
-  double* my = ws + lane * kLS;
-  double* hand = lds + 64 * W * kLS;  // [wavefront][cell]: lane 63's values
+  constexpr bool ALN = line_aligned(PL, W);
+  constexpr int kSL = ALN ? kLA : kLS;
+  double* ws = lds + w * 64 * kSL;  // this wavefront's slots: [line][step of the group] / [line][cell mod 16]
+  double* my = ws + lane * kSL;
+  double* hand = lds + 64 * W * kSL;  // [wavefront][cell]: lane 63's values
   double* ring = hand + W * nx;       // [thread][step mod kLR]: every line's values (P > 1)
   // staging lanes: lane 8j + c moves cell c of the group for local line 8k + j;
   // rows of the current group (stores) and of the next (loads), -1 off the block
   const int sj = lane >> 3, sc = lane & 7;
-  int rcur[8], rnext[8];
-  auto stage_rows = [&](int t1) {
+  int rcur[8], rnext[8], cnext[8];
+  // ALN = false: cells t1 - j - lag + 0..7 of the group starting at step t1;
+  // ALN = true: the aligned chunk 8 m + 0..7, m = (t1 - j - lag + off) >> 3
+  // (off = 7: the chunk a group's inputs first reach; off = 0 and t1 = t0:
+  // the newest chunk complete after group t0)
+  auto stage_rows = [&](int t1, int off, bool first = false) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int jj = 64 * jw + 8 * k + sj;
-      const int ip = t1 + sc - jj - lag;
-      const bool v = wact && jj < ny && (unsigned)ip < (unsigned)nx;
+      const int ip = ALN ? (((t1 - jj - lag + off) >> 3) << 3) + sc : t1 + sc - jj - lag;
+      // first: the chunk of a line's first cell, unless that cell starts a chunk
+      // (then the next call's chunk is the same one, loaded once)
+      const bool v = wact && jj < ny && (unsigned)ip < (unsigned)nx && !(first && ((t1 - jj - lag) & 7) == 0);
       rnext[k] = v ? line_row(jj) + (FWD ? ip : nx - 1 - ip) : -1;
+      cnext[k] = ip;
     }
   };
   double va[8], vb[8];
@@ -921,13 +941,15 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
         z0 = fma(-alpha, vb[k], va[k]);
         *(rnext[k] >= 0 ? rout + rnext[k] : sink) = z0;
       }
-      ws[(8 * k + sj) * kLS + sc] = z0;
+      // aligned ring: cells off the block go to the padding slot 16
+      ws[(8 * k + sj) * kSL + (ALN ? (rnext[k] >= 0 ? (cnext[k] & 15) : 16) : sc)] = z0;
       rcur[k] = rnext[k];
     }
   };
   auto stage_store = [&]() {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) *(rcur[k] >= 0 ? s + rcur[k] : sink) = ws[(8 * k + sj) * kLS + sc];
+    for (int k = 0; k < 8; ++k)
+      *(rcur[k] >= 0 ? s + rcur[k] : sink) = ws[(8 * k + sj) * kSL + (ALN ? (cnext[k] & 15) : sc)];
   };
   // factor loads: anti-diagonal dl = t - lag - 64 jw of the wavefront's own
   // lines for the next load and the entry offset of its line 0 on it
@@ -984,7 +1006,7 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
   LineOps R[kLG];
   // one group of kLG steps: the next group's vectors are in flight during it
   auto group = [&](int t0) {
-    stage_rows(t0 + kLG);
+    stage_rows(t0 + kLG, ALN ? 7 : 0);
     stage_load();
     wave_lds_sync();
     // the group's inputs and hand-over values, read before its first step
@@ -992,32 +1014,56 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
     double av[kLG], hv[kLG], pv[kLG];
 #pragma unroll
     for (int u = 0; u < kLG; ++u) {
-      av[u] = my[u];
+      av[u] = my[ALN ? ((t0 + u - j - lag) & 15) : u];
       const int ip = min(max(t0 + u - j - lag, 0), nx - 1);
       hv[u] = (W > 1 && jw > 0) ? hand[(w - 1) * nx + ip] : kZero;
       if (PL) pv[u] = (kp > 0) ? ring[(tid - 64 * WP) * kLR + (t0 + u + kLR - kLPlag) % kLR] : kZero;
     }
 #pragma unroll
     for (int u = 0; u < kLG; ++u) {
-      my[u] = step(t0 + u, av[u], hv[u], PL ? pv[u] : kZero, R[u]);
+      const int ip = t0 + u - j - lag;
+      const bool on = lact && (unsigned)ip < (unsigned)nx;
+      my[ALN ? (on ? (ip & 15) : 16) : u] = step(t0 + u, av[u], hv[u], PL ? pv[u] : kZero, R[u]);
       load(t0 + u + kLG, R[u]);
     }
     wave_lds_sync();
-    stage_store();
+    if (ALN) {
+      // the newest complete chunk of every line: its cells were the ring's
+      // since the group before (in-place: input, then the row's value)
+      const int rl[8] = {rnext[0], rnext[1], rnext[2], rnext[3], rnext[4], rnext[5], rnext[6], rnext[7]};
+      const int cl[8] = {cnext[0], cnext[1], cnext[2], cnext[3], cnext[4], cnext[5], cnext[6], cnext[7]};
+      stage_rows(t0, 0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) rcur[k] = rnext[k];
+      stage_store();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) { rnext[k] = rl[k]; cnext[k] = cl[k]; }
+    } else {
+      stage_store();
+    }
     wave_lds_sync();
     stage_fill();
     if (W > 1) lds_barrier();
   };
-  stage_rows(0);
+  if (ALN) {
+    // the chunks group 0 reads: cells a .. a + 7 of every line lie in chunks
+    // a >> 3 and (a + 7) >> 3
+    stage_rows(0, 0, true);
+    stage_load();
+    stage_fill();
+    wave_lds_sync();
+  }
+  stage_rows(0, ALN ? 7 : 0);
   stage_load();
   stage_fill();
 #pragma unroll
   for (int u = 0; u < kLG; ++u) load(u, R[u]);
   // the first group runs outside the loop, so that the loop is entered with
   // the memory operations of a whole group in flight on both of its paths
-  // (the compiler's wait counts then match the steady state)
+  // (the compiler's wait counts then match the steady state); with aligned
+  // chunks one more group stores the lines' last chunks
   group(0);
-  for (int t0 = kLG; t0 < nsteps; t0 += kLG) group(t0);
+  for (int t0 = kLG; t0 < nsteps + (ALN ? kLG - 1 : 0); t0 += kLG) group(t0);
 }
 
 // One workgroup of 64 W threads per block: forward then backward sweep.  The
