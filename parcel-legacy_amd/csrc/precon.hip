@@ -908,20 +908,41 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
   // rows of the current group (stores) and of the next (loads), -1 off the block
   const int sj = lane >> 3, sc = lane & 7;
   int rcur[8], rnext[8], cnext[8];
-  // ALN = false: cells t1 - j - lag + 0..7 of the group starting at step t1;
-  // ALN = true: the aligned chunk 8 m + 0..7, m = (t1 - j - lag + off) >> 3
-  // (off = 7: the chunk a group's inputs first reach; off = 0 and t1 = t0:
-  // the newest chunk complete after group t0)
-  auto stage_rows = [&](int t1, int off, bool first = false) {
+  // cells t1 - j - lag + 0..7 of the group starting at step t1 (ALN = false)
+  auto stage_rows = [&](int t1, int off) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int jj = 64 * jw + 8 * k + sj;
-      const int ip = ALN ? (((t1 - jj - lag + off) >> 3) << 3) + sc : t1 + sc - jj - lag;
-      // first: the chunk of a line's first cell, unless that cell starts a chunk
-      // (then the next call's chunk is the same one, loaded once)
-      const bool v = wact && jj < ny && (unsigned)ip < (unsigned)nx && !(first && ((t1 - jj - lag) & 7) == 0);
+      const int ip = t1 + off + sc - jj - lag;
+      const bool v = wact && jj < ny && (unsigned)ip < (unsigned)nx;
       rnext[k] = v ? line_row(jj) + (FWD ? ip : nx - 1 - ip) : -1;
       cnext[k] = ip;
+    }
+  };
+  // ALN: the chunk cells advance by exactly 8 per group, so each staging lane
+  // carries the cell of its input chunk (loaded during a group for the next)
+  // and of its output chunk (stored after a group) and the first row of its
+  // line (backward: the row of cell 0, rows descending)
+  int lrow[8], cin[8], cout[8];
+  bool lv[8];
+  if (ALN) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int jj = 64 * jw + 8 * k + sj;
+      const int a0 = -jj - lag;  // the line's cell at step 0
+      lv[k] = wact && jj < ny;
+      lrow[k] = line_row(min(jj, ny - 1)) + (FWD ? 0 : nx - 1);
+      cout[k] = ((a0 >> 3) << 3) + sc;
+      cin[k] = (((a0 + 15) >> 3) << 3) + sc;
+    }
+  }
+  auto chunk_rows = [&](const int (&cell)[8], bool first) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int jj = 64 * jw + 8 * k + sj;
+      const bool v = lv[k] && (unsigned)cell[k] < (unsigned)nx && !(first && ((-jj - lag) & 7) == 0);
+      rnext[k] = v ? (FWD ? lrow[k] + cell[k] : lrow[k] - cell[k]) : -1;
+      cnext[k] = cell[k];
     }
   };
   double va[8], vb[8];
@@ -1006,7 +1027,8 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
   LineOps R[kLG];
   // one group of kLG steps: the next group's vectors are in flight during it
   auto group = [&](int t0) {
-    stage_rows(t0 + kLG, ALN ? 7 : 0);
+    if (ALN) chunk_rows(cin, false);
+    else stage_rows(t0 + kLG, 0);
     stage_load();
     wave_lds_sync();
     // the group's inputs and hand-over values, read before its first step
@@ -1032,12 +1054,17 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
       // since the group before (in-place: input, then the row's value)
       const int rl[8] = {rnext[0], rnext[1], rnext[2], rnext[3], rnext[4], rnext[5], rnext[6], rnext[7]};
       const int cl[8] = {cnext[0], cnext[1], cnext[2], cnext[3], cnext[4], cnext[5], cnext[6], cnext[7]};
-      stage_rows(t0, 0);
+      chunk_rows(cout, false);
 #pragma unroll
       for (int k = 0; k < 8; ++k) rcur[k] = rnext[k];
       stage_store();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) { rnext[k] = rl[k]; cnext[k] = cl[k]; }
+      for (int k = 0; k < 8; ++k) {
+        rnext[k] = rl[k];
+        cnext[k] = cl[k];
+        cin[k] += kLG;
+        cout[k] += kLG;
+      }
     } else {
       stage_store();
     }
@@ -1047,13 +1074,18 @@ __device__ __forceinline__ void line_sweep(const LineArgs& A, int lo, int ny, in
   };
   if (ALN) {
     // the chunks group 0 reads: cells a .. a + 7 of every line lie in chunks
-    // a >> 3 and (a + 7) >> 3
-    stage_rows(0, 0, true);
+    // a >> 3 (cout's) and (a + 7) >> 3 (cin's minus one)
+    chunk_rows(cout, true);
     stage_load();
     stage_fill();
     wave_lds_sync();
+    int c1[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c1[k] = cin[k] - kLG;
+    chunk_rows(c1, false);
+  } else {
+    stage_rows(0, 0);
   }
-  stage_rows(0, ALN ? 7 : 0);
   stage_load();
   stage_fill();
 #pragma unroll
