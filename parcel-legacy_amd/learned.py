@@ -73,7 +73,7 @@ def solve_learned(net, rp, col, vals, b, x=None, rtol=1e-10, itrmax=20000, preci
     FP32 by default; precision 32: FP64 refinement around FP32 PCG, itrmax
     then counts corrections).  A system that ends with iret != 0 is solved
     again by the plain FP64 PCG + block-Jacobi ILU(0) (fallback_blocks
-    blocks), which then judges its convergence.  Returns (x, niter, iret,
+    blocks), which then decides its convergence.  Returns (x, niter, iret,
     stats, seconds); stats["fell_back"] lists the systems that fell back."""
     nsys, n = b.shape
     with ph.Batch(rp, col, vals, ph.BATCH_FACTOR) as B:
